@@ -1,0 +1,420 @@
+/*
+ * FP64 CPU restatement of pygetm's 3D tracer-transport step (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py as the checker; never part of the product path.
+ *
+ * Written from the reference algorithm, keeping its FP64 operation order so
+ * that a build without FMA contraction (-ffp-contract=off, x86-64 baseline)
+ * reproduces the Fortran bit for bit:
+ *   limiters           src/operators/advection_schemes/limiters.h:1-44
+ *   x / y sweeps       src/operators/advection_schemes/advection.F90.template:23-221
+ *   z sweep            src/operators/advection_schemes/advection.F90.template:223-374
+ *   diffusion          src/operators/diffusion.F90:135-314
+ *   layer loop         python/src/wrapper.F90:213-273
+ *   stage order        python/pygetm/operators.py:95-193
+ *   tracer step        python/pygetm/tracer.py:193-276
+ * MIN/MAX follow flang's lowering (max(a,b) = a > b ? a : b, folded left to
+ * right) and CEILING saturates to the int32 range with NaN -> 0, as measured
+ * on amdflang 22, so NaN edge cases (HSIMT at cfl = 1) match too.
+ */
+#include "getm_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum { HSIMT = 1, MUSCL = 2, P2_PDM = 3, SPLMAX13 = 4, SUPERBEE = 5, UPSTREAM = 6 };
+
+static inline double fmax2(double a, double b) { return a > b ? a : b; }
+static inline double fmin2(double a, double b) { return a < b ? a : b; }
+static inline double fmin3(double a, double b, double c) { return fmin2(fmin2(a, b), c); }
+
+static int fceiling(double x)
+{
+    if (x != x) return 0;
+    double c = ceil(x);
+    if (c >= 2147483647.0) return INT_MAX;
+    if (c <= -2147483648.0) return INT_MIN;
+    return (int)c;
+}
+
+/* limiters.h:1-44 */
+double oracle_limiter(int scheme, double cfl, double ratio)
+{
+    switch (scheme) {
+    case HSIMT: {
+        double kappa = 1.0 - cfl;
+        double x = 0.25 * (kappa - (1.0 / 3.0) / kappa);
+        double lim = (0.5 + x) + (0.5 - x) * ratio;
+        lim = fmax2(0.0, lim);
+        return fmin3(2.0 * ratio, lim, 2.0);
+    }
+    case MUSCL:
+        return fmin3(2.0 * ratio, 0.5 * (1.0 + ratio), 2.0);
+    case P2_PDM: {
+        double x = (1.0 / 6.0) * (1.0 - 2.0 * cfl);
+        double lim = (0.5 + x) + (0.5 - x) * ratio;
+        return fmin3((2.0 * ratio) / (cfl + 1.e-10), lim, 2.0 / (1.0 - cfl));
+    }
+    case SPLMAX13:
+        return fmin3(2.0 * ratio, (1.0 / 3.0) * fmax2(1.0 + 2.0 * ratio, 2.0 + ratio), 2.0);
+    case SUPERBEE:
+        return fmax2(fmin2(2.0 * ratio, 1.0), fmin2(ratio, 2.0));
+    default: /* UPSTREAM */
+        return 0.0;
+    }
+}
+
+static inline int is_flux_face(int m) { return m == 1 || m == 2; }
+
+/* Limited upwind face value (template:66-88, 166-188): f_up2 = upstream-upstream,
+ * f_up = upstream, f_dn = downstream. */
+static inline double limited_face_value(int scheme, double cfl, double fu, double fuu, double fd)
+{
+    double deltaf = fd - fu;
+    double deltafu = fu - fuu;
+    if (deltaf * deltafu > 0) {
+        double ratio = deltafu / deltaf;
+        double limiter = oracle_limiter(scheme, cfl, ratio);
+        fu = fu + ((0.5 * limiter) * (1.0 - cfl)) * deltaf;
+    }
+    return fu;
+}
+
+static int valid_scheme(int s) { return s >= HSIMT && s <= UPSTREAM; }
+
+/* Per-layer x or y half-sweep.  dir 1: stride between neighbours = 1 (i);
+ * dir 2: stride = nx_ (j).  Faces run over [min-1, max] along the sweep
+ * direction and [min, max] across it; cells over [min, max] x [min, max]. */
+static void sweep_layer(int dir, int scheme, int nx_, int ny_, int halox, int haloy,
+                        const int* uvmask, const double* metric1, const double* metric2,
+                        const double* huv, const double* uv, const int* tmask, const double* iarea,
+                        const double* ah, double dt, double* h, double* f, double* QU, double* flux)
+{
+    const int imin = halox, imax = nx_ - halox - 1, jmin = haloy, jmax = ny_ - haloy - 1;
+    const int s = dir == 1 ? 1 : nx_;
+    const int fi0 = dir == 1 ? imin - 1 : imin, fj0 = dir == 1 ? jmin : jmin - 1;
+    for (int j = fj0; j <= jmax; j++) {
+        for (int i = fi0; i <= imax; i++) {
+            const int p = j * nx_ + i;
+            if (is_flux_face(uvmask[p])) {
+                const double vel = uv[p];
+                double q, cfl;
+                if (dir == 1) { /* QU = u*hu*dyu, cfl = |u*dt*idxu| */
+                    q = (vel * huv[p]) * metric2[p];
+                    cfl = fabs((vel * dt) * metric1[p]);
+                } else {        /* QV = v*hv*dxv, cfl = |v*dt*idyv| */
+                    q = (vel * huv[p]) * metric1[p];
+                    cfl = fabs((vel * dt) * metric2[p]);
+                }
+                double fu, fuu, fd;
+                if (vel > 0) {
+                    fu = f[p];
+                    fuu = fu;
+                    if (is_flux_face(uvmask[p - s])) fuu = f[p - s];
+                    fd = f[p + s];
+                } else {
+                    fu = f[p + s];
+                    fuu = fu;
+                    if (is_flux_face(uvmask[p + s])) fuu = f[p + 2 * s];
+                    fd = f[p];
+                }
+                fu = limited_face_value(scheme, cfl, fu, fuu, fd);
+                QU[p] = q;
+                flux[p] = q * fu;
+            } else {
+                QU[p] = 0.0;
+                flux[p] = 0.0;
+            }
+        }
+    }
+    if (ah) {
+        for (int j = fj0; j <= jmax; j++) {
+            for (int i = fi0; i <= imax; i++) {
+                const int p = j * nx_ + i;
+                if (is_flux_face(uvmask[p])) {
+                    if (dir == 1)
+                        flux[p] = flux[p] - (((ah[p] * huv[p]) * (f[p + s] - f[p])) * metric1[p]) * metric2[p];
+                    else
+                        flux[p] = flux[p] - (((ah[p] * huv[p]) * (f[p + s] - f[p])) * metric2[p]) * metric1[p];
+                }
+            }
+        }
+    }
+    for (int j = jmin; j <= jmax; j++) {
+        for (int i = imin; i <= imax; i++) {
+            const int p = j * nx_ + i;
+            if (tmask[p] == 1) {
+                double hfo = h[p] * f[p];
+                h[p] = h[p] - (dt * (QU[p] - QU[p - s])) * iarea[p];
+                double advn = (flux[p] - flux[p - s]) * iarea[p];
+                f[p] = (hfo - dt * advn) / h[p];
+            }
+        }
+    }
+}
+
+int oracle_advection_uv(int direction, int scheme, int nx, int ny, int nk, int halox, int haloy,
+                        const int* uvmask, const double* metric1, const double* metric2,
+                        const double* huv, const double* uv, const int* tmask, const double* iarea,
+                        const double* ah, double dt, double* h, double* var)
+{
+    if ((direction != 1 && direction != 2) || !valid_scheme(scheme) || nx < 1 || ny < 1 || nk < 1 ||
+        halox < 2 || haloy < 2)
+        return 1;
+    const int nx_ = nx + 2 * halox, ny_ = ny + 2 * haloy;
+    const size_t n2 = (size_t)nx_ * ny_;
+    double* QU = calloc(n2, sizeof(double));
+    double* flux = calloc(n2, sizeof(double));
+    if (!QU || !flux) { free(QU); free(flux); return 2; }
+    for (int k = 0; k < nk; k++)
+        sweep_layer(direction, scheme, nx_, ny_, halox, haloy, uvmask, metric1, metric2,
+                    huv + k * n2, uv + k * n2, tmask, iarea, ah, dt, h + k * n2, var + k * n2, QU, flux);
+    free(QU);
+    free(flux);
+    return 0;
+}
+
+/* One column of the z sweep; h and f are strided by n2 (layer stride). */
+static int w_column(int scheme, int nz, size_t n2, const double* w, const double* w_var, double dt,
+                    double* h, double* f, double* flux, double* cfl0, double* cfls, double* haux,
+                    double* faux)
+{
+    const int itersmax = 200;
+    const int km = nz; /* Fortran kmax; Fortran index k -> C k-1 for layers, k for interfaces */
+    flux[0] = 0.0;
+    flux[km] = 0.0;
+    for (int k = 1; k <= km - 1; k++)
+        cfl0[k] = (fabs(w_var[k * n2]) * dt) / fmin2(h[(k - 1) * n2], h[k * n2]);
+
+    int iters = 1;
+    double itersm1 = 1.0, dtk = dt;
+    int it = 1;
+    while (it <= iters) {
+        if (it == 1) {
+            for (int k = 1; k <= km; k++) {
+                haux[k] = h[(k - 1) * n2];
+                faux[k] = f[(k - 1) * n2];
+            }
+            for (int k = 1; k <= km - 1; k++) cfls[k] = cfl0[k];
+        } else {
+            for (int k = 1; k <= km - 1; k++)
+                cfls[k] = (fabs(w_var[k * n2]) * dt) / fmin2(haux[k], haux[k + 1]);
+        }
+        if (iters < itersmax) {
+            /* MAXVAL of an empty array is -HUGE */
+            double cfl = -1.7976931348623157e308;
+            for (int k = 1; k <= km - 1; k++) cfl = fmax2(cfl, cfls[k]);
+            int c = fceiling(cfl);
+            int iters_new = c > 1 ? c : 1;
+            if (iters_new > iters) {
+                iters = iters_new > itersmax ? itersmax : iters_new;
+                itersm1 = 1.0 / iters;
+                dtk = dt * itersm1;
+                if (it > 1) {
+                    it = 1;
+                    continue;
+                }
+            }
+        }
+        for (int k = 1; k <= km - 1; k++) {
+            double cfl = cfls[k] * itersm1;
+            double fu, fuu, fd;
+            if (w_var[k * n2] > 0) {
+                fu = faux[k];
+                fuu = k > 1 ? faux[k - 1] : fu;
+                fd = faux[k + 1];
+            } else {
+                fu = faux[k + 1];
+                fuu = k < km - 1 ? faux[k + 2] : fu;
+                fd = faux[k];
+            }
+            fu = limited_face_value(scheme, cfl, fu, fuu, fd);
+            flux[k] = w_var[k * n2] * fu;
+        }
+        const int last = it == iters;
+        for (int k = 1; k <= km; k++) {
+            double hfo = haux[k] * faux[k];
+            double hnew = haux[k] - dtk * (w[k * n2] - w[(k - 1) * n2]);
+            double advn = flux[k] - flux[k - 1];
+            double fnew = (hfo - dtk * advn) / hnew;
+            if (last) {
+                h[(k - 1) * n2] = hnew;
+                f[(k - 1) * n2] = fnew;
+            } else {
+                haux[k] = hnew;
+                faux[k] = fnew;
+            }
+        }
+        it = it + 1;
+    }
+    return iters;
+}
+
+int oracle_advection_w(int scheme, int nx, int ny, int nz, int halox, int haloy,
+                       const double* w, const double* w_var, const int* tmask, double dt,
+                       double* h, double* var, int* max_iters)
+{
+    if (!valid_scheme(scheme) || nx < 1 || ny < 1 || nz < 1 || halox < 0 || haloy < 0) return 1;
+    const int nx_ = nx + 2 * halox, ny_ = ny + 2 * haloy;
+    const size_t n2 = (size_t)nx_ * ny_;
+    double* buf = calloc((size_t)5 * (nz + 2), sizeof(double));
+    if (!buf) return 2;
+    double *flux = buf, *cfl0 = buf + (nz + 2), *cfls = buf + 2 * (nz + 2), *haux = buf + 3 * (nz + 2),
+           *faux = buf + 4 * (nz + 2);
+    int mi = 0;
+    for (int j = haloy; j < ny_ - haloy; j++) {
+        for (int i = halox; i < nx_ - halox; i++) {
+            const size_t p = (size_t)j * nx_ + i;
+            if (tmask[p] != 1) continue;
+            int it = w_column(scheme, nz, n2, w + p, w_var + p, dt, h + p, var + p, flux, cfl0, cfls,
+                              haux, faux);
+            if (it > mi) mi = it;
+        }
+    }
+    free(buf);
+    if (max_iters) *max_iters = mi;
+    return 0;
+}
+
+int oracle_vdiff_prepare(int nx, int ny, int nz, int halox, int haloy, double molecular,
+                         const double* nuh, double dt, double cnpar, const double* dzo,
+                         const double* dzn, double* auxo, double* auxn)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || halox < 0 || haloy < 0) return 1;
+    if (nz == 1) return 0;
+    const int nx_ = nx + 2 * halox, ny_ = ny + 2 * haloy;
+    const size_t n2 = (size_t)nx_ * ny_;
+    for (int k = 0; k < nz - 1; k++) {
+        for (int j = haloy; j < ny_ - haloy; j++) {
+            for (int i = halox; i < nx_ - halox; i++) {
+                const size_t p = (size_t)j * nx_ + i;
+                /* nuh passed as nuh(:,:,1:nz-1): layer-index k uses interface k+1 */
+                double x = (2.0 * dt) * (nuh[(k + 1) * n2 + p] + molecular);
+                auxo[k * n2 + p] = ((1.0 - cnpar) * x) / (dzo[(k + 1) * n2 + p] + dzo[k * n2 + p]);
+                auxn[k * n2 + p] = (cnpar * x) / (dzn[(k + 1) * n2 + p] + dzn[k * n2 + p]);
+            }
+        }
+    }
+    return 0;
+}
+
+int oracle_vdiff_apply(int nx, int ny, int nz, int halox, int haloy, const int* mask,
+                       const double* dzo, const double* dzn, const double* auxo,
+                       const double* auxn, double* var, const double* ea2, const double* ea4)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || halox < 0 || haloy < 0) return 1;
+    const int nx_ = nx + 2 * halox, ny_ = ny + 2 * haloy;
+    const size_t n2 = (size_t)nx_ * ny_;
+    if (nz == 1) {
+        /* diffusion.F90:224-231 operates on the whole array, halos and land included */
+        for (size_t p = 0; p < n2; p++) {
+            double a4 = dzo[p] * var[p];
+            double a2 = dzn[p];
+            if (ea4) a4 = a4 + ea4[p];
+            if (ea2) a2 = a2 - ea2[p];
+            var[p] = a4 / a2;
+        }
+        return 0;
+    }
+    double* buf = malloc((size_t)4 * nz * sizeof(double));
+    if (!buf) return 2;
+    double *a2 = buf, *a4 = buf + nz, *ru = buf + 2 * nz, *qu = buf + 3 * nz;
+    for (int j = haloy; j < ny_ - haloy; j++) {
+        for (int i = halox; i < nx_ - halox; i++) {
+            const size_t p = (size_t)j * nx_ + i;
+            if (mask[p] != 1) continue;
+            const double* c = var + p;
+#define V(k) c[(size_t)(k) * n2]
+#define AO(k) auxo[(size_t)(k) * n2 + p]
+#define AN(k) auxn[(size_t)(k) * n2 + p]
+            const int kt = nz - 1;
+            a2[kt] = dzn[kt * n2 + p] + AN(kt - 1);
+            a4[kt] = V(kt) * dzo[kt * n2 + p] + (V(kt - 1) - V(kt)) * AO(kt - 1);
+            for (int k = 1; k < kt; k++) {
+                a2[k] = (dzn[k * n2 + p] + AN(k)) + AN(k - 1);
+                a4[k] = (V(k) * dzo[k * n2 + p] + (V(k + 1) - V(k)) * AO(k)) + (V(k - 1) - V(k)) * AO(k - 1);
+            }
+            a2[0] = dzn[p] + AN(0);
+            a4[0] = V(0) * dzo[p] + (V(1) - V(0)) * AO(0);
+            if (ea2)
+                for (int k = 0; k < nz; k++) a2[k] = a2[k] - ea2[k * n2 + p];
+            if (ea4)
+                for (int k = 0; k < nz; k++) a4[k] = a4[k] + ea4[k * n2 + p];
+            /* a1(k) = -auxn(k-1) (k >= 1), a3(k) = -auxn(k) (k <= nz-2) */
+            ru[kt] = (-AN(kt - 1)) / a2[kt];
+            qu[kt] = a4[kt] / a2[kt];
+            for (int k = kt - 1; k >= 1; k--) {
+                double den = a2[k] - (-AN(k)) * ru[k + 1];
+                ru[k] = (-AN(k - 1)) / den;
+                qu[k] = (a4[k] - (-AN(k)) * qu[k + 1]) / den;
+            }
+            qu[0] = (a4[0] - (-AN(0)) * qu[1]) / (a2[0] - (-AN(0)) * ru[1]);
+            double* vo = var + p;
+            vo[0] = qu[0];
+            for (int k = 1; k <= kt; k++) vo[(size_t)k * n2] = qu[k] - ru[k] * vo[(size_t)(k - 1) * n2];
+#undef V
+#undef AO
+#undef AN
+        }
+    }
+    free(buf);
+    return 0;
+}
+
+int oracle_transport_step(int scheme, int nx, int ny, int nz, int halox, int haloy,
+                          const int* tmask, const int* umask, const int* vmask,
+                          const double* idxu, const double* dyu, const double* dxv,
+                          const double* idyv, const double* iarea,
+                          const double* ho, const double* hn, const double* hu, const double* hv,
+                          const double* u, const double* v, const double* w, const double* nuh,
+                          const double* ah_u, const double* ah_v, double dt, double cnpar,
+                          double molecular, int ntracers, double* tracers, double* h_out)
+{
+    if (!valid_scheme(scheme) || ntracers < 0 || nx < 1 || ny < 1 || nz < 1) return 1;
+    const int nx_ = nx + 2 * halox, ny_ = ny + 2 * haloy;
+    const size_t n3 = (size_t)nx_ * ny_ * nz;
+    double* cur = malloc(n3 * sizeof(double));
+    double* h = malloc(n3 * sizeof(double));
+    double* auxo = calloc(n3, sizeof(double));
+    double* auxn = calloc(n3, sizeof(double));
+    int rc = 0;
+    if (!cur || !h || !auxo || !auxn) { rc = 2; goto out; }
+    memcpy(cur, ho, n3 * sizeof(double));
+    const double half = 0.5 * dt;
+    /* ufirst = False: adv1 = v_3d, adv2 = u_3d (operators.py:159-162) */
+    for (int stage = 0; stage < 5; stage++) {
+        for (int t = 0; t < ntracers; t++) {
+            double* var = tracers + (size_t)t * n3;
+            memcpy(h, cur, n3 * sizeof(double));
+            if (stage == 0 || stage == 4)
+                rc = oracle_advection_uv(2, scheme, nx, ny, nz, halox, haloy, vmask, dxv, idyv, hv, v,
+                                         tmask, iarea, ah_v, half, h, var);
+            else if (stage == 1 || stage == 3)
+                rc = oracle_advection_uv(1, scheme, nx, ny, nz, halox, haloy, umask, idxu, dyu, hu, u,
+                                         tmask, iarea, ah_u, half, h, var);
+            else
+                rc = oracle_advection_w(scheme, nx, ny, nz, halox, haloy, w, w, tmask, dt, h, var, NULL);
+            if (rc) goto out;
+        }
+        if (ntracers > 0) memcpy(cur, h, n3 * sizeof(double));
+    }
+    if (h_out) memcpy(h_out, cur, n3 * sizeof(double));
+    if (ntracers > 0) {
+        rc = oracle_vdiff_prepare(nx, ny, nz, halox, haloy, molecular, nuh, dt, cnpar, hn, hn, auxo, auxn);
+        if (rc) goto out;
+        for (int t = 0; t < ntracers; t++) {
+            rc = oracle_vdiff_apply(nx, ny, nz, halox, haloy, tmask, hn, hn, auxo, auxn,
+                                    tracers + (size_t)t * n3, NULL, NULL);
+            if (rc) goto out;
+        }
+    }
+out:
+    free(cur);
+    free(h);
+    free(auxo);
+    free(auxn);
+    return rc;
+}

@@ -1,0 +1,70 @@
+/*
+ * FP64 CPU restatement of pygetm's 3D tracer-transport step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP product path is
+ * compared against; the product never links, loads or calls it.
+ *
+ * Layout: every 3D field is a C-order (nz, ny_, nx_) block (Fortran
+ * (nx_, ny_, nz), i fastest), 2D fields are (ny_, nx_), with nx_ = nx + 2*halox
+ * and ny_ = ny + 2*haloy.  Interface fields (w, nuh) have nz+1 levels,
+ * level 0 = bottom.  Masks are int32.
+ *
+ * All functions return 0 on success, nonzero on bad arguments.
+ */
+#ifndef GETM_ORACLE_H
+#define GETM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One directional half-sweep over nk layers.
+ * direction 1 (x, u2d): metric1 = idxu, metric2 = dyu   (advection.F90.template:23-121)
+ * direction 2 (y, v2d): metric1 = dxv,  metric2 = idyv  (advection.F90.template:123-221)
+ * ah may be NULL (no horizontal diffusion). */
+int oracle_advection_uv(int direction, int scheme, int nx, int ny, int nk, int halox, int haloy,
+                        const int* uvmask, const double* metric1, const double* metric2,
+                        const double* huv, const double* uv, const int* tmask, const double* iarea,
+                        const double* ah, double dt, double* h, double* var);
+
+/* Vertical sweep with CFL sub-stepping (advection.F90.template:223-374).
+ * Returns the largest iteration count used in *max_iters if non-NULL. */
+int oracle_advection_w(int scheme, int nx, int ny, int nz, int halox, int haloy,
+                       const double* w, const double* w_var, const int* tmask, double dt,
+                       double* h, double* var, int* max_iters);
+
+/* theta-method matrix set-up (diffusion.F90:135-204): fills auxo, auxn (nz, ny_, nx_);
+ * level k holds interface k+1 (between layers k and k+1).  nuh has nz+1 levels. */
+int oracle_vdiff_prepare(int nx, int ny, int nz, int halox, int haloy, double molecular,
+                         const double* nuh, double dt, double cnpar, const double* dzo,
+                         const double* dzn, double* auxo, double* auxn);
+
+/* RHS + Thomas solve per mask==1 column (diffusion.F90:206-314).  ea2/ea4 may be NULL. */
+int oracle_vdiff_apply(int nx, int ny, int nz, int halox, int haloy, const int* mask,
+                       const double* dzo, const double* dzn, const double* auxo,
+                       const double* auxn, double* var, const double* ea2, const double* ea4);
+
+/* One tracer-transport step for ntracers tracers (stacked (T, nz, ny_, nx_)),
+ * in the order of Advection.apply_3d_batch (operators.py:135-193, ufirst=False:
+ * v dt/2, u dt/2, w dt, u dt/2, v dt/2, every tracer restarting each stage
+ * from the shared thickness) followed by TracerCollection._diffuse
+ * (tracer.py:248-276: prepare once with dzo = dzn = hn, apply per tracer,
+ * no sources).  ah_u/ah_v may be NULL.  h_out (nz, ny_, nx_) receives the
+ * advected thickness (the operator's work array) if non-NULL. */
+int oracle_transport_step(int scheme, int nx, int ny, int nz, int halox, int haloy,
+                          const int* tmask, const int* umask, const int* vmask,
+                          const double* idxu, const double* dyu, const double* dxv,
+                          const double* idyv, const double* iarea,
+                          const double* ho, const double* hn, const double* hu, const double* hv,
+                          const double* u, const double* v, const double* w, const double* nuh,
+                          const double* ah_u, const double* ah_v, double dt, double cnpar,
+                          double molecular, int ntracers, double* tracers, double* h_out);
+
+/* Limiter value for a given scheme, cfl and slope ratio (limiters.h:1-44);
+ * exported for the closed-form tests. */
+double oracle_limiter(int scheme, double cfl, double ratio);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
