@@ -1,0 +1,74 @@
+// Internal object layouts and launcher declarations shared by the C-ABI
+// (capi.hip) and the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/getm_amd.h"
+
+namespace getm {
+
+// A horizontal grid (T, U or V points) with halo, as create_grid builds it
+// (python/src/wrapper.F90:48-59).  Field pointers are caller-owned device
+// buffers registered by name with grid_set_array.
+struct Grid {
+    int imin, imax, jmin, jmax, kmin, kmax, halox, haloy;
+    int nx_, ny_, nz;  // padded extents
+    const int* mask = nullptr;
+    const double* dx = nullptr;
+    const double* dy = nullptr;
+    const double* idx = nullptr;
+    const double* idy = nullptr;
+    const double* iarea = nullptr;
+    const double* area = nullptr;
+    const double* hn = nullptr;
+    const double* ho = nullptr;
+    size_t n2() const { return (size_t)nx_ * ny_; }
+    size_t n3() const { return n2() * nz; }
+};
+
+// Growable device scratch owned by an operator object.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+};
+
+struct Advection {
+    int scheme;
+    const Grid* tgrid;
+    DeviceBuffer work;    // out-of-place target of the x/y sweeps (nk layers)
+    DeviceBuffer wsh;     // z-sweep sub-iteration thickness
+    DeviceBuffer wsf;     // z-sweep sub-iteration tracer
+    DeviceBuffer stats;   // int32[2]: largest sub-iteration count, columns capped at 200
+};
+
+struct VerticalDiffusion {
+    const Grid* tgrid;
+    DeviceBuffer auxo, auxn, ru;
+    int prepared = 0;
+    int nz_prepared = 0;
+};
+
+// ---- FP64 kernels (transport_fp64.hip) ----
+int launch_adv_uv_f64(int direction, int scheme, int nk, const Grid& t, const Grid& uv,
+                      const double* vel, const double* ah, double dt, double* h, const double* huv,
+                      const double* f_in, double* f_out, hipStream_t s);
+int launch_adv_w_f64(int scheme, const Grid& t, const double* w, const double* wv, double dt, double* h,
+                     double* f, double* wsh, double* wsf, int* stats, hipStream_t s);
+int launch_vdiff_prepare_f64(const Grid& t, int nz, double molecular, const double* nuh, double dt,
+                             double cnpar, const double* dzo, const double* dzn, double* auxo, double* auxn,
+                             hipStream_t s);
+int launch_vdiff_apply_f64(const Grid& t, int nz, const int* mask, const double* dzo, const double* dzn,
+                           const double* auxo, const double* auxn, double* var, const double* ea2,
+                           const double* ea4, double* ru, hipStream_t s);
+
+}  // namespace getm
+
+#define GETM_HIP_OK(expr)                                  \
+    do {                                                   \
+        if ((expr) != hipSuccess) return GETM_ERR_HIP;     \
+    } while (0)

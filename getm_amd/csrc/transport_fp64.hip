@@ -1,0 +1,368 @@
+// FP64 kernels of the tracer-transport step, bit-identical to the reference.
+//
+// Compiled with -ffp-contract=off: every multiply and add rounds on its own,
+// divisions are IEEE correctly rounded, and the operation order restates the
+// Fortran (file:line at each kernel), so results equal the reference's bits.
+//
+// Layout: C-order (nz, ny_, nx_), i fastest; one lane per (i, j) so a wave
+// reads 64 consecutive i and every load/store coalesces.
+#include "common.h"
+#include "internal.h"
+
+namespace getm {
+
+// ---------------------------------------------------------------------------
+// x / y half-sweep, one lane per cell, out of place on the tracer.
+// advection.F90.template:23-121 (u2d) and :123-221 (v2d).
+//
+// The reference first computes every face flux of a layer, then updates the
+// cells.  Here each lane recomputes its two faces; a face computed by two
+// neighbouring lanes is evaluated with identical operands in identical order,
+// so both see the same value as the reference's single evaluation.  Because
+// neighbours read the old tracer, the tracer is written to a separate buffer
+// (f_out); cells that the sweep does not update are copied through.  The
+// thickness h of a cell depends only on its own faces, so it is updated in
+// place.
+// ---------------------------------------------------------------------------
+template <int S, int DIR>
+__device__ __forceinline__ void face_flux_f64(const int* __restrict__ uvmask, const double* __restrict__ m1,
+                                              const double* __restrict__ m2, const double* __restrict__ huv,
+                                              const double* __restrict__ vel, const double* __restrict__ ah,
+                                              const double* __restrict__ f, size_t p2, size_t p3,
+                                              ptrdiff_t s2, ptrdiff_t s3, double dt, double& q, double& flux)
+{
+    if (!open_face(uvmask[p2])) {
+        q = 0.0;
+        flux = 0.0;
+        return;
+    }
+    const double u = vel[p3];
+    double cfl;
+    if (DIR == 1) {  // QU = u*hu*dyu ; cfl = |u*dt*idxu|   (m1 = idxu, m2 = dyu)
+        q = (u * huv[p3]) * m2[p2];
+        cfl = fabs((u * dt) * m1[p2]);
+    } else {         // QV = v*hv*dxv ; cfl = |v*dt*idyv|   (m1 = dxv, m2 = idyv)
+        q = (u * huv[p3]) * m1[p2];
+        cfl = fabs((u * dt) * m2[p2]);
+    }
+    double fu, fuu, fd;
+    if (u > 0.0) {
+        fu = f[p3];
+        fuu = fu;
+        if (open_face(uvmask[p2 - s2])) fuu = f[p3 - s3];
+        fd = f[p3 + s3];
+    } else {
+        fu = f[p3 + s3];
+        fuu = fu;
+        if (open_face(uvmask[p2 + s2])) fuu = f[p3 + 2 * s3];
+        fd = f[p3];
+    }
+    fu = face_value<S, double>(cfl, fu, fuu, fd);
+    flux = q * fu;
+    if (ah) {  // template:97-105 / 197-205
+        const double df = f[p3 + s3] - f[p3];
+        if (DIR == 1)
+            flux = flux - (((ah[p2] * huv[p3]) * df) * m1[p2]) * m2[p2];
+        else
+            flux = flux - (((ah[p2] * huv[p3]) * df) * m2[p2]) * m1[p2];
+    }
+}
+
+template <int S, int DIR>
+__global__ void __launch_bounds__(256) k_adv_uv_f64(
+    int nx_, int ny_, int imin, int imax, int jmin, int jmax, const int* __restrict__ uvmask,
+    const double* __restrict__ m1, const double* __restrict__ m2, const double* __restrict__ huv,
+    const double* __restrict__ vel, const int* __restrict__ tmask, const double* __restrict__ iarea,
+    const double* __restrict__ ah, double dt, double* __restrict__ h, const double* __restrict__ f,
+    double* __restrict__ fout)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx_ || j >= ny_) return;
+    const size_t n2 = (size_t)nx_ * ny_;
+    const size_t p2 = (size_t)j * nx_ + i;
+    const size_t p3 = (size_t)k * n2 + p2;
+    const bool interior = i >= imin && i <= imax && j >= jmin && j <= jmax;
+    if (!interior || tmask[p2] != 1) {
+        fout[p3] = f[p3];
+        return;
+    }
+    const ptrdiff_t s2 = DIR == 1 ? 1 : nx_;
+    double q0, fl0, q1, fl1;
+    face_flux_f64<S, DIR>(uvmask, m1, m2, huv, vel, ah, f, p2 - s2, p3 - s2, s2, s2, dt, q0, fl0);
+    face_flux_f64<S, DIR>(uvmask, m1, m2, huv, vel, ah, f, p2, p3, s2, s2, dt, q1, fl1);
+    const double hfo = h[p3] * f[p3];
+    const double hnew = h[p3] - (dt * (q1 - q0)) * iarea[p2];
+    h[p3] = hnew;
+    const double advn = (fl1 - fl0) * iarea[p2];
+    fout[p3] = (hfo - dt * advn) / hnew;
+}
+
+int launch_adv_uv_f64(int direction, int scheme, int nk, const Grid& t, const Grid& uv, const double* vel,
+                      const double* ah, double dt, double* h, const double* huv, const double* f_in,
+                      double* f_out, hipStream_t s)
+{
+    const int nx_ = t.nx_, ny_ = t.ny_;
+    dim3 block(64, 4, 1);
+    dim3 grid((nx_ + 63) / 64, (ny_ + 3) / 4, nk);
+    // direction 1 reads (idxu, dyu) from the U grid, direction 2 (dxv, idyv) from the V grid
+    const double* m1 = direction == 1 ? uv.idx : uv.dx;
+    const double* m2 = direction == 1 ? uv.dy : uv.idy;
+    if (!uv.mask || !m1 || !m2 || !t.mask || !t.iarea) return GETM_ERR_ARG;
+    GETM_SCHEME_SWITCH(scheme, {
+        if (direction == 1)
+            hipLaunchKernelGGL((k_adv_uv_f64<S_, 1>), grid, block, 0, s, nx_, ny_, t.imin, t.imax, t.jmin,
+                               t.jmax, uv.mask, m1, m2, huv, vel, t.mask, t.iarea, ah, dt, h, f_in, f_out);
+        else
+            hipLaunchKernelGGL((k_adv_uv_f64<S_, 2>), grid, block, 0, s, nx_, ny_, t.imin, t.imax, t.jmin,
+                               t.jmax, uv.mask, m1, m2, huv, vel, t.mask, t.iarea, ah, dt, h, f_in, f_out);
+    });
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// z sweep, one lane per water column, in place.
+// advection.F90.template:223-374.
+//
+// Iteration control is the reference's: iters = max(1, ceil(max cfl)) capped at
+// 200, and a restart from the original column whenever a later sub-step needs
+// more iterations.  Intermediate sub-step states live in wsh/wsf (same layout
+// as h, f); the first sub-step reads h, f and the last writes them, so the
+// common single-iteration case touches nothing else.  Within a sub-step the
+// column is swept bottom to top with a 4-value register window, which lets the
+// update write layer k while the old values of k-1..k+2 are still held.
+// ---------------------------------------------------------------------------
+template <int S>
+__global__ void __launch_bounds__(256) k_adv_w_f64(
+    int nx_, int ny_, int nz, int imin, int imax, int jmin, int jmax, const int* __restrict__ tmask,
+    const double* __restrict__ w, const double* __restrict__ wv, double dt, double* __restrict__ h,
+    double* __restrict__ f, double* __restrict__ wsh, double* __restrict__ wsf, int* __restrict__ stats)
+{
+    const int i = imin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = jmin + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > imax || j > jmax) return;
+    const size_t n2 = (size_t)nx_ * ny_;
+    const size_t p = (size_t)j * nx_ + i;
+    if (tmask[p] != 1) return;
+
+    int iters = 1, capped = 0;
+    double itersm1 = 1.0, dtk = dt;
+    int it = 1;
+    while (it <= iters) {
+        const double* sh = (it == 1 ? h : wsh) + p;
+        const double* sf = (it == 1 ? f : wsf) + p;
+        if (iters < kItersMax) {
+            double c = -1.7976931348623157e308;  // MAXVAL of an empty array
+            double hk = sh[0];
+            for (int k = 1; k <= nz - 1; k++) {
+                const double hk1 = sh[(size_t)k * n2];
+                c = fmax2(c, (fabs(wv[(size_t)k * n2 + p]) * dt) / fmin2(hk, hk1));
+                hk = hk1;
+            }
+            const int ce = fceiling(c);
+            const int iters_new = ce > 1 ? ce : 1;
+            if (iters_new > iters) {
+                if (iters_new > kItersMax) capped = 1;
+                iters = iters_new > kItersMax ? kItersMax : iters_new;
+                itersm1 = 1.0 / (double)iters;
+                dtk = dt * itersm1;
+                if (it > 1) {
+                    it = 1;
+                    continue;
+                }
+            }
+        }
+        const bool last = it == iters;
+        double* dh = (last ? h : wsh) + p;
+        double* df = (last ? f : wsf) + p;
+        // window over Fortran layers: fm1 = faux(k-1), f0 = faux(k), f1 = faux(k+1), h0/h1 likewise
+        double fm1 = 0.0, f0 = sf[0], h0 = sh[0];
+        double f1 = nz > 1 ? sf[n2] : 0.0, h1 = nz > 1 ? sh[n2] : 0.0;
+        double flux_prev = 0.0, wprev = w[p];
+        for (int k = 1; k <= nz; k++) {
+            double f2 = 0.0, h2 = 0.0;
+            if (k + 2 <= nz) {
+                f2 = sf[(size_t)(k + 1) * n2];
+                h2 = sh[(size_t)(k + 1) * n2];
+            }
+            double fluxk = 0.0;
+            if (k <= nz - 1) {
+                const double wvk = wv[(size_t)k * n2 + p];
+                const double cfl = ((fabs(wvk) * dt) / fmin2(h0, h1)) * itersm1;
+                double fu, fuu, fd;
+                if (wvk > 0.0) {
+                    fu = f0;
+                    fuu = k > 1 ? fm1 : fu;
+                    fd = f1;
+                } else {
+                    fu = f1;
+                    fuu = k < nz - 1 ? f2 : fu;
+                    fd = f0;
+                }
+                fu = face_value<S, double>(cfl, fu, fuu, fd);
+                fluxk = wvk * fu;
+            }
+            const double wk = w[(size_t)k * n2 + p];
+            const double hfo = h0 * f0;
+            const double hnew = h0 - dtk * (wk - wprev);
+            const double advn = fluxk - flux_prev;
+            dh[(size_t)(k - 1) * n2] = hnew;
+            df[(size_t)(k - 1) * n2] = (hfo - dtk * advn) / hnew;
+            fm1 = f0; f0 = f1; f1 = f2;
+            h0 = h1; h1 = h2;
+            flux_prev = fluxk;
+            wprev = wk;
+        }
+        it = it + 1;
+    }
+    if (iters > 1) atomicMax(&stats[0], iters);
+    if (capped) atomicAdd(&stats[1], 1);
+}
+
+int launch_adv_w_f64(int scheme, const Grid& t, const double* w, const double* wv, double dt, double* h,
+                     double* f, double* wsh, double* wsf, int* stats, hipStream_t s)
+{
+    const int ni = t.imax - t.imin + 1, nj = t.jmax - t.jmin + 1;
+    dim3 block(64, 4, 1);
+    dim3 grid((ni + 63) / 64, (nj + 3) / 4, 1);
+    if (!t.mask) return GETM_ERR_ARG;
+    GETM_SCHEME_SWITCH(scheme, {
+        hipLaunchKernelGGL((k_adv_w_f64<S_>), grid, block, 0, s, t.nx_, t.ny_, t.nz, t.imin, t.imax, t.jmin,
+                           t.jmax, t.mask, w, wv, dt, h, f, wsh, wsf, stats);
+    });
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// theta-method vertical diffusion.  diffusion.F90:135-314.
+// ---------------------------------------------------------------------------
+
+// auxo/auxn at interior points, levels 0..nz-2 (interface k+1).  diffusion.F90:163-171.
+__global__ void __launch_bounds__(256) k_vdiff_prepare_f64(
+    int nx_, int ny_, int nz, int imin, int imax, int jmin, int jmax, double molecular,
+    const double* __restrict__ nuh, double dt, double cnpar, const double* __restrict__ dzo,
+    const double* __restrict__ dzn, double* __restrict__ auxo, double* __restrict__ auxn)
+{
+    const int i = imin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = jmin + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > imax || j > jmax) return;
+    const size_t n2 = (size_t)nx_ * ny_;
+    const size_t p = (size_t)k * n2 + (size_t)j * nx_ + i;
+    const double x = (2.0 * dt) * (nuh[p + n2] + molecular);
+    auxo[p] = ((1.0 - cnpar) * x) / (dzo[p + n2] + dzo[p]);
+    auxn[p] = (cnpar * x) / (dzn[p + n2] + dzn[p]);
+}
+
+int launch_vdiff_prepare_f64(const Grid& t, int nz, double molecular, const double* nuh, double dt,
+                             double cnpar, const double* dzo, const double* dzn, double* auxo, double* auxn,
+                             hipStream_t s)
+{
+    if (nz <= 1) return GETM_OK;  // diffusion.F90:155
+    const int ni = t.imax - t.imin + 1, nj = t.jmax - t.jmin + 1;
+    dim3 block(64, 4, 1);
+    dim3 grid((ni + 63) / 64, (nj + 3) / 4, nz - 1);
+    hipLaunchKernelGGL(k_vdiff_prepare_f64, grid, block, 0, s, t.nx_, t.ny_, nz, t.imin, t.imax, t.jmin,
+                       t.jmax, molecular, nuh, dt, cnpar, dzo, dzn, auxo, auxn);
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+// Matrix + Thomas solve per mask==1 column (diffusion.F90:238-307), with
+// a1(k) = -auxn(k-1), a3(k) = -auxn(k).  Elimination runs top to bottom as in
+// the reference; qu(k) overwrites var(k) once its old value is no longer
+// needed and ru(k) goes to scratch, then back-substitution runs bottom to top.
+__global__ void __launch_bounds__(256) k_vdiff_apply_f64(
+    int nx_, int ny_, int nz, int imin, int imax, int jmin, int jmax, const int* __restrict__ mask,
+    const double* __restrict__ dzo, const double* __restrict__ dzn, const double* __restrict__ auxo,
+    const double* __restrict__ auxn, double* __restrict__ var, const double* __restrict__ ea2,
+    const double* __restrict__ ea4, double* __restrict__ ru)
+{
+    const int i = imin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = jmin + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > imax || j > jmax) return;
+    const size_t n2 = (size_t)nx_ * ny_;
+    const size_t p = (size_t)j * nx_ + i;
+    if (mask[p] != 1) return;
+    const int kt = nz - 1;
+#define AT(a, k) (a)[(size_t)(k) * n2 + p]
+    // surface layer
+    double vk1 = AT(var, kt);      // var(k+1) in the loop below
+    double vk = AT(var, kt - 1);   // var(k)
+    double a2 = AT(dzn, kt) + AT(auxn, kt - 1);
+    double a4 = vk1 * AT(dzo, kt) + (vk - vk1) * AT(auxo, kt - 1);
+    if (ea2) a2 = a2 - AT(ea2, kt);
+    if (ea4) a4 = a4 + AT(ea4, kt);
+    double rup = (-AT(auxn, kt - 1)) / a2;
+    double qup = a4 / a2;
+    AT(ru, kt) = rup;
+    AT(var, kt) = qup;
+    for (int k = kt - 1; k >= 1; k--) {
+        const double vkm1 = AT(var, k - 1);
+        const double ank = AT(auxn, k), ankm1 = AT(auxn, k - 1);
+        a2 = (AT(dzn, k) + ank) + ankm1;
+        a4 = (vk * AT(dzo, k) + (vk1 - vk) * AT(auxo, k)) + (vkm1 - vk) * AT(auxo, k - 1);
+        if (ea2) a2 = a2 - AT(ea2, k);
+        if (ea4) a4 = a4 + AT(ea4, k);
+        const double den = a2 - (-ank) * rup;
+        const double ruk = (-ankm1) / den;
+        const double quk = (a4 - (-ank) * qup) / den;
+        AT(ru, k) = ruk;
+        AT(var, k) = quk;
+        rup = ruk;
+        qup = quk;
+        vk1 = vk;
+        vk = vkm1;
+    }
+    // bottom layer
+    a2 = AT(dzn, 0) + AT(auxn, 0);
+    a4 = vk * AT(dzo, 0) + (vk1 - vk) * AT(auxo, 0);
+    if (ea2) a2 = a2 - AT(ea2, 0);
+    if (ea4) a4 = a4 + AT(ea4, 0);
+    double v = (a4 - (-AT(auxn, 0)) * qup) / (a2 - (-AT(auxn, 0)) * rup);
+    AT(var, 0) = v;
+    for (int k = 1; k <= kt; k++) {
+        v = AT(var, k) - AT(ru, k) * v;
+        AT(var, k) = v;
+    }
+#undef AT
+}
+
+// nz == 1: var = (dzo*var + ea4)/(dzn - ea2) on the whole padded array (diffusion.F90:224-231).
+__global__ void __launch_bounds__(256) k_vdiff_single_f64(size_t n, const double* __restrict__ dzo,
+                                                          const double* __restrict__ dzn, double* __restrict__ var,
+                                                          const double* __restrict__ ea2,
+                                                          const double* __restrict__ ea4)
+{
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double a4 = dzo[p] * var[p];
+    double a2 = dzn[p];
+    if (ea4) a4 = a4 + ea4[p];
+    if (ea2) a2 = a2 - ea2[p];
+    var[p] = a4 / a2;
+}
+
+int launch_vdiff_apply_f64(const Grid& t, int nz, const int* mask, const double* dzo, const double* dzn,
+                           const double* auxo, const double* auxn, double* var, const double* ea2,
+                           const double* ea4, double* ru, hipStream_t s)
+{
+    if (nz == 1) {
+        const size_t n = t.n2();
+        hipLaunchKernelGGL(k_vdiff_single_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, dzo, dzn, var,
+                           ea2, ea4);
+    } else {
+        const int ni = t.imax - t.imin + 1, nj = t.jmax - t.jmin + 1;
+        dim3 block(64, 4, 1);
+        dim3 grid((ni + 63) / 64, (nj + 3) / 4, 1);
+        hipLaunchKernelGGL(k_vdiff_apply_f64, grid, block, 0, s, t.nx_, t.ny_, nz, t.imin, t.imax, t.jmin, t.jmax,
+                           mask, dzo, dzn, auxo, auxn, var, ea2, ea4, ru);
+    }
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+}  // namespace getm

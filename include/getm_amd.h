@@ -1,0 +1,129 @@
+/*
+ * getm_amd: MI355X (gfx950) tracer-transport step for pygetm.
+ *
+ * C-ABI drop-in for the hot-path entry points that pygetm's Cython layer binds
+ * (python/_pygetm.pyx:24-66, implemented bind(c) in python/src/wrapper.F90).
+ * Each entry point keeps the reference's name and argument meaning; the
+ * differences are stated per function and are the same throughout:
+ *
+ *   - every field pointer is a DEVICE pointer (HBM) in the reference layout:
+ *     C-order (nz, ny_, nx_) = Fortran (nx_, ny_, nz), i fastest, halo
+ *     included; interface fields (w, nuh) have nz+1 levels, level 0 = bottom;
+ *     masks are int32, everything else float64;
+ *   - a trailing `stream` (a hipStream_t, may be NULL for the null stream):
+ *     work is enqueued on it and the call returns without synchronising;
+ *   - an int status instead of `void`: GETM_OK or one of the GETM_ERR_*
+ *     codes below (the reference has no status and silently ignores an
+ *     invalid scheme; here the create call returns NULL instead).
+ *
+ * Fields are caller-owned device buffers that are registered with a grid by
+ * their pygetm names (grid_set_array), where the reference's Fortran grid owns
+ * them and hands out pointers (get_array, wrapper.F90:70-125).
+ *
+ * Objects keep no global state and may be used from several host threads as
+ * long as one object is not used concurrently on two streams.
+ */
+#ifndef GETM_AMD_H
+#define GETM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GETM_OK = 0,
+    GETM_ERR_ARG = 1,          /* null handle/pointer, bad direction, shape mismatch */
+    GETM_ERR_SCHEME = 2,       /* scheme code outside 1..6 */
+    GETM_ERR_NOT_PREPARED = 3, /* vertical diffusion applied before prepare (diffusion.F90:154) */
+    GETM_ERR_HIP = 4,          /* a HIP allocation or launch failed */
+    GETM_ERR_CAPACITY = 5      /* a size exceeds what the entry point supports */
+};
+
+/* Advection schemes, pygetm.operators.AdvectionScheme (operators.py:11-18). */
+enum {
+    GETM_HSIMT = 1,
+    GETM_MUSCL = 2,
+    GETM_P2_PDM = 3,
+    GETM_SPLMAX13 = 4,
+    GETM_SUPERBEE = 5,
+    GETM_UPSTREAM = 6
+};
+
+/* ---- grids ------------------------------------------------------------- */
+
+/* Replaces create_grid (python/src/wrapper.F90:48-59).  Fortran index bounds;
+ * padded extents are nx_ = imax-imin+1+2*halox, ny_ likewise, nz = kmax-kmin+1. */
+void* create_grid(int imin, int imax, int jmin, int jmax, int kmin, int kmax, int halox, int haloy);
+
+/* Replaces grid_finalize (wrapper.F90:61-68). */
+void grid_finalize(void* grid);
+
+/* Counterpart of get_array (wrapper.F90:70-125) with ownership inverted: the
+ * caller registers a device buffer under the reference's field name.  Names
+ * used by the transport step: "mask" (int32 2D), "dx", "dy", "idx", "idy",
+ * "area", "iarea" (2D), "hn", "ho" (3D). */
+int grid_set_array(void* grid, const char* name, void* device_ptr);
+
+/* ---- advection ------------------------------------------------------------ */
+
+/* Replaces advection_create (wrapper.F90:188-202).  NULL for an invalid scheme
+ * or a NULL grid. */
+void* advection_create(int scheme, void* tgrid);
+
+/* Replaces advection_finalize (wrapper.F90:204-211). */
+void advection_finalize(void* adv);
+
+/* Replaces advection_uv_calculate (wrapper.F90:213-254): one x (direction 1,
+ * u2d, advection.F90.template:23-121) or y (direction 2, v2d, :123-221)
+ * half-sweep over nk layers, in place on h and var.  uvgrid is the U grid for
+ * direction 1 (reads mask, idx, dy) and the V grid for direction 2 (mask, dx,
+ * idy).  Ah (2D on uvgrid) may be NULL: no horizontal diffusion.  The FP64
+ * result equals the reference bit for bit. */
+int advection_uv_calculate(int direction, int nk, void* adv, void* tgrid, void* uvgrid, const double* uv,
+                           const double* Ah, double timestep, double* h, const double* huv, double* var,
+                           void* stream);
+
+/* Replaces advection_w_calculate (wrapper.F90:256-273): vertical sweep with
+ * CFL sub-stepping (w3d, advection.F90.template:223-374), in place. */
+int advection_w_calculate(void* adv, void* tgrid, const double* w, const double* w_var, double timestep,
+                          double* h, double* var, void* stream);
+
+/* No reference counterpart (it is a sequence of the calls above in
+ * Advection.apply_3d, operators.py:95-133): the five stages
+ * v dt/2, u dt/2, w dt, u dt/2, v dt/2 (ufirst = 0) or u-v-w-v-u (ufirst = 1)
+ * on one tracer with h starting from its current contents, enqueued without
+ * intermediate copies.  w_var may equal w. */
+int advection_apply_3d(void* adv, void* tgrid, void* ugrid, void* vgrid, const double* u, const double* v,
+                       const double* w, const double* w_var, const double* Ah_u, const double* Ah_v,
+                       double timestep, double* h, double* var, int ufirst, void* stream);
+
+/* Sub-stepping statistics of the z sweep since the last reset (no reference
+ * counterpart; the reference caps at 200 iterations silently,
+ * advection.F90.template:298-310).  Synchronises the stream. */
+int advection_get_stats(void* adv, int* max_iters, int* capped_columns, int reset, void* stream);
+
+/* ---- vertical diffusion ---------------------------------------------------- */
+
+/* Replaces vertical_diffusion_create (wrapper.F90:127-138). */
+void* vertical_diffusion_create(void* tgrid);
+
+/* Replaces vertical_diffusion_finalize (wrapper.F90:140-147). */
+void vertical_diffusion_finalize(void* vd);
+
+/* Replaces c_vertical_diffusion_prepare (wrapper.F90:149-161,
+ * diffusion.F90:135-204).  nx, ny are the padded extents nx_, ny_; nuh has
+ * nz+1 levels of which 1..nz-1 are read. */
+int c_vertical_diffusion_prepare(void* vd, int nx, int ny, int nz, double molecular, const double* nuh,
+                                 double timestep, double cnpar, const int* mask, const double* ho,
+                                 const double* hn, void* stream);
+
+/* Replaces c_vertical_diffusion_apply (wrapper.F90:163-186,
+ * diffusion.F90:206-314).  ea2/ea4 may be NULL. */
+int c_vertical_diffusion_apply(void* vd, int nx, int ny, int nz, const int* mask, const double* ho,
+                               const double* hn, double* var, const double* ea2, const double* ea4,
+                               void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

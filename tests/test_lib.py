@@ -1,0 +1,43 @@
+"""The C-ABI library builds, loads without a GPU and exports every entry point
+declared in include/getm_amd.h (no compute call is made here)."""
+import ctypes
+import os
+import re
+
+from getm_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_in_headers():
+    names = []
+    inc = os.path.join(ROOT, "include")
+    for fn in sorted(os.listdir(inc)):
+        if fn.endswith(".h"):
+            text = open(os.path.join(inc, fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"^\s*(?:void\s*\*|void|int)\s+(\w+)\s*\(", text, flags=re.M)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(_lib.LIB_PATH), "build first: python -m getm_amd.build"
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_in_headers()
+    assert len(names) >= 13
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding declares exactly the header's entry points
+    assert sorted(names) == sorted(_lib.declared_symbols())
+
+
+def test_create_rejects_bad_scheme_without_gpu():
+    lib = _lib.load()
+    # object creation that needs no device memory must still validate its input
+    assert lib.create_grid(1, 0, 1, 4, 1, 3, 2, 2) is None
+    g = lib.create_grid(1, 5, 1, 4, 1, 3, 2, 2)
+    assert g
+    assert lib.grid_set_array(g, b"nonsense", None) == 1
+    assert lib.advection_create(0, g) is None
+    assert lib.advection_create(7, g) is None
+    lib.grid_finalize(g)

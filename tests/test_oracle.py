@@ -1,0 +1,267 @@
+"""Pins the FP64 CPU oracle before anything is compared against it.
+
+* x/y/z sweeps and all six limiters: bit-identical to the reference Fortran,
+  both through the committed golden vectors (tests/golden/make_golden.py) and,
+  where oracle/_ref is built, live on fresh seeds.
+* limiters: textbook closed forms of the named limiters at and between kinks.
+* sweeps: constant states stay constant; tracer content is conserved.
+* diffusion (not buildable from the reference here): the dense closed form of
+  the theta scheme, and the invariants of python/tests/test_vertical_diffusion.py.
+* whole step: advected thickness equals hn and inventory is conserved
+  (python/tests/test_3d_barotropic.py:121-134).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from getm_amd import inputs
+from helpers import GOLDEN_CASES, assert_bitwise, golden_domain, theta_dense_solve, vdiff_grid
+
+SCHEMES = range(1, 7)
+
+
+# ---------------------------------------------------------------- advection vs reference
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_sweeps_match_reference_golden(oracle_mod, golden, case, scheme):
+    d = golden_domain(golden, case)
+    ah = d.ah
+    h, f = d.ho.copy(), d.tracer.copy()
+    half = 0.5 * d.dt
+    o = oracle_mod
+    o.advection_uv(2, scheme, d.vmask, d.dxv, d.idyv, d.hv, d.v, d.tmask, d.iarea, ah, half, h, f)
+    o.advection_uv(1, scheme, d.umask, d.idxu, d.dyu, d.hu, d.u, d.tmask, d.iarea, ah, half, h, f)
+    o.advection_w(scheme, d.w, d.w, d.tmask, d.dt, h, f)
+    o.advection_uv(1, scheme, d.umask, d.idxu, d.dyu, d.hu, d.u, d.tmask, d.iarea, ah, half, h, f)
+    o.advection_uv(2, scheme, d.vmask, d.dxv, d.idyv, d.hv, d.v, d.tmask, d.iarea, ah, half, h, f)
+    assert_bitwise(f, golden[f"{case}/s{scheme}/f"], "tracer")
+    assert_bitwise(h, golden[f"{case}/s{scheme}/h"], "h")
+    hw, fw = d.ho.copy(), d.tracer.copy()
+    o.advection_w(scheme, d.w, d.w, d.tmask, d.dt, hw, fw)
+    assert_bitwise(fw, golden[f"{case}/s{scheme}/f_w"], "tracer after w")
+    assert_bitwise(hw, golden[f"{case}/s{scheme}/h_w"], "h after w")
+
+
+def test_golden_cases_exercise_substepping(oracle_mod, golden):
+    """The fixtures visit single, multiple and capped sub-step counts."""
+    seen = set()
+    for case in GOLDEN_CASES:
+        d = golden_domain(golden, case)
+        h, f = d.ho.copy(), d.tracer.copy()
+        seen.add(oracle_mod.advection_w(5, d.w, d.w, d.tmask, d.dt, h, f))
+    assert min(seen) > 1 and max(seen) == 200
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_sweeps_match_reference_live(oracle_mod, seed, scheme):
+    if not oracle_mod.ref_available():
+        pytest.skip("oracle/_ref not built (needs /root/reference and amdflang)")
+    d = inputs.tiny(nx=30, ny=22, nz=9, seed=seed, cfl=0.5 if seed == 21 else 0.1, wscale=3.0,
+                    mask_codes=seed == 22, ntracers=1)
+    ah = np.full_like(d.idxu, 25.0) if seed == 22 else None
+    res = {}
+    for impl in ("c", "ref"):
+        h, f = d.ho.copy(), d.tracers[0].copy()
+        oracle_mod.advection_uv(2, scheme, d.vmask, d.dxv, d.idyv, d.hv, d.v, d.tmask, d.iarea, ah, 0.5 * d.dt, h,
+                                f, impl=impl)
+        oracle_mod.advection_uv(1, scheme, d.umask, d.idxu, d.dyu, d.hu, d.u, d.tmask, d.iarea, ah, 0.5 * d.dt, h,
+                                f, impl=impl)
+        oracle_mod.advection_w(scheme, d.w, d.w, d.tmask, d.dt, h, f, impl=impl)
+        res[impl] = (h, f)
+    assert_bitwise(res["c"][0], res["ref"][0], "h")
+    assert_bitwise(res["c"][1], res["ref"][1], "tracer")
+
+
+def test_hsimt_cfl_one_edge_matches_reference(oracle_mod):
+    """HSIMT divides by kappa = 1 - cfl; an exactly integer column CFL gives cfl = 1
+    in the w sweep.  The restatement follows flang's MIN/MAX NaN selection."""
+    if not oracle_mod.ref_available():
+        pytest.skip("oracle/_ref not built")
+    d = inputs.tiny(nx=6, ny=5, nz=4, seed=3, ntracers=1, wscale=0.0)
+    h = np.where(d.tmask == 1, 1.0, 0.0)[None].repeat(d.nz, 0) * 1.0
+    w = np.zeros_like(d.w)
+    w[1:-1] = np.where(d.tmask == 1, 2.0 / d.dt, 0.0)  # |w| dt / min(h) = 2 exactly
+    f = d.tracers[0].copy()
+    f[:, :, :] = np.array([0.0, 1.0, 3.0, 4.0])[:, None, None]
+    outs = []
+    for impl in ("c", "ref"):
+        hh, ff = h.copy(), f.copy()
+        oracle_mod.advection_w(1, w, w, d.tmask, d.dt, hh, ff, impl=impl)
+        outs.append((hh, ff))
+    assert_bitwise(outs[0][0], outs[1][0], "h")
+    assert_bitwise(outs[0][1], outs[1][1], "tracer")
+
+
+# ---------------------------------------------------------------- limiter closed forms
+def _textbook(scheme, cfl, r):
+    if scheme == 5:  # Superbee (Roe 1985), r > 0
+        return max(min(2 * r, 1.0), min(r, 2.0))
+    if scheme == 2:  # MUSCL / monotonised central (van Leer 1977)
+        return min(2 * r, 0.5 * (1 + r), 2.0)
+    if scheme == 4:  # SPLMAX13
+        return min(2 * r, max(1 + 2 * r, 2 + r) / 3.0, 2.0)
+    if scheme == 6:
+        return 0.0
+    if scheme == 1:  # HSIMT (Wu & Zhu 2010): kappa = 1 - cfl, third-order weights, TVD clip
+        kap = 1 - cfl
+        x = 0.25 * (kap - 1 / (3 * kap))
+        return min(max(0.0, 0.5 + x + (0.5 - x) * r), 2 * r, 2.0)
+    if scheme == 3:  # P2-PDM: third-order (1-2c)/6 weights, PDM bounds 2r/c and 2/(1-c)
+        x = (1 - 2 * cfl) / 6
+        return min(2 * r / (cfl + 1e-10), 0.5 + x + (0.5 - x) * r, 2 / (1 - cfl))
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_limiters_closed_form(oracle_mod, scheme):
+    for cfl in (0.0, 0.05, 0.3, 0.7, 0.95):
+        for r in (1e-3, 0.25, 1 / 3, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 10.0):
+            got = oracle_mod.limiter(scheme, cfl, r)
+            assert got == pytest.approx(_textbook(scheme, cfl, r), rel=1e-14, abs=1e-15), (cfl, r)
+
+
+def test_superbee_breakpoints(oracle_mod):
+    assert [oracle_mod.limiter(5, 0.2, r) for r in (0.25, 0.5, 1.0, 1.5, 2.0, 4.0)] == [0.5, 1.0, 1.0, 1.5, 2.0, 2.0]
+    assert [oracle_mod.limiter(2, 0.2, r) for r in (0.25, 1.0, 3.0, 5.0)] == [0.5, 1.0, 2.0, 2.0]
+    assert oracle_mod.limiter(4, 0.2, 1.0) == 1.0
+
+
+# ---------------------------------------------------------------- sweep invariants
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_constant_state_preserved_and_content_conserved(oracle_mod, scheme):
+    d = inputs.tiny(nx=20, ny=16, nz=6, seed=5, ntracers=1, wscale=1.0)
+    wet3 = np.broadcast_to(d.tmask == 1, d.ho.shape)
+    h, f = d.ho.copy(), np.full_like(d.ho, 3.25)
+    o = oracle_mod
+    o.advection_uv(2, scheme, d.vmask, d.dxv, d.idyv, d.hv, d.v, d.tmask, d.iarea, None, 0.5 * d.dt, h, f)
+    o.advection_uv(1, scheme, d.umask, d.idxu, d.dyu, d.hu, d.u, d.tmask, d.iarea, None, 0.5 * d.dt, h, f)
+    o.advection_w(scheme, d.w, d.w, d.tmask, d.dt, h, f)
+    assert np.abs(f[wet3] / 3.25 - 1).max() < 1e-13
+    # content of a random tracer over the closed domain
+    tr = d.tracers[0].copy()
+    h = d.ho.copy()
+    before = math.fsum((h * tr * d.area)[wet3])
+    o.advection_uv(2, scheme, d.vmask, d.dxv, d.idyv, d.hv, d.v, d.tmask, d.iarea, None, 0.5 * d.dt, h, tr)
+    o.advection_uv(1, scheme, d.umask, d.idxu, d.dyu, d.hu, d.u, d.tmask, d.iarea, None, 0.5 * d.dt, h, tr)
+    o.advection_w(scheme, d.w, d.w, d.tmask, d.dt, h, tr)
+    after = math.fsum((h * tr * d.area)[wet3])
+    assert abs(after - before) <= 1e-13 * abs(before)
+
+
+# ---------------------------------------------------------------- diffusion
+@pytest.mark.parametrize("cnpar", [0.5, 0.75, 1.0])
+@pytest.mark.parametrize("zoom", [{}, {"ddu": 1.0}, {"ddl": 1.0}])
+def test_vdiff_dense_closed_form(oracle_mod, cnpar, zoom):
+    rng = np.random.default_rng(7)
+    mask, hn = vdiff_grid(rng, nz=12, nx=9, ny=7, **zoom)
+    hn = hn * rng.uniform(0.7, 1.3, size=hn.shape)  # make columns differ
+    nuh = 10.0 ** rng.uniform(-5, -1, size=(hn.shape[0] + 1,) + hn.shape[1:])
+    var = rng.uniform(0, 1, size=hn.shape)
+    out = var.copy()
+    oracle_mod.vertical_diffusion(nuh, 600.0, out, mask, hn, cnpar=cnpar, molecular=1e-6)
+    for j, i in zip(*np.nonzero(mask == 1)):
+        ref = theta_dense_solve(var[:, j, i], hn[:, j, i], nuh[:, j, i], 600.0, cnpar, 1e-6)
+        np.testing.assert_allclose(out[:, j, i], ref, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(out[:, mask != 1], var[:, mask != 1])
+
+
+def _diffuse_like_reference(o, tracer_in, mask, hn, nuh, cnpar, nstep=100, dt=600.0, sources=None):
+    """The loop of test_vertical_diffusion.py:69-141 (NaN at masked points and at the
+    surface/bottom interfaces to prove they are never read)."""
+    nuh = nuh.copy()
+    nuh[:, mask != 1] = np.nan
+    nuh[0] = np.nan
+    nuh[-1] = np.nan
+    tr = tracer_in.copy()
+    for _ in range(nstep):
+        o.vertical_diffusion(nuh, dt, tr, mask, hn, cnpar=cnpar, ea4=sources)
+    return tr
+
+
+@pytest.mark.parametrize("cnpar", [0.5, 0.75, 1.0])
+@pytest.mark.parametrize("zoom", [{}, {"ddu": 1.0}, {"ddl": 1.0}])
+@pytest.mark.parametrize("kind", ["bottom", "surface", "random", "source"])
+def test_vdiff_reference_invariants(oracle_mod, cnpar, zoom, kind):
+    rng = np.random.default_rng(hash((cnpar, str(zoom), kind)) % 2**32)
+    mask, hn = vdiff_grid(rng, **zoom)
+    nz = hn.shape[0]
+    nuh = np.full((nz + 1,) + hn.shape[1:], 0.01)
+    tr = np.zeros_like(hn)
+    sources = None
+    if kind == "bottom":
+        tr[0] = 1.0
+    elif kind == "surface":
+        tr[-1] = 1.0
+    elif kind == "random":
+        tr[...] = rng.uniform(0.0, 1.0, (nz, 1, 1))
+    else:
+        sources = (1.0 / 600.0) * 600.0 * hn
+    out = _diffuse_like_reference(oracle_mod, tr, mask, hn, nuh, cnpar, sources=sources)
+    wet = mask == 1
+    assert np.isfinite(out[:, wet]).all()
+    cols = out[:, wet]
+    assert (cols.max(axis=1) - cols.min(axis=1)).max() == 0.0  # exact horizontal homogeneity
+    tol = 1e-13
+    if sources is None:
+        ini = tr[:, wet]
+        eps = tol * max(abs(cols.min()), abs(cols.max()))
+        assert cols.max() <= ini.max() + eps and cols.min() >= ini.min() - eps
+    expected = (tr * hn).sum(axis=0)[wet]
+    if sources is not None:
+        expected = expected + sources.sum(axis=0)[wet] * 100
+    integral = (out * hn).sum(axis=0)[wet]
+    assert np.abs(integral - expected).max() / np.abs(expected).max() <= tol
+
+
+@pytest.mark.parametrize("zoom", [{}, {"ddu": 1.0}, {"ddl": 1.0}])
+def test_vdiff_relative_source_closed_form(oracle_mod, zoom):
+    """test_vertical_diffusion.py:178-207: x -> x / (1 - r) per step."""
+    rng = np.random.default_rng(3)
+    mask, hn = vdiff_grid(rng, **zoom)
+    nz = hn.shape[0]
+    nuh = np.full((nz + 1,) + hn.shape[1:], 0.01)
+    tr = np.ones_like(hn)
+    r = 0.1
+    ea2 = (r / 600.0) * 600.0 * hn
+    for _ in range(100):
+        oracle_mod.vertical_diffusion(nuh, 600.0, tr, mask, hn, cnpar=1.0, ea2=ea2)
+    expected = 1.0 / (1.0 - r) ** 100
+    rel = (tr[:, mask == 1] - expected) / expected
+    assert not (rel.min(axis=1) - rel.max(axis=1)).any()
+    assert np.abs(rel).max() <= 1e-13
+
+
+def test_vdiff_single_layer(oracle_mod):
+    rng = np.random.default_rng(4)
+    mask, hn = vdiff_grid(rng, nz=1, nx=6, ny=5)
+    var = rng.uniform(size=hn.shape)
+    ea2 = rng.uniform(0, 0.1, size=hn.shape)
+    ea4 = rng.uniform(size=hn.shape)
+    out = var.copy()
+    oracle_mod.vertical_diffusion(np.zeros((2,) + hn.shape[1:]), 60.0, out, mask, hn, ea2=ea2, ea4=ea4)
+    # diffusion.F90:224-231 updates every point, halos and land included
+    np.testing.assert_array_equal(out, (hn * var + ea4) / (hn - ea2))
+
+
+# ---------------------------------------------------------------- whole step
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_step_thickness_and_inventory(oracle_mod, scheme):
+    d = inputs.seamount(nx=40, ny=36, nz=10, scheme=scheme, ntracers=2)
+    inv0 = d.inventory(h=d.ho)
+    tr = d.tracers.copy()
+    h = oracle_mod.transport_step(d, tr, return_h=True)
+    wet = np.broadcast_to(d.tmask == 1, d.ho.shape)
+    np.testing.assert_allclose(h[wet], d.hn[wet], rtol=1e-14, atol=1e-14)
+    inv1 = d.inventory(tr, h=d.hn)
+    np.testing.assert_allclose(inv1, inv0, rtol=1e-14)
+
+
+def test_step_coastal_inventory(oracle_mod):
+    d = inputs.coastal(nx=48, ny=40, nz=8, scheme=3, ntracers=2)
+    inv0 = d.inventory(h=d.ho)
+    tr = d.tracers.copy()
+    h = oracle_mod.transport_step(d, tr, return_h=True)
+    wet = np.broadcast_to(d.tmask == 1, d.ho.shape)
+    np.testing.assert_allclose(h[wet], d.hn[wet], rtol=1e-13)
+    np.testing.assert_allclose(d.inventory(tr), inv0, rtol=1e-13)
