@@ -40,7 +40,37 @@ _SIGNATURES = {
     "vertical_diffusion_finalize": (None, [_vp]),
     "c_vertical_diffusion_prepare": (_i, [_vp, _i, _i, _i, _d, _vp, _d, _d, _vp, _vp, _vp, _vp]),
     "c_vertical_diffusion_apply": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "transport_create": (_vp, [_vp, _vp, _vp, _i, _d, _i, _i]),
+    "transport_finalize": (None, [_vp]),
+    "transport_step": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp]),
+    "transport_get_stats": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _i, _vp]),
+    "getm_profile_enable": (_i, [_i]),
+    "getm_profile_read": (_i, [_i, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_long), _i]),
 }
+
+# kernel identifiers of getm_profile_read (include/getm_amd.h)
+KERNELS = {
+    "adv_uv_f64": 0,
+    "adv_w_f64": 1,
+    "vdiff_prepare_f64": 2,
+    "vdiff_apply_f64": 3,
+    "hpair_vu": 4,
+    "hpair_uv": 5,
+    "wcol": 6,
+    "vdiff_f32": 7,
+}
+
+
+def profile_enable(on: bool) -> None:
+    check(load().getm_profile_enable(int(on)), "getm_profile_enable")
+
+
+def profile_read(kernel: str, reset: bool = True):
+    """(total milliseconds, launches) of one kernel since the last reset."""
+    ms, n = ctypes.c_double(0.0), ctypes.c_long(0)
+    check(load().getm_profile_read(KERNELS[kernel], ctypes.byref(ms), ctypes.byref(n), int(reset)),
+          "getm_profile_read")
+    return ms.value, n.value
 
 _lib = None
 

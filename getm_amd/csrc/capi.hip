@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <new>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -259,6 +260,147 @@ int c_vertical_diffusion_apply(void* vd, int nx, int ny, int nz, const int* mask
     return launch_vdiff_apply_f64(*t, nz, mask, ho, hn, static_cast<const double*>(d->auxo.ptr),
                                   static_cast<const double*>(d->auxn.ptr), var, ea2, ea4,
                                   static_cast<double*>(d->ru.ptr), as_stream(stream));
+}
+
+void* transport_create(void* tgrid, void* ugrid, void* vgrid, int scheme, double cnpar, int precision, int ufirst)
+{
+    if (scheme < GETM_HSIMT || scheme > GETM_UPSTREAM || !tgrid || !ugrid || !vgrid) return nullptr;
+    if (precision != GETM_F64 && precision != GETM_F32) return nullptr;
+    const Grid* t = static_cast<const Grid*>(tgrid);
+    const Grid* gu = static_cast<const Grid*>(ugrid);
+    const Grid* gv = static_cast<const Grid*>(vgrid);
+    if (check_grids(t, gu) || check_grids(t, gv)) return nullptr;
+    Transport* tr = new (std::nothrow) Transport();
+    if (!tr) return nullptr;
+    tr->scheme = scheme;
+    tr->precision = precision;
+    tr->ufirst = ufirst ? 1 : 0;
+    tr->cnpar = cnpar;
+    tr->tgrid = t;
+    tr->ugrid = gu;
+    tr->vgrid = gv;
+    if (tr->stats.ensure(2 * sizeof(int)) != GETM_OK || hipMemset(tr->stats.ptr, 0, 2 * sizeof(int)) != hipSuccess) {
+        delete tr;
+        return nullptr;
+    }
+    return tr;
+}
+
+void transport_finalize(void* p)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr) return;
+    for (DeviceBuffer* b : {&tr->scratch, &tr->hA, &tr->hB, &tr->stats, &tr->auxo, &tr->auxn, &tr->ru}) b->release();
+    delete tr;
+}
+
+}  // extern "C"
+
+template <typename R>
+static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, const double* w, const double* nuh,
+                            double dt, double molecular, double* h_out, hipStream_t s)
+{
+    const Grid* t = tr->tgrid;
+    const size_t n3 = t->n3();
+    int rc;
+    if ((rc = tr->scratch.ensure(n3 * (size_t)ntr * sizeof(R))) || (rc = tr->hA.ensure(n3 * sizeof(double))) ||
+        (rc = tr->hB.ensure(n3 * sizeof(double))))
+        return rc;
+    R* S = static_cast<R*>(tr->scratch.ptr);
+    double* hA = static_cast<double*>(tr->hA.ptr);
+    double* hB = static_cast<double*>(tr->hB.ptr);
+    double* hfinal = h_out ? h_out : hA;
+    const double half = 0.5 * dt;
+    const int d1 = tr->ufirst ? 1 : 2;  // first direction of stages 1-2 (operators.py:118-121)
+    const int d2 = 3 - d1;
+    if ((rc = launch_hpair<R>(d1, tr->scheme, A, half, ntr, f, S, t->ho, hA, s))) return rc;
+    if ((rc = launch_wcol<R>(tr->scheme, A, dt, ntr, S, f, hA, hB, w, w, static_cast<int*>(tr->stats.ptr), s)))
+        return rc;
+    if ((rc = launch_hpair<R>(d2, tr->scheme, A, half, ntr, f, S, hB, hfinal, s))) return rc;
+    if constexpr (std::is_same<R, float>::value) {
+        return launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, S, f, s);
+    } else {
+        // FP64: the reference-order per-tracer diffusion kernels, prepared once (tracer.py:273-276)
+        GETM_HIP_OK(hipMemcpyAsync(f, S, n3 * (size_t)ntr * sizeof(R), hipMemcpyDeviceToDevice, s));
+        if ((rc = tr->auxo.ensure(n3 * sizeof(double))) || (rc = tr->auxn.ensure(n3 * sizeof(double))) ||
+            (rc = tr->ru.ensure(n3 * sizeof(double))))
+            return rc;
+        double* auxo = static_cast<double*>(tr->auxo.ptr);
+        double* auxn = static_cast<double*>(tr->auxn.ptr);
+        if ((rc = launch_vdiff_prepare_f64(*t, t->nz, molecular, nuh, dt, tr->cnpar, t->hn, t->hn, auxo, auxn, s)))
+            return rc;
+        for (int i = 0; i < ntr; i++)
+            if ((rc = launch_vdiff_apply_f64(*t, t->nz, t->mask, t->hn, t->hn, auxo, auxn, f + (size_t)i * n3, nullptr,
+                                             nullptr, static_cast<double*>(tr->ru.ptr), s)))
+                return rc;
+        return GETM_OK;
+    }
+}
+
+extern "C" {
+
+int transport_step(void* p, int ntracers, void* tracers, const double* u, const double* v, const double* w,
+                   const double* nuh, const double* Ah_u, const double* Ah_v, double timestep, double molecular,
+                   double* h_out, void* stream)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr) return GETM_ERR_ARG;
+    if (ntracers < 0 || (ntracers > 0 && !tracers) || !u || !v || !w || !nuh) return GETM_ERR_ARG;
+    if ((Ah_u == nullptr) != (Ah_v == nullptr)) return GETM_ERR_ARG;
+    const Grid* t = tr->tgrid;
+    if (!t->ho || !t->hn || !t->mask || !t->iarea || !tr->ugrid->hn || !tr->vgrid->hn || !tr->ugrid->mask ||
+        !tr->vgrid->mask || !tr->ugrid->idx || !tr->ugrid->dy || !tr->vgrid->dx || !tr->vgrid->idy)
+        return GETM_ERR_ARG;
+    if (t->nz > 64 && tr->precision == GETM_F32) return GETM_ERR_CAPACITY;
+    hipStream_t s = as_stream(stream);
+    if (ntracers == 0) {
+        if (h_out) GETM_HIP_OK(hipMemcpyAsync(h_out, t->ho, t->n3() * sizeof(double), hipMemcpyDeviceToDevice, s));
+        return GETM_OK;
+    }
+    StepArgs A;
+    A.nx_ = t->nx_;
+    A.ny_ = t->ny_;
+    A.nz = t->nz;
+    A.imin = t->imin;
+    A.imax = t->imax;
+    A.jmin = t->jmin;
+    A.jmax = t->jmax;
+    A.n2 = t->n2();
+    A.n3 = t->n3();
+    A.tmask = t->mask;
+    A.umask = tr->ugrid->mask;
+    A.vmask = tr->vgrid->mask;
+    A.idxu = tr->ugrid->idx;
+    A.dyu = tr->ugrid->dy;
+    A.dxv = tr->vgrid->dx;
+    A.idyv = tr->vgrid->idy;
+    A.iarea = t->iarea;
+    A.u = u;
+    A.v = v;
+    A.hu = tr->ugrid->hn;
+    A.hv = tr->vgrid->hn;
+    A.hn = t->hn;
+    A.ah_u = Ah_u;
+    A.ah_v = Ah_v;
+    if (tr->precision == GETM_F32)
+        return transport_step_t<float>(tr, A, ntracers, static_cast<float*>(tracers), w, nuh, timestep, molecular,
+                                       h_out, s);
+    return transport_step_t<double>(tr, A, ntracers, static_cast<double*>(tracers), w, nuh, timestep, molecular, h_out,
+                                    s);
+}
+
+int transport_get_stats(void* p, int* max_iters, int* capped_columns, int reset, void* stream)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr) return GETM_ERR_ARG;
+    int host[2] = {0, 0};
+    hipStream_t s = as_stream(stream);
+    GETM_HIP_OK(hipMemcpyAsync(host, tr->stats.ptr, sizeof(host), hipMemcpyDeviceToHost, s));
+    GETM_HIP_OK(hipStreamSynchronize(s));
+    if (max_iters) *max_iters = host[0] > 1 ? host[0] : 1;
+    if (capped_columns) *capped_columns = host[1];
+    if (reset) GETM_HIP_OK(hipMemsetAsync(tr->stats.ptr, 0, sizeof(host), s));
+    return GETM_OK;
 }
 
 }  // extern "C"

@@ -10,6 +10,20 @@
 
 namespace getm {
 
+// RAII bracket of one kernel launch with two HIP events on its stream when
+// profiling is enabled (profile.hip); a no-op otherwise.
+class ProfileScope {
+   public:
+    ProfileScope(int id, hipStream_t s);
+    ~ProfileScope();
+
+   private:
+    int id_;
+    hipStream_t s_;
+    void* a_ = nullptr;
+    void* b_ = nullptr;
+};
+
 // A horizontal grid (T, U or V points) with halo, as create_grid builds it
 // (python/src/wrapper.F90:48-59).  Field pointers are caller-owned device
 // buffers registered by name with grid_set_array.
@@ -52,6 +66,50 @@ struct VerticalDiffusion {
     int prepared = 0;
     int nz_prepared = 0;
 };
+
+// Everything the batched step kernels read besides the tracers, by value.
+struct StepArgs {
+    int nx_, ny_, nz, imin, imax, jmin, jmax;
+    size_t n2, n3;
+    const int* tmask;
+    const int* umask;
+    const int* vmask;
+    const double* idxu;
+    const double* dyu;
+    const double* dxv;
+    const double* idyv;
+    const double* iarea;
+    const double* u;
+    const double* v;
+    const double* hu;
+    const double* hv;
+    const double* hn;
+    const double* ah_u;  // both NULL or both set
+    const double* ah_v;
+};
+
+// Batched T-tracer step object (transport_create).
+struct Transport {
+    int scheme, precision, ufirst;
+    double cnpar;
+    const Grid* tgrid;
+    const Grid* ugrid;
+    const Grid* vgrid;
+    DeviceBuffer scratch;  // tracer ping-pong buffer
+    DeviceBuffer hA, hB;   // FP64 thickness between stages
+    DeviceBuffer stats;    // int32[2] as in Advection
+    DeviceBuffer auxo, auxn, ru;  // FP64 diffusion (precision GETM_F64)
+};
+
+// ---- batched kernels (transport_batched.hip) ----
+template <typename R>
+int launch_hpair(int d1, int scheme, const StepArgs& A, double dt, int ntr, const R* fin, R* fout,
+                 const double* hin, double* hout, hipStream_t s);
+template <typename R>
+int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, const R* fin, R* fout, const double* hin,
+                double* hout, const double* w, const double* wv, int* stats, hipStream_t s);
+int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr,
+                     const float* fin, float* fout, hipStream_t s);
 
 // ---- FP64 kernels (transport_fp64.hip) ----
 int launch_adv_uv_f64(int direction, int scheme, int nk, const Grid& t, const Grid& uv,

@@ -110,6 +110,7 @@ int launch_adv_uv_f64(int direction, int scheme, int nk, const Grid& t, const Gr
     const double* m1 = direction == 1 ? uv.idx : uv.dx;
     const double* m2 = direction == 1 ? uv.dy : uv.idy;
     if (!uv.mask || !m1 || !m2 || !t.mask || !t.iarea) return GETM_ERR_ARG;
+    ProfileScope ps(GETM_K_ADV_UV_F64, s);
     GETM_SCHEME_SWITCH(scheme, {
         if (direction == 1)
             hipLaunchKernelGGL((k_adv_uv_f64<S_, 1>), grid, block, 0, s, nx_, ny_, t.imin, t.imax, t.jmin,
@@ -228,6 +229,7 @@ int launch_adv_w_f64(int scheme, const Grid& t, const double* w, const double* w
     dim3 block(64, 4, 1);
     dim3 grid((ni + 63) / 64, (nj + 3) / 4, 1);
     if (!t.mask) return GETM_ERR_ARG;
+    ProfileScope ps(GETM_K_ADV_W_F64, s);
     GETM_SCHEME_SWITCH(scheme, {
         hipLaunchKernelGGL((k_adv_w_f64<S_>), grid, block, 0, s, t.nx_, t.ny_, t.nz, t.imin, t.imax, t.jmin,
                            t.jmax, t.mask, w, wv, dt, h, f, wsh, wsf, stats);
@@ -265,6 +267,7 @@ int launch_vdiff_prepare_f64(const Grid& t, int nz, double molecular, const doub
     const int ni = t.imax - t.imin + 1, nj = t.jmax - t.jmin + 1;
     dim3 block(64, 4, 1);
     dim3 grid((ni + 63) / 64, (nj + 3) / 4, nz - 1);
+    ProfileScope ps(GETM_K_VDIFF_PREPARE_F64, s);
     hipLaunchKernelGGL(k_vdiff_prepare_f64, grid, block, 0, s, t.nx_, t.ny_, nz, t.imin, t.imax, t.jmin,
                        t.jmax, molecular, nuh, dt, cnpar, dzo, dzn, auxo, auxn);
     GETM_HIP_OK(hipGetLastError());
@@ -350,6 +353,7 @@ int launch_vdiff_apply_f64(const Grid& t, int nz, const int* mask, const double*
                            const double* auxo, const double* auxn, double* var, const double* ea2,
                            const double* ea4, double* ru, hipStream_t s)
 {
+    ProfileScope ps(GETM_K_VDIFF_APPLY_F64, s);
     if (nz == 1) {
         const size_t n = t.n2();
         hipLaunchKernelGGL(k_vdiff_single_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, dzo, dzn, var,

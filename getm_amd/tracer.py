@@ -1,0 +1,121 @@
+"""Tracer transport driver mirroring pygetm.tracer.TracerCollection
+(python/pygetm/tracer.py:125-276): split advection of every tracer followed by
+implicit vertical diffusion of every tracer.
+
+Tracers live in one stacked device tensor ``(T, nz, ny_, nx_)``.  With
+``batched=True`` (default) a step is one ``transport_step`` call of the
+C-ABI, which streams all tracers through shared stage coefficients; with
+``batched=False`` it is the reference's call sequence of per-tracer operators
+(FP64 only).  ``precision="f32"`` stores tracers in FP32 with the thickness
+update kept in FP64.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional
+
+import torch
+
+from . import _lib, operators
+from .core import CENTERS, Array, Grid
+
+_DTYPES = {"f64": (torch.float64, 1), "f32": (torch.float32, 2)}
+
+
+class TracerCollection:
+    def __init__(self, grid: Grid, advection_scheme=operators.AdvectionScheme.DEFAULT, cnpar: float = 1.0,
+                 precision: str = "f64", batched: bool = True, ufirst: bool = False):
+        if precision not in _DTYPES:
+            raise ValueError(f"precision must be one of {list(_DTYPES)}")
+        if precision == "f32" and not batched:
+            raise ValueError("the FP32 form is only available as the batched step")
+        self.grid = grid
+        self.precision = precision
+        self.batched = batched
+        self.cnpar = float(cnpar)
+        self.dtype, code = _DTYPES[precision]
+        self._molecular: List[float] = []
+        self._names: List[str] = []
+        self.values = torch.empty((0, grid.nz_, grid.ny_, grid.nx_), dtype=self.dtype, device=grid.device)
+        self.h = torch.empty((grid.nz_, grid.ny_, grid.nx_), dtype=torch.float64, device=grid.device)
+        self.Ah_u: Optional[Array] = None
+        self.Ah_v: Optional[Array] = None
+        lib = _lib.load()
+        if batched:
+            self.p = lib.transport_create(grid.p, grid.ugrid.p, grid.vgrid.p, int(advection_scheme), self.cnpar, code,
+                                          int(ufirst))
+            if not self.p:
+                raise ValueError(f"transport_create failed (scheme {advection_scheme!r})")
+        else:
+            self.p = None
+            self._advection = operators.Advection(grid, scheme=advection_scheme)
+            self._advection.ufirst = ufirst
+            self._vertical_diffusion = operators.VerticalDiffusion(grid, cnpar=cnpar)
+
+    def __del__(self):
+        p = getattr(self, "p", None)
+        if p and _lib._lib is not None:
+            _lib._lib.transport_finalize(p)
+            self.p = None
+
+    def __len__(self) -> int:
+        return self.values.shape[0]
+
+    def __getitem__(self, i: int) -> Array:
+        return Array(self.grid, self.values[i], CENTERS, self._names[i])
+
+    def add(self, values=None, molecular_diffusivity: float = 0.0, name: str = "") -> Array:
+        """Append a tracer (device tensor or array of shape (nz, ny_, nx_)); returns its view."""
+        g = self.grid
+        new = torch.empty((len(self) + 1, g.nz_, g.ny_, g.nx_), dtype=self.dtype, device=g.device)
+        new[:-1].copy_(self.values)
+        if values is not None:
+            new[-1].copy_(torch.as_tensor(values, device=g.device))
+        self.values = new
+        self._molecular.append(float(molecular_diffusivity))
+        self._names.append(name)
+        return self[len(self) - 1]
+
+    def set_values(self, values: torch.Tensor, molecular_diffusivity: float = 0.0) -> None:
+        """Replace all tracers by a stacked (T, nz, ny_, nx_) tensor (copied)."""
+        g = self.grid
+        if tuple(values.shape[1:]) != (g.nz_, g.ny_, g.nx_):
+            raise ValueError(f"expected (T, {g.nz_}, {g.ny_}, {g.nx_}), got {tuple(values.shape)}")
+        self.values = values.to(device=g.device, dtype=self.dtype).contiguous().clone()
+        self._molecular = [float(molecular_diffusivity)] * values.shape[0]
+        self._names = [""] * values.shape[0]
+
+    def advance(self, timestep: float, u: Array, v: Array, w: Array, diffusivity: Array) -> None:
+        """tracer.py:193-246: advection of all tracers with velocities at t + dt/2,
+        then vertical diffusion with dzo = dzn = hn (tracer.py:248-276)."""
+        if not self.batched:
+            tracers = [self[i] for i in range(len(self))]
+            self._advection.apply_3d_batch(u, v, w, timestep, tracers, Ah_u=self.Ah_u, Ah_v=self.Ah_v)
+            self.h.copy_(self._advection.h)
+            avmol = -1.0
+            for tracer, mol in zip(tracers, self._molecular):
+                if mol != avmol:
+                    avmol = mol
+                    self._vertical_diffusion.prepare(diffusivity, timestep, avmol)
+                self._vertical_diffusion.apply(tracer)
+            return
+        mols = set(self._molecular)
+        if len(mols) > 1:
+            raise ValueError("the batched step needs one molecular diffusivity for all tracers")
+        mol = mols.pop() if mols else 0.0
+        rc = _lib.load().transport_step(
+            self.p, len(self), _lib.ptr(self.values), _lib.ptr(u.all_values), _lib.ptr(v.all_values),
+            _lib.ptr(w.all_values), _lib.ptr(diffusivity.all_values),
+            _lib.ptr(self.Ah_u.all_values) if self.Ah_u is not None else None,
+            _lib.ptr(self.Ah_v.all_values) if self.Ah_v is not None else None, float(timestep), mol,
+            _lib.ptr(self.h), _lib.stream_handle())
+        _lib.check(rc, "transport_step")
+
+    def stats(self, reset: bool = True):
+        """(largest z sub-iteration count, columns capped at 200) since the last reset."""
+        if not self.batched:
+            return self._advection.stats(reset)
+        mi, cap = ctypes.c_int(0), ctypes.c_int(0)
+        _lib.check(_lib.load().transport_get_stats(self.p, ctypes.byref(mi), ctypes.byref(cap), int(reset),
+                                                   _lib.stream_handle()), "transport_get_stats")
+        return mi.value, cap.value

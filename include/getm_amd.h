@@ -123,6 +123,60 @@ int c_vertical_diffusion_apply(void* vd, int nx, int ny, int nz, const int* mask
                                const double* hn, double* var, const double* ea2, const double* ea4,
                                void* stream);
 
+/* ---- batched tracer step ------------------------------------------------- */
+
+/* Tracer storage/arithmetic precision of a transport object. */
+enum { GETM_F64 = 1, GETM_F32 = 2 };
+
+/* No single reference counterpart: replaces the loop of
+ * TracerCollection.advance (python/pygetm/tracer.py:193-276), i.e.
+ * Advection.apply_3d_batch (operators.py:135-193) followed by
+ * VerticalDiffusion prepare/apply per tracer, for T tracers that share one
+ * velocity field, one layer-thickness evolution (starting from the T grid's
+ * "ho") and one diffusion matrix (dzo = dzn = the T grid's "hn").
+ * ugrid/vgrid must have "hn" (face thicknesses) registered.
+ * precision GETM_F64: tracers are double and results equal the per-tracer
+ * calls bit for bit.  GETM_F32: tracers are float, stencil/limiter/tridiagonal
+ * arithmetic is FP32, the thickness update is FP64. */
+void* transport_create(void* tgrid, void* ugrid, void* vgrid, int scheme, double cnpar, int precision, int ufirst);
+void transport_finalize(void* tr);
+
+/* One step for ntracers stacked tracers (ntracers x nz x ny_ x nx_, float or
+ * double as created), in place.  w is used as w_var for every tracer; Ah_u and
+ * Ah_v are both NULL (no horizontal diffusion) or both set.  h_out (FP64
+ * nz x ny_ x nx_, may be NULL) receives the advected thickness, the
+ * reference's Advection.h after apply_3d_batch. */
+int transport_step(void* tr, int ntracers, void* tracers, const double* u, const double* v, const double* w,
+                   const double* nuh, const double* Ah_u, const double* Ah_v, double timestep, double molecular,
+                   double* h_out, void* stream);
+
+/* z-sweep sub-stepping statistics of a transport object (as advection_get_stats). */
+int transport_get_stats(void* tr, int* max_iters, int* capped_columns, int reset, void* stream);
+
+/* ---- instrumentation ---------------------------------------------------- */
+
+/* Kernel identifiers for getm_profile_read. */
+enum {
+    GETM_K_ADV_UV_F64 = 0,
+    GETM_K_ADV_W_F64 = 1,
+    GETM_K_VDIFF_PREPARE_F64 = 2,
+    GETM_K_VDIFF_APPLY_F64 = 3,
+    GETM_K_HPAIR_VU = 4,  /* batched stages v dt/2 + u dt/2 (or u + v) */
+    GETM_K_HPAIR_UV = 5,  /* batched stages u dt/2 + v dt/2 (or v + u) */
+    GETM_K_WCOL = 6,      /* batched w stage */
+    GETM_K_VDIFF_F32 = 7, /* batched FP32 vertical diffusion */
+    GETM_NUM_KERNELS = 16
+};
+
+/* No reference counterpart (the reference accumulates cpu_time around its
+ * loops, diffusion.F90:161-311).  When enabled, every kernel launch of this
+ * library is bracketed by HIP events recorded on the launch stream;
+ * getm_profile_read returns the summed elapsed milliseconds and the number of
+ * launches of one kernel (synchronising on the recorded events) and optionally
+ * resets them.  Process-wide instrumentation, off by default. */
+int getm_profile_enable(int enable);
+int getm_profile_read(int kernel, double* total_ms, long* launches, int reset);
+
 #ifdef __cplusplus
 }
 #endif

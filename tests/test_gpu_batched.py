@@ -1,0 +1,132 @@
+"""Batched T-tracer step (transport_step) against the oracle.
+
+FP64: bit-identical to the oracle's TracerCollection.advance restatement for
+every scheme, with sub-stepping, restarts, the 200 cap, mask codes and Ah.
+FP32: within 1e-5 of the FP64 oracle (max |error| / max |value| per tracer).
+"""
+import numpy as np
+import pytest
+
+from getm_amd import inputs
+from helpers import assert_bitwise
+from test_gpu_parity import dev, host, make_grid
+
+pytestmark = pytest.mark.gpu
+SCHEMES = range(1, 7)
+F32_TOL = 1e-5
+
+
+def run_batched(d, precision, nsteps=1, ah=None, ufirst=False):
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.tracer import TracerCollection
+
+    grid = make_grid(d)
+    u = grid.ugrid.wrap(dev(d.u), CENTERS)
+    v = grid.vgrid.wrap(dev(d.v), CENTERS)
+    w = grid.wrap(dev(d.w), INTERFACES)
+    nuh = grid.wrap(dev(d.nuh), INTERFACES)
+    tc = TracerCollection(grid, advection_scheme=d.scheme, cnpar=d.cnpar, precision=precision, ufirst=ufirst)
+    tc.set_values(dev(d.tracers))
+    if ah is not None:
+        tc.Ah_u = grid.ugrid.wrap(dev(ah))
+        tc.Ah_v = grid.vgrid.wrap(dev(ah))
+    for _ in range(nsteps):
+        tc.advance(d.dt, u, v, w, nuh)
+    return host(tc.values).astype(np.float64), host(tc.h), tc
+
+
+def run_oracle(oracle_mod, d, nsteps=1, ah=None):
+    tr = d.tracers.copy()
+    h = None
+    for _ in range(nsteps):
+        h = oracle_mod.transport_step(d, tr, ah_u=ah, ah_v=ah, return_h=True)
+    return tr, h
+
+
+CASES = {
+    "gentle": dict(nx=45, ny=38, nz=9, cfl=0.2, wscale=1.0),
+    "violent": dict(nx=33, ny=27, nz=7, cfl=0.9, wscale=0.0),
+    "codes": dict(nx=41, ny=35, nz=8, cfl=0.4, wscale=2.0, mask_codes=True),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_batched_f64_bitwise(oracle_mod, case, scheme):
+    d = inputs.tiny(seed=50 + scheme, scheme=scheme, ntracers=3, **CASES[case])
+    ah = np.random.default_rng(scheme).uniform(0, 30, size=d.idxu.shape) if case == "codes" else None
+    got, h, tc = run_batched(d, "f64", ah=ah)
+    ref, href = run_oracle(oracle_mod, d, ah=ah)
+    assert_bitwise(got, ref, "tracers")
+    assert_bitwise(h, href, "advected thickness")
+    if case == "violent":
+        mi, capped = tc.stats()
+        assert mi > 1
+
+
+def test_batched_f64_matches_per_tracer_operators():
+    """The batched step equals the reference's per-tracer call sequence (batched=False)."""
+    d = inputs.seamount(nx=48, ny=40, nz=12, ntracers=2)
+    a, ha, _ = run_batched(d, "f64")
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.tracer import TracerCollection
+
+    grid = make_grid(d)
+    tc = TracerCollection(grid, advection_scheme=d.scheme, cnpar=d.cnpar, precision="f64", batched=False)
+    tc.set_values(dev(d.tracers))
+    tc.advance(d.dt, grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS),
+               grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES))
+    assert_bitwise(a, host(tc.values), "tracers")
+    assert_bitwise(ha, host(tc.h), "thickness")
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_batched_f32_error(oracle_mod, scheme):
+    d = inputs.coastal(nx=96, ny=72, nz=12, scheme=scheme, ntracers=3, tracer_kind="blobs", seed=scheme)
+    got, h, _ = run_batched(d, "f32", nsteps=10)
+    ref, href = run_oracle(oracle_mod, d, nsteps=10)
+    assert_bitwise(h, href, "FP64 thickness in the FP32 form")
+    wet = np.broadcast_to(d.tmask == 1, d.ho.shape)
+    for t in range(d.ntracers):
+        err = np.abs(got[t][wet] - ref[t][wet]).max() / np.abs(ref[t][wet]).max()
+        assert err <= F32_TOL, (t, err)
+
+
+@pytest.mark.parametrize("nz", [1, 2, 16, 17, 33, 48, 64])
+def test_batched_f32_depths(oracle_mod, nz):
+    d = inputs.tiny(nx=30, ny=26, nz=nz, seed=nz, ntracers=2, cfl=0.3, wscale=0.5)
+    d.cnpar = 0.7
+    got, h, _ = run_batched(d, "f32")
+    ref, href = run_oracle(oracle_mod, d)
+    assert_bitwise(h, href, "thickness")
+    fin = np.isfinite(ref)
+    np.testing.assert_array_equal(np.isfinite(got), fin)
+    scale = np.abs(ref[fin]).max()
+    assert np.abs(got[fin] - ref[fin]).max() / scale <= F32_TOL
+
+
+def test_batched_f32_with_ah_and_ufirst(oracle_mod):
+    d = inputs.tiny(nx=40, ny=31, nz=10, seed=77, ntracers=2, cfl=0.4, wscale=1.0, mask_codes=True)
+    ah = np.random.default_rng(1).uniform(0, 30, size=d.idxu.shape)
+    got, _, _ = run_batched(d, "f32", ah=ah)
+    ref, _ = run_oracle(oracle_mod, d, ah=ah)
+    fin = np.isfinite(ref)
+    assert np.abs(got[fin] - ref[fin]).max() / np.abs(ref[fin]).max() <= F32_TOL
+
+
+def test_batched_f64_ufirst(oracle_mod):
+    """ufirst = True runs u-v-w-v-u; compare with single sweeps in that order."""
+    d = inputs.tiny(nx=29, ny=23, nz=6, seed=5, ntracers=1, cfl=0.4)
+    got, h, _ = run_batched(d, "f64", ufirst=True)
+    hh, f = d.ho.copy(), d.tracers[0].copy()
+    half = 0.5 * d.dt
+    ua = (d.umask, d.idxu, d.dyu, d.hu, d.u, d.tmask, d.iarea, None, half)
+    va = (d.vmask, d.dxv, d.idyv, d.hv, d.v, d.tmask, d.iarea, None, half)
+    oracle_mod.advection_uv(1, d.scheme, *ua, hh, f)
+    oracle_mod.advection_uv(2, d.scheme, *va, hh, f)
+    oracle_mod.advection_w(d.scheme, d.w, d.w, d.tmask, d.dt, hh, f)
+    oracle_mod.advection_uv(2, d.scheme, *va, hh, f)
+    oracle_mod.advection_uv(1, d.scheme, *ua, hh, f)
+    oracle_mod.vertical_diffusion(d.nuh, d.dt, f, d.tmask, d.hn, cnpar=d.cnpar)
+    assert_bitwise(got[0], f, "tracer")
+    assert_bitwise(h, hh, "thickness")
