@@ -32,8 +32,16 @@ NX, NY, NZ = 1024, 768, 40
 
 
 def algorithmic_bytes(kernel: str, n_wet_cols: int, nz: int, s: int, tracers: int) -> float:
-    """Compulsory HBM bytes of one launch, counting wet cells/columns only."""
+    """Compulsory HBM bytes of one launch, counting wet cells/columns only
+    (s = bytes per tracer value, tracers = tracers per launch)."""
     cells = n_wet_cols * nz
+    tr = tracers * 2 * s  # every tracer read once and written once
+    if kernel in ("hpair_vu", "hpair_uv"):  # + h in/out, u, v, hu, hv; 3 masks + 5 metrics per column
+        return cells * (tr + 16 + 32) + n_wet_cols * 52
+    if kernel == "wcol":  # + h in/out, w at interfaces; mask
+        return cells * (tr + 16) + n_wet_cols * (8 * (nz + 1) + 4)
+    if kernel == "vdiff_f32":  # + hn, nuh at the nz-1 inner interfaces; mask
+        return cells * (tr + 8) + n_wet_cols * (8 * (nz - 1) + 4)
     if kernel == "adv_uv_f64":   # f, h read+write; velocity, face thickness; masks + 3 metrics per column
         return cells * (16 + 16 + 16) + n_wet_cols * (8 + 24)
     if kernel == "adv_w_f64":    # f, h read+write; w at interfaces; mask
@@ -84,14 +92,44 @@ def cpu_baseline(seconds: float, cores: int):
 
 
 # ------------------------------------------------------------------ GPU
+def blob_tracers(ntr, grid, seed, dtype):
+    """Tracers generated on the device: each a sum of 4 Gaussian blobs with
+    amplitude 10**U(-3, 1) (as inputs.make_tracers(kind="blobs"))."""
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(seed)
+    nz, ny_, nx_ = grid.nz_, grid.ny_, grid.nx_
+    dev = grid.device
+    xs = torch.arange(nx_, device=dev, dtype=torch.float64)
+    ys = torch.arange(ny_, device=dev, dtype=torch.float64)
+    zs = torch.arange(nz, device=dev, dtype=torch.float64)
+    out = torch.empty((ntr, nz, ny_, nx_), dtype=dtype, device=dev)
+    for t in range(ntr):
+        acc = torch.zeros((nz, ny_, nx_), dtype=torch.float64, device=dev)
+        for _ in range(4):
+            amp = 10.0 ** rng.uniform(-3.0, 1.0)
+            x0, y0, z0 = rng.uniform(0, nx_), rng.uniform(0, ny_), rng.uniform(0, nz)
+            sx, sy, sz = rng.uniform(0.03, 0.15) * nx_, rng.uniform(0.03, 0.15) * ny_, rng.uniform(0.1, 0.5) * nz
+            gx = torch.exp(-0.5 * ((xs - x0) / sx) ** 2)
+            gy = torch.exp(-0.5 * ((ys - y0) / sy) ** 2)
+            gz = torch.exp(-0.5 * ((zs - z0) / sz) ** 2)
+            acc += amp * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+        out[t] = acc.to(dtype)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", choices=["f64"], default="f64")
-    ap.add_argument("--tracers", type=int, default=1)
-    ap.add_argument("--scheme", type=int, default=5)
+    ap.add_argument("--workload", choices=["coastal", "multi"], default="coastal",
+                    help="coastal: 1024x768x40 with T and S, SUPERBEE; multi: same grid, 256 blob tracers, P2_PDM")
+    ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--tracers", type=int, default=0, help="override the workload's tracer count")
+    ap.add_argument("--scheme", type=int, default=0, help="override the workload's scheme")
+    ap.add_argument("--per-tracer", action="store_true", help="FP64 per-tracer operator sequence (reference order)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -108,9 +146,16 @@ def main():
     from getm_amd.core import CENTERS, INTERFACES, Grid
     from getm_amd.tracer import TracerCollection
 
+    if args.workload == "coastal":
+        ntr, scheme, kind = 2, 5, "TS"
+    else:
+        ntr, scheme, kind = 256, 3, "blobs"
+    ntr = args.tracers or ntr
+    scheme = args.scheme or scheme
+    args.tracers, args.scheme = ntr, scheme
     t_gen = time.perf_counter()
-    d = inputs.coastal(nx=NX, ny=NY, nz=NZ, scheme=args.scheme, ntracers=args.tracers,
-                       tracer_kind="TS" if args.tracers <= 2 else "blobs")
+    d = inputs.coastal(nx=NX, ny=NY, nz=NZ, scheme=scheme, ntracers=0 if kind == "blobs" else ntr,
+                       tracer_kind="TS")
     t_gen = time.perf_counter() - t_gen
     print(f"[bench] inputs generated in {t_gen:.1f} s: {NX}x{NY}x{NZ}, {d.n_wet_columns} wet columns",
           file=sys.stderr, flush=True)
@@ -121,11 +166,14 @@ def main():
     v = grid.vgrid.wrap(dev(d.v), CENTERS)
     w = grid.wrap(dev(d.w), INTERFACES)
     nuh = grid.wrap(dev(d.nuh), INTERFACES)
-    tc = TracerCollection(grid, advection_scheme=args.scheme, cnpar=1.0)
-    for t in range(args.tracers):
-        tc.add(dev(d.tracers[t]))
-    del d.u, d.v, d.w, d.nuh
-    s_bytes = 8
+    tc = TracerCollection(grid, advection_scheme=scheme, cnpar=1.0, precision=args.precision,
+                          batched=not args.per_tracer)
+    if kind == "blobs":
+        tc.set_values(blob_tracers(ntr, grid, seed=7, dtype=tc.dtype))
+    else:
+        tc.set_values(dev(d.tracers))
+    del d.u, d.v, d.w, d.nuh, d.tracers
+    s_bytes = 4 if args.precision == "f32" else 8
 
     def step():
         tc.advance(d.dt, u, v, w, nuh)
@@ -149,10 +197,12 @@ def main():
     n_wet_cols = d.n_wet_columns
     updates = n_wet_cols * NZ * args.tracers * args.steps
     value = updates / elapsed
+    ktimes = {k: v for k, v in ktimes.items() if v[1] > 0}
     dom = max(ktimes, key=lambda k: ktimes[k][0])
     dom_ms, dom_n = ktimes[dom]
     avg_s = dom_ms / 1e3 / max(dom_n, 1)
-    alg = algorithmic_bytes(dom, n_wet_cols, NZ, s_bytes, args.tracers)
+    per_launch_tracers = 1 if dom.endswith("_f64") and dom != "vdiff_prepare_f64" else args.tracers
+    alg = algorithmic_bytes(dom, n_wet_cols, NZ, s_bytes, per_launch_tracers)
     achieved = alg / avg_s / 1e9
     b_step = step_bytes(n_wet_cols, NZ, s_bytes, args.tracers)
     print("[bench] kernel times (ms total, launches): " +
@@ -178,11 +228,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": args.precision,
         "data": "synthetic (seeded coastal generator, continuity-consistent)",
         "config": {
-            "workload": f"coastal {NX}x{NY}x{NZ}, 2 km grid, 30% land, SUPERBEE, dt 30 s, "
-                        f"{args.tracers} tracer(s), FP64 bit-exact path",
+            "workload": f"coastal {NX}x{NY}x{NZ}, 2 km grid, 30% land, scheme {scheme}, dt 30 s, "
+                        f"{ntr} tracer(s), " + ("FP32 tracers with FP64 thickness" if args.precision == "f32"
+                                                 else "FP64 bit-exact") +
+                        (", per-tracer operators" if args.per_tracer else ", batched step"),
             "grid": [NX, NY, NZ],
             "wet_columns": n_wet_cols,
             "tracers": args.tracers,

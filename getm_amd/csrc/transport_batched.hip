@@ -398,7 +398,8 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
 #pragma unroll
                     for (int b = 0; b < TB; b++) fl[b] = R(0);
                 }
-                if (t0 == 0 || it < iters) ho[(size_t)(k - 1) * n2] = hnew;
+                // sub-stepped columns replay h in place per batch, so every batch writes it
+                if (t0 == 0 || iters > 1) ho[(size_t)(k - 1) * n2] = hnew;
                 if constexpr (std::is_same<R, double>::value) {
 #pragma unroll
                     for (int b = 0; b < TB; b++) {
