@@ -313,12 +313,13 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     const double half = 0.5 * dt;
     const int d1 = tr->ufirst ? 1 : 2;  // first direction of stages 1-2 (operators.py:118-121)
     const int d2 = 3 - d1;
-    if ((rc = launch_hpair<R>(d1, tr->scheme, A, half, ntr, f, S, t->ho, hA, s))) return rc;
-    if ((rc = launch_wcol<R>(tr->scheme, A, dt, ntr, S, f, hA, hB, w, w, static_cast<int*>(tr->stats.ptr), s)))
+    if ((rc = launch_hpair<R>(d1, tr->scheme, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, s))) return rc;
+    if ((rc = launch_wcol<R>(tr->scheme, A, dt, ntr, tr->tpb_w, S, f, hA, hB, w, w, static_cast<int*>(tr->stats.ptr),
+                             s)))
         return rc;
-    if ((rc = launch_hpair<R>(d2, tr->scheme, A, half, ntr, f, S, hB, hfinal, s))) return rc;
+    if ((rc = launch_hpair<R>(d2, tr->scheme, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, s))) return rc;
     if constexpr (std::is_same<R, float>::value) {
-        return launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, S, f, s);
+        return launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
     } else {
         // FP64: the reference-order per-tracer diffusion kernels, prepared once (tracer.py:273-276)
         GETM_HIP_OK(hipMemcpyAsync(f, S, n3 * (size_t)ntr * sizeof(R), hipMemcpyDeviceToDevice, s));
@@ -387,6 +388,16 @@ int transport_step(void* p, int ntracers, void* tracers, const double* u, const 
                                        h_out, s);
     return transport_step_t<double>(tr, A, ntracers, static_cast<double*>(tracers), w, nuh, timestep, molecular, h_out,
                                     s);
+}
+
+int transport_set_tuning(void* p, int tracers_per_block_h, int tracers_per_block_w, int tracers_per_block_d)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr || tracers_per_block_h < 0 || tracers_per_block_w < 0 || tracers_per_block_d < 0) return GETM_ERR_ARG;
+    tr->tpb_h = tracers_per_block_h;
+    tr->tpb_w = tracers_per_block_w;
+    tr->tpb_d = tracers_per_block_d;
+    return GETM_OK;
 }
 
 int transport_get_stats(void* p, int* max_iters, int* capped_columns, int reset, void* stream)

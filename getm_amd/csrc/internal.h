@@ -99,16 +99,18 @@ struct Transport {
     DeviceBuffer hA, hB;   // FP64 thickness between stages
     DeviceBuffer stats;    // int32[2] as in Advection
     DeviceBuffer auxo, auxn, ru;  // FP64 diffusion (precision GETM_F64)
+    int tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tracers per block of the three kernel kinds (0 = all)
 };
 
 // ---- batched kernels (transport_batched.hip) ----
+// tpb = tracers per block (tracer batches go into the grid); <= 0 means all
 template <typename R>
-int launch_hpair(int d1, int scheme, const StepArgs& A, double dt, int ntr, const R* fin, R* fout,
+int launch_hpair(int d1, int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
                  const double* hin, double* hout, hipStream_t s);
 template <typename R>
-int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, const R* fin, R* fout, const double* hin,
+int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout, const double* hin,
                 double* hout, const double* w, const double* wv, int* stats, hipStream_t s);
-int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr,
+int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
                      const float* fin, float* fout, hipStream_t s);
 
 // ---- FP64 kernels (transport_fp64.hip) ----

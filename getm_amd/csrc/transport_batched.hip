@@ -25,11 +25,20 @@
 namespace getm {
 
 // ---------------------------------------------------------------------------
-// Horizontal stage pair on one layer tile.
+// Horizontal stage pair on one layer tile (v dt/2 then u dt/2, or u then v).
+//
+// Per block: a 64 x 16 output tile of one layer and its input tile with halo 2.
+// Tracer-independent per-face and per-cell terms are formed once and kept in
+// registers; for every tracer each face is limited once into an LDS flux row,
+// cells are updated from it, and the next tracer's input tile is already in
+// flight (register prefetch) while the current one is computed.
+// Face (x, y) of direction D sits between tile cells (x, y) and (x, y) + e_D.
+// Faces outside the reference's face loops (template:61-62, 161-162) are never
+// read by an updated cell and are left closed.
 // ---------------------------------------------------------------------------
-constexpr int TX = 64, TY = 16;     // output tile
+constexpr int TX = 64, TY = 16;          // output tile
 constexpr int HX = TX + 4, HY = TY + 4;  // input tile incl. halo 2
-constexpr int NTHR = 256;
+constexpr int NTHR = 1024;
 
 struct FaceC {
     double q;     // transport Q = vel*h*metric (FP64, also drives the h update)
@@ -38,54 +47,121 @@ struct FaceC {
 };
 
 // Face (gi, gj) of direction DIR (1: U face east of T(gi,gj); 2: V face north of it).
+// Only called for faces inside the reference's face loops, where the neighbour
+// masks at p +- s exist; all loads are issued before any is used.
 template <int DIR>
 __device__ __forceinline__ FaceC face_coef(const StepArgs& A, int gi, int gj, int k, double dt)
 {
     FaceC fc;
     const size_t p = (size_t)gj * A.nx_ + gi;
+    const size_t p3 = (size_t)k * A.n2 + p;
+    const ptrdiff_t s = DIR == 1 ? 1 : A.nx_;
     const int* mask = DIR == 1 ? A.umask : A.vmask;
-    if (!open_face(mask[p])) {
+    const int m0 = mask[p], mlo = mask[p - s], mhi = mask[p + s];
+    const double vel = DIR == 1 ? A.u[p3] : A.v[p3];
+    const double hf = DIR == 1 ? A.hu[p3] : A.hv[p3];
+    const double mq = DIR == 1 ? A.dyu[p] : A.dxv[p];   // metric of the transport
+    const double mc = DIR == 1 ? A.idxu[p] : A.idyv[p];  // inverse metric of the CFL number
+    if (!open_face(m0)) {
         fc.q = 0.0;
         fc.cfl = 0.0;
         fc.flags = 0;
         return fc;
     }
-    const size_t p3 = (size_t)k * A.n2 + p;
-    const ptrdiff_t s = DIR == 1 ? 1 : A.nx_;
-    double vel;
-    if (DIR == 1) {
-        vel = A.u[p3];
-        fc.q = (vel * A.hu[p3]) * A.dyu[p];
-        fc.cfl = fabs((vel * dt) * A.idxu[p]);
-    } else {
-        vel = A.v[p3];
-        fc.q = (vel * A.hv[p3]) * A.dxv[p];
-        fc.cfl = fabs((vel * dt) * A.idyv[p]);
-    }
+    fc.q = (vel * hf) * mq;
+    fc.cfl = fabs((vel * dt) * mc);
     const bool pos = vel > 0.0;
-    const bool fuu_ok = pos ? open_face(mask[p - s]) : open_face(mask[p + s]);
+    const bool fuu_ok = pos ? open_face(mlo) : open_face(mhi);
     fc.flags = 1 | (pos ? 2 : 0) | (fuu_ok ? 4 : 0);
     return fc;
 }
 
-// Limited flux through a face from the tracer values of the four cells around
-// it along the sweep direction: fm = lower-lower, f0 = lower, f1 = upper,
-// fp = upper-upper.
-template <int S, typename R>
-__device__ __forceinline__ R face_flux(R q, R cfl, int flags, R fm, R f0, R f1, R fp)
+// Face of the reference's face loops?  x faces: i in [imin-1, imax], j in [jmin, jmax];
+// y faces: i in [imin, imax], j in [jmin-1, jmax].
+template <int DIR>
+__device__ __forceinline__ bool face_in_loop(const StepArgs& A, int gi, int gj)
 {
-    if (!(flags & 1)) return R(0);
-    R fu, fuu, fd;
-    if (flags & 2) {
-        fu = f0;
-        fuu = (flags & 4) ? fm : fu;
-        fd = f1;
-    } else {
-        fu = f1;
-        fuu = (flags & 4) ? fp : fu;
-        fd = f0;
+    if (DIR == 1) return gi >= A.imin - 1 && gi <= A.imax && gj >= A.jmin && gj <= A.jmax;
+    return gi >= A.imin && gi <= A.imax && gj >= A.jmin - 1 && gj <= A.jmax;
+}
+
+// Per-face terms that do not depend on the tracer.
+//   FP64: Q, cfl -> the reference's limiter and face value, bit for bit.
+//   FP32: Q, 0.5 (1 - cfl) and the cfl-only parts of the limiter (HSIMT:
+//   0.5 +- x; P2_PDM: 0.5 +- x, 2/(cfl + 1e-10), 2/(1 - cfl)), evaluated in FP64 once.
+template <int S, typename R>
+struct FaceK;
+
+template <int S>
+struct FaceK<S, double> {
+    double q, cfl;
+    int flags;
+    __device__ void set(const FaceC& f)
+    {
+        q = f.q;
+        cfl = f.cfl;
+        flags = f.flags;
     }
-    return q * face_value<S, R>(cfl, fu, fuu, fd);
+    __device__ __forceinline__ double value(double fu, double fuu, double fd) const
+    {
+        return face_value<S, double>(cfl, fu, fuu, fd);
+    }
+};
+
+template <int S>
+struct FaceK<S, float> {
+    static constexpr int NL = S == HSIMT ? 2 : (S == P2_PDM ? 4 : 0);
+    float q, ch;
+    float l[NL > 0 ? NL : 1];
+    int flags;
+    __device__ void set(const FaceC& f)
+    {
+        q = float(f.q);
+        ch = float(0.5 * (1.0 - f.cfl));
+        flags = f.flags;
+        if constexpr (S == HSIMT) {
+            const double kappa = 1.0 - f.cfl;
+            const double x = 0.25 * (kappa - (1.0 / 3.0) / kappa);
+            l[0] = float(0.5 + x);
+            l[1] = float(0.5 - x);
+        } else if constexpr (S == P2_PDM) {
+            const double x = (1.0 / 6.0) * (1.0 - 2.0 * f.cfl);
+            l[0] = float(0.5 + x);
+            l[1] = float(0.5 - x);
+            l[2] = float(2.0 / (f.cfl + 1.e-10));
+            l[3] = float(2.0 / (1.0 - f.cfl));
+        }
+    }
+    __device__ __forceinline__ float value(float fu, float fuu, float fd) const
+    {
+        if constexpr (S == UPSTREAM) {
+            return fu;
+        } else {
+            // branch-free: the limited value is formed always and selected
+            const float deltaf = fd - fu;
+            const float deltafu = fu - fuu;
+            const float r = deltafu * __frcp_rn(deltaf);
+            float lim;
+            if constexpr (S == SUPERBEE) lim = fmax2(fmin2(2.f * r, 1.f), fmin2(r, 2.f));
+            else if constexpr (S == MUSCL) lim = fmin3(2.f * r, 0.5f * (1.f + r), 2.f);
+            else if constexpr (S == SPLMAX13) lim = fmin3(2.f * r, (1.f / 3.f) * fmax2(1.f + 2.f * r, 2.f + r), 2.f);
+            else if constexpr (S == HSIMT) lim = fmin3(2.f * r, fmax2(0.f, l[0] + l[1] * r), 2.f);
+            else lim = fmin3(r * l[2], l[0] + l[1] * r, l[3]);  // P2_PDM
+            const float lim_fu = fu + (lim * ch) * deltaf;
+            return deltaf * deltafu > 0.f ? lim_fu : fu;
+        }
+    }
+};
+
+template <int S, typename R>
+__device__ __forceinline__ R face_flux(const FaceK<S, R>& f, R fm, R f0, R f1, R fp)
+{
+    const bool pos = f.flags & 2, uu = f.flags & 4;
+    const R fu = pos ? f0 : f1;
+    const R fd = pos ? f1 : f0;
+    const R fuu = uu ? (pos ? fm : fp) : fu;
+    const R val = f.q * f.value(fu, fuu, fd);
+    return (f.flags & 1) ? val : R(0);
 }
 
 // Horizontal diffusion flux correction (template:97-105, 197-205), read from
@@ -104,49 +180,33 @@ __device__ __forceinline__ R ah_flux(const StepArgs& A, int gi, int gj, int k, R
     return (((R(ah[p] * hf)) * (f1 - f0)) * R(m1)) * R(m2);
 }
 
-// Cell coefficients of one stage: the cell is updated iff interior and wet.
+// Per-cell terms of one stage; the cell is updated iff interior and wet.
+//   FP64: hold, hnew, iA for the reference order (h f - dt (dF iA)) / hnew
+//   FP32: a = hold/hnew, b = dt iA/hnew:  f_new = a f - b dF
 template <typename R>
-struct CellC {
-    R q0, q1, c0, c1;      // face transports (as R) and CFLs
-    int fl0, fl1;
-    int upd;
-    // FP64: hold, hnew, iA kept for the reference order (hfo - dt*advn)/hnew
-    // FP32: a = hold/hnew, b = dt*iA/hnew, f_new = a*f - b*(fl1 - fl0)
+struct CellK {
     R a, b, iA;
+    int upd;
 };
 
-template <int DIR, typename R>
-__device__ __forceinline__ void cell_coef(const StepArgs& A, int gi, int gj, int k, double dt, double hold,
-                                          CellC<R>& c, double& hnew)
+template <typename R>
+__device__ __forceinline__ void cell_set(CellK<R>& c, bool upd, double hold, double hnew, double iA, double dt)
 {
-    const bool inside = gi >= A.imin && gi <= A.imax && gj >= A.jmin && gj <= A.jmax;
-    const size_t p = (size_t)gj * A.nx_ + gi;
-    c.upd = inside && A.tmask[p] == 1;
-    hnew = hold;
-    if (!c.upd) return;
-    const FaceC f0 = DIR == 1 ? face_coef<1>(A, gi - 1, gj, k, dt) : face_coef<2>(A, gi, gj - 1, k, dt);
-    const FaceC f1 = face_coef<DIR>(A, gi, gj, k, dt);
-    const double iA = A.iarea[p];
-    hnew = hold - (dt * (f1.q - f0.q)) * iA;
-    c.q0 = R(f0.q);
-    c.q1 = R(f1.q);
-    c.c0 = R(f0.cfl);
-    c.c1 = R(f1.cfl);
-    c.fl0 = f0.flags;
-    c.fl1 = f1.flags;
+    c.upd = upd;
+    if (!upd) return;
     if constexpr (std::is_same<R, double>::value) {
         c.a = hold;
         c.b = hnew;
         c.iA = iA;
     } else {
-        c.a = R(hold / hnew);
-        c.b = R((dt * iA) / hnew);
-        c.iA = R(iA);
+        c.a = float(hold / hnew);
+        c.b = float((dt * iA) / hnew);
+        c.iA = 0.f;
     }
 }
 
 template <typename R>
-__device__ __forceinline__ R cell_update(const CellC<R>& c, double dt, R f, R fl0, R fl1)
+__device__ __forceinline__ R cell_update(const CellK<R>& c, double dt, R f, R fl0, R fl1)
 {
     if constexpr (std::is_same<R, double>::value) {
         const double hfo = c.a * f;
@@ -157,132 +217,246 @@ __device__ __forceinline__ R cell_update(const CellC<R>& c, double dt, R f, R fl
     }
 }
 
-// Local coordinates: tile-local (xl, yl), output tile [0,TX) x [0,TY), input
-// tile [-2,TX+2) x [-2,TY+2) stored at s_in[yl+2][xl+2].
+// Tile-local coordinates: output [0,TX) x [0,TY); input [-2,TX+2) x [-2,TY+2)
+// stored at s_in[y+2][x+2].  Region A (first stage) cells: D1 = y -> x in
+// [-2,TX+2), y in [0,TY); D1 = x -> x in [0,TX), y in [-2,TY+2).
 template <int S, typename R, int D1>
-__global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int ntr, const R* __restrict__ fin,
-                                                R* __restrict__ fout, const double* __restrict__ hin,
-                                                double* __restrict__ hout)
+__global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb,
+                                                const R* __restrict__ fin_all, R* __restrict__ fout_all,
+                                                const double* __restrict__ hin, double* __restrict__ hout)
 {
+    // blockIdx.z = layer + nz * tracer batch; every batch block writes the same thickness
+    const int k = blockIdx.z % A.nz, tbatch = blockIdx.z / A.nz;
+    const int tb0 = tbatch * tpb;
+    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
+    const R* fin = fin_all + (size_t)tb0 * A.n3;
+    R* fout = fout_all + (size_t)tb0 * A.n3;
     constexpr int D2 = 3 - D1;
-    // region A (first stage): D1 = y -> x in [-2,TX+2), y in [0,TY); D1 = x -> x in [0,TX), y in [-2,TY+2)
+    // region A cells
     constexpr int AX0 = D1 == 2 ? -2 : 0, AW = D1 == 2 ? HX : TX;
     constexpr int AY0 = D1 == 2 ? 0 : -2, AH = D1 == 2 ? TY : HY;
     constexpr int NA = AW * AH, NPA = (NA + NTHR - 1) / NTHR;
+    // region A faces: D1 = y -> x in [-2,TX+2), y in [-1,TY); D1 = x -> x in [-1,TX), y in [-2,TY+2)
+    constexpr int FAX0 = D1 == 2 ? -2 : -1, FAW = D1 == 2 ? HX : TX + 1;
+    constexpr int FAY0 = D1 == 2 ? -1 : -2, FAH = D1 == 2 ? TY + 1 : HY;
+    constexpr int NFA = FAW * FAH, NPFA = (NFA + NTHR - 1) / NTHR;
+    // region B faces: D2 = x -> x in [-1,TX), y in [0,TY); D2 = y -> x in [0,TX), y in [-1,TY)
+    constexpr int FBX0 = D2 == 1 ? -1 : 0, FBW = D2 == 1 ? TX + 1 : TX;
+    constexpr int FBY0 = D2 == 1 ? 0 : -1, FBH = D2 == 1 ? TY : TY + 1;
+    constexpr int NFB = FBW * FBH, NPFB = (NFB + NTHR - 1) / NTHR;
     constexpr int NB = TX * TY, NPB = NB / NTHR;
-    __shared__ R s_in[HY][HX];
+    constexpr int NIN = HX * HY, NPIN = (NIN + NTHR - 1) / NTHR;
+
+    __shared__ R s_in[2][HY][HX];
     __shared__ R s_a[HY][HX];
+    __shared__ R s_fa[NFA];
+    __shared__ R s_fb[NFB];
+    __shared__ double s_q[NFA > NFB ? NFA : NFB];  // FP64 face transports of the stage being set up
+    __shared__ double s_h[HY][HX];                 // FP64 thickness after stage A
 
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, k = blockIdx.z;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const size_t lay = (size_t)k * A.n2;
+    auto inarr = [&](int gi, int gj) { return gi >= 0 && gi < A.nx_ && gj >= 0 && gj < A.ny_; };
+    auto wet = [&](int gi, int gj) {
+        return gi >= A.imin && gi <= A.imax && gj >= A.jmin && gj <= A.jmax && A.tmask[(size_t)gj * A.nx_ + gi] == 1;
+    };
 
-    CellC<R> ca[NPA];
-    CellC<R> cb[NPB];
+    // ---- tracer-independent terms.  Each face transport is formed once and
+    // shared through LDS; h_new = h_old - (dt (Q1 - Q0)) iA as in the reference.
+    FaceK<S, R> fa[NPFA];
+#pragma unroll
+    for (int m = 0; m < NPFA; m++) {
+        const int c = tid + m * NTHR;
+        FaceC fc{0.0, 0.0, 0};
+        if (c < NFA) {
+            const int gi = x0 + FAX0 + c % FAW, gj = y0 + FAY0 + c / FAW;
+            if (face_in_loop<D1>(A, gi, gj)) fc = face_coef<D1>(A, gi, gj, k, dt);
+            s_q[c] = fc.q;
+        }
+        fa[m].set(fc);
+    }
+    __syncthreads();
+    CellK<R> ca[NPA];
 #pragma unroll
     for (int m = 0; m < NPA; m++) {
         const int c = tid + m * NTHR;
         ca[m].upd = 0;
         if (c < NA) {
-            const int gi = x0 + AX0 + c % AW, gj = y0 + AY0 + c / AW;
-            if (gi >= 0 && gi < A.nx_ && gj >= 0 && gj < A.ny_) {
-                double hn1;
-                cell_coef<D1, R>(A, gi, gj, k, dt, hin[lay + (size_t)gj * A.nx_ + gi], ca[m], hn1);
+            const int xl = AX0 + c % AW, yl = AY0 + c / AW;
+            const int gi = x0 + xl, gj = y0 + yl;
+            double h1 = 0.0;
+            if (inarr(gi, gj)) {
+                const size_t p = (size_t)gj * A.nx_ + gi;
+                const double h0 = hin[lay + p];
+                const double iA = A.iarea[p];
+                const bool upd = wet(gi, gj);
+                h1 = h0;
+                if (upd) {
+                    const int f1 = (yl - FAY0) * FAW + (xl - FAX0);
+                    const int f0 = D1 == 1 ? f1 - 1 : f1 - FAW;
+                    h1 = h0 - (dt * (s_q[f1] - s_q[f0])) * iA;
+                }
+                cell_set<R>(ca[m], upd, h0, h1, iA, dt);
             }
+            s_h[yl + 2][xl + 2] = h1;
         }
     }
+    __syncthreads();
+    FaceK<S, R> fb[NPFB];
+#pragma unroll
+    for (int m = 0; m < NPFB; m++) {
+        const int c = tid + m * NTHR;
+        FaceC fc{0.0, 0.0, 0};
+        if (c < NFB) {
+            const int gi = x0 + FBX0 + c % FBW, gj = y0 + FBY0 + c / FBW;
+            if (face_in_loop<D2>(A, gi, gj)) fc = face_coef<D2>(A, gi, gj, k, dt);
+            s_q[c] = fc.q;
+        }
+        fb[m].set(fc);
+    }
+    __syncthreads();
+    CellK<R> cb[NPB];
 #pragma unroll
     for (int m = 0; m < NPB; m++) {
         const int c = tid + m * NTHR;
-        const int gi = x0 + c % TX, gj = y0 + c / TX;
+        const int xl = c % TX, yl = c / TX;
+        const int gi = x0 + xl, gj = y0 + yl;
         cb[m].upd = 0;
         if (gi < A.nx_ && gj < A.ny_) {
-            const size_t p3 = lay + (size_t)gj * A.nx_ + gi;
-            CellC<R> tmp;
-            double h1, h2;
-            cell_coef<D1, R>(A, gi, gj, k, dt, hin[p3], tmp, h1);  // recomputed: identical operands
-            cell_coef<D2, R>(A, gi, gj, k, dt, h1, cb[m], h2);
-            hout[p3] = h2;
+            const size_t p = (size_t)gj * A.nx_ + gi;
+            const double h1 = s_h[yl + 2][xl + 2];
+            const double iA = A.iarea[p];
+            const bool upd = wet(gi, gj);
+            double h2 = h1;
+            if (upd) {
+                const int f1 = (yl - FBY0) * FBW + (xl - FBX0);
+                const int f0 = D2 == 1 ? f1 - 1 : f1 - FBW;
+                h2 = h1 - (dt * (s_q[f1] - s_q[f0])) * iA;
+            }
+            cell_set<R>(cb[m], upd, h1, h2, iA, dt);
+            hout[lay + p] = h2;
         }
     }
 
-    for (int t = 0; t < ntr; t++) {
+    // ---- tracer loop with a register prefetch of the next input tile
+    R pf[NPIN];
+    auto load_tile = [&](int t) {
         const R* fi = fin + (size_t)t * A.n3 + lay;
-        for (int c = tid; c < HX * HY; c += NTHR) {
-            const int xl = c % HX, yl = c / HX;
-            const int gi = x0 - 2 + xl, gj = y0 - 2 + yl;
+#pragma unroll
+        for (int m = 0; m < NPIN; m++) {
+            const int c = tid + m * NTHR;
             R val = R(0);
-            if (gi >= 0 && gi < A.nx_ && gj >= 0 && gj < A.ny_) val = fi[(size_t)gj * A.nx_ + gi];
-            s_in[yl][xl] = val;
+            if (c < NIN) {
+                const int gi = x0 - 2 + c % HX, gj = y0 - 2 + c / HX;
+                if (inarr(gi, gj)) val = fi[(size_t)gj * A.nx_ + gi];
+            }
+            pf[m] = val;
+        }
+    };
+    auto store_tile = [&](int buf) {
+#pragma unroll
+        for (int m = 0; m < NPIN; m++) {
+            const int c = tid + m * NTHR;
+            if (c < NIN) s_in[buf][c / HX][c % HX] = pf[m];
+        }
+    };
+    if (ntr > 0) {
+        load_tile(0);
+        store_tile(0);
+    }
+    __syncthreads();
+    for (int t = 0; t < ntr; t++) {
+        const int buf = t & 1;
+        if (t + 1 < ntr) load_tile(t + 1);
+        // stage A faces
+#pragma unroll
+        for (int m = 0; m < NPFA; m++) {
+            const int c = tid + m * NTHR;
+            if (c < NFA) {
+                const int xs = FAX0 + c % FAW + 2, ys = FAY0 + c / FAW + 2;  // s_in index of the lower cell
+                R v[4];
+#pragma unroll
+                for (int o = 0; o < 4; o++) v[o] = D1 == 1 ? s_in[buf][ys][xs - 1 + o] : s_in[buf][ys - 1 + o][xs];
+                R fl = face_flux<S, R>(fa[m], v[0], v[1], v[2], v[3]);
+                if (A.ah_u && (fa[m].flags & 1)) {
+                    const int gi = x0 + xs - 2, gj = y0 + ys - 2;
+                    fl = fl - ah_flux<D1, R>(A, gi, gj, k, v[1], v[2]);
+                }
+                s_fa[c] = fl;
+            }
         }
         __syncthreads();
+        // stage A cells
 #pragma unroll
         for (int m = 0; m < NPA; m++) {
             const int c = tid + m * NTHR;
             if (c < NA) {
-                const int xs = AX0 + c % AW + 2, ys = AY0 + c / AW + 2;  // s_in indices
-                R out = s_in[ys][xs];
+                const int xl = AX0 + c % AW, yl = AY0 + c / AW;
+                R out = s_in[buf][yl + 2][xl + 2];
                 if (ca[m].upd) {
-                    R v[5];
-#pragma unroll
-                    for (int o = 0; o < 5; o++) v[o] = D1 == 1 ? s_in[ys][xs - 2 + o] : s_in[ys - 2 + o][xs];
-                    R fl0 = face_flux<S, R>(ca[m].q0, ca[m].c0, ca[m].fl0, v[0], v[1], v[2], v[3]);
-                    R fl1 = face_flux<S, R>(ca[m].q1, ca[m].c1, ca[m].fl1, v[1], v[2], v[3], v[4]);
-                    if (A.ah_u) {
-                        const int gi = x0 + xs - 2, gj = y0 + ys - 2;
-                        if (D1 == 1) {
-                            fl0 = fl0 - ah_flux<1, R>(A, gi - 1, gj, k, v[1], v[2]);
-                            fl1 = fl1 - ah_flux<1, R>(A, gi, gj, k, v[2], v[3]);
-                        } else {
-                            fl0 = fl0 - ah_flux<2, R>(A, gi, gj - 1, k, v[1], v[2]);
-                            fl1 = fl1 - ah_flux<2, R>(A, gi, gj, k, v[2], v[3]);
-                        }
-                    }
-                    out = cell_update<R>(ca[m], dt, v[2], fl0, fl1);
+                    // faces of this cell in the region-A face array
+                    const int fx = xl - FAX0, fy = yl - FAY0;
+                    const int f1 = fy * FAW + fx;
+                    const int f0 = D1 == 1 ? f1 - 1 : f1 - FAW;
+                    out = cell_update<R>(ca[m], dt, out, s_fa[f0], s_fa[f1]);
                 }
-                s_a[ys][xs] = out;
+                s_a[yl + 2][xl + 2] = out;
             }
         }
         __syncthreads();
+        // stage B faces
+#pragma unroll
+        for (int m = 0; m < NPFB; m++) {
+            const int c = tid + m * NTHR;
+            if (c < NFB) {
+                const int xs = FBX0 + c % FBW + 2, ys = FBY0 + c / FBW + 2;
+                R v[4];
+#pragma unroll
+                for (int o = 0; o < 4; o++) v[o] = D2 == 1 ? s_a[ys][xs - 1 + o] : s_a[ys - 1 + o][xs];
+                R fl = face_flux<S, R>(fb[m], v[0], v[1], v[2], v[3]);
+                if (A.ah_u && (fb[m].flags & 1)) {
+                    const int gi = x0 + xs - 2, gj = y0 + ys - 2;
+                    fl = fl - ah_flux<D2, R>(A, gi, gj, k, v[1], v[2]);
+                }
+                s_fb[c] = fl;
+            }
+        }
+        __syncthreads();
+        // stage B cells -> HBM
         R* fo = fout + (size_t)t * A.n3 + lay;
 #pragma unroll
         for (int m = 0; m < NPB; m++) {
             const int c = tid + m * NTHR;
-            const int xs = c % TX + 2, ys = c / TX + 2;
-            const int gi = x0 + xs - 2, gj = y0 + ys - 2;
-            R out = s_a[ys][xs];
+            const int xl = c % TX, yl = c / TX;
+            const int gi = x0 + xl, gj = y0 + yl;
+            R out = s_a[yl + 2][xl + 2];
             if (cb[m].upd) {
-                R v[5];
-#pragma unroll
-                for (int o = 0; o < 5; o++) v[o] = D2 == 1 ? s_a[ys][xs - 2 + o] : s_a[ys - 2 + o][xs];
-                R fl0 = face_flux<S, R>(cb[m].q0, cb[m].c0, cb[m].fl0, v[0], v[1], v[2], v[3]);
-                R fl1 = face_flux<S, R>(cb[m].q1, cb[m].c1, cb[m].fl1, v[1], v[2], v[3], v[4]);
-                if (A.ah_u) {
-                    if (D2 == 1) {
-                        fl0 = fl0 - ah_flux<1, R>(A, gi - 1, gj, k, v[1], v[2]);
-                        fl1 = fl1 - ah_flux<1, R>(A, gi, gj, k, v[2], v[3]);
-                    } else {
-                        fl0 = fl0 - ah_flux<2, R>(A, gi, gj - 1, k, v[1], v[2]);
-                        fl1 = fl1 - ah_flux<2, R>(A, gi, gj, k, v[2], v[3]);
-                    }
-                }
-                out = cell_update<R>(cb[m], dt, v[2], fl0, fl1);
+                const int fx = xl - FBX0, fy = yl - FBY0;
+                const int f1 = fy * FBW + fx;
+                const int f0 = D2 == 1 ? f1 - 1 : f1 - FBW;
+                out = cell_update<R>(cb[m], dt, out, s_fb[f0], s_fb[f1]);
             }
             if (gi < A.nx_ && gj < A.ny_) fo[(size_t)gj * A.nx_ + gi] = out;
         }
+        if (t + 1 < ntr) store_tile(buf ^ 1);
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
 // Vertical stage, one lane per column, TB tracers per sweep.
-// advection.F90.template:223-374 with the sub-step count decided once from the
-// thickness alone: the count and every restart depend only on h and w, so the
-// h-only replay of the reference's control flow ends with the same count, and
-// the tracer pass then repeats the reference's final (restart-free) pass.
+// advection.F90.template:223-374.  The sub-step count and every restart depend
+// only on h and w, so the count is decided once from a thickness-only replay
+// of the reference's control flow, and the tracer pass then repeats the
+// reference's final (restart-free) pass.  The thickness after s sub-steps is
+// recomputed per layer by repeating the reference's own update
+// h <- h - dtk (w_k - w_{k-1}) s times, so nothing is kept in memory and
+// blocks that handle different tracer batches of a column never share
+// scratch; batch 0 writes the final thickness.
 // ---------------------------------------------------------------------------
 template <int S, typename R, int TB>
-__global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int ntr, const R* __restrict__ fin,
+__global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int ntr, int tpb, const R* __restrict__ fin,
                                               R* __restrict__ fout, const double* __restrict__ hin,
                                               double* __restrict__ hout, const double* __restrict__ w,
                                               const double* __restrict__ wv, int* __restrict__ stats)
@@ -290,33 +464,45 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= A.nx_ || j >= A.ny_) return;
+    const int tb0 = blockIdx.z * tpb;
+    const int tb1 = tb0 + tpb < ntr ? tb0 + tpb : ntr;
+    const bool first = blockIdx.z == 0;
     const size_t n2 = A.n2, n3 = A.n3;
     const int nz = A.nz;
     const size_t p = (size_t)j * A.nx_ + i;
     const bool inside = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax;
     if (!inside || A.tmask[p] != 1) {
-        for (int k = 0; k < nz; k++) hout[(size_t)k * n2 + p] = hin[(size_t)k * n2 + p];
-        for (int t = 0; t < ntr; t++)
+        if (first)
+            for (int k = 0; k < nz; k++) hout[(size_t)k * n2 + p] = hin[(size_t)k * n2 + p];
+        for (int t = tb0; t < tb1; t++)
             for (int k = 0; k < nz; k++) fout[(size_t)t * n3 + (size_t)k * n2 + p] = fin[(size_t)t * n3 + (size_t)k * n2 + p];
         return;
     }
     const double* hi = hin + p;
-    double* ho = hout + p;
     const double* wp = w + p;
     const double* wvp = wv + p;
+    double dtk = dt;
+    // thickness of 0-based layer kk after s sub-steps
+    auto h_at = [&](int kk, int steps) {
+        double h = hi[(size_t)kk * n2];
+        if (steps > 0) {
+            const double dw = wp[(size_t)(kk + 1) * n2] - wp[(size_t)kk * n2];
+            for (int st = 0; st < steps; st++) h = h - dtk * dw;
+        }
+        return h;
+    };
 
-    // ---- h-only replay of the iteration control
+    // ---- thickness-only replay of the iteration control (template:277-322)
     int iters = 1, capped = 0;
-    double itersm1 = 1.0, dtk = dt;
+    double itersm1 = 1.0;
     {
         int it = 1;
         while (it <= iters) {
-            const double* sh = it == 1 ? hi : ho;
             if (iters < kItersMax) {
-                double c = -1.7976931348623157e308;
-                double hk = sh[0];
+                double c = -1.7976931348623157e308;  // MAXVAL of an empty array
+                double hk = h_at(0, it - 1);
                 for (int k = 1; k <= nz - 1; k++) {
-                    const double hk1 = sh[(size_t)k * n2];
+                    const double hk1 = h_at(k, it - 1);
                     c = fmax2(c, (fabs(wvp[(size_t)k * n2]) * dt) / fmin2(hk, hk1));
                     hk = hk1;
                 }
@@ -333,30 +519,24 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
                     }
                 }
             }
-            if (iters == 1) break;  // single sub-step: no intermediate thickness needed
-            if (it < iters) {
-                double wprev = wp[0];
-                for (int k = 1; k <= nz; k++) {
-                    const double wk = wp[(size_t)k * n2];
-                    ho[(size_t)(k - 1) * n2] = sh[(size_t)(k - 1) * n2] - dtk * (wk - wprev);
-                    wprev = wk;
-                }
-            }
+            if (iters == 1) break;
             it++;
         }
     }
-    if (iters > 1) atomicMax(&stats[0], iters);
-    if (capped) atomicAdd(&stats[1], 1);
+    if (first) {
+        if (iters > 1) atomicMax(&stats[0], iters);
+        if (capped) atomicAdd(&stats[1], 1);
+    }
 
-    // ---- tracer passes, TB tracers at a time, sharing the h sweep
-    for (int t0 = 0; t0 < ntr; t0 += TB) {
-        const int nb = ntr - t0 < TB ? ntr - t0 : TB;
+    // ---- tracer passes, TB tracers at a time, sharing the thickness sweep
+    for (int t0 = tb0; t0 < tb1; t0 += TB) {
+        const int nb = tb1 - t0 < TB ? tb1 - t0 : TB;
         for (int it = 1; it <= iters; it++) {
-            const double* sh = it == 1 ? hi : ho;
             const R* sf = (it == 1 ? fin : fout) + (size_t)t0 * n3 + p;
             R* df = fout + (size_t)t0 * n3 + p;
+            const bool write_h = first && t0 == tb0 && it == iters;
             R fm1[TB], f0[TB], f1[TB], fprev[TB];
-            double h0 = sh[0], h1 = nz > 1 ? sh[n2] : 0.0;
+            double h0 = h_at(0, it - 1), h1 = nz > 1 ? h_at(1, it - 1) : 0.0;
 #pragma unroll
             for (int b = 0; b < TB; b++) {
                 fm1[b] = R(0);
@@ -369,7 +549,7 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
                 double h2 = 0.0;
                 R f2[TB];
                 const bool has2 = k + 2 <= nz;
-                if (has2) h2 = sh[(size_t)(k + 1) * n2];
+                if (has2) h2 = h_at(k + 1, it - 1);
 #pragma unroll
                 for (int b = 0; b < TB; b++) f2[b] = (has2 && b < nb) ? sf[(size_t)b * n3 + (size_t)(k + 1) * n2] : R(0);
                 const double wk = wp[(size_t)k * n2];
@@ -377,8 +557,12 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
                 R fl[TB];
                 if (k <= nz - 1) {
                     const double wvk = wvp[(size_t)k * n2];
-                    const double cfl = ((fabs(wvk) * dt) / fmin2(h0, h1)) * itersm1;
-                    const R rcfl = R(cfl), rw = R(wvk);
+                    R rcfl;
+                    if constexpr (std::is_same<R, double>::value)
+                        rcfl = ((fabs(wvk) * dt) / fmin2(h0, h1)) * itersm1;
+                    else
+                        rcfl = float(fabs(wvk) * dt * itersm1) * __frcp_rn(float(fmin2(h0, h1)));
+                    const R rw = R(wvk);
                     const bool pos = wvk > 0.0;
 #pragma unroll
                     for (int b = 0; b < TB; b++) {
@@ -398,8 +582,7 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
 #pragma unroll
                     for (int b = 0; b < TB; b++) fl[b] = R(0);
                 }
-                // sub-stepped columns replay h in place per batch, so every batch writes it
-                if (t0 == 0 || iters > 1) ho[(size_t)(k - 1) * n2] = hnew;
+                if (write_h) hout[(size_t)(k - 1) * n2 + p] = hnew;
                 if constexpr (std::is_same<R, double>::value) {
 #pragma unroll
                     for (int b = 0; b < TB; b++) {
@@ -408,7 +591,8 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
                         if (b < nb) df[(size_t)b * n3 + (size_t)(k - 1) * n2] = (hfo - dtk * advn) / hnew;
                     }
                 } else {
-                    const R ca = R(h0 / hnew), cb = R(dtk / hnew);
+                    const double rh = 1.0 / hnew;
+                    const R ca = R(h0 * rh), cb = R(dtk * rh);
 #pragma unroll
                     for (int b = 0; b < TB; b++)
                         if (b < nb) df[(size_t)b * n3 + (size_t)(k - 1) * n2] = ca * f0[b] - cb * (fl[b] - fprev[b]);
@@ -440,9 +624,13 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
 // ---------------------------------------------------------------------------
 template <int NZMAX, bool CN1>
 __global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, double cnpar, double molecular,
-                                                  const double* __restrict__ nuh, int ntr, const float* __restrict__ fin,
-                                                  float* __restrict__ fout)
+                                                  const double* __restrict__ nuh, int ntr_all, int tpb,
+                                                  const float* __restrict__ fin_all, float* __restrict__ fout_all)
 {
+    const int tb0 = blockIdx.z * tpb;
+    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
+    const float* fin = fin_all + (size_t)tb0 * A.n3;
+    float* fout = fout_all + (size_t)tb0 * A.n3;
     extern __shared__ float s_coef[];  // [ncoef][nz][64]
     const int lane = threadIdx.x;
     const int i = blockIdx.x * 64 + lane;
@@ -492,13 +680,21 @@ __global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, d
             ru_up = ruk;
         }
     }
-    for (int t = 0; t < ntr; t++) {
-        const float* fi = fin + (size_t)t * n3 + p;
-        float* fo = fout + (size_t)t * n3 + p;
-        float vr[NZMAX];
+    // the next tracer's column is loaded while the current one is solved
+    float vr[NZMAX], vn[NZMAX];
+    if (ntr > 0) {
 #pragma unroll
         for (int k = 0; k < NZMAX; k++)
-            if (k < nz) vr[k] = fi[(size_t)k * n2];
+            if (k < nz) vr[k] = fin[p + (size_t)k * n2];
+    }
+    for (int t = 0; t < ntr; t++) {
+        float* fo = fout + (size_t)t * n3 + p;
+        if (t + 1 < ntr) {
+            const float* fi = fin + (size_t)(t + 1) * n3 + p;
+#pragma unroll
+            for (int k = 0; k < NZMAX; k++)
+                if (k < nz) vn[k] = fi[(size_t)k * n2];
+        }
         if (nz == 1) {
             // diffusion.F90:224-231: every point, halos and land included
             const double dz = A.hn[p];
@@ -533,59 +729,72 @@ __global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, d
 #pragma unroll
         for (int k = 0; k < NZMAX; k++)
             if (k < nz) fo[(size_t)k * n2] = vr[k];
+#pragma unroll
+        for (int k = 0; k < NZMAX; k++) vr[k] = vn[k];
     }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static inline int nbatch(int ntr, int tpb) { return tpb <= 0 ? 1 : (ntr + tpb - 1) / tpb; }
+
 template <typename R>
-int launch_hpair(int d1, int scheme, const StepArgs& A, double dt, int ntr, const R* fin, R* fout,
+int launch_hpair(int d1, int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
                  const double* hin, double* hout, hipStream_t s)
 {
-    dim3 grid((A.nx_ + TX - 1) / TX, (A.ny_ + TY - 1) / TY, A.nz);
+    if (tpb <= 0 || tpb > ntr) tpb = ntr;
+    const int nb = nbatch(ntr, tpb);
+    if ((long)A.nz * nb > 65535) return GETM_ERR_CAPACITY;
+    dim3 grid((A.nx_ + TX - 1) / TX, (A.ny_ + TY - 1) / TY, A.nz * nb);
     ProfileScope ps(d1 == 2 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
     GETM_SCHEME_SWITCH(scheme, {
         if (d1 == 2)
-            hipLaunchKernelGGL((k_hpair<S_, R, 2>), grid, dim3(NTHR), 0, s, A, dt, ntr, fin, fout, hin, hout);
+            hipLaunchKernelGGL((k_hpair<S_, R, 2>), grid, dim3(NTHR), 0, s, A, dt, ntr, tpb, fin, fout, hin, hout);
         else
-            hipLaunchKernelGGL((k_hpair<S_, R, 1>), grid, dim3(NTHR), 0, s, A, dt, ntr, fin, fout, hin, hout);
+            hipLaunchKernelGGL((k_hpair<S_, R, 1>), grid, dim3(NTHR), 0, s, A, dt, ntr, tpb, fin, fout, hin, hout);
     });
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }
 
 template <typename R>
-int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, const R* fin, R* fout, const double* hin,
+int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout, const double* hin,
                 double* hout, const double* w, const double* wv, int* stats, hipStream_t s)
 {
+    if (tpb <= 0 || tpb > ntr) tpb = ntr;
+    const int nb = nbatch(ntr, tpb);
+    if (nb > 65535) return GETM_ERR_CAPACITY;
     dim3 block(64, 4, 1);
-    dim3 grid((A.nx_ + 63) / 64, (A.ny_ + 3) / 4, 1);
+    dim3 grid((A.nx_ + 63) / 64, (A.ny_ + 3) / 4, nb);
     ProfileScope ps(GETM_K_WCOL, s);
     constexpr int TB = std::is_same<R, double>::value ? 2 : 4;
     GETM_SCHEME_SWITCH(scheme, {
-        hipLaunchKernelGGL((k_wcol<S_, R, TB>), grid, block, 0, s, A, dt, ntr, fin, fout, hin, hout, w, wv, stats);
+        hipLaunchKernelGGL((k_wcol<S_, R, TB>), grid, block, 0, s, A, dt, ntr, tpb, fin, fout, hin, hout, w, wv, stats);
     });
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }
 
-int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr,
+int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
                      const float* fin, float* fout, hipStream_t s)
 {
     if (A.nz > 64) return GETM_ERR_CAPACITY;
+    if (tpb <= 0 || tpb > ntr) tpb = ntr;
+    const int nb = nbatch(ntr, tpb);
+    if (nb > 65535) return GETM_ERR_CAPACITY;
     const bool cn1 = cnpar == 1.0;
     const size_t shm = (size_t)(cn1 ? 3 : 5) * A.nz * 64 * sizeof(float);
-    dim3 grid((A.nx_ + 63) / 64, A.ny_, 1);
+    dim3 grid((A.nx_ + 63) / 64, A.ny_, nb);
     ProfileScope ps(GETM_K_VDIFF_F32, s);
-#define GETM_VD(NZM)                                                                                          \
-    do {                                                                                                      \
-        if (cn1)                                                                                              \
+#define GETM_VD(NZM)                                                                                             \
+    do {                                                                                                         \
+        if (cn1)                                                                                                 \
             hipLaunchKernelGGL((k_vdiff_f32<NZM, true>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, ntr, \
-                               fin, fout);                                                                    \
-        else                                                                                                  \
+                               tpb, fin, fout);                                                                  \
+        else                                                                                                     \
             hipLaunchKernelGGL((k_vdiff_f32<NZM, false>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, ntr, \
-                               fin, fout);                                                                    \
+                               tpb, fin, fout);                                                                  \
     } while (0)
     if (A.nz <= 16) GETM_VD(16);
     else if (A.nz <= 32) GETM_VD(32);
@@ -596,13 +805,13 @@ int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecula
     return GETM_OK;
 }
 
-template int launch_hpair<float>(int, int, const StepArgs&, double, int, const float*, float*, const double*, double*,
-                                 hipStream_t);
-template int launch_hpair<double>(int, int, const StepArgs&, double, int, const double*, double*, const double*,
+template int launch_hpair<float>(int, int, const StepArgs&, double, int, int, const float*, float*, const double*,
+                                 double*, hipStream_t);
+template int launch_hpair<double>(int, int, const StepArgs&, double, int, int, const double*, double*, const double*,
                                   double*, hipStream_t);
-template int launch_wcol<float>(int, const StepArgs&, double, int, const float*, float*, const double*, double*,
+template int launch_wcol<float>(int, const StepArgs&, double, int, int, const float*, float*, const double*, double*,
                                 const double*, const double*, int*, hipStream_t);
-template int launch_wcol<double>(int, const StepArgs&, double, int, const double*, double*, const double*, double*,
-                                 const double*, const double*, int*, hipStream_t);
+template int launch_wcol<double>(int, const StepArgs&, double, int, int, const double*, double*, const double*,
+                                 double*, const double*, const double*, int*, hipStream_t);
 
 }  // namespace getm

@@ -111,6 +111,15 @@ class TracerCollection:
             _lib.ptr(self.h), _lib.stream_handle())
         _lib.check(rc, "transport_step")
 
+    def set_tuning(self, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
+                   tracers_per_block_d: int = 16) -> None:
+        """Launch shape of the batched kernels (tracers per block; 0 = all).  Does not
+        change results."""
+        if not self.batched:
+            return
+        _lib.check(_lib.load().transport_set_tuning(self.p, int(tracers_per_block_h), int(tracers_per_block_w),
+                                                    int(tracers_per_block_d)), "transport_set_tuning")
+
     def stats(self, reset: bool = True):
         """(largest z sub-iteration count, columns capped at 200) since the last reset."""
         if not self.batched:

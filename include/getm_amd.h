@@ -150,6 +150,12 @@ int transport_step(void* tr, int ntracers, void* tracers, const double* u, const
                    const double* nuh, const double* Ah_u, const double* Ah_v, double timestep, double molecular,
                    double* h_out, void* stream);
 
+/* No reference counterpart: launch shape of the three batched kernel kinds
+ * (horizontal stage pairs, w stage, FP32 diffusion) as tracers per block; the
+ * tracer batches of one step are spread over the grid.  0 = all tracers in
+ * every block.  Results do not depend on these values. */
+int transport_set_tuning(void* tr, int tracers_per_block_h, int tracers_per_block_w, int tracers_per_block_d);
+
 /* z-sweep sub-stepping statistics of a transport object (as advection_get_stats). */
 int transport_get_stats(void* tr, int* max_iters, int* capped_columns, int reset, void* stream);
 

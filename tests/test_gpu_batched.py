@@ -131,3 +131,24 @@ def test_batched_f64_ufirst(oracle_mod):
     oracle_mod.vertical_diffusion(d.nuh, d.dt, f, d.tmask, d.hn, cnpar=d.cnpar)
     assert_bitwise(got[0], f, "tracer")
     assert_bitwise(h, hh, "thickness")
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_batched_launch_shape_does_not_change_results(precision):
+    """Tracer batches spread over the grid (transport_set_tuning) give identical bits."""
+    d = inputs.tiny(nx=37, ny=30, nz=9, seed=91, ntracers=7, cfl=0.5, wscale=2.0)
+    outs = []
+    for tuning in [(0, 0, 0), (1, 1, 1), (2, 3, 5), (4, 16, 16)]:
+        from getm_amd.core import CENTERS, INTERFACES
+        from getm_amd.tracer import TracerCollection
+
+        grid = make_grid(d)
+        tc = TracerCollection(grid, advection_scheme=d.scheme, cnpar=d.cnpar, precision=precision)
+        tc.set_tuning(*tuning)
+        tc.set_values(dev(d.tracers))
+        tc.advance(d.dt, grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS),
+                   grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES))
+        outs.append((host(tc.values), host(tc.h)))
+    for o in outs[1:]:
+        assert_bitwise(o[0], outs[0][0], "tracers")
+        assert_bitwise(o[1], outs[0][1], "thickness")
