@@ -28,16 +28,22 @@ namespace getm {
 // Horizontal stage pair on one layer tile (v dt/2 then u dt/2, or u then v).
 //
 // Per block: a 64 x 16 output tile of one layer and its input tile with halo 2.
-// Tracer-independent per-face and per-cell terms are formed once and kept in
-// registers; for every tracer each face is limited once into an LDS flux row,
-// cells are updated from it, and the next tracer's input tile is already in
-// flight (register prefetch) while the current one is computed.
+// Set-up (tracer-independent, once per block): every global load of the
+// set-up -- face fields of both stages, cell fields and the first tracer's
+// tile -- is issued before any is used, so the block waits for HBM once.  Face
+// transports are formed once and shared through LDS for the FP64 thickness
+// update; each face keeps its transport, the cfl-only limiter terms and the
+// LDS offsets of its upstream, upstream-upstream and downstream cells, so the
+// tracer loop only loads, limits and updates.  Per tracer, each face is
+// limited once into an LDS flux row and the next tracer's tile is in flight
+// (register prefetch) while the current one is computed.
 // Face (x, y) of direction D sits between tile cells (x, y) and (x, y) + e_D.
 // Faces outside the reference's face loops (template:61-62, 161-162) are never
 // read by an updated cell and are left closed.
 // ---------------------------------------------------------------------------
 constexpr int TX = 64, TY = 16;          // output tile
 constexpr int HX = TX + 4, HY = TY + 4;  // input tile incl. halo 2
+constexpr int NT = HX * HY;              // cells of the input tile
 constexpr int NTHR = 1024;
 
 struct FaceC {
@@ -46,38 +52,15 @@ struct FaceC {
     int flags;    // bit0 open, bit1 vel > 0, bit2 upstream-upstream face open
 };
 
-// Face (gi, gj) of direction DIR (1: U face east of T(gi,gj); 2: V face north of it).
-// Only called for faces inside the reference's face loops, where the neighbour
-// masks at p +- s exist; all loads are issued before any is used.
-template <int DIR>
-__device__ __forceinline__ FaceC face_coef(const StepArgs& A, int gi, int gj, int k, double dt)
-{
-    FaceC fc;
-    const size_t p = (size_t)gj * A.nx_ + gi;
-    const size_t p3 = (size_t)k * A.n2 + p;
-    const ptrdiff_t s = DIR == 1 ? 1 : A.nx_;
-    const int* mask = DIR == 1 ? A.umask : A.vmask;
-    const int m0 = mask[p], mlo = mask[p - s], mhi = mask[p + s];
-    const double vel = DIR == 1 ? A.u[p3] : A.v[p3];
-    const double hf = DIR == 1 ? A.hu[p3] : A.hv[p3];
-    const double mq = DIR == 1 ? A.dyu[p] : A.dxv[p];   // metric of the transport
-    const double mc = DIR == 1 ? A.idxu[p] : A.idyv[p];  // inverse metric of the CFL number
-    if (!open_face(m0)) {
-        fc.q = 0.0;
-        fc.cfl = 0.0;
-        fc.flags = 0;
-        return fc;
-    }
-    fc.q = (vel * hf) * mq;
-    fc.cfl = fabs((vel * dt) * mc);
-    const bool pos = vel > 0.0;
-    const bool fuu_ok = pos ? open_face(mlo) : open_face(mhi);
-    fc.flags = 1 | (pos ? 2 : 0) | (fuu_ok ? 4 : 0);
-    return fc;
-}
+// Global fields of one face, loaded before use.
+struct FaceRaw {
+    int inloop, m0, mlo, mhi;
+    double vel, hf, mq, mc;
+};
 
 // Face of the reference's face loops?  x faces: i in [imin-1, imax], j in [jmin, jmax];
-// y faces: i in [imin, imax], j in [jmin-1, jmax].
+// y faces: i in [imin, imax], j in [jmin-1, jmax].  There the neighbour masks at
+// p +- s exist (halo >= 2).
 template <int DIR>
 __device__ __forceinline__ bool face_in_loop(const StepArgs& A, int gi, int gj)
 {
@@ -85,51 +68,102 @@ __device__ __forceinline__ bool face_in_loop(const StepArgs& A, int gi, int gj)
     return gi >= A.imin && gi <= A.imax && gj >= A.jmin - 1 && gj <= A.jmax;
 }
 
-// Per-face terms that do not depend on the tracer.
+// Face (gi, gj) of direction DIR (1: U face east of T(gi,gj); 2: V face north of it).
+template <int DIR>
+__device__ __forceinline__ void face_load(const StepArgs& A, int gi, int gj, int k, FaceRaw& r)
+{
+    r.inloop = face_in_loop<DIR>(A, gi, gj);
+    if (!r.inloop) return;
+    const size_t p = (size_t)gj * A.nx_ + gi;
+    const size_t p3 = (size_t)k * A.n2 + p;
+    const ptrdiff_t s = DIR == 1 ? 1 : A.nx_;
+    const int* mask = DIR == 1 ? A.umask : A.vmask;
+    r.m0 = mask[p];
+    r.mlo = mask[p - s];
+    r.mhi = mask[p + s];
+    r.vel = DIR == 1 ? A.u[p3] : A.v[p3];
+    r.hf = DIR == 1 ? A.hu[p3] : A.hv[p3];
+    r.mq = DIR == 1 ? A.dyu[p] : A.dxv[p];   // metric of the transport
+    r.mc = DIR == 1 ? A.idxu[p] : A.idyv[p];  // inverse metric of the CFL number
+}
+
+// QU = u*hu*dyu, cfl = |u*dt*idxu| (template:64-65; QV/cfl likewise :164-165)
+__device__ __forceinline__ FaceC face_make(const FaceRaw& r, double dt)
+{
+    FaceC fc{0.0, 0.0, 0};
+    if (!r.inloop || !open_face(r.m0)) return fc;
+    fc.q = (r.vel * r.hf) * r.mq;
+    fc.cfl = fabs((r.vel * dt) * r.mc);
+    const bool pos = r.vel > 0.0;
+    const bool fuu_ok = pos ? open_face(r.mlo) : open_face(r.mhi);
+    fc.flags = 1 | (pos ? 2 : 0) | (fuu_ok ? 4 : 0);
+    return fc;
+}
+
+// Per-face terms that do not depend on the tracer, plus the LDS offsets of the
+// upstream (o_u), upstream-upstream (o_uu) and downstream (o_d) cells, chosen by
+// the velocity sign and the neighbour mask (template:66-80).
 //   FP64: Q, cfl -> the reference's limiter and face value, bit for bit.
 //   FP32: Q, 0.5 (1 - cfl) and the cfl-only parts of the limiter (HSIMT:
 //   0.5 +- x; P2_PDM: 0.5 +- x, 2/(cfl + 1e-10), 2/(1 - cfl)), evaluated in FP64 once.
 template <int S, typename R>
 struct FaceK;
 
-template <int S>
-struct FaceK<S, double> {
-    double q, cfl;
-    int flags;
-    __device__ void set(const FaceC& f)
+struct FaceOff {
+    int open, o_u, o_uu, o_d;
+    // base = LDS offset of the lower cell, sd = stride along the sweep.
+    // Branch-free so the per-face terms stay in registers.
+    __device__ __forceinline__ void set(const FaceC& f, int base, int sd)
     {
-        q = f.q;
-        cfl = f.cfl;
-        flags = f.flags;
-    }
-    __device__ __forceinline__ double value(double fu, double fuu, double fd) const
-    {
-        return face_value<S, double>(cfl, fu, fuu, fd);
+        const bool op = f.flags & 1, pos = f.flags & 2, uu = f.flags & 4;
+        const int ou = pos ? base : base + sd;
+        const int od = pos ? base + sd : base;
+        const int ouu = uu ? (pos ? base - sd : base + 2 * sd) : ou;
+        open = op;
+        o_u = op ? ou : base;
+        o_d = op ? od : base;
+        o_uu = op ? ouu : base;
     }
 };
 
 template <int S>
-struct FaceK<S, float> {
-    static constexpr int NL = S == HSIMT ? 2 : (S == P2_PDM ? 4 : 0);
-    float q, ch;
-    float l[NL > 0 ? NL : 1];
-    int flags;
-    __device__ void set(const FaceC& f)
+struct FaceK<S, double> : FaceOff {
+    double q, cfl;
+    __device__ void set(const FaceC& f, int base, int sd)
     {
-        q = float(f.q);
-        ch = float(0.5 * (1.0 - f.cfl));
-        flags = f.flags;
+        FaceOff::set(f, base, sd);
+        q = f.q;
+        cfl = f.cfl;
+    }
+    __device__ __forceinline__ double flux(const double* s) const
+    {
+        const double fl = q * face_value<S, double>(cfl, s[o_u], s[o_uu], s[o_d]);
+        return open ? fl : 0.0;
+    }
+};
+
+// FP32 limited face value with the cfl-only terms formed once in FP64:
+// 0.5 (1 - cfl), and HSIMT: 0.5 +- x; P2_PDM: 0.5 +- x, 2/(cfl + 1e-10), 2/(1 - cfl)
+// (limiters.h:1-44).  Branch-free: the limited value is formed always and selected.
+template <int S>
+struct LimF {
+    static constexpr int NL = S == HSIMT ? 2 : (S == P2_PDM ? 4 : 0);
+    float ch;
+    float l[NL > 0 ? NL : 1];
+    __device__ __forceinline__ void set(double cfl)
+    {
+        ch = float(0.5 * (1.0 - cfl));
         if constexpr (S == HSIMT) {
-            const double kappa = 1.0 - f.cfl;
+            const double kappa = 1.0 - cfl;
             const double x = 0.25 * (kappa - (1.0 / 3.0) / kappa);
             l[0] = float(0.5 + x);
             l[1] = float(0.5 - x);
         } else if constexpr (S == P2_PDM) {
-            const double x = (1.0 / 6.0) * (1.0 - 2.0 * f.cfl);
+            const double x = (1.0 / 6.0) * (1.0 - 2.0 * cfl);
             l[0] = float(0.5 + x);
             l[1] = float(0.5 - x);
-            l[2] = float(2.0 / (f.cfl + 1.e-10));
-            l[3] = float(2.0 / (1.0 - f.cfl));
+            l[2] = float(2.0 / (cfl + 1.e-10));
+            l[3] = float(2.0 / (1.0 - cfl));
         }
     }
     __device__ __forceinline__ float value(float fu, float fuu, float fd) const
@@ -137,7 +171,6 @@ struct FaceK<S, float> {
         if constexpr (S == UPSTREAM) {
             return fu;
         } else {
-            // branch-free: the limited value is formed always and selected
             const float deltaf = fd - fu;
             const float deltafu = fu - fuu;
             const float r = deltafu * __frcp_rn(deltaf);
@@ -153,16 +186,22 @@ struct FaceK<S, float> {
     }
 };
 
-template <int S, typename R>
-__device__ __forceinline__ R face_flux(const FaceK<S, R>& f, R fm, R f0, R f1, R fp)
-{
-    const bool pos = f.flags & 2, uu = f.flags & 4;
-    const R fu = pos ? f0 : f1;
-    const R fd = pos ? f1 : f0;
-    const R fuu = uu ? (pos ? fm : fp) : fu;
-    const R val = f.q * f.value(fu, fuu, fd);
-    return (f.flags & 1) ? val : R(0);
-}
+template <int S>
+struct FaceK<S, float> : FaceOff {
+    float q;
+    LimF<S> lim;
+    __device__ void set(const FaceC& f, int base, int sd)
+    {
+        FaceOff::set(f, base, sd);
+        q = float(f.q);
+        lim.set(f.cfl);
+    }
+    __device__ __forceinline__ float flux(const float* s) const
+    {
+        const float val = lim.value(s[o_u], s[o_uu], s[o_d]);
+        return open ? q * val : 0.f;
+    }
+};
 
 // Horizontal diffusion flux correction (template:97-105, 197-205), read from
 // global memory: only used when Ah is given.
@@ -180,13 +219,20 @@ __device__ __forceinline__ R ah_flux(const StepArgs& A, int gi, int gj, int k, R
     return (((R(ah[p] * hf)) * (f1 - f0)) * R(m1)) * R(m2);
 }
 
+// Global fields of one cell, loaded before use.
+struct CellRaw {
+    int inarr, tm;
+    double h0, iA;
+};
+
 // Per-cell terms of one stage; the cell is updated iff interior and wet.
 //   FP64: hold, hnew, iA for the reference order (h f - dt (dF iA)) / hnew
 //   FP32: a = hold/hnew, b = dt iA/hnew:  f_new = a f - b dF
+// own = LDS offset of the cell, fl1 = offset of its upper face flux.
 template <typename R>
 struct CellK {
     R a, b, iA;
-    int upd;
+    int upd, own, fl1;
 };
 
 template <typename R>
@@ -218,8 +264,8 @@ __device__ __forceinline__ R cell_update(const CellK<R>& c, double dt, R f, R fl
 }
 
 // Tile-local coordinates: output [0,TX) x [0,TY); input [-2,TX+2) x [-2,TY+2)
-// stored at s_in[y+2][x+2].  Region A (first stage) cells: D1 = y -> x in
-// [-2,TX+2), y in [0,TY); D1 = x -> x in [0,TX), y in [-2,TY+2).
+// stored at LDS offset (y+2)*HX + (x+2).  Region A (first stage) cells:
+// D1 = y -> x in [-2,TX+2), y in [0,TY); D1 = x -> x in [0,TX), y in [-2,TY+2).
 template <int S, typename R, int D1>
 __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb,
                                                 const R* __restrict__ fin_all, R* __restrict__ fout_all,
@@ -231,7 +277,9 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
     const R* fin = fin_all + (size_t)tb0 * A.n3;
     R* fout = fout_all + (size_t)tb0 * A.n3;
+
     constexpr int D2 = 3 - D1;
+    constexpr int SD1 = D1 == 1 ? 1 : HX, SD2 = D2 == 1 ? 1 : HX;  // LDS strides along the sweeps
     // region A cells
     constexpr int AX0 = D1 == 2 ? -2 : 0, AW = D1 == 2 ? HX : TX;
     constexpr int AY0 = D1 == 2 ? 0 : -2, AH = D1 == 2 ? TY : HY;
@@ -245,101 +293,34 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
     constexpr int FBY0 = D2 == 1 ? 0 : -1, FBH = D2 == 1 ? TY : TY + 1;
     constexpr int NFB = FBW * FBH, NPFB = (NFB + NTHR - 1) / NTHR;
     constexpr int NB = TX * TY, NPB = NB / NTHR;
-    constexpr int NIN = HX * HY, NPIN = (NIN + NTHR - 1) / NTHR;
+    constexpr int NPIN = (NT + NTHR - 1) / NTHR;
 
-    __shared__ R s_in[2][HY][HX];
-    __shared__ R s_a[HY][HX];
+    __shared__ R s_in[3][NT];
+    __shared__ R s_a[NT];
     __shared__ R s_fa[NFA];
     __shared__ R s_fb[NFB];
     __shared__ double s_q[NFA > NFB ? NFA : NFB];  // FP64 face transports of the stage being set up
-    __shared__ double s_h[HY][HX];                 // FP64 thickness after stage A
+    __shared__ double s_h[NT];                     // FP64 thickness after stage A
 
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const size_t lay = (size_t)k * A.n2;
     auto inarr = [&](int gi, int gj) { return gi >= 0 && gi < A.nx_ && gj >= 0 && gj < A.ny_; };
-    auto wet = [&](int gi, int gj) {
-        return gi >= A.imin && gi <= A.imax && gj >= A.jmin && gj <= A.jmax && A.tmask[(size_t)gj * A.nx_ + gi] == 1;
+    auto cell_load = [&](int gi, int gj, CellRaw& r) {
+        r.inarr = inarr(gi, gj);
+        if (!r.inarr) return;
+        const size_t p = (size_t)gj * A.nx_ + gi;
+        r.h0 = hin[lay + p];
+        r.iA = A.iarea[p];
+        r.tm = A.tmask[p];
+    };
+    auto wet = [&](int gi, int gj, int tm) {
+        return gi >= A.imin && gi <= A.imax && gj >= A.jmin && gj <= A.jmax && tm == 1;
     };
 
-    // ---- tracer-independent terms.  Each face transport is formed once and
-    // shared through LDS; h_new = h_old - (dt (Q1 - Q0)) iA as in the reference.
-    FaceK<S, R> fa[NPFA];
-#pragma unroll
-    for (int m = 0; m < NPFA; m++) {
-        const int c = tid + m * NTHR;
-        FaceC fc{0.0, 0.0, 0};
-        if (c < NFA) {
-            const int gi = x0 + FAX0 + c % FAW, gj = y0 + FAY0 + c / FAW;
-            if (face_in_loop<D1>(A, gi, gj)) fc = face_coef<D1>(A, gi, gj, k, dt);
-            s_q[c] = fc.q;
-        }
-        fa[m].set(fc);
-    }
-    __syncthreads();
-    CellK<R> ca[NPA];
-#pragma unroll
-    for (int m = 0; m < NPA; m++) {
-        const int c = tid + m * NTHR;
-        ca[m].upd = 0;
-        if (c < NA) {
-            const int xl = AX0 + c % AW, yl = AY0 + c / AW;
-            const int gi = x0 + xl, gj = y0 + yl;
-            double h1 = 0.0;
-            if (inarr(gi, gj)) {
-                const size_t p = (size_t)gj * A.nx_ + gi;
-                const double h0 = hin[lay + p];
-                const double iA = A.iarea[p];
-                const bool upd = wet(gi, gj);
-                h1 = h0;
-                if (upd) {
-                    const int f1 = (yl - FAY0) * FAW + (xl - FAX0);
-                    const int f0 = D1 == 1 ? f1 - 1 : f1 - FAW;
-                    h1 = h0 - (dt * (s_q[f1] - s_q[f0])) * iA;
-                }
-                cell_set<R>(ca[m], upd, h0, h1, iA, dt);
-            }
-            s_h[yl + 2][xl + 2] = h1;
-        }
-    }
-    __syncthreads();
-    FaceK<S, R> fb[NPFB];
-#pragma unroll
-    for (int m = 0; m < NPFB; m++) {
-        const int c = tid + m * NTHR;
-        FaceC fc{0.0, 0.0, 0};
-        if (c < NFB) {
-            const int gi = x0 + FBX0 + c % FBW, gj = y0 + FBY0 + c / FBW;
-            if (face_in_loop<D2>(A, gi, gj)) fc = face_coef<D2>(A, gi, gj, k, dt);
-            s_q[c] = fc.q;
-        }
-        fb[m].set(fc);
-    }
-    __syncthreads();
-    CellK<R> cb[NPB];
-#pragma unroll
-    for (int m = 0; m < NPB; m++) {
-        const int c = tid + m * NTHR;
-        const int xl = c % TX, yl = c / TX;
-        const int gi = x0 + xl, gj = y0 + yl;
-        cb[m].upd = 0;
-        if (gi < A.nx_ && gj < A.ny_) {
-            const size_t p = (size_t)gj * A.nx_ + gi;
-            const double h1 = s_h[yl + 2][xl + 2];
-            const double iA = A.iarea[p];
-            const bool upd = wet(gi, gj);
-            double h2 = h1;
-            if (upd) {
-                const int f1 = (yl - FBY0) * FBW + (xl - FBX0);
-                const int f0 = D2 == 1 ? f1 - 1 : f1 - FBW;
-                h2 = h1 - (dt * (s_q[f1] - s_q[f0])) * iA;
-            }
-            cell_set<R>(cb[m], upd, h1, h2, iA, dt);
-            hout[lay + p] = h2;
-        }
-    }
-
-    // ---- tracer loop with a register prefetch of the next input tile
+    // ---- every set-up load in flight at once
+    FaceRaw rfa[NPFA], rfb[NPFB];
+    CellRaw rca[NPA], rcb[NPB];
     R pf[NPIN];
     auto load_tile = [&](int t) {
         const R* fi = fin + (size_t)t * A.n3 + lay;
@@ -347,7 +328,7 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
         for (int m = 0; m < NPIN; m++) {
             const int c = tid + m * NTHR;
             R val = R(0);
-            if (c < NIN) {
+            if (c < NT) {
                 const int gi = x0 - 2 + c % HX, gj = y0 - 2 + c / HX;
                 if (inarr(gi, gj)) val = fi[(size_t)gj * A.nx_ + gi];
             }
@@ -358,30 +339,118 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
 #pragma unroll
         for (int m = 0; m < NPIN; m++) {
             const int c = tid + m * NTHR;
-            if (c < NIN) s_in[buf][c / HX][c % HX] = pf[m];
+            if (c < NT) s_in[buf][c] = pf[m];
         }
     };
-    if (ntr > 0) {
-        load_tile(0);
-        store_tile(0);
+#pragma unroll
+    for (int m = 0; m < NPFA; m++) {
+        const int c = tid + m * NTHR;
+        rfa[m].inloop = 0;
+        if (c < NFA) face_load<D1>(A, x0 + FAX0 + c % FAW, y0 + FAY0 + c / FAW, k, rfa[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < NPFB; m++) {
+        const int c = tid + m * NTHR;
+        rfb[m].inloop = 0;
+        if (c < NFB) face_load<D2>(A, x0 + FBX0 + c % FBW, y0 + FBY0 + c / FBW, k, rfb[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < NPA; m++) {
+        const int c = tid + m * NTHR;
+        rca[m].inarr = 0;
+        if (c < NA) cell_load(x0 + AX0 + c % AW, y0 + AY0 + c / AW, rca[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < NPB; m++) {
+        const int c = tid + m * NTHR;
+        cell_load(x0 + c % TX, y0 + c / TX, rcb[m]);
+    }
+    if (ntr > 0) load_tile(0);
+
+    // ---- stage A faces: transports to LDS for the thickness update
+    FaceK<S, R> fa[NPFA];
+#pragma unroll
+    for (int m = 0; m < NPFA; m++) {
+        const int c = tid + m * NTHR;
+        const FaceC fc = face_make(rfa[m], dt);
+        if (c < NFA) s_q[c] = fc.q;
+        const int base = (FAY0 + c / FAW + 2) * HX + (FAX0 + c % FAW + 2);
+        fa[m].set(fc, c < NFA ? base : 0, SD1);
+    }
+    if (ntr > 0) store_tile(0);
+    __syncthreads();
+    // ---- stage A cells: h1 = h0 - (dt (Q1 - Q0)) iA
+    CellK<R> ca[NPA];
+#pragma unroll
+    for (int m = 0; m < NPA; m++) {
+        const int c = tid + m * NTHR;
+        ca[m].upd = 0;
+        if (c < NA) {
+            const int xl = AX0 + c % AW, yl = AY0 + c / AW;
+            const int own = (yl + 2) * HX + (xl + 2);
+            const int f1 = (yl - FAY0) * FAW + (xl - FAX0);
+            ca[m].own = own;
+            ca[m].fl1 = f1;
+            double h1 = 0.0;
+            if (rca[m].inarr) {
+                const bool upd = wet(x0 + xl, y0 + yl, rca[m].tm);
+                h1 = rca[m].h0;
+                if (upd) h1 = rca[m].h0 - (dt * (s_q[f1] - s_q[f1 - (D1 == 1 ? 1 : FAW)])) * rca[m].iA;
+                cell_set<R>(ca[m], upd, rca[m].h0, h1, rca[m].iA, dt);
+            }
+            s_h[own] = h1;
+        }
     }
     __syncthreads();
+    // ---- stage B faces
+    FaceK<S, R> fb[NPFB];
+#pragma unroll
+    for (int m = 0; m < NPFB; m++) {
+        const int c = tid + m * NTHR;
+        const FaceC fc = face_make(rfb[m], dt);
+        if (c < NFB) s_q[c] = fc.q;
+        const int base = (FBY0 + c / FBW + 2) * HX + (FBX0 + c % FBW + 2);
+        fb[m].set(fc, c < NFB ? base : 0, SD2);
+    }
+    __syncthreads();
+    // ---- stage B cells: h2 = h1 - (dt (Q1 - Q0)) iA, written as the pair's thickness
+    CellK<R> cb[NPB];
+#pragma unroll
+    for (int m = 0; m < NPB; m++) {
+        const int c = tid + m * NTHR;
+        const int xl = c % TX, yl = c / TX;
+        const int own = (yl + 2) * HX + (xl + 2);
+        const int f1 = (yl - FBY0) * FBW + (xl - FBX0);
+        cb[m].upd = 0;
+        cb[m].own = own;
+        cb[m].fl1 = f1;
+        if (rcb[m].inarr) {
+            const double h1 = s_h[own];
+            const bool upd = wet(x0 + xl, y0 + yl, rcb[m].tm);
+            double h2 = h1;
+            if (upd) h2 = h1 - (dt * (s_q[f1] - s_q[f1 - (D2 == 1 ? 1 : FBW)])) * rcb[m].iA;
+            cell_set<R>(cb[m], upd, h1, h2, rcb[m].iA, dt);
+            hout[lay + (size_t)(y0 + yl) * A.nx_ + x0 + xl] = h2;
+        }
+    }
+
+    // ---- tracer loop: tile t+1 was loaded during iteration t-1 and is stored to
+    // LDS now, tile t+2 is loaded during this iteration (three LDS tile buffers)
+    if (ntr > 1) load_tile(1);
     for (int t = 0; t < ntr; t++) {
-        const int buf = t & 1;
-        if (t + 1 < ntr) load_tile(t + 1);
+        const R* sin = s_in[t % 3];
+        if (t + 1 < ntr) store_tile((t + 1) % 3);
+        if (t + 2 < ntr) load_tile(t + 2);
         // stage A faces
 #pragma unroll
         for (int m = 0; m < NPFA; m++) {
             const int c = tid + m * NTHR;
             if (c < NFA) {
-                const int xs = FAX0 + c % FAW + 2, ys = FAY0 + c / FAW + 2;  // s_in index of the lower cell
-                R v[4];
-#pragma unroll
-                for (int o = 0; o < 4; o++) v[o] = D1 == 1 ? s_in[buf][ys][xs - 1 + o] : s_in[buf][ys - 1 + o][xs];
-                R fl = face_flux<S, R>(fa[m], v[0], v[1], v[2], v[3]);
-                if (A.ah_u && (fa[m].flags & 1)) {
-                    const int gi = x0 + xs - 2, gj = y0 + ys - 2;
-                    fl = fl - ah_flux<D1, R>(A, gi, gj, k, v[1], v[2]);
+                R fl = fa[m].flux(sin);
+                if (A.ah_u && fa[m].open) {
+                    const int xs = FAX0 + c % FAW, ys = FAY0 + c / FAW;
+                    const int base = (ys + 2) * HX + xs + 2;
+                    fl = fl - ah_flux<D1, R>(A, x0 + xs, y0 + ys, k, sin[base], sin[base + SD1]);
                 }
                 s_fa[c] = fl;
             }
@@ -392,16 +461,10 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
         for (int m = 0; m < NPA; m++) {
             const int c = tid + m * NTHR;
             if (c < NA) {
-                const int xl = AX0 + c % AW, yl = AY0 + c / AW;
-                R out = s_in[buf][yl + 2][xl + 2];
-                if (ca[m].upd) {
-                    // faces of this cell in the region-A face array
-                    const int fx = xl - FAX0, fy = yl - FAY0;
-                    const int f1 = fy * FAW + fx;
-                    const int f0 = D1 == 1 ? f1 - 1 : f1 - FAW;
-                    out = cell_update<R>(ca[m], dt, out, s_fa[f0], s_fa[f1]);
-                }
-                s_a[yl + 2][xl + 2] = out;
+                R out = sin[ca[m].own];
+                if (ca[m].upd)
+                    out = cell_update<R>(ca[m], dt, out, s_fa[ca[m].fl1 - (D1 == 1 ? 1 : FAW)], s_fa[ca[m].fl1]);
+                s_a[ca[m].own] = out;
             }
         }
         __syncthreads();
@@ -410,14 +473,11 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
         for (int m = 0; m < NPFB; m++) {
             const int c = tid + m * NTHR;
             if (c < NFB) {
-                const int xs = FBX0 + c % FBW + 2, ys = FBY0 + c / FBW + 2;
-                R v[4];
-#pragma unroll
-                for (int o = 0; o < 4; o++) v[o] = D2 == 1 ? s_a[ys][xs - 1 + o] : s_a[ys - 1 + o][xs];
-                R fl = face_flux<S, R>(fb[m], v[0], v[1], v[2], v[3]);
-                if (A.ah_u && (fb[m].flags & 1)) {
-                    const int gi = x0 + xs - 2, gj = y0 + ys - 2;
-                    fl = fl - ah_flux<D2, R>(A, gi, gj, k, v[1], v[2]);
+                R fl = fb[m].flux(s_a);
+                if (A.ah_u && fb[m].open) {
+                    const int xs = FBX0 + c % FBW, ys = FBY0 + c / FBW;
+                    const int base = (ys + 2) * HX + xs + 2;
+                    fl = fl - ah_flux<D2, R>(A, x0 + xs, y0 + ys, k, s_a[base], s_a[base + SD2]);
                 }
                 s_fb[c] = fl;
             }
@@ -428,18 +488,11 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
 #pragma unroll
         for (int m = 0; m < NPB; m++) {
             const int c = tid + m * NTHR;
-            const int xl = c % TX, yl = c / TX;
-            const int gi = x0 + xl, gj = y0 + yl;
-            R out = s_a[yl + 2][xl + 2];
-            if (cb[m].upd) {
-                const int fx = xl - FBX0, fy = yl - FBY0;
-                const int f1 = fy * FBW + fx;
-                const int f0 = D2 == 1 ? f1 - 1 : f1 - FBW;
-                out = cell_update<R>(cb[m], dt, out, s_fb[f0], s_fb[f1]);
-            }
+            const int gi = x0 + c % TX, gj = y0 + c / TX;
+            R out = s_a[cb[m].own];
+            if (cb[m].upd) out = cell_update<R>(cb[m], dt, out, s_fb[cb[m].fl1 - (D2 == 1 ? 1 : FBW)], s_fb[cb[m].fl1]);
             if (gi < A.nx_ && gj < A.ny_) fo[(size_t)gj * A.nx_ + gi] = out;
         }
-        if (t + 1 < ntr) store_tile(buf ^ 1);
         __syncthreads();
     }
 }
@@ -557,13 +610,11 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
                 R fl[TB];
                 if (k <= nz - 1) {
                     const double wvk = wvp[(size_t)k * n2];
-                    R rcfl;
-                    if constexpr (std::is_same<R, double>::value)
-                        rcfl = ((fabs(wvk) * dt) / fmin2(h0, h1)) * itersm1;
-                    else
-                        rcfl = float(fabs(wvk) * dt * itersm1) * __frcp_rn(float(fmin2(h0, h1)));
+                    const double cfl = ((fabs(wvk) * dt) / fmin2(h0, h1)) * itersm1;
                     const R rw = R(wvk);
                     const bool pos = wvk > 0.0;
+                    LimF<S> lf;
+                    if constexpr (!std::is_same<R, double>::value) lf.set(cfl);
 #pragma unroll
                     for (int b = 0; b < TB; b++) {
                         R fu, fuu, fd;
@@ -576,7 +627,10 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
                             fuu = k < nz - 1 ? f2[b] : fu;
                             fd = f0[b];
                         }
-                        fl[b] = rw * face_value<S, R>(rcfl, fu, fuu, fd);
+                        if constexpr (std::is_same<R, double>::value)
+                            fl[b] = rw * face_value<S, double>(cfl, fu, fuu, fd);
+                        else
+                            fl[b] = rw * lf.value(fu, fuu, fd);
                     }
                 } else {
 #pragma unroll
@@ -680,57 +734,74 @@ __global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, d
             ru_up = ruk;
         }
     }
-    // the next tracer's column is loaded while the current one is solved
-    float vr[NZMAX], vn[NZMAX];
-    if (ntr > 0) {
+    // two tracers per pass: their recurrences interleave and share each factor load
+    for (int t = 0; t < ntr; t += 2) {
+        const bool two = t + 1 < ntr;
+        const float* fi0 = fin + (size_t)t * n3 + p;
+        const float* fi1 = fin + (size_t)(two ? t + 1 : t) * n3 + p;
+        float va[NZMAX], vb[NZMAX];
 #pragma unroll
         for (int k = 0; k < NZMAX; k++)
-            if (k < nz) vr[k] = fin[p + (size_t)k * n2];
-    }
-    for (int t = 0; t < ntr; t++) {
-        float* fo = fout + (size_t)t * n3 + p;
-        if (t + 1 < ntr) {
-            const float* fi = fin + (size_t)(t + 1) * n3 + p;
-#pragma unroll
-            for (int k = 0; k < NZMAX; k++)
-                if (k < nz) vn[k] = fi[(size_t)k * n2];
-        }
+            if (k < nz) {
+                va[k] = fi0[(size_t)k * n2];
+                vb[k] = fi1[(size_t)k * n2];
+            }
         if (nz == 1) {
             // diffusion.F90:224-231: every point, halos and land included
             const double dz = A.hn[p];
-            vr[0] = float((dz * (double)vr[0]) / dz);
+            va[0] = float((dz * (double)va[0]) / dz);
+            vb[0] = float((dz * (double)vb[0]) / dz);
         } else if (solve) {
-            // forward (top-down) elimination, qu overwrites vr with a window of old values
-            float v_up = 0.f, qu_up = 0.f;
+            // forward (top-down) elimination, qu overwrites the column with a window of old values
+            float a_up = 0.f, qa_up = 0.f, b_up = 0.f, qb_up = 0.f;
 #pragma unroll
             for (int k = NZMAX - 1; k >= 0; k--) {
                 if (k < nz) {
-                    const float vk = vr[k];
-                    float q = g1[k * 64 + lane] * vk;
-                    if (k < nz - 1) q += g2[k * 64 + lane] * qu_up;
+                    const float c1 = g1[k * 64 + lane];
+                    const float c2 = k < nz - 1 ? g2[k * 64 + lane] : 0.f;
+                    const float ak = va[k], bk = vb[k];
+                    float qa = c1 * ak + c2 * qa_up;
+                    float qb = c1 * bk + c2 * qb_up;
                     if (!CN1) {
-                        if (k < nz - 1) q += g3[k * 64 + lane] * (v_up - vk);
-                        if (k > 0) q += g4[k * 64 + lane] * (vr[k - 1 >= 0 ? k - 1 : 0] - vk);
+                        if (k < nz - 1) {
+                            const float c3 = g3[k * 64 + lane];
+                            qa += c3 * (a_up - ak);
+                            qb += c3 * (b_up - bk);
+                        }
+                        if (k > 0) {
+                            const float c4 = g4[k * 64 + lane];
+                            qa += c4 * (va[k - 1 >= 0 ? k - 1 : 0] - ak);
+                            qb += c4 * (vb[k - 1 >= 0 ? k - 1 : 0] - bk);
+                        }
                     }
-                    vr[k] = q;
-                    qu_up = q;
-                    v_up = vk;
+                    va[k] = qa;
+                    vb[k] = qb;
+                    qa_up = qa;
+                    qb_up = qb;
+                    a_up = ak;
+                    b_up = bk;
                 }
             }
-            float prev = vr[0];
+            float pa = va[0], pb = vb[0];
 #pragma unroll
             for (int k = 1; k < NZMAX; k++) {
                 if (k < nz) {
-                    prev = vr[k] - ru[k * 64 + lane] * prev;
-                    vr[k] = prev;
+                    const float r = ru[k * 64 + lane];
+                    pa = va[k] - r * pa;
+                    pb = vb[k] - r * pb;
+                    va[k] = pa;
+                    vb[k] = pb;
                 }
             }
         }
+        float* fo0 = fout + (size_t)t * n3 + p;
+        float* fo1 = fout + (size_t)(t + 1) * n3 + p;
 #pragma unroll
         for (int k = 0; k < NZMAX; k++)
-            if (k < nz) fo[(size_t)k * n2] = vr[k];
-#pragma unroll
-        for (int k = 0; k < NZMAX; k++) vr[k] = vn[k];
+            if (k < nz) {
+                fo0[(size_t)k * n2] = va[k];
+                if (two) fo1[(size_t)k * n2] = vb[k];
+            }
     }
 }
 
