@@ -45,6 +45,8 @@ _SIGNATURES = {
     "transport_step": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp]),
     "transport_get_stats": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _i, _vp]),
     "transport_set_tuning": (_i, [_vp, _i, _i, _i]),
+    "transport_inventory": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "transport_conserve": (_i, [_vp, _i, _vp, _vp, _vp]),
     "getm_profile_enable": (_i, [_i]),
     "getm_profile_read": (_i, [_i, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_long), _i]),
 }
@@ -59,6 +61,8 @@ KERNELS = {
     "hpair_uv": 5,
     "wcol": 6,
     "vdiff_f32": 7,
+    "inventory": 8,
+    "fix": 9,
 }
 
 

@@ -290,11 +290,43 @@ void transport_finalize(void* p)
 {
     Transport* tr = static_cast<Transport*>(p);
     if (!tr) return;
-    for (DeviceBuffer* b : {&tr->scratch, &tr->hA, &tr->hB, &tr->stats, &tr->auxo, &tr->auxn, &tr->ru}) b->release();
+    for (DeviceBuffer* b : {&tr->scratch, &tr->hA, &tr->hB, &tr->stats, &tr->auxo, &tr->auxn, &tr->ru, &tr->part, &tr->part2, &tr->ratio, &tr->resid})
+        b->release();
     delete tr;
 }
 
 }  // extern "C"
+
+static StepArgs step_args(const Transport* tr, const double* u, const double* v, const double* Ah_u, const double* Ah_v)
+{
+    const Grid* t = tr->tgrid;
+    StepArgs A;
+    A.nx_ = t->nx_;
+    A.ny_ = t->ny_;
+    A.nz = t->nz;
+    A.imin = t->imin;
+    A.imax = t->imax;
+    A.jmin = t->jmin;
+    A.jmax = t->jmax;
+    A.n2 = t->n2();
+    A.n3 = t->n3();
+    A.tmask = t->mask;
+    A.umask = tr->ugrid->mask;
+    A.vmask = tr->vgrid->mask;
+    A.idxu = tr->ugrid->idx;
+    A.dyu = tr->ugrid->dy;
+    A.dxv = tr->vgrid->dx;
+    A.idyv = tr->vgrid->idy;
+    A.iarea = t->iarea;
+    A.u = u;
+    A.v = v;
+    A.hu = tr->ugrid->hn;
+    A.hv = tr->vgrid->hn;
+    A.hn = t->hn;
+    A.ah_u = Ah_u;
+    A.ah_v = Ah_v;
+    return A;
+}
 
 template <typename R>
 static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, const double* w, const double* nuh,
@@ -358,36 +390,56 @@ int transport_step(void* p, int ntracers, void* tracers, const double* u, const 
         if (h_out) GETM_HIP_OK(hipMemcpyAsync(h_out, t->ho, t->n3() * sizeof(double), hipMemcpyDeviceToDevice, s));
         return GETM_OK;
     }
-    StepArgs A;
-    A.nx_ = t->nx_;
-    A.ny_ = t->ny_;
-    A.nz = t->nz;
-    A.imin = t->imin;
-    A.imax = t->imax;
-    A.jmin = t->jmin;
-    A.jmax = t->jmax;
-    A.n2 = t->n2();
-    A.n3 = t->n3();
-    A.tmask = t->mask;
-    A.umask = tr->ugrid->mask;
-    A.vmask = tr->vgrid->mask;
-    A.idxu = tr->ugrid->idx;
-    A.dyu = tr->ugrid->dy;
-    A.dxv = tr->vgrid->dx;
-    A.idyv = tr->vgrid->idy;
-    A.iarea = t->iarea;
-    A.u = u;
-    A.v = v;
-    A.hu = tr->ugrid->hn;
-    A.hv = tr->vgrid->hn;
-    A.hn = t->hn;
-    A.ah_u = Ah_u;
-    A.ah_v = Ah_v;
+    StepArgs A = step_args(tr, u, v, Ah_u, Ah_v);
     if (tr->precision == GETM_F32)
         return transport_step_t<float>(tr, A, ntracers, static_cast<float*>(tracers), w, nuh, timestep, molecular,
                                        h_out, s);
     return transport_step_t<double>(tr, A, ntracers, static_cast<double*>(tracers), w, nuh, timestep, molecular, h_out,
                                     s);
+}
+
+int transport_inventory(void* p, int ntracers, const void* tracers, const double* h, double* inventory, void* stream)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr || ntracers < 1 || !tracers || !inventory) return GETM_ERR_ARG;
+    const Grid* t = tr->tgrid;
+    if (!t->area || !t->mask || !(h || t->hn)) return GETM_ERR_ARG;
+    if (ntracers > 65535) return GETM_ERR_CAPACITY;
+    const StepArgs A = step_args(tr, nullptr, nullptr, nullptr, nullptr);
+    int rc;
+    if ((rc = tr->part.ensure(inventory_part_doubles(A, ntracers) * sizeof(double)))) return rc;
+    double* part = static_cast<double*>(tr->part.ptr);
+    const double* hh = h ? h : t->hn;
+    hipStream_t s = as_stream(stream);
+    if (tr->precision == GETM_F32)
+        return launch_inventory<float>(A, hh, t->area, ntracers, static_cast<const float*>(tracers), part, inventory,
+                                       nullptr, nullptr, s);
+    return launch_inventory<double>(A, hh, t->area, ntracers, static_cast<const double*>(tracers), part, inventory,
+                                    nullptr, nullptr, s);
+}
+
+int transport_conserve(void* p, int ntracers, void* tracers, const double* target, void* stream)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr || ntracers < 1 || !tracers || !target) return GETM_ERR_ARG;
+    if (tr->precision != GETM_F32) return GETM_OK;
+    const Grid* t = tr->tgrid;
+    if (!t->area || !t->mask || !t->hn) return GETM_ERR_ARG;
+    if (ntracers > 65535) return GETM_ERR_CAPACITY;
+    const StepArgs A = step_args(tr, nullptr, nullptr, nullptr, nullptr);
+    int rc;
+    const size_t np = inventory_part_doubles(A, ntracers);
+    if ((rc = tr->part.ensure(np * sizeof(double))) || (rc = tr->part2.ensure(np * sizeof(double))) ||
+        (rc = tr->ratio.ensure((size_t)ntracers * sizeof(double))) ||
+        (rc = tr->resid.ensure((size_t)ntracers * sizeof(double))))
+        return rc;
+    double* part = static_cast<double*>(tr->part.ptr);
+    double* ratio = static_cast<double*>(tr->ratio.ptr);
+    hipStream_t s = as_stream(stream);
+    float* f = static_cast<float*>(tracers);
+    if ((rc = launch_inventory<float>(A, t->hn, t->area, ntracers, f, part, nullptr, target, ratio, s))) return rc;
+    return launch_fix_f32(A, ntracers, f, part, ratio, t->area, static_cast<double*>(tr->part2.ptr), target,
+                          static_cast<double*>(tr->resid.ptr), s);
 }
 
 int transport_set_tuning(void* p, int tracers_per_block_h, int tracers_per_block_w, int tracers_per_block_d)

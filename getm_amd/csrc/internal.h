@@ -100,6 +100,7 @@ struct Transport {
     DeviceBuffer stats;    // int32[2] as in Advection
     DeviceBuffer auxo, auxn, ru;  // FP64 diffusion (precision GETM_F64)
     int tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tracers per block of the three kernel kinds (0 = all)
+    DeviceBuffer part, part2, ratio, resid;  // inventory partial sums, fixer ratios and residuals
 };
 
 // ---- batched kernels (transport_batched.hip) ----
@@ -112,6 +113,14 @@ int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, cons
                 double* hout, const double* w, const double* wv, int* stats, hipStream_t s);
 int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
                      const float* fin, float* fout, hipStream_t s);
+
+// ---- inventories (inventory.hip) ----
+template <typename R>
+int launch_inventory(const StepArgs& A, const double* h, const double* area, int ntr, const R* f, double* part,
+                     double* inv, const double* target, double* ratio, hipStream_t s);
+int launch_fix_f32(const StepArgs& A, int ntr, float* f, const double* part, const double* ratio, const double* area,
+                   double* part2, const double* target, double* resid, hipStream_t s);
+size_t inventory_part_doubles(const StepArgs& A, int ntr);
 
 // ---- FP64 kernels (transport_fp64.hip) ----
 int launch_adv_uv_f64(int direction, int scheme, int nk, const Grid& t, const Grid& uv,

@@ -24,7 +24,7 @@ _DTYPES = {"f64": (torch.float64, 1), "f32": (torch.float32, 2)}
 
 class TracerCollection:
     def __init__(self, grid: Grid, advection_scheme=operators.AdvectionScheme.DEFAULT, cnpar: float = 1.0,
-                 precision: str = "f64", batched: bool = True, ufirst: bool = False):
+                 precision: str = "f64", batched: bool = True, ufirst: bool = False, conserve: bool = False):
         if precision not in _DTYPES:
             raise ValueError(f"precision must be one of {list(_DTYPES)}")
         if precision == "f32" and not batched:
@@ -40,6 +40,9 @@ class TracerCollection:
         self.h = torch.empty((grid.nz_, grid.ny_, grid.nx_), dtype=torch.float64, device=grid.device)
         self.Ah_u: Optional[Array] = None
         self.Ah_v: Optional[Array] = None
+        # FP32 only: restore every tracer's inventory to its value before the step
+        self.conserve = bool(conserve) and precision == "f32"
+        self._target: Optional[torch.Tensor] = None
         lib = _lib.load()
         if batched:
             self.p = lib.transport_create(grid.p, grid.ugrid.p, grid.vgrid.p, int(advection_scheme), self.cnpar, code,
@@ -103,6 +106,9 @@ class TracerCollection:
         if len(mols) > 1:
             raise ValueError("the batched step needs one molecular diffusivity for all tracers")
         mol = mols.pop() if mols else 0.0
+        if self.conserve and len(self):
+            # closed domain without sources: sum(area ho c) before = sum(area hn c) after
+            self._target = self.inventory_device(self.grid.ho.all_values)
         rc = _lib.load().transport_step(
             self.p, len(self), _lib.ptr(self.values), _lib.ptr(u.all_values), _lib.ptr(v.all_values),
             _lib.ptr(w.all_values), _lib.ptr(diffusivity.all_values),
@@ -110,6 +116,24 @@ class TracerCollection:
             _lib.ptr(self.Ah_v.all_values) if self.Ah_v is not None else None, float(timestep), mol,
             _lib.ptr(self.h), _lib.stream_handle())
         _lib.check(rc, "transport_step")
+        if self.conserve and len(self):
+            _lib.check(_lib.load().transport_conserve(self.p, len(self), _lib.ptr(self.values), _lib.ptr(self._target),
+                                                      _lib.stream_handle()), "transport_conserve")
+
+    def inventory_device(self, h: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Per-tracer sum of area * h * c over wet cells (FP64, compensated,
+        deterministic) as a device tensor; h defaults to the T grid's hn."""
+        if not self.batched:
+            raise ValueError("inventories are computed by the batched transport object")
+        out = torch.empty(len(self), dtype=torch.float64, device=self.grid.device)
+        _lib.check(_lib.load().transport_inventory(self.p, len(self), _lib.ptr(self.values),
+                                                   _lib.ptr(h) if h is not None else None, _lib.ptr(out),
+                                                   _lib.stream_handle()), "transport_inventory")
+        return out
+
+    def inventory(self, h: Optional[torch.Tensor] = None):
+        """Host copy of inventory_device (simulation.py:1408-1466 report these)."""
+        return self.inventory_device(h).cpu().numpy()
 
     def set_tuning(self, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
                    tracers_per_block_d: int = 16) -> None:

@@ -156,6 +156,22 @@ int transport_step(void* tr, int ntracers, void* tracers, const double* u, const
  * every block.  Results do not depend on these values. */
 int transport_set_tuning(void* tr, int tracers_per_block_h, int tracers_per_block_w, int tracers_per_block_d);
 
+/* Tracer inventories (the reference's domain integrals of tracers,
+ * simulation.py:1408-1466): inventory[t] = sum over wet cells of
+ * area * h * c for each of ntracers stacked tracers, accumulated in FP64
+ * with compensation in a fixed order (deterministic).  h = NULL uses the T
+ * grid's "hn"; the T grid must have "area" registered.  inventory is a device
+ * array of ntracers doubles. */
+int transport_inventory(void* tr, int ntracers, const void* tracers, const double* h, double* inventory,
+                        void* stream);
+
+/* No reference counterpart (the FP64 reference conserves by construction):
+ * for FP32 tracers, restores each tracer's inventory (with "hn") to target[t]
+ * (device, ntracers doubles) by changing a few cells by one ulp, chosen by
+ * rounding a mass-proportional prefix sum over a fixed cell order.  Leaves the
+ * inventory within one cell's rounding of the target.  No-op for FP64. */
+int transport_conserve(void* tr, int ntracers, void* tracers, const double* target, void* stream);
+
 /* z-sweep sub-stepping statistics of a transport object (as advection_get_stats). */
 int transport_get_stats(void* tr, int* max_iters, int* capped_columns, int reset, void* stream);
 
@@ -171,6 +187,8 @@ enum {
     GETM_K_HPAIR_UV = 5,  /* batched stages u dt/2 + v dt/2 (or v + u) */
     GETM_K_WCOL = 6,      /* batched w stage */
     GETM_K_VDIFF_F32 = 7, /* batched FP32 vertical diffusion */
+    GETM_K_INVENTORY = 8, /* tracer inventories */
+    GETM_K_FIX = 9,       /* FP32 inventory fixer */
     GETM_NUM_KERNELS = 16
 };
 
