@@ -130,7 +130,7 @@ def main():
     ap.add_argument("--tracers", type=int, default=0, help="override the workload's tracer count")
     ap.add_argument("--scheme", type=int, default=0, help="override the workload's scheme")
     ap.add_argument("--per-tracer", action="store_true", help="FP64 per-tracer operator sequence (reference order)")
-    ap.add_argument("--tuning", default="", help="tracers per block h,w,d of the batched kernels (0 = all)")
+    ap.add_argument("--tuning", default="", help="tracers per block h,w,d of the batched kernels (0 = all)[,tile height]")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")

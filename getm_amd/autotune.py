@@ -1,0 +1,99 @@
+"""Launch-shape autotuner for the batched step and ensembles.
+
+Searches the height of the horizontal-stage tiles (4, 8 or 16 rows of 64
+columns) and the tracers per block of the three kernel kinds, timing each
+candidate with HIP events on the current stream, and caches the best shape per
+workload signature (grid size bucket, nz, tracers) in a JSON file.  Launch shapes
+never change results (see tests), so tuning is safe at any time.
+
+The column solver is always the per-lane Thomas elimination (the reference's
+algorithm, diffusion.F90:276-313); a wave-parallel cyclic-reduction variant is
+not implemented, so there is no solver axis to search.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import math
+import os
+from typing import Callable, Dict, Iterable, Optional, Tuple
+
+import torch
+
+Shape = Tuple[int, int, int, int]  # (tile_y, tracers/block h, w, d)
+
+DEFAULT_CANDIDATES = [
+    (ty, h, w, d)
+    for ty, h, (w, d) in itertools.product((16, 8, 4), (0, 32), ((16, 16), (64, 64)))
+]
+
+
+def signature(ncells: int, nz: int, ntracers: int) -> str:
+    """Bucketed workload key: log2 of the column count, nz rounded up to 8, log2 of T."""
+    cols = max(1, ncells // max(nz, 1))
+    return f"c{int(math.log2(cols))}-z{8 * math.ceil(nz / 8)}-t{int(math.log2(max(ntracers, 1)))}"
+
+
+def time_step(step: Callable[[], None], reps: int = 3) -> float:
+    """Mean milliseconds of ``step`` over ``reps`` calls (after one warm-up call)."""
+    step()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        step()
+    stop.record()
+    stop.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+class Autotuner:
+    def __init__(self, cache_path: Optional[str] = None):
+        self.cache_path = cache_path
+        self.cache: Dict[str, Shape] = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as fh:
+                self.cache = {k: tuple(v) for k, v in json.load(fh).items()}
+
+    def _save(self):
+        if self.cache_path:
+            tmp = self.cache_path + ".tmp"
+            with open(tmp, "w") as fh:
+                json.dump(self.cache, fh, indent=1)
+            os.replace(tmp, self.cache_path)
+
+    def tune(self, key: str, apply: Callable[[Shape], None], step: Callable[[], None],
+             candidates: Iterable[Shape] = DEFAULT_CANDIDATES, reps: int = 3, force: bool = False) -> Shape:
+        """Apply the cached shape for ``key`` or time every candidate and keep the fastest.
+        ``apply(shape)`` must make the next ``step()`` use the shape."""
+        if key in self.cache and not force:
+            apply(self.cache[key])
+            return self.cache[key]
+        best, best_ms = None, float("inf")
+        self.timings = {}
+        for cand in candidates:
+            apply(cand)
+            ms = time_step(step, reps)
+            self.timings[cand] = ms
+            if ms < best_ms:
+                best, best_ms = cand, ms
+        apply(best)
+        self.cache[key] = best
+        self._save()
+        return best
+
+    def tune_collection(self, tc, advance: Callable[[], None], **kw) -> Shape:
+        """Tune a batched getm_amd.tracer.TracerCollection; ``advance`` runs one step."""
+        g = tc.grid
+        key = "step-" + tc.precision + "-" + signature(g.nx_ * g.ny_ * g.nz_, g.nz_, len(tc))
+        return self.tune(key, lambda s: tc.set_tuning(s[1], s[2], s[3], tile_y=s[0]), advance, **kw)
+
+    def tune_ensemble(self, ens, **kw) -> Shape:
+        """Tune a getm_amd.ensemble.Ensemble (rebuilds its work tables per candidate)."""
+        ncells = sum(g.nx_ * g.ny_ * g.nz_ for g in ens.grids)
+        key = "ens-" + signature(ncells, max(g.nz_ for g in ens.grids), sum(ens.ntracers)) + f"-d{len(ens)}"
+
+        def apply(s):
+            ens.set_tuning(*s)
+            ens.commit()
+
+        return self.tune(key, apply, ens.step, **kw)

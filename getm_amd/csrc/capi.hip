@@ -5,6 +5,7 @@
 
 #include <new>
 #include <type_traits>
+#include <vector>
 
 #include "internal.h"
 
@@ -345,11 +346,11 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     const double half = 0.5 * dt;
     const int d1 = tr->ufirst ? 1 : 2;  // first direction of stages 1-2 (operators.py:118-121)
     const int d2 = 3 - d1;
-    if ((rc = launch_hpair<R>(d1, tr->scheme, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, s))) return rc;
+    if ((rc = launch_hpair<R>(d1, tr->scheme, tr->tile_y, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, s))) return rc;
     if ((rc = launch_wcol<R>(tr->scheme, A, dt, ntr, tr->tpb_w, S, f, hA, hB, w, w, static_cast<int*>(tr->stats.ptr),
                              s)))
         return rc;
-    if ((rc = launch_hpair<R>(d2, tr->scheme, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, s))) return rc;
+    if ((rc = launch_hpair<R>(d2, tr->scheme, tr->tile_y, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, s))) return rc;
     if constexpr (std::is_same<R, float>::value) {
         return launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
     } else {
@@ -442,10 +443,12 @@ int transport_conserve(void* p, int ntracers, void* tracers, const double* targe
                           static_cast<double*>(tr->resid.ptr), s);
 }
 
-int transport_set_tuning(void* p, int tracers_per_block_h, int tracers_per_block_w, int tracers_per_block_d)
+int transport_set_tuning(void* p, int tile_y, int tracers_per_block_h, int tracers_per_block_w, int tracers_per_block_d)
 {
     Transport* tr = static_cast<Transport*>(p);
     if (!tr || tracers_per_block_h < 0 || tracers_per_block_w < 0 || tracers_per_block_d < 0) return GETM_ERR_ARG;
+    if (tile_y != 4 && tile_y != 8 && tile_y != 16) return GETM_ERR_ARG;
+    tr->tile_y = tile_y;
     tr->tpb_h = tracers_per_block_h;
     tr->tpb_w = tracers_per_block_w;
     tr->tpb_d = tracers_per_block_d;
@@ -464,6 +467,184 @@ int transport_get_stats(void* p, int* max_iters, int* capped_columns, int reset,
     if (capped_columns) *capped_columns = host[1];
     if (reset) GETM_HIP_OK(hipMemsetAsync(tr->stats.ptr, 0, sizeof(host), s));
     return GETM_OK;
+}
+
+
+void* ensemble_create(int ndomains, void* const* tgrids, void* const* ugrids, void* const* vgrids,
+                      const int* ntracers, int scheme, double cnpar, int ufirst)
+{
+    if (ndomains < 1 || !tgrids || !ugrids || !vgrids || !ntracers) return nullptr;
+    if (scheme < GETM_HSIMT || scheme > GETM_UPSTREAM) return nullptr;
+    for (int d = 0; d < ndomains; d++) {
+        const Grid* t = static_cast<const Grid*>(tgrids[d]);
+        if (check_grids(t, static_cast<const Grid*>(ugrids[d])) || check_grids(t, static_cast<const Grid*>(vgrids[d])))
+            return nullptr;
+        if (ntracers[d] < 1 || t->nz > 64) return nullptr;
+    }
+    Ensemble* e = new (std::nothrow) Ensemble();
+    if (!e) return nullptr;
+    e->ndom = ndomains;
+    e->scheme = scheme;
+    e->cnpar = cnpar;
+    e->ufirst = ufirst ? 1 : 0;
+    e->host = new (std::nothrow) EnsDom<float>[ndomains]();
+    e->tg = new (std::nothrow) const Grid*[ndomains];
+    e->ug = new (std::nothrow) const Grid*[ndomains];
+    e->vg = new (std::nothrow) const Grid*[ndomains];
+    if (!e->host || !e->tg || !e->ug || !e->vg || e->stats.ensure(2 * sizeof(int)) != GETM_OK ||
+        hipMemset(e->stats.ptr, 0, 2 * sizeof(int)) != hipSuccess) {
+        ensemble_finalize(e);
+        return nullptr;
+    }
+    for (int d = 0; d < ndomains; d++) {
+        e->tg[d] = static_cast<const Grid*>(tgrids[d]);
+        e->ug[d] = static_cast<const Grid*>(ugrids[d]);
+        e->vg[d] = static_cast<const Grid*>(vgrids[d]);
+        e->host[d].ntr = ntracers[d];
+    }
+    return e;
+}
+
+void ensemble_finalize(void* p)
+{
+    Ensemble* e = static_cast<Ensemble*>(p);
+    if (!e) return;
+    for (DeviceBuffer* b : {&e->doms, &e->work_h, &e->work_w, &e->work_d, &e->scratch, &e->hbuf, &e->stats}) b->release();
+    delete[] e->host;
+    delete[] e->tg;
+    delete[] e->ug;
+    delete[] e->vg;
+    delete e;
+}
+
+int ensemble_bind(void* p, int d, float* tracers, const double* u, const double* v, const double* w, const double* nuh,
+                  double timestep, double molecular)
+{
+    Ensemble* e = static_cast<Ensemble*>(p);
+    if (!e || d < 0 || d >= e->ndom || !tracers || !u || !v || !w || !nuh) return GETM_ERR_ARG;
+    const Grid* t = e->tg[d];
+    if (!t->ho || !t->hn || !t->mask || !t->iarea || !e->ug[d]->hn || !e->vg[d]->hn || !e->ug[d]->mask ||
+        !e->vg[d]->mask || !e->ug[d]->idx || !e->ug[d]->dy || !e->vg[d]->dx || !e->vg[d]->idy)
+        return GETM_ERR_ARG;
+    Transport view;  // only to reuse step_args
+    view.tgrid = t;
+    view.ugrid = e->ug[d];
+    view.vgrid = e->vg[d];
+    EnsDom<float>& D = e->host[d];
+    D.A = step_args(&view, u, v, nullptr, nullptr);
+    D.tracers = tracers;
+    D.ho = t->ho;
+    D.w = w;
+    D.nuh = nuh;
+    D.dt = timestep;
+    D.cnpar = e->cnpar;
+    D.molecular = molecular;
+    e->committed = 0;
+    return GETM_OK;
+}
+
+int ensemble_set_tuning(void* p, int tile_y, int tpb_h, int tpb_w, int tpb_d)
+{
+    Ensemble* e = static_cast<Ensemble*>(p);
+    if (!e || tpb_h < 0 || tpb_w < 0 || tpb_d < 0 || (tile_y != 4 && tile_y != 8 && tile_y != 16)) return GETM_ERR_ARG;
+    e->tile_y = tile_y;
+    e->tpb_h = tpb_h;
+    e->tpb_w = tpb_w;
+    e->tpb_d = tpb_d;
+    e->committed = 0;
+    return GETM_OK;
+}
+
+static int upload_work(DeviceBuffer& buf, const std::vector<int4>& w)
+{
+    if (w.empty()) return GETM_OK;
+    int rc = buf.ensure(w.size() * sizeof(int4));
+    if (rc) return rc;
+    GETM_HIP_OK(hipMemcpy(buf.ptr, w.data(), w.size() * sizeof(int4), hipMemcpyHostToDevice));
+    return GETM_OK;
+}
+
+int ensemble_commit(void* p, void* stream)
+{
+    Ensemble* e = static_cast<Ensemble*>(p);
+    if (!e) return GETM_ERR_ARG;
+    (void)stream;
+    size_t nscr = 0, nh = 0;
+    for (int d = 0; d < e->ndom; d++) {
+        if (!e->host[d].tracers) return GETM_ERR_ARG;  // not bound
+        nscr += e->tg[d]->n3() * (size_t)e->host[d].ntr;
+        nh += 2 * e->tg[d]->n3();
+    }
+    int rc;
+    if ((rc = e->scratch.ensure(nscr * sizeof(float))) || (rc = e->hbuf.ensure(nh * sizeof(double)))) return rc;
+    float* scr = static_cast<float*>(e->scratch.ptr);
+    double* hb = static_cast<double*>(e->hbuf.ptr);
+    std::vector<int4> wh, ww, wd;
+    e->nzmax = 1;
+    for (int d = 0; d < e->ndom; d++) {
+        EnsDom<float>& D = e->host[d];
+        const Grid* t = e->tg[d];
+        D.scratch = scr;
+        scr += t->n3() * (size_t)D.ntr;
+        D.hA = hb;
+        D.hB = hb + t->n3();
+        hb += 2 * t->n3();
+        D.stats = static_cast<int*>(e->stats.ptr);
+        D.tpb_h = (e->tpb_h <= 0 || e->tpb_h > D.ntr) ? D.ntr : e->tpb_h;
+        D.tpb_w = (e->tpb_w <= 0 || e->tpb_w > D.ntr) ? D.ntr : e->tpb_w;
+        D.tpb_d = (e->tpb_d <= 0 || e->tpb_d > D.ntr) ? D.ntr : e->tpb_d;
+        if (t->nz > e->nzmax) e->nzmax = t->nz;
+        const int nbh = (D.ntr + D.tpb_h - 1) / D.tpb_h;
+        const int nbw = (D.ntr + D.tpb_w - 1) / D.tpb_w;
+        const int nbd = (D.ntr + D.tpb_d - 1) / D.tpb_d;
+        if (nbh > 32767 || t->nz > 65535) return GETM_ERR_CAPACITY;
+        for (int b = 0; b < nbh; b++)
+            for (int k = 0; k < t->nz; k++)
+                for (int y0 = 0; y0 < t->ny_; y0 += e->tile_y)
+                    for (int x0 = 0; x0 < t->nx_; x0 += 64) wh.push_back(make_int4(d, x0, y0, k | (b << 16)));
+        for (int b = 0; b < nbw; b++)
+            for (int by = 0; by < (t->ny_ + 3) / 4; by++)
+                for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) ww.push_back(make_int4(d, bx, by, b));
+        for (int b = 0; b < nbd; b++)
+            for (int j = 0; j < t->ny_; j++)
+                for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) wd.push_back(make_int4(d, bx, j, b));
+    }
+    if (wh.size() > 0x7fffffff || ww.size() > 0x7fffffff || wd.size() > 0x7fffffff) return GETM_ERR_CAPACITY;
+    if ((rc = upload_work(e->work_h, wh)) || (rc = upload_work(e->work_w, ww)) || (rc = upload_work(e->work_d, wd)))
+        return rc;
+    e->nwork_h = (int)wh.size();
+    e->nwork_w = (int)ww.size();
+    e->nwork_d = (int)wd.size();
+    if ((rc = e->doms.ensure((size_t)e->ndom * sizeof(EnsDom<float>)))) return rc;
+    GETM_HIP_OK(hipMemcpy(e->doms.ptr, e->host, (size_t)e->ndom * sizeof(EnsDom<float>), hipMemcpyHostToDevice));
+    e->committed = 1;
+    return GETM_OK;
+}
+
+int ensemble_step(void* p, void* stream)
+{
+    Ensemble* e = static_cast<Ensemble*>(p);
+    if (!e) return GETM_ERR_ARG;
+    if (!e->committed) return GETM_ERR_NOT_PREPARED;
+    hipStream_t s = as_stream(stream);
+    const EnsDom<float>* doms = static_cast<const EnsDom<float>*>(e->doms.ptr);
+    const int d1 = e->ufirst ? 1 : 2;
+    int rc;
+    if ((rc = launch_hpair_multi(1, d1, e->scheme, e->tile_y, doms, static_cast<const int4*>(e->work_h.ptr), e->nwork_h,
+                                 s)))
+        return rc;
+    if ((rc = launch_wcol_multi(e->scheme, doms, static_cast<const int4*>(e->work_w.ptr), e->nwork_w, s))) return rc;
+    if ((rc = launch_hpair_multi(2, 3 - d1, e->scheme, e->tile_y, doms, static_cast<const int4*>(e->work_h.ptr),
+                                 e->nwork_h, s)))
+        return rc;
+    return launch_vdiff_multi(e->cnpar == 1.0, e->nzmax, doms, static_cast<const int4*>(e->work_d.ptr), e->nwork_d, s);
+}
+
+const double* ensemble_thickness(void* p, int d)
+{
+    Ensemble* e = static_cast<Ensemble*>(p);
+    if (!e || d < 0 || d >= e->ndom || !e->committed) return nullptr;
+    return e->host[d].hA;
 }
 
 }  // extern "C"

@@ -88,6 +88,22 @@ struct StepArgs {
     const double* ah_v;
 };
 
+// One domain of an ensemble as the ensemble kernels read it (device table).
+template <typename R>
+struct EnsDom {
+    StepArgs A;
+    R* tracers;      // ntr x nz x ny_ x nx_
+    R* scratch;      // same size, stage ping-pong
+    const double* ho;
+    double* hA;      // thickness after stages 1-2, and the final thickness
+    double* hB;      // thickness after stage 3
+    const double* w;
+    const double* nuh;
+    double dt, cnpar, molecular;
+    int ntr, tpb_h, tpb_w, tpb_d;
+    int* stats;
+};
+
 // Batched T-tracer step object (transport_create).
 struct Transport {
     int scheme, precision, ufirst;
@@ -100,14 +116,34 @@ struct Transport {
     DeviceBuffer stats;    // int32[2] as in Advection
     DeviceBuffer auxo, auxn, ru;  // FP64 diffusion (precision GETM_F64)
     int tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tracers per block of the three kernel kinds (0 = all)
+    int tile_y = 16;                        // height of the horizontal-stage tiles (4, 8 or 16)
     DeviceBuffer part, part2, ratio, resid;  // inventory partial sums, fixer ratios and residuals
+};
+
+// Ensemble of independent domains stepped by one launch per kernel kind
+// (ensemble_create); FP32 tracers with FP64 thickness.
+struct Ensemble {
+    int ndom = 0, scheme = 5, ufirst = 0;
+    double cnpar = 1.0;
+    int tile_y = 16, tpb_h = 0, tpb_w = 16, tpb_d = 16;
+    int nwork_h = 0, nwork_w = 0, nwork_d = 0, nzmax = 1;
+    int committed = 0;
+    EnsDom<float>* host = nullptr;  // host copy of the domain table
+    const Grid** tg = nullptr;
+    const Grid** ug = nullptr;
+    const Grid** vg = nullptr;
+    DeviceBuffer doms, work_h, work_w, work_d, scratch, hbuf, stats;
 };
 
 // ---- batched kernels (transport_batched.hip) ----
 // tpb = tracers per block (tracer batches go into the grid); <= 0 means all
 template <typename R>
-int launch_hpair(int d1, int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
+int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
                  const double* hin, double* hout, hipStream_t s);
+int launch_hpair_multi(int stage, int d1, int scheme, int ty, const EnsDom<float>* doms, const int4* work, int nwork,
+                       hipStream_t s);
+int launch_wcol_multi(int scheme, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
+int launch_vdiff_multi(bool cn1, int nzmax, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
 template <typename R>
 int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout, const double* hin,
                 double* hout, const double* w, const double* wv, int* stats, hipStream_t s);

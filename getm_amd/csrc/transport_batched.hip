@@ -41,10 +41,8 @@ namespace getm {
 // Faces outside the reference's face loops (template:61-62, 161-162) are never
 // read by an updated cell and are left closed.
 // ---------------------------------------------------------------------------
-constexpr int TX = 64, TY = 16;          // output tile
-constexpr int HX = TX + 4, HY = TY + 4;  // input tile incl. halo 2
-constexpr int NT = HX * HY;              // cells of the input tile
-constexpr int NTHR = 1024;
+constexpr int TX = 64;      // output tile width (one wave); the height TY is a template parameter
+constexpr int HX = TX + 4;  // input tile width incl. halo 2
 
 struct FaceC {
     double q;     // transport Q = vel*h*metric (FP64, also drives the h update)
@@ -266,18 +264,14 @@ __device__ __forceinline__ R cell_update(const CellK<R>& c, double dt, R f, R fl
 // Tile-local coordinates: output [0,TX) x [0,TY); input [-2,TX+2) x [-2,TY+2)
 // stored at LDS offset (y+2)*HX + (x+2).  Region A (first stage) cells:
 // D1 = y -> x in [-2,TX+2), y in [0,TY); D1 = x -> x in [0,TX), y in [-2,TY+2).
-template <int S, typename R, int D1>
-__global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb,
-                                                const R* __restrict__ fin_all, R* __restrict__ fout_all,
-                                                const double* __restrict__ hin, double* __restrict__ hout)
+template <int S, typename R, int D1, int TY>
+__device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0, int y0, int k, int ntr,
+                                           const R* __restrict__ fin, R* __restrict__ fout,
+                                           const double* __restrict__ hin, double* __restrict__ hout)
 {
-    // blockIdx.z = layer + nz * tracer batch; every batch block writes the same thickness
-    const int k = blockIdx.z % A.nz, tbatch = blockIdx.z / A.nz;
-    const int tb0 = tbatch * tpb;
-    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
-    const R* fin = fin_all + (size_t)tb0 * A.n3;
-    R* fout = fout_all + (size_t)tb0 * A.n3;
-
+    constexpr int HY = TY + 4;       // input tile height incl. halo 2
+    constexpr int NT = HX * HY;      // cells of the input tile
+    constexpr int NTHR = TX * TY;    // one lane per output cell
     constexpr int D2 = 3 - D1;
     constexpr int SD1 = D1 == 1 ? 1 : HX, SD2 = D2 == 1 ? 1 : HX;  // LDS strides along the sweeps
     // region A cells
@@ -303,7 +297,6 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
     __shared__ double s_h[NT];                     // FP64 thickness after stage A
 
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const size_t lay = (size_t)k * A.n2;
     auto inarr = [&](int gi, int gj) { return gi >= 0 && gi < A.nx_ && gj >= 0 && gj < A.ny_; };
     auto cell_load = [&](int gi, int gj, CellRaw& r) {
@@ -497,6 +490,36 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
     }
 }
 
+// Single domain: blockIdx = (x tile, y tile, layer + nz * tracer batch); every
+// tracer-batch block writes the same thickness.
+template <int S, typename R, int D1, int TY>
+__global__ void __launch_bounds__(TX * TY) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb,
+                                                   const R* __restrict__ fin_all, R* __restrict__ fout_all,
+                                                   const double* __restrict__ hin, double* __restrict__ hout)
+{
+    const int k = blockIdx.z % A.nz, tbatch = blockIdx.z / A.nz;
+    const int tb0 = tbatch * tpb;
+    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
+    hpair_body<S, R, D1, TY>(A, dt, blockIdx.x * TX, blockIdx.y * TY, k, ntr, fin_all + (size_t)tb0 * A.n3,
+                             fout_all + (size_t)tb0 * A.n3, hin, hout);
+}
+
+// Ensemble: one work item per block {domain, x0, y0, layer | batch << 16}.
+// STAGE 1: stages 1-2 (tracers -> scratch, ho -> hA); STAGE 2: stages 4-5
+// (tracers -> scratch, hB -> hA).
+template <int S, typename R, int D1, int TY, int STAGE>
+__global__ void __launch_bounds__(TX * TY) k_hpair_multi(const EnsDom<R>* __restrict__ doms,
+                                                         const int4* __restrict__ work)
+{
+    const int4 wk = work[blockIdx.x];
+    const EnsDom<R>& D = doms[wk.x];
+    const int k = wk.w & 0xffff, tb0 = (wk.w >> 16) * D.tpb_h;
+    const int ntr = D.ntr - tb0 < D.tpb_h ? D.ntr - tb0 : D.tpb_h;
+    const size_t off = (size_t)tb0 * D.A.n3;
+    hpair_body<S, R, D1, TY>(D.A, 0.5 * D.dt, wk.y, wk.z, k, ntr, D.tracers + off, D.scratch + off,
+                             STAGE == 1 ? D.ho : D.hB, D.hA);
+}
+
 // ---------------------------------------------------------------------------
 // Vertical stage, one lane per column, TB tracers per sweep.
 // advection.F90.template:223-374.  The sub-step count and every restart depend
@@ -509,17 +532,13 @@ __global__ void __launch_bounds__(NTHR) k_hpair(const StepArgs A, double dt, int
 // scratch; batch 0 writes the final thickness.
 // ---------------------------------------------------------------------------
 template <int S, typename R, int TB>
-__global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int ntr, int tpb, const R* __restrict__ fin,
-                                              R* __restrict__ fout, const double* __restrict__ hin,
-                                              double* __restrict__ hout, const double* __restrict__ w,
-                                              const double* __restrict__ wv, int* __restrict__ stats)
+__device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, int j, int tb0, int tb1, bool first,
+                                          const R* __restrict__ fin, R* __restrict__ fout,
+                                          const double* __restrict__ hin, double* __restrict__ hout,
+                                          const double* __restrict__ w, const double* __restrict__ wv,
+                                          int* __restrict__ stats)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= A.nx_ || j >= A.ny_) return;
-    const int tb0 = blockIdx.z * tpb;
-    const int tb1 = tb0 + tpb < ntr ? tb0 + tpb : ntr;
-    const bool first = blockIdx.z == 0;
     const size_t n2 = A.n2, n3 = A.n3;
     const int nz = A.nz;
     const size_t p = (size_t)j * A.nx_ + i;
@@ -666,6 +685,31 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
     }
 }
 
+// Single domain: blockIdx.z = tracer batch.
+template <int S, typename R, int TB>
+__global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int ntr, int tpb, const R* __restrict__ fin,
+                                              R* __restrict__ fout, const double* __restrict__ hin,
+                                              double* __restrict__ hout, const double* __restrict__ w,
+                                              const double* __restrict__ wv, int* __restrict__ stats)
+{
+    const int tb0 = blockIdx.z * tpb;
+    const int tb1 = tb0 + tpb < ntr ? tb0 + tpb : ntr;
+    wcol_body<S, R, TB>(A, dt, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y, tb0, tb1,
+                        blockIdx.z == 0, fin, fout, hin, hout, w, wv, stats);
+}
+
+// Ensemble: work item {domain, bx, by, batch}; scratch -> tracers, hA -> hB.
+template <int S, typename R, int TB>
+__global__ void __launch_bounds__(256) k_wcol_multi(const EnsDom<R>* __restrict__ doms, const int4* __restrict__ work)
+{
+    const int4 wk = work[blockIdx.x];
+    const EnsDom<R>& D = doms[wk.x];
+    const int tb0 = wk.w * D.tpb_w;
+    const int tb1 = tb0 + D.tpb_w < D.ntr ? tb0 + D.tpb_w : D.ntr;
+    wcol_body<S, R, TB>(D.A, D.dt, wk.y * 64 + threadIdx.x, wk.z * 4 + threadIdx.y, tb0, tb1, wk.w == 0, D.scratch,
+                        D.tracers, D.hA, D.hB, D.w, D.w, D.stats);
+}
+
 // ---------------------------------------------------------------------------
 // FP32 vertical diffusion of all tracers, one lane per column, the column held
 // in registers.  Tracer-independent parts of the theta scheme
@@ -677,18 +721,11 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
 // g3 = auxo_k/den, g4 = auxo_{k-1}/den (g3 = g4 = 0 when cnpar = 1).
 // ---------------------------------------------------------------------------
 template <int NZMAX, bool CN1>
-__global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, double cnpar, double molecular,
-                                                  const double* __restrict__ nuh, int ntr_all, int tpb,
-                                                  const float* __restrict__ fin_all, float* __restrict__ fout_all)
+__device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double cnpar, double molecular,
+                                           const double* __restrict__ nuh, int i, int j, int lane, int ntr,
+                                           const float* __restrict__ fin, float* __restrict__ fout)
 {
-    const int tb0 = blockIdx.z * tpb;
-    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
-    const float* fin = fin_all + (size_t)tb0 * A.n3;
-    float* fout = fout_all + (size_t)tb0 * A.n3;
     extern __shared__ float s_coef[];  // [ncoef][nz][64]
-    const int lane = threadIdx.x;
-    const int i = blockIdx.x * 64 + lane;
-    const int j = blockIdx.y;
     const int nz = A.nz;
     const size_t n2 = A.n2, n3 = A.n3;
     constexpr int NC = CN1 ? 3 : 5;
@@ -805,26 +842,64 @@ __global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, d
     }
 }
 
+// Single domain: blockIdx = (64-column chunk, row, tracer batch).
+template <int NZMAX, bool CN1>
+__global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, double cnpar, double molecular,
+                                                  const double* __restrict__ nuh, int ntr_all, int tpb,
+                                                  const float* __restrict__ fin_all, float* __restrict__ fout_all)
+{
+    const int tb0 = blockIdx.z * tpb;
+    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
+    vdiff_body<NZMAX, CN1>(A, dt, cnpar, molecular, nuh, blockIdx.x * 64 + threadIdx.x, blockIdx.y, threadIdx.x, ntr,
+                           fin_all + (size_t)tb0 * A.n3, fout_all + (size_t)tb0 * A.n3);
+}
+
+// Ensemble: work item {domain, bx, row, batch}; scratch -> tracers.
+template <int NZMAX, bool CN1>
+__global__ void __launch_bounds__(64) k_vdiff_multi(const EnsDom<float>* __restrict__ doms,
+                                                    const int4* __restrict__ work)
+{
+    const int4 wk = work[blockIdx.x];
+    const EnsDom<float>& D = doms[wk.x];
+    const int tb0 = wk.w * D.tpb_d;
+    const int ntr = D.ntr - tb0 < D.tpb_d ? D.ntr - tb0 : D.tpb_d;
+    const size_t off = (size_t)tb0 * D.A.n3;
+    vdiff_body<NZMAX, CN1>(D.A, D.dt, D.cnpar, D.molecular, D.nuh, wk.y * 64 + threadIdx.x, wk.z, threadIdx.x, ntr,
+                           D.scratch + off, D.tracers + off);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static inline int nbatch(int ntr, int tpb) { return tpb <= 0 ? 1 : (ntr + tpb - 1) / tpb; }
 
 template <typename R>
-int launch_hpair(int d1, int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
+int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
                  const double* hin, double* hout, hipStream_t s)
 {
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
     if ((long)A.nz * nb > 65535) return GETM_ERR_CAPACITY;
-    dim3 grid((A.nx_ + TX - 1) / TX, (A.ny_ + TY - 1) / TY, A.nz * nb);
+    if (ty != 4 && ty != 8 && ty != 16) return GETM_ERR_ARG;
+    dim3 grid((A.nx_ + TX - 1) / TX, (A.ny_ + ty - 1) / ty, A.nz * nb);
     ProfileScope ps(d1 == 2 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
-    GETM_SCHEME_SWITCH(scheme, {
-        if (d1 == 2)
-            hipLaunchKernelGGL((k_hpair<S_, R, 2>), grid, dim3(NTHR), 0, s, A, dt, ntr, tpb, fin, fout, hin, hout);
-        else
-            hipLaunchKernelGGL((k_hpair<S_, R, 1>), grid, dim3(NTHR), 0, s, A, dt, ntr, tpb, fin, fout, hin, hout);
-    });
+#define GETM_HP(TYV)                                                                                           \
+    GETM_SCHEME_SWITCH(scheme, {                                                                               \
+        if (d1 == 2)                                                                                           \
+            hipLaunchKernelGGL((k_hpair<S_, R, 2, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, fin, fout, hin, \
+                               hout);                                                                          \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_hpair<S_, R, 1, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, fin, fout, hin, \
+                               hout);                                                                          \
+    })
+    if (ty == 16) {
+        GETM_HP(16);
+    } else if (ty == 8) {
+        GETM_HP(8);
+    } else {
+        GETM_HP(4);
+    }
+#undef GETM_HP
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }
@@ -876,10 +951,76 @@ int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecula
     return GETM_OK;
 }
 
-template int launch_hpair<float>(int, int, const StepArgs&, double, int, int, const float*, float*, const double*,
-                                 double*, hipStream_t);
-template int launch_hpair<double>(int, int, const StepArgs&, double, int, int, const double*, double*, const double*,
-                                  double*, hipStream_t);
+template int launch_hpair<float>(int, int, int, const StepArgs&, double, int, int, const float*, float*,
+                                 const double*, double*, hipStream_t);
+template int launch_hpair<double>(int, int, int, const StepArgs&, double, int, int, const double*, double*,
+                                  const double*, double*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// ensemble launchers: one launch per kernel kind covers every domain
+// ---------------------------------------------------------------------------
+int launch_hpair_multi(int stage, int d1, int scheme, int ty, const EnsDom<float>* doms, const int4* work, int nwork,
+                       hipStream_t s)
+{
+    if (nwork <= 0) return GETM_OK;
+    ProfileScope ps(stage == 1 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
+#define GETM_HPM(TYV)                                                                                              \
+    GETM_SCHEME_SWITCH(scheme, {                                                                                   \
+        if (stage == 1 && d1 == 2)                                                                                 \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 1>), dim3(nwork), dim3(TX * TYV), 0, s, doms, work); \
+        else if (stage == 1)                                                                                       \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 1>), dim3(nwork), dim3(TX * TYV), 0, s, doms, work); \
+        else if (d1 == 2)                                                                                          \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 2>), dim3(nwork), dim3(TX * TYV), 0, s, doms, work); \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 2>), dim3(nwork), dim3(TX * TYV), 0, s, doms, work); \
+    })
+    if (ty == 16) {
+        GETM_HPM(16);
+    } else if (ty == 8) {
+        GETM_HPM(8);
+    } else if (ty == 4) {
+        GETM_HPM(4);
+    } else {
+        return GETM_ERR_ARG;
+    }
+#undef GETM_HPM
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+int launch_wcol_multi(int scheme, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s)
+{
+    if (nwork <= 0) return GETM_OK;
+    ProfileScope ps(GETM_K_WCOL, s);
+    GETM_SCHEME_SWITCH(scheme, {
+        hipLaunchKernelGGL((k_wcol_multi<S_, float, 4>), dim3(nwork), dim3(64, 4, 1), 0, s, doms, work);
+    });
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+int launch_vdiff_multi(bool cn1, int nzmax, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s)
+{
+    if (nwork <= 0) return GETM_OK;
+    if (nzmax > 64) return GETM_ERR_CAPACITY;
+    const size_t shm = (size_t)(cn1 ? 3 : 5) * nzmax * 64 * sizeof(float);
+    ProfileScope ps(GETM_K_VDIFF_F32, s);
+#define GETM_VDM(NZM)                                                                                            \
+    do {                                                                                                        \
+        if (cn1)                                                                                                \
+            hipLaunchKernelGGL((k_vdiff_multi<NZM, true>), dim3(nwork), dim3(64), shm, s, doms, work);          \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_vdiff_multi<NZM, false>), dim3(nwork), dim3(64), shm, s, doms, work);         \
+    } while (0)
+    if (nzmax <= 16) GETM_VDM(16);
+    else if (nzmax <= 32) GETM_VDM(32);
+    else if (nzmax <= 48) GETM_VDM(48);
+    else GETM_VDM(64);
+#undef GETM_VDM
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
 template int launch_wcol<float>(int, const StepArgs&, double, int, int, const float*, float*, const double*, double*,
                                 const double*, const double*, int*, hipStream_t);
 template int launch_wcol<double>(int, const StepArgs&, double, int, int, const double*, double*, const double*,

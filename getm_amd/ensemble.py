@@ -1,0 +1,100 @@
+"""Ensembles of independent domains stepped together.
+
+Each domain has its own grid (nx, ny, nz <= 64, mask, metrics, thicknesses),
+velocities, w, nuh and 1..T FP32 tracers; one ``step()`` is
+TracerCollection.advance (python/pygetm/tracer.py:193-276) of every domain, run
+as four launches that cover all domains (``ensemble_step`` in the C-ABI).
+Per-domain results do not depend on the other domains or on the launch shape
+and equal the single-domain FP32 batched step bit for bit.
+
+``graph_chunk(n)`` captures n steps in a hipGraph (via torch.cuda.CUDAGraph on
+the current device) for replay without per-launch host overhead.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence
+
+import torch
+
+from . import _lib
+from .core import Array, Grid
+
+
+class Ensemble:
+    def __init__(self, grids: Sequence[Grid], ntracers: Sequence[int], scheme: int = 5, cnpar: float = 1.0,
+                 ufirst: bool = False):
+        if len(grids) != len(ntracers) or not grids:
+            raise ValueError("one tracer count per grid is needed")
+        self.grids: List[Grid] = list(grids)
+        self.ntracers = [int(n) for n in ntracers]
+        n = len(grids)
+        arr = ctypes.c_void_p * n
+        tg = arr(*[g.p for g in grids])
+        ug = arr(*[g.ugrid.p for g in grids])
+        vg = arr(*[g.vgrid.p for g in grids])
+        nt = (ctypes.c_int * n)(*self.ntracers)
+        self.p = _lib.load().ensemble_create(n, tg, ug, vg, nt, int(scheme), float(cnpar), int(ufirst))
+        if not self.p:
+            raise ValueError("ensemble_create failed (scheme, grids, nz <= 64 and ntracers >= 1 are required)")
+        self._keep = [None] * n  # tensors bound to the library, kept alive here
+        self.tracers: List[Optional[torch.Tensor]] = [None] * n
+        self.tuning = (16, 0, 16, 16)
+
+    def __del__(self):
+        p = getattr(self, "p", None)
+        if p and _lib._lib is not None:
+            _lib._lib.ensemble_finalize(p)
+            self.p = None
+
+    def __len__(self) -> int:
+        return len(self.grids)
+
+    def bind(self, d: int, tracers: torch.Tensor, u: Array, v: Array, w: Array, nuh: Array, timestep: float,
+             molecular: float = 0.0) -> None:
+        """Fields of domain d; tracers is a (ntracers[d], nz, ny_, nx_) float32 device tensor
+        that is stepped in place."""
+        g = self.grids[d]
+        if tracers.dtype != torch.float32 or not tracers.is_contiguous():
+            raise ValueError("ensemble tracers must be contiguous float32")
+        if tuple(tracers.shape) != (self.ntracers[d], g.nz_, g.ny_, g.nx_):
+            raise ValueError(f"domain {d}: expected {(self.ntracers[d], g.nz_, g.ny_, g.nx_)}, "
+                             f"got {tuple(tracers.shape)}")
+        rc = _lib.load().ensemble_bind(self.p, d, _lib.ptr(tracers), _lib.ptr(u.all_values), _lib.ptr(v.all_values),
+                                       _lib.ptr(w.all_values), _lib.ptr(nuh.all_values), float(timestep),
+                                       float(molecular))
+        _lib.check(rc, "ensemble_bind")
+        self._keep[d] = (tracers, u, v, w, nuh)
+        self.tracers[d] = tracers
+
+    def set_tuning(self, tile_y: int = 16, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
+                   tracers_per_block_d: int = 16) -> None:
+        _lib.check(_lib.load().ensemble_set_tuning(self.p, int(tile_y), int(tracers_per_block_h),
+                                                   int(tracers_per_block_w), int(tracers_per_block_d)),
+                   "ensemble_set_tuning")
+        self.tuning = (tile_y, tracers_per_block_h, tracers_per_block_w, tracers_per_block_d)
+
+    def commit(self) -> None:
+        """Allocate scratch and upload the work tables (synchronous)."""
+        _lib.check(_lib.load().ensemble_commit(self.p, _lib.stream_handle()), "ensemble_commit")
+
+    def step(self) -> None:
+        _lib.check(_lib.load().ensemble_step(self.p, _lib.stream_handle()), "ensemble_step")
+
+    def graph_chunk(self, nsteps: int) -> "torch.cuda.CUDAGraph":
+        """Capture nsteps steps as one hipGraph; replay with ``g.replay()``."""
+        self.step()  # every lazily sized buffer exists before capture
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(nsteps):
+                self.step()
+        return g
+
+    @property
+    def cells(self) -> int:
+        """Wet tracer cells updated per step (wet columns x nz x tracers, summed)."""
+        tot = 0
+        for g, n in zip(self.grids, self.ntracers):
+            tot += int((g.mask.all_values == 1).sum().item()) * g.nz_ * n
+        return tot

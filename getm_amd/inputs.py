@@ -345,3 +345,21 @@ def tiny(nx=24, ny=20, nz=8, seed=0, scheme=5, dt=None, land_fraction=0.25, cfl=
         tmask = tm
     return Domain(nx, ny, nz, h, tmask, umask, vmask, m["idxu"], m["dyu"], m["dxv"], m["idyv"], m["iarea"],
                   m["area"], H, z, sigma, ho, hn, hu, hv, u, v, w, nuh, tr, dt, scheme, 1.0, "tiny")
+
+
+def ensemble_domains(ndomains=96, seed=3, size_range=(64, 768), nz_range=(10, 60), tracer_range=(1, 32),
+                     scheme=5, dt=30.0):
+    """Uneven ensemble: per domain nx, ny log-uniform in ``size_range``, nz uniform in
+    ``nz_range``, 1..32 blob tracers, fields generated as by :func:`coastal` from a
+    per-domain seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    lo, hi = math.log(size_range[0]), math.log(size_range[1])
+    for d in range(ndomains):
+        nx = int(round(math.exp(rng.uniform(lo, hi))))
+        ny = int(round(math.exp(rng.uniform(lo, hi))))
+        nz = int(rng.integers(nz_range[0], nz_range[1] + 1))
+        nt = int(rng.integers(tracer_range[0], tracer_range[1] + 1))
+        out.append(coastal(nx=nx, ny=ny, nz=nz, dx=2000.0, dt=dt, scheme=scheme, seed=1000 + d, ntracers=nt,
+                           tracer_kind="blobs"))
+    return out

@@ -136,13 +136,15 @@ class TracerCollection:
         return self.inventory_device(h).cpu().numpy()
 
     def set_tuning(self, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
-                   tracers_per_block_d: int = 16) -> None:
-        """Launch shape of the batched kernels (tracers per block; 0 = all).  Does not
-        change results."""
+                   tracers_per_block_d: int = 16, tile_y: int = 16) -> None:
+        """Launch shape of the batched kernels: tracers per block of the three kernel
+        kinds (0 = all) and the height of the horizontal-stage tiles (4, 8, 16).  Does
+        not change results."""
         if not self.batched:
             return
-        _lib.check(_lib.load().transport_set_tuning(self.p, int(tracers_per_block_h), int(tracers_per_block_w),
-                                                    int(tracers_per_block_d)), "transport_set_tuning")
+        _lib.check(_lib.load().transport_set_tuning(self.p, int(tile_y), int(tracers_per_block_h),
+                                                    int(tracers_per_block_w), int(tracers_per_block_d)),
+                   "transport_set_tuning")
 
     def stats(self, reset: bool = True):
         """(largest z sub-iteration count, columns capped at 200) since the last reset."""

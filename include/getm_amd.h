@@ -150,11 +150,13 @@ int transport_step(void* tr, int ntracers, void* tracers, const double* u, const
                    const double* nuh, const double* Ah_u, const double* Ah_v, double timestep, double molecular,
                    double* h_out, void* stream);
 
-/* No reference counterpart: launch shape of the three batched kernel kinds
- * (horizontal stage pairs, w stage, FP32 diffusion) as tracers per block; the
- * tracer batches of one step are spread over the grid.  0 = all tracers in
- * every block.  Results do not depend on these values. */
-int transport_set_tuning(void* tr, int tracers_per_block_h, int tracers_per_block_w, int tracers_per_block_d);
+/* No reference counterpart: launch shape of the batched kernels -- the height
+ * of the horizontal-stage tiles (4, 8 or 16 rows of 64 columns) and, for the
+ * three kernel kinds (horizontal stage pairs, w stage, FP32 diffusion), the
+ * tracers per block (the tracer batches of one step are spread over the grid;
+ * 0 = all tracers in every block).  Results do not depend on these values. */
+int transport_set_tuning(void* tr, int tile_y, int tracers_per_block_h, int tracers_per_block_w,
+                         int tracers_per_block_d);
 
 /* Tracer inventories (the reference's domain integrals of tracers,
  * simulation.py:1408-1466): inventory[t] = sum over wet cells of
@@ -174,6 +176,39 @@ int transport_conserve(void* tr, int ntracers, void* tracers, const double* targ
 
 /* z-sweep sub-stepping statistics of a transport object (as advection_get_stats). */
 int transport_get_stats(void* tr, int* max_iters, int* capped_columns, int reset, void* stream);
+
+/* ---- ensembles of independent domains -------------------------------------- */
+
+/* No reference counterpart: many independent domains (each its own T/U/V
+ * grids, nx, ny, nz <= 64, mask and ntracers[d] FP32 tracers) stepped together
+ * with one launch per kernel kind, i.e. TracerCollection.advance of every
+ * domain (tracer.py:193-276) in four launches.  Per-domain arithmetic does not
+ * depend on the other domains: results equal transport_step on each domain
+ * with GETM_F32, bit for bit.  Usage: create, bind every domain, commit (all
+ * allocation and table uploads, synchronous), then ensemble_step any number of
+ * times -- it only enqueues kernels, so it can be captured in a hipGraph. */
+void* ensemble_create(int ndomains, void* const* tgrids, void* const* ugrids, void* const* vgrids,
+                      const int* ntracers, int scheme, double cnpar, int ufirst);
+void ensemble_finalize(void* ens);
+
+/* Fields of domain d: tracers (ntracers[d] x nz x ny_ x nx_ float), u, v (on the
+ * domain's U/V grids), w, nuh (nz+1 levels), time step and molecular
+ * diffusivity.  Call ensemble_commit after (re)binding. */
+int ensemble_bind(void* ens, int domain, float* tracers, const double* u, const double* v, const double* w,
+                  const double* nuh, double timestep, double molecular);
+
+/* Launch shape as transport_set_tuning; takes effect at the next commit. */
+int ensemble_set_tuning(void* ens, int tile_y, int tracers_per_block_h, int tracers_per_block_w,
+                        int tracers_per_block_d);
+
+/* Allocates scratch, builds and uploads the work tables (synchronous). */
+int ensemble_commit(void* ens, void* stream);
+
+/* One step of every domain. */
+int ensemble_step(void* ens, void* stream);
+
+/* Device pointer of domain d's advected thickness after the last step. */
+const double* ensemble_thickness(void* ens, int domain);
 
 /* ---- instrumentation ---------------------------------------------------- */
 

@@ -138,7 +138,7 @@ def test_batched_launch_shape_does_not_change_results(precision):
     """Tracer batches spread over the grid (transport_set_tuning) give identical bits."""
     d = inputs.tiny(nx=37, ny=30, nz=9, seed=91, ntracers=7, cfl=0.5, wscale=2.0)
     outs = []
-    for tuning in [(0, 0, 0), (1, 1, 1), (2, 3, 5), (4, 16, 16)]:
+    for tuning in [(0, 0, 0, 16), (1, 1, 1, 8), (2, 3, 5, 4), (4, 16, 16, 16)]:
         from getm_amd.core import CENTERS, INTERFACES
         from getm_amd.tracer import TracerCollection
 

@@ -16,7 +16,7 @@ def declared_in_headers():
         if fn.endswith(".h"):
             text = open(os.path.join(inc, fn)).read()
             text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-            names += re.findall(r"^\s*(?:void\s*\*|void|int)\s+(\w+)\s*\(", text, flags=re.M)
+            names += re.findall(r"^\s*(?:const\s+)?(?:void|int|double|float)\s*\*?\s*(\w+)\s*\(", text, flags=re.M)
     return names
 
 
