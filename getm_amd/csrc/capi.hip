@@ -430,9 +430,10 @@ int transport_conserve(void* p, int ntracers, void* tracers, const double* targe
     const StepArgs A = step_args(tr, nullptr, nullptr, nullptr, nullptr);
     int rc;
     const size_t np = inventory_part_doubles(A, ntracers);
-    if ((rc = tr->part.ensure(np * sizeof(double))) || (rc = tr->part2.ensure(np * sizeof(double))) ||
+    if ((rc = tr->part.ensure(np * sizeof(double))) ||
+        (rc = tr->part2.ensure(fix_part_doubles(A, ntracers) * sizeof(double))) ||
         (rc = tr->ratio.ensure((size_t)ntracers * sizeof(double))) ||
-        (rc = tr->resid.ensure((size_t)ntracers * sizeof(double))))
+        (rc = tr->resid.ensure(2 * (size_t)ntracers * sizeof(double))))
         return rc;
     double* part = static_cast<double*>(tr->part.ptr);
     double* ratio = static_cast<double*>(tr->ratio.ptr);
@@ -579,7 +580,7 @@ int ensemble_commit(void* p, void* stream)
     if ((rc = e->scratch.ensure(nscr * sizeof(float))) || (rc = e->hbuf.ensure(nh * sizeof(double)))) return rc;
     float* scr = static_cast<float*>(e->scratch.ptr);
     double* hb = static_cast<double*>(e->hbuf.ptr);
-    std::vector<int4> wh, ww, wd;
+    std::vector<int4> wh, ww, wd_class[8];
     e->nzmax = 1;
     for (int d = 0; d < e->ndom; d++) {
         EnsDom<float>& D = e->host[d];
@@ -605,9 +606,16 @@ int ensemble_commit(void* p, void* stream)
         for (int b = 0; b < nbw; b++)
             for (int by = 0; by < (t->ny_ + 3) / 4; by++)
                 for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) ww.push_back(make_int4(d, bx, by, b));
+        // diffusion work grouped by depth class, so each launch's kernel is unrolled for its columns
+        std::vector<int4>& wd = wd_class[(t->nz - 1) / 8];
         for (int b = 0; b < nbd; b++)
             for (int j = 0; j < t->ny_; j++)
                 for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) wd.push_back(make_int4(d, bx, j, b));
+    }
+    std::vector<int4> wd;
+    for (int c = 0; c < 8; c++) {
+        e->nwork_d_class[c] = (int)wd_class[c].size();
+        wd.insert(wd.end(), wd_class[c].begin(), wd_class[c].end());
     }
     if (wh.size() > 0x7fffffff || ww.size() > 0x7fffffff || wd.size() > 0x7fffffff) return GETM_ERR_CAPACITY;
     if ((rc = upload_work(e->work_h, wh)) || (rc = upload_work(e->work_w, ww)) || (rc = upload_work(e->work_d, wd)))
@@ -637,7 +645,13 @@ int ensemble_step(void* p, void* stream)
     if ((rc = launch_hpair_multi(2, 3 - d1, e->scheme, e->tile_y, doms, static_cast<const int4*>(e->work_h.ptr),
                                  e->nwork_h, s)))
         return rc;
-    return launch_vdiff_multi(e->cnpar == 1.0, e->nzmax, doms, static_cast<const int4*>(e->work_d.ptr), e->nwork_d, s);
+    const int4* wd = static_cast<const int4*>(e->work_d.ptr);
+    for (int c = 0; c < 8; c++) {
+        if (!e->nwork_d_class[c]) continue;
+        if ((rc = launch_vdiff_multi(e->cnpar == 1.0, c, doms, wd, e->nwork_d_class[c], s))) return rc;
+        wd += e->nwork_d_class[c];
+    }
+    return GETM_OK;
 }
 
 const double* ensemble_thickness(void* p, int d)

@@ -127,6 +127,7 @@ struct Ensemble {
     double cnpar = 1.0;
     int tile_y = 16, tpb_h = 0, tpb_w = 16, tpb_d = 16;
     int nwork_h = 0, nwork_w = 0, nwork_d = 0, nzmax = 1;
+    int nwork_d_class[8] = {};  // diffusion work items per depth class (8 c-7 .. 8 c + 8 layers), in this order
     int committed = 0;
     EnsDom<float>* host = nullptr;  // host copy of the domain table
     const Grid** tg = nullptr;
@@ -143,7 +144,8 @@ int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int n
 int launch_hpair_multi(int stage, int d1, int scheme, int ty, const EnsDom<float>* doms, const int4* work, int nwork,
                        hipStream_t s);
 int launch_wcol_multi(int scheme, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
-int launch_vdiff_multi(bool cn1, int nzmax, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
+// one depth class: every domain of the work items has 8 (cls + 1) - 7 .. 8 (cls + 1) layers
+int launch_vdiff_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
 template <typename R>
 int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout, const double* hin,
                 double* hout, const double* w, const double* wv, int* stats, hipStream_t s);
@@ -156,6 +158,7 @@ int launch_inventory(const StepArgs& A, const double* h, const double* area, int
                      double* inv, const double* target, double* ratio, hipStream_t s);
 int launch_fix_f32(const StepArgs& A, int ntr, float* f, const double* part, const double* ratio, const double* area,
                    double* part2, const double* target, double* resid, hipStream_t s);
+size_t fix_part_doubles(const StepArgs& A, int ntr);
 size_t inventory_part_doubles(const StepArgs& A, int ntr);
 
 // ---- FP64 kernels (transport_fp64.hip) ----

@@ -14,10 +14,15 @@
 // c/ulp in a fixed cell order, cell i changes by
 //   n_i = rint((D/I) S_i) - rint((D/I) S_{i-1})  ulps   (n_i in {-1, 0, 1} in practice),
 // which is exact in FP32 and touches ~|D/I| * 2^23 cells per tracer.  Flipped
-// cells carry different masses, so a residual of a few cell-ulps remains; the
-// fix kernel accumulates the fixed inventory on the way, and a sequential
-// greedy pass over the first wet cells of each tracer removes the residual to
-// within half of one cell's ulp mass (relative ~1e-7 / wet cells).
+// cells carry different masses, so a residual of a few cell-ulps remains (of
+// the order of the largest cell's ulp mass when a tracer sits in a few cells,
+// e.g. a fresh point release).  The fix kernel accumulates the fixed
+// inventory and each block's largest-mass cell on the way; a greedy pass (one
+// wave per tracer, cells decided in a fixed order) then walks the wet cells
+// from the column of the tracer's largest-mass cell, changing each by
+// rint(R / ulp mass) ulps (at most 4096, i.e. 5e-4 of the cell's own value), so
+// the residual R never grows and ends below half the ulp mass of the smallest
+// cell it passes.
 #include <type_traits>
 
 #include "common.h"
@@ -89,25 +94,59 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, con
     }
 }
 
-// One lane per tracer: inventory = compensated sum over blocks; for the fixer
-// also the exclusive prefix of the per-block c/ulp sums and the ratio D / I.
-__global__ void k_inventory_reduce(int ntr, int nblk, double* __restrict__ part, double* __restrict__ inv,
-                                   const double* __restrict__ target, double* __restrict__ ratio)
+// Fixed-order combination of per-lane compensated sums (s, c) over a wave:
+// lane 0's pair, then lane 1's, ... (every lane ends with the same total).
+__device__ __forceinline__ double wave_neumaier_total(double s, double c)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double S = 0.0, C = 0.0;
+    for (int l = 0; l < 64; l++) {
+        neumaier(S, C, __shfl(s, l));
+        C += __shfl(c, l);
+    }
+    return S + C;
+}
+
+// Exclusive prefix over the wave's lanes, in lane order (same value order every call).
+__device__ __forceinline__ double wave_exclusive_prefix(double x, int lane)
+{
+    double run = 0.0, mine = 0.0;
+    for (int l = 0; l < 64; l++) {
+        if (l == lane) mine = run;
+        run += __shfl(x, l);
+    }
+    return mine;
+}
+
+// One wave per tracer: inventory = compensated sum over blocks; for the fixer
+// also the exclusive prefix of the per-block c/ulp sums and the ratio D / I.
+// Lane l owns the contiguous block range [l seg, (l + 1) seg), so the prefix is
+// a lane-local running sum started at the lane's exclusive offset.
+__global__ void __launch_bounds__(64) k_inventory_reduce(int ntr, int nblk, double* __restrict__ part,
+                                                         double* __restrict__ inv, const double* __restrict__ target,
+                                                         double* __restrict__ ratio)
+{
+    const int t = blockIdx.x, lane = threadIdx.x;
     if (t >= ntr) return;
     double* pt = part + (size_t)t * 3 * nblk;
+    const int seg = (nblk + 63) / 64;
+    const int b0 = lane * seg, b1 = b0 + seg < nblk ? b0 + seg : nblk;
     double S = 0.0, C = 0.0, Q = 0.0;
-    for (int b = 0; b < nblk; b++) {
+    for (int b = b0; b < b1; b++) {
         neumaier(S, C, pt[b]);
         C += pt[nblk + b];
-        const double qb = pt[2 * nblk + b];
-        pt[2 * nblk + b] = Q;  // exclusive prefix, in place
-        Q += qb;
+        Q += pt[2 * nblk + b];
     }
-    const double I = S + C;
-    if (inv) inv[t] = I;
-    if (target && ratio) ratio[t] = (I != 0.0) ? (target[t] - I) / I : 0.0;
+    double run = wave_exclusive_prefix(Q, lane);
+    for (int b = b0; b < b1; b++) {
+        const double qb = pt[2 * nblk + b];
+        pt[2 * nblk + b] = run;  // exclusive prefix, in place
+        run += qb;
+    }
+    const double I = wave_neumaier_total(S, C);
+    if (lane == 0) {
+        if (inv) inv[t] = I;
+        if (target && ratio) ratio[t] = (I != 0.0) ? (target[t] - I) / I : 0.0;
+    }
 }
 
 // Same block decomposition and summation order as k_inventory, so the block
@@ -117,7 +156,7 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_f32(const StepArgs A, int n
                                                          const double* __restrict__ ratio,
                                                          const double* __restrict__ area, double* __restrict__ part2)
 {
-    __shared__ double s_q[INV_THREADS], s_m[INV_THREADS], s_c[INV_THREADS];
+    __shared__ double s_q[INV_THREADS], s_m[INV_THREADS], s_c[INV_THREADS], s_x[INV_THREADS];
     const int tid = threadIdx.y * IBX + threadIdx.x;
     const int i = blockIdx.x * IBX + threadIdx.x, j = blockIdx.y * IBY + threadIdx.y;
     const int t = blockIdx.z;
@@ -147,7 +186,7 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_f32(const StepArgs A, int n
         }
     }
     __syncthreads();
-    double m = 0.0, mc = 0.0;
+    double m = 0.0, mc = 0.0, mx = 0.0;
     if (wet) {
         const double a = area[p];
         double S0 = s_q[tid];
@@ -160,77 +199,151 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_f32(const StepArgs A, int n
                 c = c + (float)(n * exp2((double)(e - 23)));
                 fc[(size_t)k * A.n2] = c;
             }
-            neumaier(m, mc, (a * A.hn[(size_t)k * A.n2 + p]) * (double)c);
+            const double cm = (a * A.hn[(size_t)k * A.n2 + p]) * (double)c;
+            neumaier(m, mc, cm);
+            mx = fmax(mx, fabs(cm));
             S0 = S1;
         }
     }
     s_m[tid] = m;
     s_c[tid] = mc;
+    s_x[tid] = mx;
     __syncthreads();
     if (tid == 0) {
-        double S = 0.0, C = 0.0;
+        double S = 0.0, C = 0.0, X = 0.0;
+        int lx = -1;
         for (int l = 0; l < INV_THREADS; l++) {
             neumaier(S, C, s_m[l]);
             C += s_c[l];
+            if (s_x[l] > X) {
+                X = s_x[l];
+                lx = l;
+            }
         }
-        double* pt = part2 + (size_t)t * 2 * nblk;
+        double* pt = part2 + (size_t)t * 4 * nblk;
         pt[blk] = S;
         pt[nblk + blk] = C;
+        pt[2 * nblk + blk] = X;
+        // column of the block's largest-mass cell (-1: no mass in this block)
+        pt[3 * nblk + blk] = lx < 0 ? -1.0
+                                    : (double)((size_t)(blockIdx.y * IBY + lx / IBX) * A.nx_ +
+                                               (blockIdx.x * IBX + lx % IBX));
     }
 }
 
-// One lane per tracer: remaining defect after the dithering pass.
-__global__ void k_residual(int ntr, int nblk, const double* __restrict__ part2, const double* __restrict__ target,
-                           double* __restrict__ resid)
+// One wave per tracer: remaining defect after the dithering pass (resid[t])
+// and the column of the largest-mass cell (resid[ntr + t], -1 if none; ties go
+// to the lowest block).
+__global__ void __launch_bounds__(64) k_residual(int ntr, int nblk, const double* __restrict__ part2,
+                                                 const double* __restrict__ target, double* __restrict__ resid)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x, lane = threadIdx.x;
     if (t >= ntr) return;
-    const double* pt = part2 + (size_t)t * 2 * nblk;
-    double S = 0.0, C = 0.0;
-    for (int b = 0; b < nblk; b++) {
+    const double* pt = part2 + (size_t)t * 4 * nblk;
+    const int seg = (nblk + 63) / 64;
+    const int b0 = lane * seg, b1 = b0 + seg < nblk ? b0 + seg : nblk;
+    double S = 0.0, C = 0.0, X = 0.0, P = -1.0;
+    for (int b = b0; b < b1; b++) {
         neumaier(S, C, pt[b]);
         C += pt[nblk + b];
+        if (pt[2 * nblk + b] > X) {
+            X = pt[2 * nblk + b];
+            P = pt[3 * nblk + b];
+        }
     }
-    // target - (S + C), compensated
-    double R = 0.0, RC = 0.0;
-    neumaier(R, RC, target[t]);
-    neumaier(R, RC, -S);
-    neumaier(R, RC, -C);
-    resid[t] = R + RC;
+    const double tot = wave_neumaier_total(S, C);
+    double XB = 0.0, PB = -1.0;
+    for (int l = 0; l < 64; l++) {
+        const double xl = __shfl(X, l), pl = __shfl(P, l);
+        if (xl > XB) {
+            XB = xl;
+            PB = pl;
+        }
+    }
+    if (lane == 0) {
+        resid[ntr + t] = PB;
+        // target - tot, compensated
+        double R = 0.0, RC = 0.0;
+        neumaier(R, RC, target[t]);
+        neumaier(R, RC, -tot);
+        resid[t] = R + RC;
+    }
 }
 
-// One lane per tracer: greedy one-ulp changes over the wet cells in memory
-// order until the residual is below half of the next cell's ulp mass.
-__global__ void k_greedy_f32(const StepArgs A, int ntr, float* __restrict__ f, const double* __restrict__ area,
-                             const double* __restrict__ resid, const double* __restrict__ target, int max_cells)
+// One wave per tracer: greedy changes of up to GREEDY_MAX_ULPS ulps per cell.
+// The walk visits chunks of 64 consecutive columns and, in each, the layers
+// bottom to top; the 64 cells of one layer are loaded together and then decided
+// one lane after the other with the running residual R, which every lane holds.
+// Chunks: first the 64 columns centred on the largest-mass cell in the rows
+// j0, j0-1, j0+1, ..., j0 +- GREEDY_ROWS/2 (a tracer that sits in a few cells,
+// such as a fresh point release, has all its cells of intermediate mass there,
+// upstream and downstream), then the whole grid in memory order from that
+// column on, wrapping round.  A cell whose ulp mass exceeds 2|R| would change by
+// 0 ulps and is skipped.  The walk ends when |R| <= tol or after max_cells
+// cells (wet or not).
+constexpr double GREEDY_MAX_ULPS = 4096.0;
+constexpr int GREEDY_ROWS = 63;
+
+__global__ void __launch_bounds__(64) k_greedy_f32(const StepArgs A, int ntr, float* __restrict__ f,
+                                                   const double* __restrict__ area, const double* __restrict__ resid,
+                                                   const double* __restrict__ target, int max_cells)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x, lane = threadIdx.x;
     if (t >= ntr) return;
     double R = resid[t];
     const double tol = 1e-14 * fabs(target[t]);  // two orders below the conservation target
     if (fabs(R) <= tol) return;
     float* ft = f + (size_t)t * A.n3;
-    int visited = 0;
-    for (int j = A.jmin; j <= A.jmax && visited < max_cells; j++) {
-        for (int i = A.imin; i <= A.imax && visited < max_cells; i++) {
-            const size_t p = (size_t)j * A.nx_ + i;
-            if (A.tmask[p] != 1) continue;
-            const double a = area[p];
-            for (int k = 0; k < A.nz && visited < max_cells; k++, visited++) {
-                const size_t pk = (size_t)k * A.n2 + p;
-                const float c = ft[pk];
-                const int e = ilogbf(c);
-                if (c == 0.f || e < -126) continue;
-                const double ulp = exp2((double)(e - 23));
-                const double wm = (a * A.hn[pk]) * ulp;  // mass of one ulp in this cell
-                if (!(wm > 0.0)) continue;
-                const double n = fmin(fmax(rint(R / wm), -1.0), 1.0);
-                if (n != 0.0) {
+    const double P = resid[ntr + t];
+    const size_t n2 = (size_t)A.n2;
+    const size_t p0 = (P >= 0.0 && P < (double)n2) ? (size_t)P : 0;
+    const int i0 = (int)(p0 % A.nx_), j0 = (int)(p0 / A.nx_);
+    const size_t nchunks = GREEDY_ROWS + (n2 + 63) / 64;
+    size_t visited = 0;
+    for (size_t s = 0; s < nchunks && visited < (size_t)max_cells; s++) {
+        size_t p = 0;
+        bool inr = false;
+        if (s < GREEDY_ROWS) {
+            const int j = j0 + ((s & 1) ? (int)(s + 1) / 2 : -(int)(s / 2));
+            const int i = i0 - 32 + lane;
+            inr = i >= 0 && i < A.nx_ && j >= 0 && j < A.ny_;
+            if (inr) p = (size_t)j * A.nx_ + i;
+        } else {
+            const size_t q = (s - GREEDY_ROWS) * 64 + lane;
+            inr = q < n2;
+            if (inr) p = (p0 + q) % n2;
+        }
+        bool col = false;
+        if (inr) {
+            const int i = (int)(p % A.nx_), j = (int)(p / A.nx_);
+            col = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[p] == 1;
+        }
+        const double a = col ? area[p] : 0.0;
+        visited += (size_t)64 * A.nz;
+        for (int k = 0; k < A.nz; k++) {
+            const size_t pk = (size_t)k * n2 + p;
+            float c = col ? ft[pk] : 0.f;
+            const double m = col ? a * A.hn[pk] : 0.0;
+            const int e = ilogbf(c);
+            const bool normal = col && c != 0.f && e >= -126;
+            const double ulp = normal ? exp2((double)(e - 23)) : 0.0;
+            const double wm = m * ulp;  // mass of one ulp in this cell
+            unsigned long long cand = __ballot(normal && wm > 0.0 && wm <= 2.0 * fabs(R));
+            while (cand) {
+                const int l = __ffsll((long long)cand) - 1;
+                cand &= cand - 1;
+                const double wl = __shfl(wm, l);
+                const double n = fmin(fmax(rint(R / wl), -GREEDY_MAX_ULPS), GREEDY_MAX_ULPS);
+                if (n == 0.0) continue;
+                double dR = 0.0;
+                if (lane == l) {
                     const float cn = c + (float)(n * ulp);
-                    R -= (a * A.hn[pk]) * ((double)cn - (double)c);
+                    dR = m * ((double)cn - (double)c);
+                    c = cn;
                     ft[pk] = cn;
-                    if (fabs(R) <= tol) return;
                 }
+                R -= __shfl(dR, l);
+                if (fabs(R) <= tol) return;
             }
         }
     }
@@ -245,7 +358,7 @@ int launch_inventory(const StepArgs& A, const double* h, const double* area, int
     const int nblk = grid.x * grid.y;
     ProfileScope ps(GETM_K_INVENTORY, s);
     hipLaunchKernelGGL((k_inventory<R>), grid, block, 0, s, A, h, area, ntr, f, part);
-    hipLaunchKernelGGL(k_inventory_reduce, dim3((ntr + 63) / 64), dim3(64), 0, s, ntr, nblk, part, inv, target, ratio);
+    hipLaunchKernelGGL(k_inventory_reduce, dim3(ntr), dim3(64), 0, s, ntr, nblk, part, inv, target, ratio);
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }
@@ -258,8 +371,8 @@ int launch_fix_f32(const StepArgs& A, int ntr, float* f, const double* part, con
     const int nblk = grid.x * grid.y;
     ProfileScope ps(GETM_K_FIX, s);
     hipLaunchKernelGGL(k_fix_f32, grid, block, 0, s, A, ntr, f, part, ratio, area, part2);
-    hipLaunchKernelGGL(k_residual, dim3((ntr + 63) / 64), dim3(64), 0, s, ntr, nblk, part2, target, resid);
-    hipLaunchKernelGGL(k_greedy_f32, dim3((ntr + 63) / 64), dim3(64), 0, s, A, ntr, f, area, resid, target, 1 << 16);
+    hipLaunchKernelGGL(k_residual, dim3(ntr), dim3(64), 0, s, ntr, nblk, part2, target, resid);
+    hipLaunchKernelGGL(k_greedy_f32, dim3(ntr), dim3(64), 0, s, A, ntr, f, area, resid, target, 1 << 18);
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }
@@ -268,6 +381,12 @@ size_t inventory_part_doubles(const StepArgs& A, int ntr)
 {
     const size_t nblk = (size_t)((A.nx_ + IBX - 1) / IBX) * ((A.ny_ + IBY - 1) / IBY);
     return nblk * 3 * (size_t)ntr;
+}
+
+size_t fix_part_doubles(const StepArgs& A, int ntr)
+{
+    const size_t nblk = (size_t)((A.nx_ + IBX - 1) / IBX) * ((A.ny_ + IBY - 1) / IBY);
+    return nblk * 4 * (size_t)ntr;
 }
 
 template int launch_inventory<float>(const StepArgs&, const double*, const double*, int, const float*, double*,

@@ -225,7 +225,13 @@ struct CellRaw {
 
 // Per-cell terms of one stage; the cell is updated iff interior and wet.
 //   FP64: hold, hnew, iA for the reference order (h f - dt (dF iA)) / hnew
-//   FP32: a = hold/hnew, b = dt iA/hnew:  f_new = a f - b dF
+//   FP32: a = (hold - hnew)/hnew, b = dt iA/hnew:  f_new = f + (a f - b dF).
+// The FP32 increment form matters: the factors are rounded once per step and,
+// in a steady flow, identically every step, so in the direct form
+// f_new = (hold/hnew) f - b dF the rounding of hold/hnew (up to 2^-25 of f)
+// would accumulate linearly with the step count; in the increment form it is
+// scaled by |hold/hnew - 1| (a Courant number) and the only O(ulp(f)) error is
+// the final, data-dependent rounding.
 // own = LDS offset of the cell, fl1 = offset of its upper face flux.
 template <typename R>
 struct CellK {
@@ -243,7 +249,7 @@ __device__ __forceinline__ void cell_set(CellK<R>& c, bool upd, double hold, dou
         c.b = hnew;
         c.iA = iA;
     } else {
-        c.a = float(hold / hnew);
+        c.a = float((hold - hnew) / hnew);
         c.b = float((dt * iA) / hnew);
         c.iA = 0.f;
     }
@@ -257,7 +263,7 @@ __device__ __forceinline__ R cell_update(const CellK<R>& c, double dt, R f, R fl
         const double advn = (fl1 - fl0) * c.iA;
         return (hfo - dt * advn) / c.b;
     } else {
-        return c.a * f - c.b * (fl1 - fl0);
+        return f + (c.a * f - c.b * (fl1 - fl0));
     }
 }
 
@@ -664,11 +670,13 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
                         if (b < nb) df[(size_t)b * n3 + (size_t)(k - 1) * n2] = (hfo - dtk * advn) / hnew;
                     }
                 } else {
+                    // increment form, as in cell_update: f + ((h0 - hnew)/hnew f - dt/hnew dF)
                     const double rh = 1.0 / hnew;
-                    const R ca = R(h0 * rh), cb = R(dtk * rh);
+                    const R ca = R((h0 - hnew) * rh), cb = R(dtk * rh);
 #pragma unroll
                     for (int b = 0; b < TB; b++)
-                        if (b < nb) df[(size_t)b * n3 + (size_t)(k - 1) * n2] = ca * f0[b] - cb * (fl[b] - fprev[b]);
+                        if (b < nb)
+                            df[(size_t)b * n3 + (size_t)(k - 1) * n2] = f0[b] + (ca * f0[b] - cb * (fl[b] - fprev[b]));
                 }
 #pragma unroll
                 for (int b = 0; b < TB; b++) {
@@ -720,21 +728,39 @@ __global__ void __launch_bounds__(256) k_wcol_multi(const EnsDom<R>* __restrict_
 // with den_k = a2_k - a3_k ru_{k+1}, g1 = dzo/den, g2 = auxn_k/den,
 // g3 = auxo_k/den, g4 = auxo_{k-1}/den (g3 = g4 = 0 when cnpar = 1).
 // ---------------------------------------------------------------------------
+// FP32 theta-method column solve in increment form.  With c = c_old + d the
+// reference system (diffusion.F90:233-313, dzo = dzn = h)
+//   -an_{k-1} c_{k-1} + (h_k + an_k + an_{k-1}) c_k - an_k c_{k+1} = h_k c_old_k + explicit part
+// becomes the same matrix acting on d with right-hand side
+//   an_k (c_{k+1} - c_k) + an_{k-1} (c_{k-1} - c_k) + ao_k (c_{k+1} - c_k) + ao_{k-1} (c_{k-1} - c_k)
+// (old values), eliminated from the surface down as the reference does:
+//   q_k = e_k (c_{k+1} - c_k + q_{k+1}) - ru_k (c_{k-1} - c_k) [+ g3_k (c_{k+1} - c_k) + g4_k (c_{k-1} - c_k)]
+//   d_0 = q_0,  d_k = q_k - ru_k d_{k-1},  c_k = c_old_k + d_k
+// with e_k = an_k/den_k, ru_k = -an_{k-1}/den_k, g3_k = ao_k/den_k, g4_k = ao_{k-1}/den_k
+// formed in FP64 and rounded once.  The factors are the same every step in a
+// steady state, so their rounding must not multiply c itself (a direct solve
+// c_k = (h_k/den_k) c_old_k + ... would drift by up to 2^-25 of c per step); here it
+// multiplies the differences only, and a well-mixed column stays exactly as it is.
+// The old column and the eliminated q_k are both kept in registers; columns of
+// nz <= 16 interleave two tracers, deeper ones take one tracer at a time.
 template <int NZMAX, bool CN1>
 __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double cnpar, double molecular,
                                            const double* __restrict__ nuh, int i, int j, int lane, int ntr,
                                            const float* __restrict__ fin, float* __restrict__ fout)
 {
     extern __shared__ float s_coef[];  // [ncoef][nz][64]
+    constexpr int NT = NZMAX <= 16 ? 2 : 1;
+    static_assert(NZMAX % 8 == 0, "depth classes are multiples of 8 layers");
     const int nz = A.nz;
+    // the caller guarantees NZMAX - 8 < nz <= NZMAX, so only the top eight
+    // unrolled levels need a run-time test
+    auto has = [&](int k) { return k < NZMAX - 8 || k < nz; };
+    auto has_up = [&](int k) { return k < NZMAX - 9 || k < nz - 1; };
     const size_t n2 = A.n2, n3 = A.n3;
-    constexpr int NC = CN1 ? 3 : 5;
-    float* g1 = s_coef;
-    float* g2 = s_coef + (size_t)nz * 64;
-    float* ru = s_coef + (size_t)2 * nz * 64;
-    float* g3 = s_coef + (size_t)3 * nz * 64;
-    float* g4 = s_coef + (size_t)4 * nz * 64;
-    (void)NC;
+    float* e2 = s_coef;
+    float* ru = s_coef + (size_t)nz * 64;
+    float* g3 = s_coef + (size_t)2 * nz * 64;
+    float* g4 = s_coef + (size_t)3 * nz * 64;
     if (i >= A.nx_) return;
     const size_t p = (size_t)j * A.nx_ + i;
     const bool inside = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax;
@@ -759,10 +785,9 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
                 ao_km1 = ((1.0 - cnpar) * x) / s;
             }
             const double a2 = (dz + an_k) + an_km1;
-            const double den = k == nz - 1 ? a2 : a2 + an_k * ru_up;  // a2 - a3 ru_{k+1}, a3 = -auxn_k
+            const double den = k == nz - 1 ? a2 : a2 + an_k * ru_up;  // a2 - a3 ru_{k+1}, a3 = -an_k
             const double ruk = -an_km1 / den;
-            g1[k * 64 + lane] = float(dz / den);
-            g2[k * 64 + lane] = float(an_k / den);
+            e2[k * 64 + lane] = float(an_k / den);
             ru[k * 64 + lane] = float(ruk);
             if (!CN1) {
                 g3[k * 64 + lane] = float(ao_k / den);
@@ -771,78 +796,78 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
             ru_up = ruk;
         }
     }
-    // two tracers per pass: their recurrences interleave and share each factor load
-    for (int t = 0; t < ntr; t += 2) {
-        const bool two = t + 1 < ntr;
-        const float* fi0 = fin + (size_t)t * n3 + p;
-        const float* fi1 = fin + (size_t)(two ? t + 1 : t) * n3 + p;
-        float va[NZMAX], vb[NZMAX];
+    for (int t = 0; t < ntr; t += NT) {
+        const int nt = ntr - t < NT ? ntr - t : NT;
+        float c[NT][NZMAX], q[NT][NZMAX];
 #pragma unroll
-        for (int k = 0; k < NZMAX; k++)
-            if (k < nz) {
-                va[k] = fi0[(size_t)k * n2];
-                vb[k] = fi1[(size_t)k * n2];
-            }
+        for (int b = 0; b < NT; b++) {
+            const float* fi = fin + (size_t)(t + (b < nt ? b : 0)) * n3 + p;
+#pragma unroll
+            for (int k = 0; k < NZMAX; k++)
+                if (has(k)) c[b][k] = fi[(size_t)k * n2];
+        }
         if (nz == 1) {
             // diffusion.F90:224-231: every point, halos and land included
             const double dz = A.hn[p];
-            va[0] = float((dz * (double)va[0]) / dz);
-            vb[0] = float((dz * (double)vb[0]) / dz);
+#pragma unroll
+            for (int b = 0; b < NT; b++) c[b][0] = float((dz * (double)c[b][0]) / dz);
         } else if (solve) {
-            // forward (top-down) elimination, qu overwrites the column with a window of old values
-            float a_up = 0.f, qa_up = 0.f, b_up = 0.f, qb_up = 0.f;
+            float q_up[NT];
+#pragma unroll
+            for (int b = 0; b < NT; b++) q_up[b] = 0.f;
 #pragma unroll
             for (int k = NZMAX - 1; k >= 0; k--) {
-                if (k < nz) {
-                    const float c1 = g1[k * 64 + lane];
-                    const float c2 = k < nz - 1 ? g2[k * 64 + lane] : 0.f;
-                    const float ak = va[k], bk = vb[k];
-                    float qa = c1 * ak + c2 * qa_up;
-                    float qb = c1 * bk + c2 * qb_up;
+                if (has(k)) {
+                    const float ek = has_up(k) ? e2[k * 64 + lane] : 0.f;
+                    const float rk = ru[k * 64 + lane];  // 0 at k = 0
+                    float e3 = 0.f, e4 = 0.f;
                     if (!CN1) {
-                        if (k < nz - 1) {
-                            const float c3 = g3[k * 64 + lane];
-                            qa += c3 * (a_up - ak);
-                            qb += c3 * (b_up - bk);
-                        }
-                        if (k > 0) {
-                            const float c4 = g4[k * 64 + lane];
-                            qa += c4 * (va[k - 1 >= 0 ? k - 1 : 0] - ak);
-                            qb += c4 * (vb[k - 1 >= 0 ? k - 1 : 0] - bk);
-                        }
+                        e3 = has_up(k) ? g3[k * 64 + lane] : 0.f;
+                        e4 = g4[k * 64 + lane];
                     }
-                    va[k] = qa;
-                    vb[k] = qb;
-                    qa_up = qa;
-                    qb_up = qb;
-                    a_up = ak;
-                    b_up = bk;
+#pragma unroll
+                    for (int b = 0; b < NT; b++) {
+                        const float ck = c[b][k];
+                        const float dup = has_up(k) ? c[b][k + 1 < NZMAX ? k + 1 : k] - ck : 0.f;
+                        const float ddn = k > 0 ? c[b][k > 0 ? k - 1 : 0] - ck : 0.f;
+                        float qk = ek * (dup + q_up[b]) - rk * ddn;
+                        if (!CN1) qk += e3 * dup + e4 * ddn;
+                        q[b][k] = qk;
+                        q_up[b] = qk;
+                    }
                 }
             }
-            float pa = va[0], pb = vb[0];
+            // compiler-only barrier: ru is read again from LDS below instead of being
+            // kept in registers from the elimination (nz more live values per lane)
+            asm volatile("" ::: "memory");
+            float d[NT];
+#pragma unroll
+            for (int b = 0; b < NT; b++) {
+                d[b] = q[b][0];
+                c[b][0] += d[b];
+            }
 #pragma unroll
             for (int k = 1; k < NZMAX; k++) {
-                if (k < nz) {
+                if (has(k)) {
                     const float r = ru[k * 64 + lane];
-                    pa = va[k] - r * pa;
-                    pb = vb[k] - r * pb;
-                    va[k] = pa;
-                    vb[k] = pb;
+#pragma unroll
+                    for (int b = 0; b < NT; b++) {
+                        d[b] = q[b][k] - r * d[b];
+                        c[b][k] += d[b];
+                    }
                 }
             }
         }
-        float* fo0 = fout + (size_t)t * n3 + p;
-        float* fo1 = fout + (size_t)(t + 1) * n3 + p;
 #pragma unroll
-        for (int k = 0; k < NZMAX; k++)
-            if (k < nz) {
-                fo0[(size_t)k * n2] = va[k];
-                if (two) fo1[(size_t)k * n2] = vb[k];
+        for (int b = 0; b < NT; b++)
+            if (b < nt) {
+                float* fo = fout + (size_t)(t + b) * n3 + p;
+#pragma unroll
+                for (int k = 0; k < NZMAX; k++)
+                    if (has(k)) fo[(size_t)k * n2] = c[b][k];
             }
     }
 }
-
-// Single domain: blockIdx = (64-column chunk, row, tracer batch).
 template <int NZMAX, bool CN1>
 __global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, double cnpar, double molecular,
                                                   const double* __restrict__ nuh, int ntr_all, int tpb,
@@ -925,12 +950,12 @@ int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, cons
 int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
                      const float* fin, float* fout, hipStream_t s)
 {
-    if (A.nz > 64) return GETM_ERR_CAPACITY;
+    if (A.nz > 64 || A.nz < 1) return GETM_ERR_CAPACITY;
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
     if (nb > 65535) return GETM_ERR_CAPACITY;
     const bool cn1 = cnpar == 1.0;
-    const size_t shm = (size_t)(cn1 ? 3 : 5) * A.nz * 64 * sizeof(float);
+    const size_t shm = (size_t)(cn1 ? 2 : 4) * A.nz * 64 * sizeof(float);
     dim3 grid((A.nx_ + 63) / 64, A.ny_, nb);
     ProfileScope ps(GETM_K_VDIFF_F32, s);
 #define GETM_VD(NZM)                                                                                             \
@@ -942,10 +967,17 @@ int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecula
             hipLaunchKernelGGL((k_vdiff_f32<NZM, false>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, ntr, \
                                tpb, fin, fout);                                                                  \
     } while (0)
-    if (A.nz <= 16) GETM_VD(16);
-    else if (A.nz <= 32) GETM_VD(32);
-    else if (A.nz <= 48) GETM_VD(48);
-    else GETM_VD(64);
+    // depth class: the kernel is unrolled for 8 (cls + 1) layers and needs nz > 8 cls
+    switch ((A.nz - 1) / 8) {
+        case 0: GETM_VD(8); break;
+        case 1: GETM_VD(16); break;
+        case 2: GETM_VD(24); break;
+        case 3: GETM_VD(32); break;
+        case 4: GETM_VD(40); break;
+        case 5: GETM_VD(48); break;
+        case 6: GETM_VD(56); break;
+        default: GETM_VD(64); break;
+    }
 #undef GETM_VD
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
@@ -1000,11 +1032,11 @@ int launch_wcol_multi(int scheme, const EnsDom<float>* doms, const int4* work, i
     return GETM_OK;
 }
 
-int launch_vdiff_multi(bool cn1, int nzmax, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s)
+int launch_vdiff_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s)
 {
     if (nwork <= 0) return GETM_OK;
-    if (nzmax > 64) return GETM_ERR_CAPACITY;
-    const size_t shm = (size_t)(cn1 ? 3 : 5) * nzmax * 64 * sizeof(float);
+    if (cls < 0 || cls > 7) return GETM_ERR_CAPACITY;
+    const size_t shm = (size_t)(cn1 ? 2 : 4) * 8 * (cls + 1) * 64 * sizeof(float);
     ProfileScope ps(GETM_K_VDIFF_F32, s);
 #define GETM_VDM(NZM)                                                                                            \
     do {                                                                                                        \
@@ -1013,10 +1045,16 @@ int launch_vdiff_multi(bool cn1, int nzmax, const EnsDom<float>* doms, const int
         else                                                                                                    \
             hipLaunchKernelGGL((k_vdiff_multi<NZM, false>), dim3(nwork), dim3(64), shm, s, doms, work);         \
     } while (0)
-    if (nzmax <= 16) GETM_VDM(16);
-    else if (nzmax <= 32) GETM_VDM(32);
-    else if (nzmax <= 48) GETM_VDM(48);
-    else GETM_VDM(64);
+    switch (cls) {
+        case 0: GETM_VDM(8); break;
+        case 1: GETM_VDM(16); break;
+        case 2: GETM_VDM(24); break;
+        case 3: GETM_VDM(32); break;
+        case 4: GETM_VDM(40); break;
+        case 5: GETM_VDM(48); break;
+        case 6: GETM_VDM(56); break;
+        default: GETM_VDM(64); break;
+    }
 #undef GETM_VDM
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;

@@ -61,6 +61,9 @@ class Domain:
     scheme: int
     cnpar: float = 1.0
     name: str = ""
+    # sparse initial tracers (steady_release(dense=False)): rows (k, j, i, value), one per tracer,
+    # every other cell zero; ``tracers`` is then empty
+    releases: Optional[np.ndarray] = None
 
     @property
     def nx_(self) -> int:
@@ -72,7 +75,7 @@ class Domain:
 
     @property
     def ntracers(self) -> int:
-        return self.tracers.shape[0]
+        return self.tracers.shape[0] if self.releases is None else self.releases.shape[0]
 
     @property
     def n_wet_columns(self) -> int:
@@ -363,3 +366,93 @@ def ensemble_domains(ndomains=96, seed=3, size_range=(64, 768), nz_range=(10, 60
         out.append(coastal(nx=nx, ny=ny, nz=nz, dx=2000.0, dt=dt, scheme=scheme, seed=1000 + d, ntracers=nt,
                            tracer_kind="blobs"))
     return out
+
+
+def steady_release(nx=96, ny=80, nz=20, dx=2000.0, dt=30.0, scheme=5, seed=0, ntracers=1, amount=1e3,
+                   land_fraction=0.3, umax=0.5, baroclinic=0.3, dense=True) -> Domain:
+    """Dye-release scenario with fields that are valid for every step.
+
+    Land and depth as in :func:`coastal`.  Layer transports are a discretely
+    divergence-free barotropic part (from a corner streamfunction) plus a
+    baroclinic part a_k * P with sum_k a_k = 0, so every layer diverges but no
+    column does: the surface stays put, ``ho == hn`` and w follows from
+    continuity -- repeated steps conserve each tracer's inventory exactly.
+    Each tracer starts as ``amount`` units released in one random wet cell;
+    with ``dense=False`` these come as the ``releases`` table instead of a dense
+    ``tracers`` array."""
+    rng = np.random.default_rng(seed)
+    h = HALO
+    ny_, nx_ = ny + 2 * h, nx + 2 * h
+    sm = _smooth_noise(rng, (ny, nx), 8.0)
+    tmask = _pad((sm > np.quantile(sm, land_fraction)).astype(np.int32), h).astype(np.int32)
+    dep = _smooth_noise(rng, (ny, nx), 8.0)
+    H = np.where(tmask == 1, _pad(10.0 ** (1.0 + dep * (math.log10(200.0) - 1.0)), h), 0.0)
+    umask, vmask = uv_masks(tmask)
+    m = _metrics(ny_, nx_, dx, dx)
+    sigma = calculate_sigma(nz, 1.0, 1.0)
+    s = sigma[:, None, None]
+    D = np.where(tmask == 1, H, 0.0)
+    Du = np.zeros_like(D)
+    Du[:, :-1] = 0.5 * (D[:, :-1] + D[:, 1:])
+    Dv = np.zeros_like(D)
+    Dv[:-1, :] = 0.5 * (D[:-1, :] + D[1:, :])
+    hu, hv, ho = s * Du, s * Dv, s * D
+    # barotropic transports per unit depth from a streamfunction on cell corners (i+1/2, j+1/2)
+    psi = _smooth_noise(rng, (ny_, nx_), 6.0) - 0.5
+    # psi[j, i] sits on corner (i+1/2, j+1/2); zero on corners touching land so that
+    # closed faces carry no transport and every column stays divergence-free
+    land = tmask != 1
+    touch = land.copy()
+    touch[:, :-1] |= land[:, 1:]
+    touch[:-1, :] |= land[1:, :]
+    touch[:-1, :-1] |= land[1:, 1:]
+    psi = np.where(touch, 0.0, psi)
+    pu = np.zeros((ny_, nx_))
+    qv = np.zeros((ny_, nx_))
+    pu[1:, :] = -(psi[1:, :] - psi[:-1, :])   # U point (i+1/2, j): corners (i+1/2, j +- 1/2)
+    qv[:, 1:] = psi[:, 1:] - psi[:, :-1]      # V point (i, j+1/2): corners (i +- 1/2, j+1/2)
+    # baroclinic, zero-sum over layers: a_k = cos(pi (k + 1/2) / nz) sigma-weighted to sum 0
+    a = np.cos(np.pi * (np.arange(nz) + 0.5) / nz)
+    a = a - (a * sigma).sum() / sigma.sum()
+    P = _smooth_noise(rng, (ny_, nx_), 6.0) - 0.5
+    open_u, open_v = (umask == 1), (vmask == 1)
+    p = np.zeros((nz, ny_, nx_))
+    q = np.zeros((nz, ny_, nx_))
+    for k in range(nz):
+        p[k] = np.where(open_u, sigma[k] * pu + baroclinic * sigma[k] * a[k] * P, 0.0)
+        q[k] = np.where(open_v, sigma[k] * qv + baroclinic * sigma[k] * a[k] * np.roll(P, 1, axis=1), 0.0)
+    # velocities reproducing these transports: u = p / (hu dyu)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        u = np.where(open_u & (hu > 0), p / (hu * m["dyu"]), 0.0)
+        v = np.where(open_v & (hv > 0), q / (hv * m["dxv"]), 0.0)
+    sc = max(np.abs(u).max(), np.abs(v).max(), 1e-300)
+    u *= umax / sc
+    v *= umax / sc
+    p = np.where(open_u, (u * hu) * m["dyu"], 0.0)
+    q = np.where(open_v, (v * hv) * m["dxv"], 0.0)
+    div = np.zeros_like(p)
+    div[:, :, 1:] += p[:, :, 1:] - p[:, :, :-1]
+    div[:, 1:, :] += q[:, 1:, :] - q[:, :-1, :]
+    div *= m["iarea"]
+    wet = tmask == 1
+    div = np.where(wet, div, 0.0)
+    w = np.zeros((nz + 1, ny_, nx_))
+    w[1:nz] = -np.cumsum(div[:-1], axis=0)
+    w[:, ~wet] = 0.0
+    nuh = 10.0 ** rng.uniform(-6.0, -2.0, size=(nz + 1, ny_, nx_))
+    jj, ii = np.nonzero(wet)
+    rel = np.zeros((ntracers, 4))
+    for t in range(ntracers):
+        c = rng.integers(len(ii))
+        k = rng.integers(nz)
+        rel[t] = (k, jj[c], ii[c], amount / (m["area"][jj[c], ii[c]] * ho[k, jj[c], ii[c]]))
+    if dense:
+        tr = np.zeros((ntracers, nz, ny_, nx_))
+        for t, (k, j, i, val) in enumerate(rel):
+            tr[t, int(k), int(j), int(i)] = val
+    else:
+        tr = np.zeros((0, nz, ny_, nx_))
+    z = np.zeros((ny_, nx_))
+    return Domain(nx, ny, nz, h, tmask, umask, vmask, m["idxu"], m["dyu"], m["dxv"], m["idyv"], m["iarea"],
+                  m["area"], H, z, sigma, ho, ho.copy(), hu, hv, u, v, w, nuh, tr, dt, scheme, 1.0, "release",
+                  None if dense else rel)
