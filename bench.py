@@ -10,6 +10,12 @@ timed region.  Prints one JSON line.
 metric: tracer cell-updates/s = wet cells (mask == 1) x nz x tracers x steps / time.
 roofline: the dominant kernel's algorithmic bytes per launch / its mean launch
 time, timed with HIP events on its launch stream during the timed steps.
+traffic: HBM bytes per launch of that kernel from the rocprofv3 FETCH_SIZE and
+WRITE_SIZE counters, calibrated on a plain copy (getm_amd.hbm_copy), as recorded
+in profiles/hbm_traffic_*.md for this workload (null if none is recorded).
+serving: a short run of the serving loop (getm_amd.serve) on 24 dye-release
+requests: saturation throughput, then Poisson arrivals at 80% of it, with
+latency percentiles, inventory drift and the error against the FP64 method.
 cpu_baseline: the FP64 CPU oracle (C restatement of the reference Fortran) on
 bounded slabs of the same workload, one process per host core.
 """
@@ -29,6 +35,49 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 NX, NY, NZ = 1024, 768, 40
+METRIC = "tracer cell-updates/s (adv+diff step), % of HBM roofline, p50/p99 solve latency"
+
+
+def recorded_traffic(kernel: str, workload: str):
+    """Bytes per launch of ``kernel`` recorded for ``workload`` in profiles/hbm_traffic_*.md
+    (lines 'traffic <workload> <kernel> <bytes>'), or None."""
+    import glob
+    import re
+
+    for fn in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.md"))):
+        for line in open(fn):
+            m = re.match(r"^traffic\s+(\S+)\s+(\S+)\s+([0-9.eE+]+)\s*$", line.strip())
+            if m and m.group(1) == workload and m.group(2) == kernel:
+                return float(m.group(3)), os.path.relpath(fn, ROOT)
+    return None
+
+
+def serving_run(nreq: int, log):
+    """Serving loop on a mix of dye-release requests (sizes 64-384, 10-60 layers,
+    1-64 tracers, 50-200 steps, SUPERBEE or HSIMT): saturation, FP64 check, 80% load."""
+    from getm_amd import serve
+
+    reqs = serve.draw_requests(nreq, seed=0, size_range=(64, 384), nz_range=(10, 60), tracer_range=(1, 64),
+                               step_range=(50, 200))
+    domains = {r.rid: r.domain() for r in reqs}
+    server = serve.Server(hbm_budget=250e9, max_slots=64)
+    rep = serve.sweep(reqs, domains, [0.8], server, verify_results=True, log=log)
+    sat, run = rep["saturation"], rep["sweep"][0]
+    return {
+        "requests": f"{nreq} dye releases: nx, ny logU[64,384], nz U[10,60], 1-64 tracers, 50-200 steps, "
+                    "SUPERBEE/HSIMT, FP32 with FP64 h and per-request inventory fixer",
+        "saturation_updates_per_s": sat["tracer_cell_updates_per_s"],
+        "load": run["fraction_of_saturation"],
+        "arrival_rate_per_s": run["arrival_rate_per_s"],
+        "updates_per_s": run["tracer_cell_updates_per_s"],
+        "latency_p50_s": run["latency_p50_s"],
+        "latency_p99_s": run["latency_p99_s"],
+        "latency_p999_s": run["latency_p999_s"],
+        "slowdown_p50": run["slowdown_p50"],
+        "slowdown_p99": run["slowdown_p99"],
+        "max_inventory_drift": max(sat["max_inventory_drift"], run["max_inventory_drift"]),
+        "max_error_vs_fp64": sat.get("max_error_vs_fp64"),
+    }
 
 
 def algorithmic_bytes(kernel: str, n_wet_cols: int, nz: int, s: int, tracers: int) -> float:
@@ -134,6 +183,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop run")
+    ap.add_argument("--serve-requests", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the step on a single undivided domain; "
@@ -211,6 +262,16 @@ def main():
     print("[bench] kernel times (ms total, launches): " +
           ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in ktimes.items()), file=sys.stderr, flush=True)
 
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    workload_key = f"{args.workload}-{args.precision}-T{args.tracers}-s{args.scheme}" + (
+        "-pertracer" if args.per_tracer else "")
+    rec = recorded_traffic(dom, workload_key)
+    serving = None
+    if not args.no_serve:
+        del tc, u, v, w, nuh, grid
+        torch.cuda.empty_cache()
+        serving = serving_run(args.serve_requests, log)
+
     cpu = None
     if not args.no_cpu:
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
@@ -221,7 +282,7 @@ def main():
                          f"~{args.cpu_seconds:.0f} s each"}
 
     out = {
-        "metric": "tracer cell-updates/s (adv+diff step)",
+        "metric": METRIC,
         "value": value,
         "unit": "tracer cell-updates/s",
         "n_gpus": 1,
@@ -246,6 +307,7 @@ def main():
             "step_hbm_fraction": b_step / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
             "kernel_ms_per_step": {k: v[0] / args.steps for k, v in ktimes.items()},
             "tuning": args.tuning or "default",
+            "key": workload_key,
         },
         "roofline": {
             "bound": "hbm",
@@ -254,8 +316,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": rec[0] if rec else None,
+            "traffic_source": rec[1] if rec else None,
+            "algorithmic_bytes": alg,
         },
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -599,10 +599,11 @@ int ensemble_commit(void* p, void* stream)
         const int nbw = (D.ntr + D.tpb_w - 1) / D.tpb_w;
         const int nbd = (D.ntr + D.tpb_d - 1) / D.tpb_d;
         if (nbh > 32767 || t->nz > 65535) return GETM_ERR_CAPACITY;
-        for (int b = 0; b < nbh; b++)
-            for (int k = 0; k < t->nz; k++)
-                for (int y0 = 0; y0 < t->ny_; y0 += e->tile_y)
-                    for (int x0 = 0; x0 < t->nx_; x0 += 64) wh.push_back(make_int4(d, x0, y0, k | (b << 16)));
+        // logical order as k_hpair: tracer batch fastest, then layer, then tile
+        for (int y0 = 0; y0 < t->ny_; y0 += e->tile_y)
+            for (int x0 = 0; x0 < t->nx_; x0 += 64)
+                for (int k = 0; k < t->nz; k++)
+                    for (int b = 0; b < nbh; b++) wh.push_back(make_int4(d, x0, y0, k | (b << 16)));
         for (int b = 0; b < nbw; b++)
             for (int by = 0; by < (t->ny_ + 3) / 4; by++)
                 for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) ww.push_back(make_int4(d, bx, by, b));
@@ -611,6 +612,17 @@ int ensemble_commit(void* p, void* stream)
         for (int b = 0; b < nbd; b++)
             for (int j = 0; j < t->ny_; j++)
                 for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) wd.push_back(make_int4(d, bx, j, b));
+    }
+    // horizontal-stage table in XCD-aware launch order: block b runs logical item
+    // (b % 8) * (n / 8) + b / 8 (n padded to a multiple of 8 with domain -1)
+    {
+        const size_t per = (wh.size() + 7) / 8;
+        std::vector<int4> launch(8 * per, make_int4(-1, 0, 0, 0));
+        for (size_t b = 0; b < launch.size(); b++) {
+            const size_t v = (b % 8) * per + b / 8;
+            if (v < wh.size()) launch[b] = wh[v];
+        }
+        wh.swap(launch);
     }
     std::vector<int4> wd;
     for (int c = 0; c < 8; c++) {

@@ -496,21 +496,37 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
     }
 }
 
-// Single domain: blockIdx = (x tile, y tile, layer + nz * tracer batch); every
+// Single domain, 1-D grid in XCD-aware order.  Work-groups go to the 8 XCDs
+// round-robin (block b to XCD b % 8), and each XCD has its own L2, so block b
+// takes logical item v = (b % 8) * (gridDim.x / 8) + b / 8: every XCD walks a
+// contiguous range of the logical order (tracer batch fastest, then layer, then
+// tile).  The blocks resident on one XCD are then the same tile at consecutive
+// layers, and the tile's 2-D fields (3 masks, 5 metrics) come from that XCD's
+// L2 instead of being fetched again from HBM for every layer.  Every
 // tracer-batch block writes the same thickness.
 template <int S, typename R, int D1, int TY>
-__global__ void __launch_bounds__(TX * TY) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb,
-                                                   const R* __restrict__ fin_all, R* __restrict__ fout_all,
-                                                   const double* __restrict__ hin, double* __restrict__ hout)
+__global__ void __launch_bounds__(TX * TY) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
+                                                   int nty, int nbatch, const R* __restrict__ fin_all,
+                                                   R* __restrict__ fout_all, const double* __restrict__ hin,
+                                                   double* __restrict__ hout)
 {
-    const int k = blockIdx.z % A.nz, tbatch = blockIdx.z / A.nz;
+    const unsigned b = blockIdx.x, per = gridDim.x / 8;
+    unsigned v = (b % 8) * per + b / 8;
+    const unsigned nitems = (unsigned)nbatch * A.nz * ntx * nty;
+    if (v >= nitems) return;
+    const int tbatch = v % nbatch;
+    v /= nbatch;
+    const int k = v % A.nz;
+    v /= A.nz;
+    const int tx = v % ntx, ty = v / ntx;
     const int tb0 = tbatch * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
-    hpair_body<S, R, D1, TY>(A, dt, blockIdx.x * TX, blockIdx.y * TY, k, ntr, fin_all + (size_t)tb0 * A.n3,
+    hpair_body<S, R, D1, TY>(A, dt, tx * TX, ty * TY, k, ntr, fin_all + (size_t)tb0 * A.n3,
                              fout_all + (size_t)tb0 * A.n3, hin, hout);
 }
 
-// Ensemble: one work item per block {domain, x0, y0, layer | batch << 16}.
+// Ensemble: one work item per block {domain, x0, y0, layer | batch << 16}; the
+// table is built in the XCD-aware order of k_hpair (domain -1 = padding).
 // STAGE 1: stages 1-2 (tracers -> scratch, ho -> hA); STAGE 2: stages 4-5
 // (tracers -> scratch, hB -> hA).
 template <int S, typename R, int D1, int TY, int STAGE>
@@ -518,6 +534,7 @@ __global__ void __launch_bounds__(TX * TY) k_hpair_multi(const EnsDom<R>* __rest
                                                          const int4* __restrict__ work)
 {
     const int4 wk = work[blockIdx.x];
+    if (wk.x < 0) return;  // padding of the XCD-ordered table
     const EnsDom<R>& D = doms[wk.x];
     const int k = wk.w & 0xffff, tb0 = (wk.w >> 16) * D.tpb_h;
     const int ntr = D.ntr - tb0 < D.tpb_h ? D.ntr - tb0 : D.tpb_h;
@@ -904,18 +921,20 @@ int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int n
 {
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
-    if ((long)A.nz * nb > 65535) return GETM_ERR_CAPACITY;
     if (ty != 4 && ty != 8 && ty != 16) return GETM_ERR_ARG;
-    dim3 grid((A.nx_ + TX - 1) / TX, (A.ny_ + ty - 1) / ty, A.nz * nb);
+    const int ntx = (A.nx_ + TX - 1) / TX, nty = (A.ny_ + ty - 1) / ty;
+    const long nitems = (long)nb * A.nz * ntx * nty;
+    if (nitems > 0x7fffff00L) return GETM_ERR_CAPACITY;
+    dim3 grid((unsigned)(8 * ((nitems + 7) / 8)));
     ProfileScope ps(d1 == 2 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
 #define GETM_HP(TYV)                                                                                           \
     GETM_SCHEME_SWITCH(scheme, {                                                                               \
         if (d1 == 2)                                                                                           \
-            hipLaunchKernelGGL((k_hpair<S_, R, 2, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, fin, fout, hin, \
-                               hout);                                                                          \
+            hipLaunchKernelGGL((k_hpair<S_, R, 2, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, nb, \
+                               fin, fout, hin, hout);                                                          \
         else                                                                                                   \
-            hipLaunchKernelGGL((k_hpair<S_, R, 1, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, fin, fout, hin, \
-                               hout);                                                                          \
+            hipLaunchKernelGGL((k_hpair<S_, R, 1, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, nb, \
+                               fin, fout, hin, hout);                                                          \
     })
     if (ty == 16) {
         GETM_HP(16);

@@ -313,6 +313,61 @@ def summary(results: Sequence[Result], wall: float, rate: Optional[float] = None
     return out
 
 
+def sweep(reqs: Sequence[Request], domains: Dict[int, inputs.Domain], fractions: Sequence[float],
+          server: Server, verify_results: bool = True, seed: int = 0, log=None) -> dict:
+    """Saturation run (every request arrives at t = 0), optionally checked against
+    the FP64 method, then one Poisson-arrival run per fraction of the measured
+    saturation throughput.  Returns the report as a dict."""
+    log = log or (lambda m: None)
+    # warm-up (kernel loading)
+    server.run([dataclasses.replace(reqs[0], rid=-1, nsteps=2)], {-1: domains[reqs[0].rid]})
+    torch.cuda.synchronize()
+    keep = server.keep_results
+    server.keep_results = verify_results
+    t = time.perf_counter()
+    sat = server.run(reqs, domains, log=log)
+    wall = time.perf_counter() - t
+    server.keep_results = keep
+    s0 = summary(sat, wall)
+    sat_rate = s0["tracer_cell_updates_per_s"]
+    s0.update(summary(sat, wall, sat_rate))
+    work = [r.wet_updates for r in sat if not r.error]
+    report = {"requests": {"n": len(reqs), "mean_work_updates": float(np.mean(work)),
+                           "budget_gb": server.hbm_budget / 1e9, "max_slots": server.max_slots},
+              "saturation": s0, "sweep": []}
+    log(f"[serve] saturation {sat_rate:.3e} tracer cell-updates/s over {wall:.1f} s, "
+        f"p50 {s0['latency_p50_s']:.3f} s, drift {s0['max_inventory_drift']:.1e}")
+    errs = {}
+    if verify_results:
+        errs = verify(sat, reqs, domains)
+        s0["max_error_vs_fp64"] = max(errs.values()) if errs else float("nan")
+        s0["verified"] = len(errs)
+        log(f"[serve] {len(errs)} results checked against FP64: max error {s0['max_error_vs_fp64']:.2e}")
+    by_id = {r.rid: r for r in reqs}
+    s0["per_request"] = [
+        {"rid": r.rid, "nx": by_id[r.rid].nx, "ny": by_id[r.rid].ny, "nz": by_id[r.rid].nz,
+         "tracers": by_id[r.rid].ntracers, "steps": r.steps, "scheme": by_id[r.rid].scheme, "latency_s": r.latency,
+         "inventory_drift": r.inventory_drift, "error_vs_fp64": errs.get(r.rid)}
+        for r in sorted(sat, key=lambda r: r.rid) if not r.error]
+    del sat
+    for frac in fractions:
+        lam = frac * sat_rate / float(np.mean(work))  # requests per second
+        rng = np.random.default_rng(seed + 1)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(reqs)))
+        batch = [dataclasses.replace(r, arrival=float(a)) for r, a in zip(reqs, arrivals)]
+        t = time.perf_counter()
+        res = server.run(batch, domains, log=log)
+        wall = time.perf_counter() - t
+        s = summary(res, wall, sat_rate)
+        s["fraction_of_saturation"] = frac
+        s["arrival_rate_per_s"] = lam
+        report["sweep"].append(s)
+        log(f"[serve] load {frac:.2f}: p50 {s['latency_p50_s']:.3f} s, p99 {s['latency_p99_s']:.3f} s, "
+            f"p99.9 {s['latency_p999_s']:.3f} s, slowdown p50 {s['slowdown_p50']:.2f} p99 {s['slowdown_p99']:.2f}, "
+            f"{s['tracer_cell_updates_per_s']:.3e} updates/s, max drift {s['max_inventory_drift']:.1e}")
+    return report
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="Serving-loop sweep: saturation throughput, then Poisson arrivals "
                                              "at fractions of it, with latency percentiles.")
@@ -336,52 +391,11 @@ def main(argv=None):
     log = lambda m: print(m, file=sys.stderr, flush=True)
     log(f"[serve] generating {len(reqs)} scenarios")
     domains = {r.rid: r.domain() for r in reqs}
-    server = Server(hbm_budget=args.budget_gb * 1e9, max_slots=args.max_slots, keep_results=not args.no_verify,
-                    conserve=not args.no_conserve)
-    # warm-up (kernel loading), then saturation: everything arrives at t = 0
-    server.run([dataclasses.replace(reqs[0], rid=-1, nsteps=2)], {-1: domains[reqs[0].rid]})
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    sat = server.run(reqs, domains, log=log)
-    wall = time.perf_counter() - t
-    s0 = summary(sat, wall)
-    sat_rate = s0["tracer_cell_updates_per_s"]
-    s0.update(summary(sat, wall, sat_rate))
-    work = [r.wet_updates for r in sat if not r.error]
-    report = {"requests": {"n": len(reqs), "size": args.size, "layers": args.layers, "tracers": args.tracers,
-                           "steps": args.steps, "mean_work_updates": float(np.mean(work)),
-                           "budget_gb": args.budget_gb, "max_slots": args.max_slots},
-              "saturation": s0, "sweep": []}
-    log(f"[serve] saturation {sat_rate:.3e} tracer cell-updates/s over {wall:.1f} s, "
-        f"p50 {s0['latency_p50_s']:.3f} s, drift {s0['max_inventory_drift']:.1e}")
-    if not args.no_verify:
-        errs = verify(sat, reqs, domains)
-        report["saturation"]["max_error_vs_fp64"] = max(errs.values()) if errs else float("nan")
-        report["saturation"]["verified"] = len(errs)
-        log(f"[serve] {len(errs)} results checked against FP64: max error {max(errs.values()):.2e}")
-    by_id = {r.rid: r for r in reqs}
-    report["saturation"]["per_request"] = [
-        {"rid": r.rid, "nx": by_id[r.rid].nx, "ny": by_id[r.rid].ny, "nz": by_id[r.rid].nz,
-         "tracers": by_id[r.rid].ntracers, "steps": r.steps, "scheme": by_id[r.rid].scheme, "latency_s": r.latency,
-         "inventory_drift": r.inventory_drift, "error_vs_fp64": None if args.no_verify else errs.get(r.rid)}
-        for r in sorted(sat, key=lambda r: r.rid) if not r.error]
-    del sat
-    server.keep_results = False
-    for frac in [float(x) for x in args.fractions.split(",") if x]:
-        lam = frac * sat_rate / float(np.mean(work))  # requests per second
-        rng = np.random.default_rng(args.seed + 1)
-        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(reqs)))
-        batch = [dataclasses.replace(r, arrival=float(a)) for r, a in zip(reqs, arrivals)]
-        t = time.perf_counter()
-        res = server.run(batch, domains, log=log)
-        wall = time.perf_counter() - t
-        s = summary(res, wall, sat_rate)
-        s["fraction_of_saturation"] = frac
-        s["arrival_rate_per_s"] = lam
-        report["sweep"].append(s)
-        log(f"[serve] load {frac:.2f}: p50 {s['latency_p50_s']:.3f} s, p99 {s['latency_p99_s']:.3f} s, "
-            f"p99.9 {s['latency_p999_s']:.3f} s, slowdown p50 {s['slowdown_p50']:.2f} p99 {s['slowdown_p99']:.2f}, "
-            f"{s['tracer_cell_updates_per_s']:.3e} updates/s, max drift {s['max_inventory_drift']:.1e}")
+    server = Server(hbm_budget=args.budget_gb * 1e9, max_slots=args.max_slots, conserve=not args.no_conserve)
+    report = sweep(reqs, domains, [float(x) for x in args.fractions.split(",") if x], server,
+                   verify_results=not args.no_verify, seed=args.seed, log=log)
+    report["requests"].update(size=args.size, layers=args.layers, tracers=args.tracers, steps=args.steps,
+                              conserve=not args.no_conserve)
     print(json.dumps(report))
     if args.out:
         with open(args.out, "w") as fh:
