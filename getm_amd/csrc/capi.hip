@@ -40,6 +40,8 @@ extern "C" {
 void* create_grid(int imin, int imax, int jmin, int jmax, int kmin, int kmax, int halox, int haloy)
 {
     if (imax < imin || jmax < jmin || kmax < kmin || halox < 0 || haloy < 0) return nullptr;
+    // in-layer byte offsets of FP64 fields are 32-bit in the kernels
+    if ((long long)(imax - imin + 1 + 2 * halox) * (jmax - jmin + 1 + 2 * haloy) * 8 >= (1LL << 32)) return nullptr;
     Grid* g = new (std::nothrow) Grid();
     if (!g) return nullptr;
     g->imin = imin;

@@ -76,6 +76,33 @@ __device__ __forceinline__ R face_value(R cfl, R fu, R fuu, R fd)
 
 __device__ __forceinline__ bool open_face(int m) { return m == 1 || m == 2; }
 
+// 1/x to about one ulp of double for x > 0: the hardware estimate refined by two
+// Newton steps.  Used only where the result is rounded to FP32 afterwards (and
+// x is a positive layer thickness), never on the bit-exact FP64 path.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+// Element at a 32-bit byte offset from a wave-uniform base: lets the compiler
+// use the scalar-base + 32-bit-offset form of global loads and stores instead
+// of 64-bit address arithmetic per lane.  Callers keep byte offsets below 2^32
+// (in-layer offsets: grids have nx_ ny_ * 8 < 2^32).
+template <typename T>
+__device__ __forceinline__ const T& at32(const T* base, unsigned byte_off)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ T& at32(T* base, unsigned byte_off)
+{
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
+}
+
 }  // namespace getm
 
 // Instantiate a templated launcher for every scheme; `body` is called with a

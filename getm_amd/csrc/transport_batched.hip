@@ -67,22 +67,23 @@ __device__ __forceinline__ bool face_in_loop(const StepArgs& A, int gi, int gj)
 }
 
 // Face (gi, gj) of direction DIR (1: U face east of T(gi,gj); 2: V face north of it).
+// In-layer offsets are 32-bit (grids have nx_ ny_ < 2^31) and every base pointer
+// is wave-uniform, so the loads take a scalar base and a 32-bit lane offset.
 template <int DIR>
-__device__ __forceinline__ void face_load(const StepArgs& A, int gi, int gj, int k, FaceRaw& r)
+__device__ __forceinline__ void face_load(const StepArgs& A, int gi, int gj, size_t lay, FaceRaw& r)
 {
     r.inloop = face_in_loop<DIR>(A, gi, gj);
     if (!r.inloop) return;
-    const size_t p = (size_t)gj * A.nx_ + gi;
-    const size_t p3 = (size_t)k * A.n2 + p;
-    const ptrdiff_t s = DIR == 1 ? 1 : A.nx_;
+    const unsigned p = (unsigned)(gj * A.nx_ + gi);
+    const unsigned s = DIR == 1 ? 1u : (unsigned)A.nx_;  // p - s >= 0 inside the face loops (halo >= 2)
     const int* mask = DIR == 1 ? A.umask : A.vmask;
-    r.m0 = mask[p];
-    r.mlo = mask[p - s];
-    r.mhi = mask[p + s];
-    r.vel = DIR == 1 ? A.u[p3] : A.v[p3];
-    r.hf = DIR == 1 ? A.hu[p3] : A.hv[p3];
-    r.mq = DIR == 1 ? A.dyu[p] : A.dxv[p];   // metric of the transport
-    r.mc = DIR == 1 ? A.idxu[p] : A.idyv[p];  // inverse metric of the CFL number
+    r.m0 = at32(mask, 4 * p);
+    r.mlo = at32(mask, 4 * (p - s));
+    r.mhi = at32(mask, 4 * (p + s));
+    r.vel = at32(DIR == 1 ? A.u + lay : A.v + lay, 8 * p);
+    r.hf = at32(DIR == 1 ? A.hu + lay : A.hv + lay, 8 * p);
+    r.mq = at32(DIR == 1 ? A.dyu : A.dxv, 8 * p);   // metric of the transport
+    r.mc = at32(DIR == 1 ? A.idxu : A.idyv, 8 * p);  // inverse metric of the CFL number
 }
 
 // QU = u*hu*dyu, cfl = |u*dt*idxu| (template:64-65; QV/cfl likewise :164-165)
@@ -171,7 +172,9 @@ struct LimF {
         } else {
             const float deltaf = fd - fu;
             const float deltafu = fu - fuu;
-            const float r = deltafu * __frcp_rn(deltaf);
+            // hardware reciprocal (1 ulp): deltaf == 0 gives inf/NaN here, but
+            // then deltaf * deltafu > 0 fails and fu is selected below
+            const float r = deltafu * __builtin_amdgcn_rcpf(deltaf);
             float lim;
             if constexpr (S == SUPERBEE) lim = fmax2(fmin2(2.f * r, 1.f), fmin2(r, 2.f));
             else if constexpr (S == MUSCL) lim = fmin3(2.f * r, 0.5f * (1.f + r), 2.f);
@@ -249,8 +252,9 @@ __device__ __forceinline__ void cell_set(CellK<R>& c, bool upd, double hold, dou
         c.b = hnew;
         c.iA = iA;
     } else {
-        c.a = float((hold - hnew) / hnew);
-        c.b = float((dt * iA) / hnew);
+        const double rh = rcp_nr(hnew);
+        c.a = float((hold - hnew) * rh);
+        c.b = float((dt * iA) * rh);
         c.iA = 0.f;
     }
 }
@@ -308,10 +312,10 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
     auto cell_load = [&](int gi, int gj, CellRaw& r) {
         r.inarr = inarr(gi, gj);
         if (!r.inarr) return;
-        const size_t p = (size_t)gj * A.nx_ + gi;
-        r.h0 = hin[lay + p];
-        r.iA = A.iarea[p];
-        r.tm = A.tmask[p];
+        const unsigned p = (unsigned)(gj * A.nx_ + gi);
+        r.h0 = at32(hin + lay, 8 * p);
+        r.iA = at32(A.iarea, 8 * p);
+        r.tm = at32(A.tmask, 4 * p);
     };
     auto wet = [&](int gi, int gj, int tm) {
         return gi >= A.imin && gi <= A.imax && gj >= A.jmin && gj <= A.jmax && tm == 1;
@@ -321,18 +325,29 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
     FaceRaw rfa[NPFA], rfb[NPFB];
     CellRaw rca[NPA], rcb[NPB];
     R pf[NPIN];
+    // in-layer offsets of this lane's input-tile cells (-1 outside the array) and
+    // output cells (-1 outside), formed once: the tracer loop only adds them to
+    // the tracer's layer base, a wave-uniform pointer
+    int toff[NPIN], ooff[NPB];
+#pragma unroll
+    for (int m = 0; m < NPIN; m++) {
+        const int c = tid + m * NTHR;
+        toff[m] = -1;
+        if (c < NT) {
+            const int gi = x0 - 2 + c % HX, gj = y0 - 2 + c / HX;
+            if (inarr(gi, gj)) toff[m] = gj * A.nx_ + gi;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NPB; m++) {
+        const int c = tid + m * NTHR;
+        const int gi = x0 + c % TX, gj = y0 + c / TX;
+        ooff[m] = (gi < A.nx_ && gj < A.ny_) ? gj * A.nx_ + gi : -1;
+    }
     auto load_tile = [&](int t) {
         const R* fi = fin + (size_t)t * A.n3 + lay;
 #pragma unroll
-        for (int m = 0; m < NPIN; m++) {
-            const int c = tid + m * NTHR;
-            R val = R(0);
-            if (c < NT) {
-                const int gi = x0 - 2 + c % HX, gj = y0 - 2 + c / HX;
-                if (inarr(gi, gj)) val = fi[(size_t)gj * A.nx_ + gi];
-            }
-            pf[m] = val;
-        }
+        for (int m = 0; m < NPIN; m++) pf[m] = toff[m] >= 0 ? at32(fi, (unsigned)toff[m] * (unsigned)sizeof(R)) : R(0);
     };
     auto store_tile = [&](int buf) {
 #pragma unroll
@@ -345,13 +360,13 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
     for (int m = 0; m < NPFA; m++) {
         const int c = tid + m * NTHR;
         rfa[m].inloop = 0;
-        if (c < NFA) face_load<D1>(A, x0 + FAX0 + c % FAW, y0 + FAY0 + c / FAW, k, rfa[m]);
+        if (c < NFA) face_load<D1>(A, x0 + FAX0 + c % FAW, y0 + FAY0 + c / FAW, lay, rfa[m]);
     }
 #pragma unroll
     for (int m = 0; m < NPFB; m++) {
         const int c = tid + m * NTHR;
         rfb[m].inloop = 0;
-        if (c < NFB) face_load<D2>(A, x0 + FBX0 + c % FBW, y0 + FBY0 + c / FBW, k, rfb[m]);
+        if (c < NFB) face_load<D2>(A, x0 + FBX0 + c % FBW, y0 + FBY0 + c / FBW, lay, rfb[m]);
     }
 #pragma unroll
     for (int m = 0; m < NPA; m++) {
@@ -429,7 +444,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             double h2 = h1;
             if (upd) h2 = h1 - (dt * (s_q[f1] - s_q[f1 - (D2 == 1 ? 1 : FBW)])) * rcb[m].iA;
             cell_set<R>(cb[m], upd, h1, h2, rcb[m].iA, dt);
-            hout[lay + (size_t)(y0 + yl) * A.nx_ + x0 + xl] = h2;
+            at32(hout + lay, 8u * (unsigned)((y0 + yl) * A.nx_ + x0 + xl)) = h2;
         }
     }
 
@@ -486,11 +501,9 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
         R* fo = fout + (size_t)t * A.n3 + lay;
 #pragma unroll
         for (int m = 0; m < NPB; m++) {
-            const int c = tid + m * NTHR;
-            const int gi = x0 + c % TX, gj = y0 + c / TX;
             R out = s_a[cb[m].own];
             if (cb[m].upd) out = cell_update<R>(cb[m], dt, out, s_fb[cb[m].fl1 - (D2 == 1 ? 1 : FBW)], s_fb[cb[m].fl1]);
-            if (gi < A.nx_ && gj < A.ny_) fo[(size_t)gj * A.nx_ + gi] = out;
+            if (ooff[m] >= 0) at32(fo, (unsigned)ooff[m] * (unsigned)sizeof(R)) = out;
         }
         __syncthreads();
     }
@@ -688,7 +701,7 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
                     }
                 } else {
                     // increment form, as in cell_update: f + ((h0 - hnew)/hnew f - dt/hnew dF)
-                    const double rh = 1.0 / hnew;
+                    const double rh = rcp_nr(hnew);
                     const R ca = R((h0 - hnew) * rh), cb = R(dtk * rh);
 #pragma unroll
                     for (int b = 0; b < TB; b++)
