@@ -348,11 +348,12 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     const double half = 0.5 * dt;
     const int d1 = tr->ufirst ? 1 : 2;  // first direction of stages 1-2 (operators.py:118-121)
     const int d2 = 3 - d1;
-    if ((rc = launch_hpair<R>(d1, tr->scheme, tr->tile_y, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, s))) return rc;
+    const int ty = tr->tile_y ? tr->tile_y : auto_tile_y(ntr);
+    if ((rc = launch_hpair<R>(d1, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, s))) return rc;
     if ((rc = launch_wcol<R>(tr->scheme, A, dt, ntr, tr->tpb_w, S, f, hA, hB, w, w, static_cast<int*>(tr->stats.ptr),
                              s)))
         return rc;
-    if ((rc = launch_hpair<R>(d2, tr->scheme, tr->tile_y, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, s))) return rc;
+    if ((rc = launch_hpair<R>(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, s))) return rc;
     if constexpr (std::is_same<R, float>::value) {
         return launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
     } else {
@@ -450,7 +451,7 @@ int transport_set_tuning(void* p, int tile_y, int tracers_per_block_h, int trace
 {
     Transport* tr = static_cast<Transport*>(p);
     if (!tr || tracers_per_block_h < 0 || tracers_per_block_w < 0 || tracers_per_block_d < 0) return GETM_ERR_ARG;
-    if (tile_y != 4 && tile_y != 8 && tile_y != 16) return GETM_ERR_ARG;
+    if (tile_y != 0 && tile_y != 4 && tile_y != 8 && tile_y != 16) return GETM_ERR_ARG;
     tr->tile_y = tile_y;
     tr->tpb_h = tracers_per_block_h;
     tr->tpb_w = tracers_per_block_w;
@@ -549,7 +550,8 @@ int ensemble_bind(void* p, int d, float* tracers, const double* u, const double*
 int ensemble_set_tuning(void* p, int tile_y, int tpb_h, int tpb_w, int tpb_d)
 {
     Ensemble* e = static_cast<Ensemble*>(p);
-    if (!e || tpb_h < 0 || tpb_w < 0 || tpb_d < 0 || (tile_y != 4 && tile_y != 8 && tile_y != 16)) return GETM_ERR_ARG;
+    if (!e || tpb_h < 0 || tpb_w < 0 || tpb_d < 0 || (tile_y != 0 && tile_y != 4 && tile_y != 8 && tile_y != 16))
+        return GETM_ERR_ARG;
     e->tile_y = tile_y;
     e->tpb_h = tpb_h;
     e->tpb_w = tpb_w;
@@ -573,11 +575,14 @@ int ensemble_commit(void* p, void* stream)
     if (!e) return GETM_ERR_ARG;
     (void)stream;
     size_t nscr = 0, nh = 0;
+    long ntr_all = 0;
     for (int d = 0; d < e->ndom; d++) {
         if (!e->host[d].tracers) return GETM_ERR_ARG;  // not bound
         nscr += e->tg[d]->n3() * (size_t)e->host[d].ntr;
         nh += 2 * e->tg[d]->n3();
+        ntr_all += e->host[d].ntr;
     }
+    e->tile_y_used = e->tile_y ? e->tile_y : auto_tile_y((int)((ntr_all + e->ndom - 1) / e->ndom));
     int rc;
     if ((rc = e->scratch.ensure(nscr * sizeof(float))) || (rc = e->hbuf.ensure(nh * sizeof(double)))) return rc;
     float* scr = static_cast<float*>(e->scratch.ptr);
@@ -602,7 +607,7 @@ int ensemble_commit(void* p, void* stream)
         const int nbd = (D.ntr + D.tpb_d - 1) / D.tpb_d;
         if (nbh > 32767 || t->nz > 65535) return GETM_ERR_CAPACITY;
         // logical order as k_hpair: tracer batch fastest, then layer, then tile
-        for (int y0 = 0; y0 < t->ny_; y0 += e->tile_y)
+        for (int y0 = 0; y0 < t->ny_; y0 += e->tile_y_used)
             for (int x0 = 0; x0 < t->nx_; x0 += 64)
                 for (int k = 0; k < t->nz; k++)
                     for (int b = 0; b < nbh; b++) wh.push_back(make_int4(d, x0, y0, k | (b << 16)));
@@ -652,11 +657,11 @@ int ensemble_step(void* p, void* stream)
     const EnsDom<float>* doms = static_cast<const EnsDom<float>*>(e->doms.ptr);
     const int d1 = e->ufirst ? 1 : 2;
     int rc;
-    if ((rc = launch_hpair_multi(1, d1, e->scheme, e->tile_y, doms, static_cast<const int4*>(e->work_h.ptr), e->nwork_h,
+    if ((rc = launch_hpair_multi(1, d1, e->scheme, e->tile_y_used, doms, static_cast<const int4*>(e->work_h.ptr), e->nwork_h,
                                  s)))
         return rc;
     if ((rc = launch_wcol_multi(e->scheme, doms, static_cast<const int4*>(e->work_w.ptr), e->nwork_w, s))) return rc;
-    if ((rc = launch_hpair_multi(2, 3 - d1, e->scheme, e->tile_y, doms, static_cast<const int4*>(e->work_h.ptr),
+    if ((rc = launch_hpair_multi(2, 3 - d1, e->scheme, e->tile_y_used, doms, static_cast<const int4*>(e->work_h.ptr),
                                  e->nwork_h, s)))
         return rc;
     const int4* wd = static_cast<const int4*>(e->work_d.ptr);

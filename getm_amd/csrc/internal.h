@@ -116,7 +116,7 @@ struct Transport {
     DeviceBuffer stats;    // int32[2] as in Advection
     DeviceBuffer auxo, auxn, ru;  // FP64 diffusion (precision GETM_F64)
     int tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tracers per block of the three kernel kinds (0 = all)
-    int tile_y = 16;                        // height of the horizontal-stage tiles (4, 8 or 16)
+    int tile_y = 0;                         // height of the horizontal-stage tiles (4, 8, 16; 0 = by tracer count)
     DeviceBuffer part, part2, ratio, resid;  // inventory partial sums, fixer ratios and residuals
 };
 
@@ -125,7 +125,8 @@ struct Transport {
 struct Ensemble {
     int ndom = 0, scheme = 5, ufirst = 0;
     double cnpar = 1.0;
-    int tile_y = 16, tpb_h = 0, tpb_w = 16, tpb_d = 16;
+    int tile_y = 0, tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tile_y 0 = by tracers per domain
+    int tile_y_used = 16;                               // the height of the committed work table
     int nwork_h = 0, nwork_w = 0, nwork_d = 0, nzmax = 1;
     int nwork_d_class[8] = {};  // diffusion work items per depth class (8 c-7 .. 8 c + 8 layers), in this order
     int committed = 0;
@@ -135,6 +136,11 @@ struct Ensemble {
     const Grid** vg = nullptr;
     DeviceBuffer doms, work_h, work_w, work_d, scratch, hbuf, stats;
 };
+
+// Default tile height of the horizontal stage pair: with few tracers the
+// per-tile set-up dominates and two 512-lane blocks per CU overlap each other's
+// barrier phases (8 rows); with many, the smaller halo share of 16-row tiles wins.
+inline int auto_tile_y(int tracers_per_domain) { return tracers_per_domain <= 8 ? 8 : 16; }
 
 // ---- batched kernels (transport_batched.hip) ----
 // tpb = tracers per block (tracer batches go into the grid); <= 0 means all

@@ -39,7 +39,7 @@ class Ensemble:
             raise ValueError("ensemble_create failed (scheme, grids, nz <= 64 and ntracers >= 1 are required)")
         self._keep = [None] * n  # tensors bound to the library, kept alive here
         self.tracers: List[Optional[torch.Tensor]] = [None] * n
-        self.tuning = (16, 0, 16, 16)
+        self.tuning = (0, 0, 16, 16)
 
     def __del__(self):
         p = getattr(self, "p", None)
@@ -67,7 +67,7 @@ class Ensemble:
         self._keep[d] = (tracers, u, v, w, nuh)
         self.tracers[d] = tracers
 
-    def set_tuning(self, tile_y: int = 16, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
+    def set_tuning(self, tile_y: int = 0, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
                    tracers_per_block_d: int = 16) -> None:
         _lib.check(_lib.load().ensemble_set_tuning(self.p, int(tile_y), int(tracers_per_block_h),
                                                    int(tracers_per_block_w), int(tracers_per_block_d)),

@@ -136,9 +136,10 @@ class TracerCollection:
         return self.inventory_device(h).cpu().numpy()
 
     def set_tuning(self, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
-                   tracers_per_block_d: int = 16, tile_y: int = 16) -> None:
+                   tracers_per_block_d: int = 16, tile_y: int = 0) -> None:
         """Launch shape of the batched kernels: tracers per block of the three kernel
-        kinds (0 = all) and the height of the horizontal-stage tiles (4, 8, 16).  Does
+        kinds (0 = all) and the height of the horizontal-stage tiles (4, 8, 16; 0 = 8
+        rows for up to 8 tracers, else 16).  Does
         not change results."""
         if not self.batched:
             return
