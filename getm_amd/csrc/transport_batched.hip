@@ -565,7 +565,10 @@ __global__ void __launch_bounds__(TX * TY) k_hpair_multi(const EnsDom<R>* __rest
 // recomputed per layer by repeating the reference's own update
 // h <- h - dtk (w_k - w_{k-1}) s times, so nothing is kept in memory and
 // blocks that handle different tracer batches of a column never share
-// scratch; batch 0 writes the final thickness.
+// scratch; batch 0 writes the final thickness.  Most columns need one
+// sub-step: the tracer pass runs first with one, taking the Courant maximum
+// the replay would take, and only a column that needs more is replayed and
+// swept again (from fin, so fin and fout must be different buffers).
 // ---------------------------------------------------------------------------
 template <int S, typename R, int TB>
 __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, int j, int tb0, int tb1, bool first,
@@ -600,45 +603,12 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
         return h;
     };
 
-    // ---- thickness-only replay of the iteration control (template:277-322)
-    int iters = 1, capped = 0;
-    double itersm1 = 1.0;
-    {
-        int it = 1;
-        while (it <= iters) {
-            if (iters < kItersMax) {
-                double c = -1.7976931348623157e308;  // MAXVAL of an empty array
-                double hk = h_at(0, it - 1);
-                for (int k = 1; k <= nz - 1; k++) {
-                    const double hk1 = h_at(k, it - 1);
-                    c = fmax2(c, (fabs(wvp[(size_t)k * n2]) * dt) / fmin2(hk, hk1));
-                    hk = hk1;
-                }
-                const int ce = fceiling(c);
-                const int iters_new = ce > 1 ? ce : 1;
-                if (iters_new > iters) {
-                    if (iters_new > kItersMax) capped = 1;
-                    iters = iters_new > kItersMax ? kItersMax : iters_new;
-                    itersm1 = 1.0 / (double)iters;
-                    dtk = dt * itersm1;
-                    if (it > 1) {
-                        it = 1;
-                        continue;
-                    }
-                }
-            }
-            if (iters == 1) break;
-            it++;
-        }
-    }
-    if (first) {
-        if (iters > 1) atomicMax(&stats[0], iters);
-        if (capped) atomicAdd(&stats[1], 1);
-    }
-
-    // ---- tracer passes, TB tracers at a time, sharing the thickness sweep
-    for (int t0 = tb0; t0 < tb1; t0 += TB) {
+    // ---- tracer pass for TB tracers at a time, sharing the thickness sweep.  With
+    // cmax != nullptr (only for iters == 1) it also returns the largest Courant
+    // number at the initial thickness, in the order of the reference's MAXVAL.
+    auto sweep = [&](int t0, int iters, double itersm1, double* cmax) {
         const int nb = tb1 - t0 < TB ? tb1 - t0 : TB;
+        double c = -1.7976931348623157e308;  // MAXVAL of an empty array
         for (int it = 1; it <= iters; it++) {
             const R* sf = (it == 1 ? fin : fout) + (size_t)t0 * n3 + p;
             R* df = fout + (size_t)t0 * n3 + p;
@@ -665,7 +635,9 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
                 R fl[TB];
                 if (k <= nz - 1) {
                     const double wvk = wvp[(size_t)k * n2];
-                    const double cfl = ((fabs(wvk) * dt) / fmin2(h0, h1)) * itersm1;
+                    const double c0 = (fabs(wvk) * dt) / fmin2(h0, h1);
+                    if (cmax) c = fmax2(c, c0);
+                    const double cfl = c0 * itersm1;
                     const R rw = R(wvk);
                     const bool pos = wvk > 0.0;
                     LimF<S> lf;
@@ -720,10 +692,59 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
                 wprev = wk;
             }
         }
-    }
-}
+        if (cmax) *cmax = c;
+    };
 
-// Single domain: blockIdx.z = tracer batch.
+    // ---- first tracer group with one sub-step, taking the Courant maximum on the
+    // way (the same expressions, in the same order, as the replay's first check):
+    // when it needs no more than one sub-step, that pass is already the result.
+    int iters = 1, capped = 0;
+    double itersm1 = 1.0;
+    {
+        double c1;
+        sweep(tb0, 1, 1.0, &c1);
+        const int ce = fceiling(c1);
+        if (ce <= 1) {
+            for (int t0 = tb0 + TB; t0 < tb1; t0 += TB) sweep(t0, 1, 1.0, nullptr);
+            return;
+        }
+    }
+
+    // ---- thickness-only replay of the iteration control (template:277-322)
+    {
+        int it = 1;
+        while (it <= iters) {
+            if (iters < kItersMax) {
+                double c = -1.7976931348623157e308;  // MAXVAL of an empty array
+                double hk = h_at(0, it - 1);
+                for (int k = 1; k <= nz - 1; k++) {
+                    const double hk1 = h_at(k, it - 1);
+                    c = fmax2(c, (fabs(wvp[(size_t)k * n2]) * dt) / fmin2(hk, hk1));
+                    hk = hk1;
+                }
+                const int ce = fceiling(c);
+                const int iters_new = ce > 1 ? ce : 1;
+                if (iters_new > iters) {
+                    if (iters_new > kItersMax) capped = 1;
+                    iters = iters_new > kItersMax ? kItersMax : iters_new;
+                    itersm1 = 1.0 / (double)iters;
+                    dtk = dt * itersm1;
+                    if (it > 1) {
+                        it = 1;
+                        continue;
+                    }
+                }
+            }
+            if (iters == 1) break;
+            it++;
+        }
+    }
+    if (first) {
+        if (iters > 1) atomicMax(&stats[0], iters);
+        if (capped) atomicAdd(&stats[1], 1);
+    }
+    for (int t0 = tb0; t0 < tb1; t0 += TB) sweep(t0, iters, itersm1, nullptr);
+}
 template <int S, typename R, int TB>
 __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int ntr, int tpb, const R* __restrict__ fin,
                                               R* __restrict__ fout, const double* __restrict__ hin,
@@ -796,34 +817,36 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
     const bool inside = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax;
     const bool solve = inside && A.tmask[p] == 1 && nz > 1;
     if (solve) {
-        // factorisation from the surface down, FP64
+        // factorisation from the surface down, FP64.  The coefficients of interface
+        // k (an, ao) serve level k as its lower and level k - 1 as its upper
+        // neighbour, so each is formed once; the reciprocals are refined hardware
+        // estimates (the factors are rounded to FP32 afterwards).
         const double* hn = A.hn + p;
-        double ru_up = 0.0;
+        double ru_up = 0.0, an_k = 0.0, ao_k = 0.0;
+        double dz = hn[(size_t)(nz - 1) * n2];
         for (int k = nz - 1; k >= 0; k--) {
-            const double dz = hn[(size_t)k * n2];
-            double an_k = 0.0, ao_k = 0.0, an_km1 = 0.0, ao_km1 = 0.0;
-            if (k <= nz - 2) {
-                const double x = (2.0 * dt) * (nuh[(size_t)(k + 1) * n2 + p] + molecular);
-                const double s = hn[(size_t)(k + 1) * n2] + dz;
-                an_k = (cnpar * x) / s;
-                ao_k = ((1.0 - cnpar) * x) / s;
-            }
+            double an_km1 = 0.0, ao_km1 = 0.0, dz_lo = 0.0;
             if (k >= 1) {
+                dz_lo = hn[(size_t)(k - 1) * n2];
                 const double x = (2.0 * dt) * (nuh[(size_t)k * n2 + p] + molecular);
-                const double s = dz + hn[(size_t)(k - 1) * n2];
-                an_km1 = (cnpar * x) / s;
-                ao_km1 = ((1.0 - cnpar) * x) / s;
+                const double rs = rcp_nr(dz + dz_lo);
+                an_km1 = (cnpar * x) * rs;
+                ao_km1 = ((1.0 - cnpar) * x) * rs;
             }
             const double a2 = (dz + an_k) + an_km1;
             const double den = k == nz - 1 ? a2 : a2 + an_k * ru_up;  // a2 - a3 ru_{k+1}, a3 = -an_k
-            const double ruk = -an_km1 / den;
-            e2[k * 64 + lane] = float(an_k / den);
+            const double rden = rcp_nr(den);
+            const double ruk = -an_km1 * rden;
+            e2[k * 64 + lane] = float(an_k * rden);
             ru[k * 64 + lane] = float(ruk);
             if (!CN1) {
-                g3[k * 64 + lane] = float(ao_k / den);
-                g4[k * 64 + lane] = float(ao_km1 / den);
+                g3[k * 64 + lane] = float(ao_k * rden);
+                g4[k * 64 + lane] = float(ao_km1 * rden);
             }
             ru_up = ruk;
+            an_k = an_km1;
+            ao_k = ao_km1;
+            dz = dz_lo;
         }
     }
     for (int t = 0; t < ntr; t += NT) {
