@@ -206,6 +206,8 @@ def main():
     ap.add_argument("--scheme", type=int, default=0, help="override the workload's scheme")
     ap.add_argument("--per-tracer", action="store_true", help="FP64 per-tracer operator sequence (reference order)")
     ap.add_argument("--tuning", default="", help="tracers per block h,w,d of the batched kernels (0 = all)[,tile height]")
+    ap.add_argument("--solver", type=int, default=0,
+                    help="wave-PCR column solve for columns of up to this many layers (0: Thomas only)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -248,6 +250,8 @@ def main():
                           batched=not args.per_tracer)
     if args.tuning:
         tc.set_tuning(*[int(x) for x in args.tuning.split(",")])
+    if args.solver:
+        tc.set_solver(args.solver)
     if kind == "blobs":
         tc.set_values(blob_tracers(ntr, grid, seed=7, dtype=tc.dtype))
     else:
@@ -336,6 +340,7 @@ def main():
             "step_hbm_fraction": b_step / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
             "kernel_ms_per_step": {k: v[0] / args.steps for k, v in ktimes.items()},
             "tuning": args.tuning or "default",
+            "pcr_max_nz": args.solver,
             "key": workload_key,
         },
         "roofline": {

@@ -45,6 +45,7 @@ _SIGNATURES = {
     "transport_step": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp]),
     "transport_get_stats": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _i, _vp]),
     "transport_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
+    "transport_set_solver": (_i, [_vp, _i]),
     "transport_inventory": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "transport_conserve": (_i, [_vp, _i, _vp, _vp, _vp]),
     "ensemble_create": (_vp, [_i, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
@@ -52,6 +53,7 @@ _SIGNATURES = {
     "ensemble_finalize": (None, [_vp]),
     "ensemble_bind": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _d, _d]),
     "ensemble_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
+    "ensemble_set_solver": (_i, [_vp, _i]),
     "ensemble_commit": (_i, [_vp, _vp]),
     "ensemble_step": (_i, [_vp, _vp]),
     "ensemble_thickness": (_vp, [_vp, _i]),

@@ -1,14 +1,13 @@
-"""Launch-shape autotuner for the batched step and ensembles.
+"""Launch-shape and solver autotuner for the batched step and ensembles.
 
 Searches the height of the horizontal-stage tiles (4, 8 or 16 rows of 64
-columns) and the tracers per block of the three kernel kinds, timing each
-candidate with HIP events on the current stream, and caches the best shape per
-workload signature (grid size bucket, nz, tracers) in a JSON file.  Launch shapes
-never change results (see tests), so tuning is safe at any time.
-
-The column solver is always the per-lane Thomas elimination (the reference's
-algorithm, diffusion.F90:276-313); a wave-parallel cyclic-reduction variant is
-not implemented, so there is no solver axis to search.
+columns), the tracers per block of the three kernel kinds, and the column
+solver by depth: per-lane Thomas elimination (the reference's algorithm,
+diffusion.F90:276-313) or wave-level parallel cyclic reduction for columns of
+up to ``pcr_max_nz`` levels.  Each candidate is timed with HIP events on the
+current stream, and the best is cached per workload signature (grid size
+bucket, nz, tracers) in a JSON file.  Launch shapes never change results; the
+two solvers agree to FP32 rounding (see tests).
 """
 from __future__ import annotations
 
@@ -20,11 +19,11 @@ from typing import Callable, Dict, Iterable, Optional, Tuple
 
 import torch
 
-Shape = Tuple[int, int, int, int]  # (tile_y, tracers/block h, w, d)
+Shape = Tuple[int, ...]  # (tile_y, tracers/block h, w, d[, pcr_max_nz])
 
 DEFAULT_CANDIDATES = [
-    (ty, h, w, d)
-    for ty, h, (w, d) in itertools.product((16, 8, 4), (0, 32), ((16, 16), (64, 64)))
+    (ty, h, w, d, pcr)
+    for ty, h, (w, d), pcr in itertools.product((16, 8, 4), (0, 32), ((16, 16), (64, 64)), (0, 16, 32))
 ]
 
 
@@ -85,7 +84,11 @@ class Autotuner:
         """Tune a batched getm_amd.tracer.TracerCollection; ``advance`` runs one step."""
         g = tc.grid
         key = "step-" + tc.precision + "-" + signature(g.nx_ * g.ny_ * g.nz_, g.nz_, len(tc))
-        return self.tune(key, lambda s: tc.set_tuning(s[1], s[2], s[3], tile_y=s[0]), advance, **kw)
+        def apply(s):
+            tc.set_tuning(s[1], s[2], s[3], tile_y=s[0])
+            tc.set_solver(s[4] if len(s) > 4 else 0)
+
+        return self.tune(key, apply, advance, **kw)
 
     def tune_ensemble(self, ens, **kw) -> Shape:
         """Tune a getm_amd.ensemble.Ensemble (rebuilds its work tables per candidate)."""
@@ -93,7 +96,8 @@ class Autotuner:
         key = "ens-" + signature(ncells, max(g.nz_ for g in ens.grids), sum(ens.ntracers)) + f"-d{len(ens)}"
 
         def apply(s):
-            ens.set_tuning(*s)
+            ens.set_tuning(*s[:4])
+            ens.set_solver(s[4] if len(s) > 4 else 0)
             ens.commit()
 
         return self.tune(key, apply, ens.step, **kw)

@@ -355,6 +355,8 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
         return rc;
     if ((rc = launch_hpair<R>(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, s))) return rc;
     if constexpr (std::is_same<R, float>::value) {
+        if (t->nz >= 2 && t->nz <= tr->pcr_max_nz)
+            return launch_vdiff_pcr(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
         return launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
     } else {
         // FP64: the reference-order per-tracer diffusion kernels, prepared once (tracer.py:273-276)
@@ -459,6 +461,14 @@ int transport_set_tuning(void* p, int tile_y, int tracers_per_block_h, int trace
     return GETM_OK;
 }
 
+int transport_set_solver(void* p, int pcr_max_nz)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr || pcr_max_nz < 0 || pcr_max_nz > 32) return GETM_ERR_ARG;
+    tr->pcr_max_nz = pcr_max_nz;
+    return GETM_OK;
+}
+
 int transport_get_stats(void* p, int* max_iters, int* capped_columns, int reset, void* stream)
 {
     Transport* tr = static_cast<Transport*>(p);
@@ -560,6 +570,15 @@ int ensemble_set_tuning(void* p, int tile_y, int tpb_h, int tpb_w, int tpb_d)
     return GETM_OK;
 }
 
+int ensemble_set_solver(void* p, int pcr_max_nz)
+{
+    Ensemble* e = static_cast<Ensemble*>(p);
+    if (!e || pcr_max_nz < 0 || pcr_max_nz > 32) return GETM_ERR_ARG;
+    e->pcr_max_nz = pcr_max_nz;
+    e->committed = 0;
+    return GETM_OK;
+}
+
 static int upload_work(DeviceBuffer& buf, const std::vector<int4>& w)
 {
     if (w.empty()) return GETM_OK;
@@ -587,7 +606,7 @@ int ensemble_commit(void* p, void* stream)
     if ((rc = e->scratch.ensure(nscr * sizeof(float))) || (rc = e->hbuf.ensure(nh * sizeof(double)))) return rc;
     float* scr = static_cast<float*>(e->scratch.ptr);
     double* hb = static_cast<double*>(e->hbuf.ptr);
-    std::vector<int4> wh, ww, wd_class[8];
+    std::vector<int4> wh, ww, wd_class[8], wp_class[3];
     e->nzmax = 1;
     for (int d = 0; d < e->ndom; d++) {
         EnsDom<float>& D = e->host[d];
@@ -615,10 +634,19 @@ int ensemble_commit(void* p, void* stream)
             for (int by = 0; by < (t->ny_ + 3) / 4; by++)
                 for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) ww.push_back(make_int4(d, bx, by, b));
         // diffusion work grouped by depth class, so each launch's kernel is unrolled for its columns
-        std::vector<int4>& wd = wd_class[(t->nz - 1) / 8];
-        for (int b = 0; b < nbd; b++)
-            for (int j = 0; j < t->ny_; j++)
-                for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) wd.push_back(make_int4(d, bx, j, b));
+        if (t->nz >= 2 && t->nz <= e->pcr_max_nz) {
+            const int cls = t->nz <= 8 ? 0 : t->nz <= 16 ? 1 : 2;
+            const size_t cols = (size_t)1024 >> (3 + cls);  // columns per PCR block
+            std::vector<int4>& wp = wp_class[cls];
+            if (t->n2() > 0x7fffffff) return GETM_ERR_CAPACITY;
+            for (int b = 0; b < nbd; b++)
+                for (size_t q0 = 0; q0 < t->n2(); q0 += cols) wp.push_back(make_int4(d, (int)q0, 0, b));
+        } else {
+            std::vector<int4>& wd = wd_class[(t->nz - 1) / 8];
+            for (int b = 0; b < nbd; b++)
+                for (int j = 0; j < t->ny_; j++)
+                    for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) wd.push_back(make_int4(d, bx, j, b));
+        }
     }
     // horizontal-stage table in XCD-aware launch order: block b runs logical item
     // (b % 8) * (n / 8) + b / 8 (n padded to a multiple of 8 with domain -1)
@@ -635,6 +663,10 @@ int ensemble_commit(void* p, void* stream)
     for (int c = 0; c < 8; c++) {
         e->nwork_d_class[c] = (int)wd_class[c].size();
         wd.insert(wd.end(), wd_class[c].begin(), wd_class[c].end());
+    }
+    for (int c = 0; c < 3; c++) {
+        e->nwork_p_class[c] = (int)wp_class[c].size();
+        wd.insert(wd.end(), wp_class[c].begin(), wp_class[c].end());
     }
     if (wh.size() > 0x7fffffff || ww.size() > 0x7fffffff || wd.size() > 0x7fffffff) return GETM_ERR_CAPACITY;
     if ((rc = upload_work(e->work_h, wh)) || (rc = upload_work(e->work_w, ww)) || (rc = upload_work(e->work_d, wd)))
@@ -669,6 +701,11 @@ int ensemble_step(void* p, void* stream)
         if (!e->nwork_d_class[c]) continue;
         if ((rc = launch_vdiff_multi(e->cnpar == 1.0, c, doms, wd, e->nwork_d_class[c], s))) return rc;
         wd += e->nwork_d_class[c];
+    }
+    for (int c = 0; c < 3; c++) {
+        if (!e->nwork_p_class[c]) continue;
+        if ((rc = launch_vdiff_pcr_multi(e->cnpar == 1.0, c, doms, wd, e->nwork_p_class[c], s))) return rc;
+        wd += e->nwork_p_class[c];
     }
     return GETM_OK;
 }

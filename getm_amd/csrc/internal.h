@@ -117,6 +117,7 @@ struct Transport {
     DeviceBuffer auxo, auxn, ru;  // FP64 diffusion (precision GETM_F64)
     int tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tracers per block of the three kernel kinds (0 = all)
     int tile_y = 0;                         // height of the horizontal-stage tiles (4, 8, 16; 0 = by tracer count)
+    int pcr_max_nz = 0;                     // FP32 column solve by wave PCR for 2 <= nz <= this (<= 32; 0: Thomas only)
     DeviceBuffer part, part2, ratio, resid;  // inventory partial sums, fixer ratios and residuals
 };
 
@@ -129,6 +130,8 @@ struct Ensemble {
     int tile_y_used = 16;                               // the height of the committed work table
     int nwork_h = 0, nwork_w = 0, nwork_d = 0, nzmax = 1;
     int nwork_d_class[8] = {};  // diffusion work items per depth class (8 c-7 .. 8 c + 8 layers), in this order
+    int pcr_max_nz = 0;         // domains with 2 <= nz <= this use the wave-PCR solve
+    int nwork_p_class[3] = {};  // PCR work items per class (8, 16, 32 levels), after the Thomas items
     int committed = 0;
     EnsDom<float>* host = nullptr;  // host copy of the domain table
     const Grid** tg = nullptr;
@@ -152,6 +155,10 @@ int launch_hpair_multi(int stage, int d1, int scheme, int ty, const EnsDom<float
 int launch_wcol_multi(int scheme, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
 // one depth class: every domain of the work items has 8 (cls + 1) - 7 .. 8 (cls + 1) layers
 int launch_vdiff_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
+// wave-PCR column solve (2 <= nz <= 32): single domain, and one PCR depth class (0..2: 8, 16, 32 levels) of an ensemble
+int launch_vdiff_pcr(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
+                     const float* fin, float* fout, hipStream_t s);
+int launch_vdiff_pcr_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
 template <typename R>
 int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout, const double* hin,
                 double* hout, const double* w, const double* wv, int* stats, hipStream_t s);

@@ -949,6 +949,153 @@ __global__ void __launch_bounds__(64) k_vdiff_multi(const EnsDom<float>* __restr
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// FP32 column solve by parallel cyclic reduction, for shallow columns (nz <= 32):
+// a block of 1024 lanes owns COLS = 1024 / NP consecutive columns (memory order),
+// one lane per (column, level) with NP = the power of two >= nz (levels nz ..
+// NP-1 are identity rows), so a wave holds 64 / NP whole columns.  The system is
+// the increment form of k_vdiff_f32 (the reference's matrix,
+// diffusion.F90:233-313, acting on d = c_new - c_old):
+//   row k:  l_k d_{k-1} + m_k d_k + u_k d_{k+1} = r_k,
+//   l_k = -an_{k-1},  m_k = h_k + an_k + an_{k-1},  u_k = -an_k,
+//   r_k = an_k (c_{k+1} - c_k) + an_{k-1} (c_{k-1} - c_k) [+ ao_k (..) + ao_{k-1} (..)].
+// Step s (stride 2^s) eliminates rows k +- 2^s from row k.  The factors a_s, g_s
+// of the log2 NP steps depend only on the matrix: each lane forms them once for
+// its row and keeps them in registers; every tracer then only reduces its
+// right-hand side, r <- r + a_s r_{k-2^s} + g_s r_{k+2^s}, d = r / m.  Tracer
+// tiles move between HBM and LDS by rows (coalesced) and are read by columns.
+// More arithmetic per cell than the per-lane Thomas elimination, but a
+// dependency chain of log2 NP instead of 2 nz and 16-32x more lanes per column:
+// it pays when there are too few columns to fill the GPU.
+// ---------------------------------------------------------------------------
+template <int NP, bool CN1>
+__device__ __forceinline__ void pcr_body(const StepArgs& A, double dt, double cnpar, double molecular,
+                                         const double* __restrict__ nuh, size_t p0, int ntr,
+                                         const float* __restrict__ fin, float* __restrict__ fout)
+{
+    constexpr int COLS = 1024 / NP, LDW = COLS + 1;  // padded rows: column reads are conflict-free
+    constexpr int LOG = NP == 8 ? 3 : NP == 16 ? 4 : 5;
+    static_assert(NP == 8 || NP == 16 || NP == 32, "wave PCR covers columns of up to 32 levels");
+    __shared__ float s_t[NP * LDW];
+    const int tid = threadIdx.x, lane = tid % 64;
+    const int k = tid % NP, cl = tid / NP;  // this lane's level and column in the tile
+    const int nz = A.nz;
+    const size_t n2 = A.n2, n3 = A.n3;
+    const size_t pc = p0 + cl;
+    const bool incol = pc < n2, lev = k < nz;
+    bool solve = false;
+    if (incol) {
+        const int i = (int)(pc % A.nx_), j = (int)(pc / A.nx_);
+        solve = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[pc] == 1;
+    }
+    // row k in FP64, rounded once
+    float l = 0.f, m = 1.f, u = 0.f, eu = 0.f, el = 0.f, ou = 0.f, ol = 0.f;
+    if (solve && lev) {
+        const double hk = A.hn[(size_t)k * n2 + pc];
+        double an_up = 0.0, an_lo = 0.0, ao_up = 0.0, ao_lo = 0.0;
+        if (k <= nz - 2) {
+            const double x = (2.0 * dt) * (nuh[(size_t)(k + 1) * n2 + pc] + molecular);
+            const double rs = rcp_nr(A.hn[(size_t)(k + 1) * n2 + pc] + hk);
+            an_up = (cnpar * x) * rs;
+            ao_up = ((1.0 - cnpar) * x) * rs;
+        }
+        if (k >= 1) {
+            const double x = (2.0 * dt) * (nuh[(size_t)k * n2 + pc] + molecular);
+            const double rs = rcp_nr(hk + A.hn[(size_t)(k - 1) * n2 + pc]);
+            an_lo = (cnpar * x) * rs;
+            ao_lo = ((1.0 - cnpar) * x) * rs;
+        }
+        l = float(-an_lo);
+        u = float(-an_up);
+        m = float((hk + an_up) + an_lo);
+        eu = float(an_up);
+        el = float(an_lo);
+        ou = float(ao_up);
+        ol = float(ao_lo);
+    }
+    // reduction factors of the matrix, kept for every tracer
+    float fa[LOG], fg[LOG];
+#pragma unroll
+    for (int st = 0; st < LOG; st++) {
+        const int sg = 1 << st;
+        const bool hl = k >= sg, hu = k + sg < NP;  // neighbours stay in this column's lanes
+        const int src_l = lane - (hl ? sg : 0), src_u = lane + (hu ? sg : 0);
+        const float ml = __shfl(m, src_l), ll = __shfl(l, src_l), ul = __shfl(u, src_l);
+        const float mu = __shfl(m, src_u), lu = __shfl(l, src_u), uu = __shfl(u, src_u);
+        const float a = hl ? -l * __builtin_amdgcn_rcpf(ml) : 0.f;
+        const float g = hu ? -u * __builtin_amdgcn_rcpf(mu) : 0.f;
+        m = m + (hl ? a * ul : 0.f) + (hu ? g * lu : 0.f);
+        l = hl ? a * ll : 0.f;
+        u = hu ? g * uu : 0.f;
+        fa[st] = a;
+        fg[st] = g;
+    }
+    const float minv = 1.f / m;
+    const int ncols = (size_t)COLS < n2 - p0 ? COLS : (int)(n2 - p0);
+    for (int t = 0; t < ntr; t++) {
+        const float* fi = fin + (size_t)t * n3 + p0;
+        float* fo = fout + (size_t)t * n3 + p0;
+        // rows in, coalesced
+        for (int e = tid; e < nz * COLS; e += 1024) {
+            const int kk = e / COLS, cc = e % COLS;
+            if (cc < ncols) s_t[kk * LDW + cc] = fi[(size_t)kk * n2 + cc];
+        }
+        __syncthreads();
+        const float c = (incol && lev) ? s_t[k * LDW + cl] : 0.f;
+        const float cdn = __shfl(c, lane - (k >= 1 ? 1 : 0));
+        const float cup = __shfl(c, lane + (k + 1 < NP ? 1 : 0));
+        float r = 0.f;
+        if (solve && lev) {
+            const float ddn = k >= 1 ? cdn - c : 0.f;
+            const float dup = k + 1 < nz ? cup - c : 0.f;
+            r = el * ddn + eu * dup;
+            if (!CN1) r += ol * ddn + ou * dup;
+        }
+#pragma unroll
+        for (int st = 0; st < LOG; st++) {
+            const int sg = 1 << st;
+            const bool hl = k >= sg, hu = k + sg < NP;
+            const float rl = __shfl(r, lane - (hl ? sg : 0));
+            const float rh = __shfl(r, lane + (hu ? sg : 0));
+            r = r + (hl ? fa[st] * rl : 0.f) + (hu ? fg[st] * rh : 0.f);
+        }
+        __syncthreads();  // every lane has read its value before the tile is overwritten
+        if (incol && lev) s_t[k * LDW + cl] = c + r * minv;
+        __syncthreads();
+        // rows out, coalesced
+        for (int e = tid; e < nz * COLS; e += 1024) {
+            const int kk = e / COLS, cc = e % COLS;
+            if (cc < ncols) fo[(size_t)kk * n2 + cc] = s_t[kk * LDW + cc];
+        }
+        __syncthreads();
+    }
+}
+
+// Single domain: blockIdx.x = tile of 1024 / NP columns, blockIdx.y = tracer batch (2 <= nz <= 32).
+template <int NP, bool CN1>
+__global__ void __launch_bounds__(1024) k_vdiff_pcr(const StepArgs A, double dt, double cnpar, double molecular,
+                                                    const double* __restrict__ nuh, int ntr_all, int tpb,
+                                                    const float* __restrict__ fin_all, float* __restrict__ fout_all)
+{
+    const int tb0 = blockIdx.y * tpb;
+    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
+    pcr_body<NP, CN1>(A, dt, cnpar, molecular, nuh, (size_t)blockIdx.x * (1024 / NP), ntr,
+                      fin_all + (size_t)tb0 * A.n3, fout_all + (size_t)tb0 * A.n3);
+}
+
+// Ensemble: one work item per block {domain, first column, 0, batch}.
+template <int NP, bool CN1>
+__global__ void __launch_bounds__(1024) k_vdiff_pcr_multi(const EnsDom<float>* __restrict__ doms,
+                                                          const int4* __restrict__ work)
+{
+    const int4 wk = work[blockIdx.x];
+    const EnsDom<float>& D = doms[wk.x];
+    const int tb0 = wk.w * D.tpb_d;
+    const int ntr = D.ntr - tb0 < D.tpb_d ? D.ntr - tb0 : D.tpb_d;
+    const size_t off = (size_t)tb0 * D.A.n3;
+    pcr_body<NP, CN1>(D.A, D.dt, D.cnpar, D.molecular, D.nuh, (size_t)wk.y, ntr, D.scratch + off, D.tracers + off);
+}
+
 static inline int nbatch(int ntr, int tpb) { return tpb <= 0 ? 1 : (ntr + tpb - 1) / tpb; }
 
 template <typename R>
@@ -1038,6 +1185,41 @@ int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecula
     return GETM_OK;
 }
 
+// PCR depth class: 0 -> 8 levels, 1 -> 16, 2 -> 32 (2 <= nz <= 32)
+static inline int pcr_class(int nz) { return nz <= 8 ? 0 : nz <= 16 ? 1 : 2; }
+
+int launch_vdiff_pcr(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
+                     const float* fin, float* fout, hipStream_t s)
+{
+    if (A.nz > 32 || A.nz < 2) return GETM_ERR_CAPACITY;
+    if (tpb <= 0 || tpb > ntr) tpb = ntr;
+    const int nb = nbatch(ntr, tpb);
+    const int cls = pcr_class(A.nz);
+    const size_t cols = (size_t)1024 >> (3 + cls);
+    const size_t ntile = ((size_t)A.n2 + cols - 1) / cols;
+    if (nb > 65535 || ntile > 0x7fffffff) return GETM_ERR_CAPACITY;
+    const bool cn1 = cnpar == 1.0;
+    dim3 grid((unsigned)ntile, nb);
+    ProfileScope ps(GETM_K_VDIFF_F32, s);
+#define GETM_PC(NP)                                                                                              \
+    do {                                                                                                         \
+        if (cn1)                                                                                                 \
+            hipLaunchKernelGGL((k_vdiff_pcr<NP, true>), grid, dim3(1024), 0, s, A, dt, cnpar, molecular, nuh, ntr, \
+                               tpb, fin, fout);                                                                  \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_vdiff_pcr<NP, false>), grid, dim3(1024), 0, s, A, dt, cnpar, molecular, nuh, ntr, \
+                               tpb, fin, fout);                                                                  \
+    } while (0)
+    switch (cls) {
+        case 0: GETM_PC(8); break;
+        case 1: GETM_PC(16); break;
+        default: GETM_PC(32); break;
+    }
+#undef GETM_PC
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
 template int launch_hpair<float>(int, int, int, const StepArgs&, double, int, int, const float*, float*,
                                  const double*, double*, hipStream_t);
 template int launch_hpair<double>(int, int, int, const StepArgs&, double, int, int, const double*, double*,
@@ -1118,5 +1300,27 @@ template int launch_wcol<float>(int, const StepArgs&, double, int, int, const fl
                                 const double*, const double*, int*, hipStream_t);
 template int launch_wcol<double>(int, const StepArgs&, double, int, int, const double*, double*, const double*,
                                  double*, const double*, const double*, int*, hipStream_t);
+
+int launch_vdiff_pcr_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s)
+{
+    if (nwork <= 0) return GETM_OK;
+    if (cls < 0 || cls > 2) return GETM_ERR_CAPACITY;
+    ProfileScope ps(GETM_K_VDIFF_F32, s);
+#define GETM_PCM(NP)                                                                                          \
+    do {                                                                                                      \
+        if (cn1)                                                                                              \
+            hipLaunchKernelGGL((k_vdiff_pcr_multi<NP, true>), dim3(nwork), dim3(1024), 0, s, doms, work);     \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_vdiff_pcr_multi<NP, false>), dim3(nwork), dim3(1024), 0, s, doms, work);    \
+    } while (0)
+    switch (cls) {
+        case 0: GETM_PCM(8); break;
+        case 1: GETM_PCM(16); break;
+        default: GETM_PCM(32); break;
+    }
+#undef GETM_PCM
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
 
 }  // namespace getm

@@ -40,6 +40,7 @@ class Ensemble:
         self._keep = [None] * n  # tensors bound to the library, kept alive here
         self.tracers: List[Optional[torch.Tensor]] = [None] * n
         self.tuning = (0, 0, 16, 16)
+        self.pcr_max_nz = 0
 
     def __del__(self):
         p = getattr(self, "p", None)
@@ -73,6 +74,11 @@ class Ensemble:
                                                    int(tracers_per_block_w), int(tracers_per_block_d)),
                    "ensemble_set_tuning")
         self.tuning = (tile_y, tracers_per_block_h, tracers_per_block_w, tracers_per_block_d)
+
+    def set_solver(self, pcr_max_nz: int = 0) -> None:
+        """Column solver per domain by depth (TracerCollection.set_solver); takes effect at commit."""
+        _lib.check(_lib.load().ensemble_set_solver(self.p, int(pcr_max_nz)), "ensemble_set_solver")
+        self.pcr_max_nz = int(pcr_max_nz)
 
     def commit(self) -> None:
         """Allocate scratch and upload the work tables (synchronous)."""

@@ -147,6 +147,15 @@ class TracerCollection:
                                                     int(tracers_per_block_w), int(tracers_per_block_d)),
                    "transport_set_tuning")
 
+    def set_solver(self, pcr_max_nz: int = 0) -> None:
+        """FP32 column solve: wave-level parallel cyclic reduction for columns of
+        2 <= nz <= pcr_max_nz (at most 32), per-lane Thomas elimination otherwise
+        (0: Thomas only).  The two agree to FP32 rounding."""
+        if not self.batched:
+            return
+        _lib.check(_lib.load().transport_set_solver(self.p, int(pcr_max_nz)), "transport_set_solver")
+        self.pcr_max_nz = int(pcr_max_nz)
+
     def stats(self, reset: bool = True):
         """(largest z sub-iteration count, columns capped at 200) since the last reset."""
         if not self.batched:

@@ -159,6 +159,13 @@ int transport_step(void* tr, int ntracers, void* tracers, const double* u, const
 int transport_set_tuning(void* tr, int tile_y, int tracers_per_block_h, int tracers_per_block_w,
                          int tracers_per_block_d);
 
+/* No reference counterpart: the FP32 column solve of the batched step.  Columns
+ * with 2 <= nz <= pcr_max_nz (at most 32) are solved by wave-level parallel
+ * cyclic reduction (one lane per level, for shallow columns), the others by per-lane Thomas
+ * elimination (the reference's algorithm, diffusion.F90:276-313); 0 = Thomas
+ * only (default).  The two agree to FP32 rounding, not bit for bit. */
+int transport_set_solver(void* tr, int pcr_max_nz);
+
 /* Tracer inventories (the reference's domain integrals of tracers,
  * simulation.py:1408-1466): inventory[t] = sum over wet cells of
  * area * h * c for each of ntracers stacked tracers, accumulated in FP64
@@ -202,6 +209,10 @@ int ensemble_bind(void* ens, int domain, float* tracers, const double* u, const 
  * domain); takes effect at the next commit. */
 int ensemble_set_tuning(void* ens, int tile_y, int tracers_per_block_h, int tracers_per_block_w,
                         int tracers_per_block_d);
+
+/* Column solver as transport_set_solver, per domain by its nz; takes effect at
+ * the next commit. */
+int ensemble_set_solver(void* ens, int pcr_max_nz);
 
 /* Allocates scratch, builds and uploads the work tables (synchronous). */
 int ensemble_commit(void* ens, void* stream);

@@ -17,7 +17,7 @@ SCHEMES = range(1, 7)
 F32_TOL = 1e-5
 
 
-def run_batched(d, precision, nsteps=1, ah=None, ufirst=False):
+def run_batched(d, precision, nsteps=1, ah=None, ufirst=False, solver=0, tuning=None):
     from getm_amd.core import CENTERS, INTERFACES
     from getm_amd.tracer import TracerCollection
 
@@ -28,6 +28,10 @@ def run_batched(d, precision, nsteps=1, ah=None, ufirst=False):
     nuh = grid.wrap(dev(d.nuh), INTERFACES)
     tc = TracerCollection(grid, advection_scheme=d.scheme, cnpar=d.cnpar, precision=precision, ufirst=ufirst)
     tc.set_values(dev(d.tracers))
+    if solver:
+        tc.set_solver(solver)
+    if tuning:
+        tc.set_tuning(*tuning)
     if ah is not None:
         tc.Ah_u = grid.ugrid.wrap(dev(ah))
         tc.Ah_v = grid.vgrid.wrap(dev(ah))
@@ -232,3 +236,44 @@ def test_f32_error_does_not_grow_in_steady_flow(scheme):
         ref = outs["f64"][t][wet]
         err = np.abs(outs["f32"][t][wet] - ref).max() / np.abs(ref).max()
         assert err <= 5e-6, (t, err)
+
+
+@pytest.mark.parametrize("cnpar", [1.0, 0.7])
+@pytest.mark.parametrize("nz", [2, 5, 8, 9, 16, 17, 31, 32])
+def test_batched_f32_pcr_solver(oracle_mod, nz, cnpar):
+    """Wave-PCR column solve (every padding class 8/16/32 and its edges): within
+    1e-5 of the FP64 oracle, like the per-lane Thomas solve it replaces."""
+    d = inputs.tiny(nx=30, ny=26, nz=nz, seed=200 + nz, ntracers=3, cfl=0.05, wscale=0.2)
+    d.cnpar = cnpar
+    got, h, _ = run_batched(d, "f32", solver=32)
+    thomas, _, _ = run_batched(d, "f32", solver=0)
+    ref, href = run_oracle(oracle_mod, d)
+    assert_bitwise(h, href, "thickness")
+    fin = np.isfinite(ref)
+    np.testing.assert_array_equal(np.isfinite(got), fin)
+    scale = np.abs(ref[fin]).max()
+    assert np.abs(got[fin] - ref[fin]).max() / scale <= F32_TOL
+    assert np.abs(got[fin] - thomas[fin]).max() / scale <= F32_TOL
+
+
+def test_pcr_solver_launch_shape_independent():
+    """Tracer batches of the PCR solve give identical bits; a solver bound below nz
+    selects the Thomas solve (identical to solver 0)."""
+    d = inputs.tiny(nx=37, ny=30, nz=12, seed=91, ntracers=7, cfl=0.3, wscale=1.0)
+    outs = [run_batched(d, "f32", solver=16, tuning=t)[0] for t in [(0, 0, 0), (1, 16, 2), (4, 4, 3)]]
+    for o in outs[1:]:
+        assert_bitwise(o, outs[0], "PCR tracers")
+    below, _, _ = run_batched(d, "f32", solver=11)
+    thomas, _, _ = run_batched(d, "f32", solver=0)
+    assert_bitwise(below, thomas, "Thomas when nz > pcr_max_nz")
+
+
+def test_pcr_solver_bound_is_checked():
+    from getm_amd import _lib
+    from getm_amd.tracer import TracerCollection
+
+    d = inputs.tiny(nx=12, ny=10, nz=4, seed=3, ntracers=1)
+    tc = TracerCollection(make_grid(d), precision="f32")
+    with pytest.raises(_lib.GetmError):
+        tc.set_solver(33)
+    tc.set_solver(32)

@@ -33,13 +33,15 @@ def make_ensemble(doms, tuning=None):
     return ens
 
 
-def single_domain(d, nsteps):
+def single_domain(d, nsteps, solver=0):
     from getm_amd.core import CENTERS, INTERFACES
     from getm_amd.tracer import TracerCollection
 
     g = make_grid(d)
     tc = TracerCollection(g, advection_scheme=d.scheme, precision="f32")
     tc.set_values(dev(d.tracers))
+    if solver:
+        tc.set_solver(solver)
     args = (g.ugrid.wrap(dev(d.u), CENTERS), g.vgrid.wrap(dev(d.v), CENTERS), g.wrap(dev(d.w), INTERFACES),
             g.wrap(dev(d.nuh), INTERFACES))
     for _ in range(nsteps):
@@ -103,3 +105,17 @@ def test_autotuner_keeps_results():
         ref.step()
     for i in range(len(doms)):
         assert_bitwise(host(ens.tracers[i]), host(ref.tracers[i]), f"domain {i}")
+
+
+def test_ensemble_pcr_by_depth_equals_single_domains():
+    """Domains of up to 24 layers take the wave-PCR solve, deeper ones Thomas, in one
+    ensemble step; each equals its single-domain step with the same solver choice."""
+    doms = small_ensemble(5, seed=11)
+    assert min(d.nz for d in doms) <= 24 < max(d.nz for d in doms)
+    ens = make_ensemble(doms)
+    ens.set_solver(24)
+    ens.commit()
+    for _ in range(3):
+        ens.step()
+    for i, d in enumerate(doms):
+        assert_bitwise(host(ens.tracers[i]), single_domain(d, 3, solver=24), f"domain {i} (nz {d.nz})")
