@@ -45,6 +45,7 @@ _SIGNATURES = {
     "transport_step": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp]),
     "transport_get_stats": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _i, _vp]),
     "transport_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
+    "c_w_momentum_3d": (_i, [_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
     "transport_set_solver": (_i, [_vp, _i]),
     "transport_inventory": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "transport_conserve": (_i, [_vp, _i, _vp, _vp, _vp]),

@@ -240,6 +240,18 @@ enum {
     GETM_NUM_KERNELS = 16
 };
 
+/* Input producer of the step: vertical velocity at layer interfaces from layer
+ * transports pk (U grid) and qk (V grid) and the thickness change ho -> hn,
+ * replacing c_w_momentum_3d (python/src/momentum.F90:191-216, bound at
+ * python/_pygetm.pyx:54, called by Momentum.update_diagnostics,
+ * momentum.py:879-883).  Same arguments: nx, ny the array extents, imin..jmax
+ * the Fortran (1-based) point range (_pygetm.pyx:454 passes halo+1 ..
+ * halo+n+1), w (nz+1, ny, nx) updated for interfaces 1 .. nz-1 of mask == 1
+ * points, w_0 and w_nz untouched.  Bit-identical to the reference. */
+int c_w_momentum_3d(int nx, int ny, int nz, int imin, int imax, int jmin, int jmax, const int* mask,
+                    const double* dyu, const double* dxv, const double* iarea, const double* ho, const double* hn,
+                    const double* pk, const double* qk, double timestep, double* w, void* stream);
+
 /* No reference counterpart (the reference accumulates cpu_time around its
  * loops, diffusion.F90:161-311).  When enabled, every kernel launch of this
  * library is bracketed by HIP events recorded on the launch stream;

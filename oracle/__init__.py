@@ -157,6 +157,18 @@ def vertical_diffusion(nuh, dt, var, mask, hn, ho=None, cnpar=1.0, molecular=0.0
     vdiff_apply(mask, dzo, hn, auxo, auxn, var, ea2, ea4, halo)
 
 
+def w_momentum_3d(mask, dyu, dxv, iarea, ho, hn, pk, qk, dt, w, halo=2, extra=1):
+    """In-place w (nz+1, ny_, nx_) from layer transports (momentum.F90:191-216).
+    Points: the interior plus ``extra`` columns/rows on the high side, as
+    _pygetm.pyx:454 passes (i in [halo, halo + nx], j likewise)."""
+    nz, ny_, nx_ = hn.shape
+    nx, ny = nx_ - 2 * halo, ny_ - 2 * halo
+    rc = lib().oracle_w_momentum_3d(nx_, ny_, nz, halo, halo + nx - 1 + extra, halo, halo + ny - 1 + extra,
+                                    _i(mask), _d(dyu), _d(dxv), _d(iarea), _d(ho), _d(hn), _d(pk), _d(qk),
+                                    ctypes.c_double(dt), _d(w))
+    _check(rc, "w_momentum_3d")
+
+
 def transport_step(d, tracers: np.ndarray, scheme=None, dt=None, cnpar=None, molecular=0.0,
                    ah_u=None, ah_v=None, return_h=False):
     """One adv+diff step on stacked FP64 tracers (T, nz, ny_, nx_), in place.

@@ -418,3 +418,29 @@ out:
     free(auxn);
     return rc;
 }
+
+/* python/src/momentum.F90:191-216.  idt = 1/dt once (:201); the loop order k, j, i
+ * (:202-204) does not matter for the result since w_k depends only on w_{k-1} of
+ * the same point; the expression keeps the Fortran's left-to-right order. */
+int oracle_w_momentum_3d(int nx, int ny, int nz, int imin, int imax, int jmin, int jmax, const int* mask,
+                         const double* dyu, const double* dxv, const double* iarea, const double* ho,
+                         const double* hn, const double* pk, const double* qk, double dt, double* w)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || imin < 1 || jmin < 1 || imax >= nx || jmax >= ny) return 1;
+    const size_t n2 = (size_t)nx * ny;
+    const double idt = 1.0 / dt;
+    for (int k = 1; k <= nz - 1; k++) {
+        const size_t L = (size_t)(k - 1) * n2;  /* layer below interface k */
+        for (int j = jmin; j <= jmax; j++) {
+            for (int i = imin; i <= imax; i++) {
+                const size_t p = (size_t)j * nx + i;
+                if (mask[p] != 1) continue;
+                w[(size_t)k * n2 + p] =
+                    w[(size_t)(k - 1) * n2 + p] - (hn[L + p] - ho[L + p]) * idt -
+                    (pk[L + p] * dyu[p] - pk[L + p - 1] * dyu[p - 1] + qk[L + p] * dxv[p] - qk[L + p - nx] * dxv[p - nx]) *
+                        iarea[p];
+            }
+        }
+    }
+    return 0;
+}

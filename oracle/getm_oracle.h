@@ -60,6 +60,18 @@ int oracle_transport_step(int scheme, int nx, int ny, int nz, int halox, int hal
                           const double* ah_u, const double* ah_v, double dt, double cnpar,
                           double molecular, int ntracers, double* tracers, double* h_out);
 
+/* Vertical velocity at layer interfaces from layer transports and the layer
+ * thickness change, python/src/momentum.F90:191-216 (c_w_momentum_3d): for the
+ * interfaces k = 1 .. nz-1 of mask == 1 points in [imin, imax] x [jmin, jmax]
+ * (0-based, inclusive),
+ *   w_k = w_{k-1} - (hn_k - ho_k) / dt
+ *         - (pk_k dyu - pk_k(i-1) dyu(i-1) + qk_k dxv - qk_k(j-1) dxv(j-1)) iarea,
+ * with hn_k, ho_k, pk_k, qk_k the layer below interface k; w_0 and w_nz are not
+ * written.  nx, ny are the array extents including halos. */
+int oracle_w_momentum_3d(int nx, int ny, int nz, int imin, int imax, int jmin, int jmax, const int* mask,
+                         const double* dyu, const double* dxv, const double* iarea, const double* ho,
+                         const double* hn, const double* pk, const double* qk, double dt, double* w);
+
 /* Limiter value for a given scheme, cfl and slope ratio (limiters.h:1-44);
  * exported for the closed-form tests. */
 double oracle_limiter(int scheme, double cfl, double ratio);

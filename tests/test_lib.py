@@ -41,3 +41,10 @@ def test_create_rejects_bad_scheme_without_gpu():
     assert lib.advection_create(0, g) is None
     assert lib.advection_create(7, g) is None
     lib.grid_finalize(g)
+
+
+def test_w_momentum_rejects_bad_range_without_gpu():
+    lib = _lib.load()
+    # imin = 1 (Fortran) would read column -1; checked before anything is launched
+    assert lib.c_w_momentum_3d(10, 8, 3, 1, 9, 2, 7, None, None, None, None, None, None, None, None, 1.0, None,
+                               None) == 1

@@ -287,3 +287,56 @@ def test_steady_release_scenario_is_closed(oracle_mod, seed):
         h = oracle_mod.transport_step(d, tr, return_h=True)
         np.testing.assert_allclose(h, d.hn, rtol=5e-14, atol=0)
     assert np.max(np.abs(d.inventory(tr) - inv0) / inv0) <= 1e-12
+
+
+def _momentum_fields(d):
+    open_u = (d.umask == 1) | (d.umask == 2)
+    open_v = (d.vmask == 1) | (d.vmask == 2)
+    pk = np.where(open_u, d.u * d.hu, 0.0)
+    qk = np.where(open_v, d.v * d.hv, 0.0)
+    return np.ascontiguousarray(pk), np.ascontiguousarray(qk)
+
+
+def test_w_momentum_single_face(oracle_mod):
+    """One transport P through the east face of cell (i0, j0) in layer L: w above
+    layer L drops by P dyu iarea in that cell and rises by P dyu iarea' in its east
+    neighbour; every other value is untouched (momentum.F90:191-216)."""
+    d = inputs.tiny(nx=12, ny=9, nz=6, seed=4, ntracers=1)
+    d.tmask[:] = 1
+    shape = d.ho.shape
+    nz = shape[0]
+    pk, qk = np.zeros(shape), np.zeros(shape)
+    i0, j0, L, P = 5, 4, 2, 0.37
+    pk[L, j0, i0] = P
+    w = np.zeros((nz + 1,) + shape[1:])
+    oracle_mod.w_momentum_3d(d.tmask, d.dyu, d.dxv, d.iarea, d.ho, d.ho.copy(), pk, qk, d.dt, w)
+    want = np.zeros_like(w)
+    want[L + 1:nz, j0, i0] = -(P * d.dyu[j0, i0]) * d.iarea[j0, i0]
+    want[L + 1:nz, j0, i0 + 1] = (P * d.dyu[j0, i0]) * d.iarea[j0, i0 + 1]
+    np.testing.assert_allclose(w, want, rtol=1e-15, atol=0)
+    # nothing in, nothing out
+    w0 = np.zeros_like(w)
+    oracle_mod.w_momentum_3d(d.tmask, d.dyu, d.dxv, d.iarea, d.ho, d.ho.copy(), 0 * pk, qk, d.dt, w0)
+    assert not w0.any()
+
+
+@pytest.mark.parametrize("seed", [1, 6])
+def test_w_momentum_column_closes_at_surface(oracle_mod, seed):
+    """Continuity-consistent inputs (surface elevation updated by the depth-integrated
+    divergence, layers by their sigma share): integrating the column from the bottom
+    leaves nothing at the surface, w_nz = w_{nz-1} - (hn - ho)_top / dt - div_top = 0."""
+    d = inputs.coastal(nx=60, ny=44, nz=12, seed=seed, ntracers=1)
+    pk, qk = _momentum_fields(d)
+    nz = d.nz
+    w = np.zeros((nz + 1,) + d.ho.shape[1:])
+    oracle_mod.w_momentum_3d(d.tmask, d.dyu, d.dxv, d.iarea, d.ho, d.hn, pk, qk, d.dt, w, extra=0)
+    top = nz - 1
+    div_top = np.zeros_like(d.iarea)
+    div_top[:, 1:] += pk[top, :, 1:] * d.dyu[:, 1:] - pk[top, :, :-1] * d.dyu[:, :-1]
+    div_top[1:, :] += qk[top, 1:, :] * d.dxv[1:, :] - qk[top, :-1, :] * d.dxv[:-1, :]
+    h = 2
+    wet = d.tmask[h:-h, h:-h] == 1
+    surf = (w[nz - 1] - (d.hn[top] - d.ho[top]) / d.dt - div_top * d.iarea)[h:-h, h:-h][wet]
+    scale = np.abs(w[1:nz, h:-h, h:-h][:, wet]).max()
+    assert scale > 0
+    assert np.abs(surf).max() <= 1e-12 * max(scale, np.abs((d.hn - d.ho) / d.dt).max())
