@@ -3,14 +3,16 @@
 Requests (dye-release scenarios: a domain, 1..T point-released FP32 tracers,
 a number of steps and a limiter) arrive over time.  A request is admitted as
 soon as a slot and HBM budget are free; its state then stays on the device.
-Every tick advances all in-flight requests by one step with one ensemble launch
-per kernel kind and limiter group (getm_amd.ensemble), so requests that
-arrive while others run share the same launches, and finished requests leave
-without waiting for the others.  Per-request arithmetic does not depend on
-what else is in flight: a result is the same alone and in the mix, bit for bit.
+Every tick advances all in-flight requests -- by a share of work each (fair
+policy: 1 to 64 steps, by request size) or by one step each (lockstep) -- with
+one ensemble launch per kernel kind and (limiter, steps per tick) group
+(getm_amd.ensemble), so requests that arrive while others run share the same
+launches, and finished requests leave without waiting for the others.
+Per-request arithmetic does not depend on what else is in flight: a result is
+the same alone and in the mix, bit for bit.
 
 With ``conserve=True`` each request's tracer inventories are restored after
-every step to their values at admission (FP64 inventories, FP32 ulp fixer),
+every tick to their values at admission (FP64 inventories, FP32 ulp fixer),
 which holds because the scenario fields keep ho == hn (closed, steady flow).
 
 Latency is measured from a request's arrival to its result being complete on
@@ -116,10 +118,16 @@ def draw_requests(n: int, seed: int = 0, size_range=(32, 192), nz_range=(10, 60)
 class _Slot:
     _serial = 0
 
-    def __init__(self, req: Request, dom: inputs.Domain, device, conserve: bool):
+    def __init__(self, req: Request, dom: inputs.Domain, device, conserve: bool, quantum: Optional[float]):
         _Slot._serial += 1
         self.serial = _Slot._serial
         self.req = req
+        # steps per loop tick: 1 (lockstep), or with a work quantum Q the power of two
+        # nearest Q / (updates per step), 1 .. 64 -- a property of the request alone
+        self.cost = dom.n_wet_columns * dom.nz * req.ntracers
+        self.k = 1
+        if quantum:
+            self.k = int(min(64, max(1, 2 ** round(math.log2(max(quantum / max(self.cost, 1), 1e-9))))))
         self.grid = Grid.from_domain(dom, device=device)
         g = self.grid
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
@@ -150,10 +158,30 @@ class _Slot:
         now = self._inventory()
         return float(((now - self.target).abs() / self.target.abs()).max().item())
 
+    def steps_this_tick(self) -> int:
+        """k, or the largest power of two not above the steps left."""
+        left = self.req.nsteps - self.steps
+        kk = self.k
+        while kk > left:
+            kk //= 2
+        return max(kk, 1)
+
 
 class Server:
+    """policy "fair": every tick each request advances about ``quantum`` tracer
+    cell-updates (k = 1 .. 64 steps, by its size), so a small request is not held
+    to the pace of the largest one in the mix; its inventory is restored once per
+    tick.  policy "lockstep": every request advances one step per tick and is
+    fixed after every step.  Either way a request's result depends only on the
+    request itself, never on the mix."""
+
     def __init__(self, hbm_budget: float = 250e9, max_slots: int = 64, conserve: bool = True,
-                 keep_results: bool = False, tuning: Optional[Sequence[int]] = None, device=None):
+                 keep_results: bool = False, tuning: Optional[Sequence[int]] = None, device=None,
+                 policy: str = "lockstep", quantum: float = 2.0 ** 25):
+        if policy not in ("fair", "lockstep"):
+            raise ValueError("policy must be 'fair' or 'lockstep'")
+        self.policy = policy
+        self.quantum = float(quantum) if policy == "fair" else None
         self.hbm_budget = float(hbm_budget)
         self.max_slots = int(max_slots)
         self.conserve = conserve
@@ -163,19 +191,22 @@ class Server:
         self._groups: Dict[int, Ensemble] = {}
         self._members: Dict[int, tuple] = {}
 
-    def _ensemble(self, scheme: int, slots: List[_Slot]) -> Ensemble:
+    def _ensemble(self, group: tuple, slots: List[_Slot]) -> Ensemble:
+        """The ensemble of one (limiter, steps per tick) group, rebuilt when its members change."""
         key = tuple(s.serial for s in slots)
-        if self._members.get(scheme) != key:
+        if self._members.get(group) != key:
             torch.cuda.synchronize()  # the previous ensemble of this group may still be running
-            ens = Ensemble([s.grid for s in slots], [s.req.ntracers for s in slots], scheme=scheme)
+            ens = Ensemble([s.grid for s in slots], [s.req.ntracers for s in slots], scheme=group[0])
             for i, s in enumerate(slots):
                 ens.bind(i, s.tracers, s.u, s.v, s.w, s.nuh, s.dt)
             if self.tuning:
-                ens.set_tuning(*self.tuning)
+                ens.set_tuning(*self.tuning[:4])
+                if len(self.tuning) > 4:
+                    ens.set_solver(self.tuning[4])
             ens.commit()
-            self._groups[scheme] = ens
-            self._members[scheme] = key
-        return self._groups[scheme]
+            self._groups[group] = ens
+            self._members[group] = key
+        return self._groups[group]
 
     def run(self, requests: Sequence[Request], domains: Optional[Dict[int, inputs.Domain]] = None,
             log=None, clock: str = "wall") -> List[Result]:
@@ -213,7 +244,7 @@ class Server:
                 if used + need > self.hbm_budget:
                     break
                 pending.pop(0)
-                active.append(_Slot(req, domains[req.rid], self.device, self.conserve))
+                active.append(_Slot(req, domains[req.rid], self.device, self.conserve, self.quantum))
                 used += need
                 admitted_at[req.rid] = now()
             self.peak_active = max(self.peak_active, len(active))
@@ -225,14 +256,20 @@ class Server:
                     else:
                         time.sleep(min(max(pending[0].arrival - now(), 0.0), 0.05))
                 continue
-            # one tick: every in-flight request advances one step
-            for scheme in sorted({s.req.scheme for s in active}):
-                group = [s for s in active if s.req.scheme == scheme]
-                self._ensemble(scheme, group).step()
+            # one tick: every in-flight request advances its steps for this tick, the
+            # requests of one (limiter, steps) group in one ensemble launch per kernel kind
+            groups: Dict[tuple, List[_Slot]] = {}
             for s in active:
-                if self.conserve:
-                    s.fix()
-                s.steps += 1
+                groups.setdefault((s.req.scheme, s.steps_this_tick()), []).append(s)
+            for group in sorted(groups):
+                ens = self._ensemble(group, groups[group])
+                for _ in range(group[1]):
+                    ens.step()
+            for group in sorted(groups):
+                for s in groups[group]:
+                    if self.conserve:
+                        s.fix()
+                    s.steps += group[1]
             ticks += 1
             done = [s for s in active if s.steps >= s.req.nsteps]
             if done:
@@ -247,10 +284,11 @@ class Server:
                     if log:
                         log(f"[serve] request {s.req.rid} done: {s.steps} steps, latency {res.latency:.3f}")
                 active = [s for s in active if s.steps < s.req.nsteps]
-                for scheme in list(self._groups):
-                    if not any(s.req.scheme == scheme for s in active):
-                        del self._groups[scheme]  # releases the group's scratch and bound tensors
-                        del self._members[scheme]
+            live = {(s.req.scheme, s.steps_this_tick()) for s in active}
+            for group in list(self._groups):
+                if group not in live:
+                    del self._groups[group]  # releases the group's scratch and bound tensors
+                    del self._members[group]
         self._groups.clear()
         self._members.clear()
         return results
@@ -381,6 +419,9 @@ def main(argv=None):
     ap.add_argument("--steps", default="50,2000")
     ap.add_argument("--no-verify", action="store_true", help="skip the FP64 check of the saturation run")
     ap.add_argument("--no-conserve", action="store_true", help="no inventory fixer (raw FP32 steps)")
+    ap.add_argument("--policy", choices=["fair", "lockstep"], default="lockstep")
+    ap.add_argument("--quantum", type=float, default=2.0 ** 25,
+                    help="tracer cell-updates per request per tick under the fair policy")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
@@ -391,11 +432,12 @@ def main(argv=None):
     log = lambda m: print(m, file=sys.stderr, flush=True)
     log(f"[serve] generating {len(reqs)} scenarios")
     domains = {r.rid: r.domain() for r in reqs}
-    server = Server(hbm_budget=args.budget_gb * 1e9, max_slots=args.max_slots, conserve=not args.no_conserve)
+    server = Server(hbm_budget=args.budget_gb * 1e9, max_slots=args.max_slots, conserve=not args.no_conserve,
+                    policy=args.policy, quantum=args.quantum)
     report = sweep(reqs, domains, [float(x) for x in args.fractions.split(",") if x], server,
                    verify_results=not args.no_verify, seed=args.seed, log=log)
     report["requests"].update(size=args.size, layers=args.layers, tracers=args.tracers, steps=args.steps,
-                              conserve=not args.no_conserve)
+                              conserve=not args.no_conserve, policy=args.policy, quantum=args.quantum)
     print(json.dumps(report))
     if args.out:
         with open(args.out, "w") as fh:

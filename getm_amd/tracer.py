@@ -17,7 +17,7 @@ from typing import List, Optional
 import torch
 
 from . import _lib, operators
-from .core import CENTERS, Array, Grid
+from .core import CENTERS, INTERFACES, Array, Grid
 
 _DTYPES = {"f64": (torch.float64, 1), "f32": (torch.float32, 2)}
 
@@ -36,6 +36,7 @@ class TracerCollection:
         self.dtype, code = _DTYPES[precision]
         self._molecular: List[float] = []
         self._names: List[str] = []
+        self._extra: List[tuple] = []  # (source, surface_flux, source_scale, vertical_velocity) per tracer
         self.values = torch.empty((0, grid.nz_, grid.ny_, grid.nx_), dtype=self.dtype, device=grid.device)
         self.h = torch.empty((grid.nz_, grid.ny_, grid.nx_), dtype=torch.float64, device=grid.device)
         self.Ah_u: Optional[Array] = None
@@ -54,6 +55,8 @@ class TracerCollection:
             self._advection = operators.Advection(grid, scheme=advection_scheme)
             self._advection.ufirst = ufirst
             self._vertical_diffusion = operators.VerticalDiffusion(grid, cnpar=cnpar)
+            self._wcomb = grid.array(z=INTERFACES, fill=0.0)  # tracer.py:143 _w
+            self._source = grid.array(z=CENTERS, fill=0.0)  # tracer.py:140 _source
 
     def __del__(self):
         p = getattr(self, "p", None)
@@ -67,9 +70,23 @@ class TracerCollection:
     def __getitem__(self, i: int) -> Array:
         return Array(self.grid, self.values[i], CENTERS, self._names[i])
 
-    def add(self, values=None, molecular_diffusivity: float = 0.0, name: str = "") -> Array:
-        """Append a tracer (device tensor or array of shape (nz, ny_, nx_)); returns its view."""
+    def add(self, values=None, molecular_diffusivity: float = 0.0, name: str = "", source: Optional[Array] = None,
+            surface_flux: Optional[Array] = None, source_scale: float = 1.0,
+            vertical_velocity: Optional[Array] = None) -> Array:
+        """Append a tracer (device tensor or array of shape (nz, ny_, nx_)); returns its view.
+
+        ``source`` (layers), ``surface_flux`` (2-D), ``source_scale`` and
+        ``vertical_velocity`` (layers; movement through the water, e.g. sinking)
+        have the meaning of pygetm.tracer.Tracer's (tracer.py:35-95) and are
+        applied by :meth:`advance` as TracerCollection.advance does; they are
+        supported by the per-tracer operator sequence (``batched=False``)."""
         g = self.grid
+        if source is not None and (source.grid is not g or source.z != CENTERS):
+            raise ValueError("source must be a layer field of the tracer's grid")
+        if surface_flux is not None and (surface_flux.grid is not g or surface_flux.z):
+            raise ValueError("surface_flux must be a 2-D field of the tracer's grid")
+        if vertical_velocity is not None and (vertical_velocity.grid is not g or vertical_velocity.z != CENTERS):
+            raise ValueError("vertical_velocity must be a layer field of the tracer's grid")
         new = torch.empty((len(self) + 1, g.nz_, g.ny_, g.nx_), dtype=self.dtype, device=g.device)
         new[:-1].copy_(self.values)
         if values is not None:
@@ -77,6 +94,7 @@ class TracerCollection:
         self.values = new
         self._molecular.append(float(molecular_diffusivity))
         self._names.append(name)
+        self._extra.append((source, surface_flux, float(source_scale), vertical_velocity))
         return self[len(self) - 1]
 
     def set_values(self, values: torch.Tensor, molecular_diffusivity: float = 0.0) -> None:
@@ -87,21 +105,49 @@ class TracerCollection:
         self.values = values.to(device=g.device, dtype=self.dtype).contiguous().clone()
         self._molecular = [float(molecular_diffusivity)] * values.shape[0]
         self._names = [""] * values.shape[0]
+        self._extra = [(None, None, 1.0, None)] * values.shape[0]
 
     def advance(self, timestep: float, u: Array, v: Array, w: Array, diffusivity: Array) -> None:
         """tracer.py:193-246: advection of all tracers with velocities at t + dt/2,
         then vertical diffusion with dzo = dzn = hn (tracer.py:248-276)."""
         if not self.batched:
             tracers = [self[i] for i in range(len(self))]
-            self._advection.apply_3d_batch(u, v, w, timestep, tracers, Ah_u=self.Ah_u, Ah_v=self.Ah_v)
+            extra = dict(zip(map(id, tracers), self._extra))
+
+            def combined_w(tracer: Array) -> Array:
+                # tracer.py:223-231: w + the tracer's own vertical velocity at interfaces
+                # (interpolation.F90:41-54: 0.5 (c_k + c_{k+1}); end interfaces kept)
+                vv = extra[id(tracer)][3]
+                if vv is None or not bool(vv.all_values.any()):
+                    return w
+                wc = self._wcomb.all_values
+                wc[1:-1] = 0.5 * (vv.all_values[:-1] + vv.all_values[1:])
+                wc += w.all_values
+                return self._wcomb
+
+            self._advection.apply_3d_batch(u, v, w, timestep, tracers, Ah_u=self.Ah_u, Ah_v=self.Ah_v,
+                                           get_w=combined_w)
             self.h.copy_(self._advection.h)
             avmol = -1.0
-            for tracer, mol in zip(tracers, self._molecular):
+            for tracer, mol, (source, flux, scale, _) in zip(tracers, self._molecular, self._extra):
+                ea4 = None
+                if source is not None or flux is not None:
+                    # tracer.py:263-272
+                    ea4 = self._source.all_values
+                    if source is not None:
+                        ea4.copy_(source.all_values)
+                    else:
+                        ea4.zero_()
+                    if flux is not None:
+                        ea4[-1] += flux.all_values
+                    ea4 *= timestep * scale
                 if mol != avmol:
                     avmol = mol
                     self._vertical_diffusion.prepare(diffusivity, timestep, avmol)
-                self._vertical_diffusion.apply(tracer)
+                self._vertical_diffusion.apply(tracer, ea4=self._source if ea4 is not None else None)
             return
+        if any(e[0] is not None or e[1] is not None or e[3] is not None for e in self._extra):
+            raise ValueError("tracer sources, surface fluxes and vertical velocities need batched=False")
         mols = set(self._molecular)
         if len(mols) > 1:
             raise ValueError("the batched step needs one molecular diffusivity for all tracers")

@@ -27,9 +27,8 @@ def domains():
     return {r.rid: r.domain() for r in REQUESTS}
 
 
-@pytest.fixture(scope="module")
-def alone(domains):
-    server = Server(keep_results=True)
+def run_alone(domains, **kw):
+    server = Server(keep_results=True, **kw)
     out = {}
     for r in REQUESTS:
         (res,) = server.run([dataclasses.replace(r, arrival=0)], domains, clock="ticks")
@@ -38,7 +37,13 @@ def alone(domains):
     return out
 
 
+@pytest.fixture(scope="module")
+def alone(domains):
+    return run_alone(domains, policy="lockstep")
+
+
 def run_mix(domains, **kw):
+    kw.setdefault("policy", "lockstep")
     server = Server(keep_results=True, **kw)
     results = server.run(REQUESTS, domains, clock="ticks")
     return server, {r.rid: r for r in results}
@@ -105,3 +110,18 @@ def test_large_point_release_conserved_and_close_to_fp64():
     assert res.inventory_drift <= 1e-12, res.inventory_drift
     errs = verify([res], [req], dom)
     assert errs[7] <= 1e-5, errs
+
+
+def test_fair_policy_mix_equals_alone_bitwise(domains):
+    """Fair sharing: small requests take several steps per tick (their share of the
+    work quantum), so they finish in fewer ticks than steps; results still do not
+    depend on the mix."""
+    kw = dict(policy="fair", quantum=6e4)
+    solo = run_alone(domains, **kw)
+    server, mix = run_mix(domains, **kw)
+    assert server.peak_active >= 2
+    for rid, res in mix.items():
+        assert torch.equal(res.tracers, solo[rid].tracers), f"request {rid} differs in the mix"
+        assert res.inventory_drift <= 1e-12
+    # at least one small request advanced more than one step per tick
+    assert any(r.finished - r.admitted < r.steps for r in mix.values())
