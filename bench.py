@@ -16,6 +16,11 @@ in profiles/hbm_traffic_*.md for this workload (null if none is recorded).
 serving: a short run of the serving loop (getm_amd.serve) on 24 dye-release
 requests: saturation throughput, then Poisson arrivals at 80% of it, with
 latency percentiles, inventory drift and the error against the FP64 method.
+accuracy (FP32 runs): the first (up to 16) tracers, which evolve independently
+of the others, rerun from the same initial values for --accuracy-steps steps in
+FP64 (the plain method), FP32 as benchmarked, and FP32 with the inventory fixer:
+max |c32 - c64| / max |c64| over wet cells, and the largest per-step inventory
+defect |I_after - I_before| / |I_before| (FP64 compensated sums).
 cpu_baseline: the reference's Fortran sweeps (oracle/_ref) with the C restatement
 of the diffusion, FP64, on bounded slabs of the same workload, one process per
 host core (the C restatement alone where oracle/_ref is not built).
@@ -51,6 +56,33 @@ def recorded_traffic(kernel: str, workload: str):
             if m and m.group(1) == workload and m.group(2) == kernel:
                 return float(m.group(3)), os.path.relpath(fn, ROOT)
     return None
+
+
+def accuracy_check(d, grid, fields, init, scheme, steps, rounding, log):
+    """FP32 error against the FP64 method and inventory defects (module docstring)."""
+    import numpy as np
+    import torch
+
+    from getm_amd import precision
+
+    wet = torch.from_numpy(np.broadcast_to(d.tmask == 1, (grid.nz_, grid.ny_, grid.nx_)).copy()).to(grid.device)
+    ref, d64, _ = precision.run(d, grid, fields, init, scheme, steps, "f64", False)
+    out = {"steps": steps, "tracers_checked": int(init.shape[0]), "fp64_max_step_inventory_defect": d64}
+
+    def err(tc):
+        e = 0.0
+        for t in range(init.shape[0]):
+            r = ref.values[t][wet]
+            e = max(e, float((tc.values[t][wet].double() - r).abs().max() / r.abs().max()))
+        return e
+
+    for name, conserve in (("plain", False), ("with_fixer", True)):
+        tc, dd, _ = precision.run(d, grid, fields, init, scheme, steps, "f32", conserve, rounding)
+        out[name] = {"max_error_vs_fp64": err(tc), "max_step_inventory_defect": dd}
+        del tc
+        torch.cuda.empty_cache()
+    log(f"[bench] accuracy over {steps} steps: {out}")
+    return out
 
 
 def serving_run(nreq: int, log):
@@ -194,13 +226,91 @@ def blob_tracers(ntr, grid, seed, dtype):
     return out
 
 
+def run_ensemble(args, log):
+    """96 uneven domains (nx, ny logU[64, 768], nz U[10, 60], 1-32 blob tracers, fields as
+    the coastal workload from per-domain seeds), SUPERBEE, FP32 with FP64 thickness,
+    stepped by one launch per kernel kind for all domains; the timed region replays one
+    hipGraph of ``steps`` steps (captured once, after the warm-up steps)."""
+    import numpy as np
+    import torch
+
+    from getm_amd import inputs
+    from getm_amd.autotune import Autotuner
+    from getm_amd.core import CENTERS, INTERFACES, Grid
+    from getm_amd.ensemble import Ensemble
+
+    torch.cuda.set_device(0)
+    specs = inputs.ensemble_specs(ndomains=args.domains, seed=3)
+    grids, tracers, keep = [], [], []
+    t_gen = time.perf_counter()
+    for nx, ny, nz, nt, seed in specs:
+        d = inputs.coastal(nx=nx, ny=ny, nz=nz, dx=2000.0, dt=30.0, scheme=5, seed=seed, ntracers=0)
+        g = Grid.from_domain(d, device="cuda:0")
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        keep.append((d.dt, g.ugrid.wrap(dev(d.u), CENTERS), g.vgrid.wrap(dev(d.v), CENTERS),
+                     g.wrap(dev(d.w), INTERFACES), g.wrap(dev(d.nuh), INTERFACES)))
+        grids.append(g)
+        tracers.append(blob_tracers(nt, g, seed, torch.float32))
+    log(f"[bench] {len(specs)} domains generated in {time.perf_counter() - t_gen:.1f} s")
+    ens = Ensemble(grids, [s[3] for s in specs], scheme=5)
+    for i, (dt, u, v, w, nuh) in enumerate(keep):
+        ens.bind(i, tracers[i], u, v, w, nuh, dt)
+    ens.set_rounding(args.rounding)
+    ens.commit()
+    shape = None
+    if args.autotune:
+        shape = Autotuner(None).tune_ensemble(ens, reps=2)
+        log(f"[bench] autotuner picked {shape}")
+    for _ in range(args.warmup):
+        ens.step()
+    torch.cuda.synchronize()
+    graph = ens.graph_chunk(args.steps)
+    graph.replay()  # first replay uploads the graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    graph.replay()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    cells = ens.cells
+    value = cells * args.steps / elapsed
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "tracer cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded coastal-like domains, device-generated blob tracers)",
+        "config": {
+            "workload": f"uneven ensemble of {len(specs)} domains (nx, ny logU[64,768], nz U[10,60], 1-32 "
+                        f"tracers), SUPERBEE, FP32 tracers with FP64 thickness, {args.steps}-step hipGraph chunk",
+            "domains": len(specs),
+            "tracer_cells_per_step": cells,
+            "tuning": list(shape) if shape else list(ens.tuning),
+            "autotuned": bool(args.autotune),
+            "rounding": args.rounding,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["coastal", "multi"], default="coastal",
-                    help="coastal: 1024x768x40 with T and S, SUPERBEE; multi: same grid, 256 blob tracers, P2_PDM")
+    ap.add_argument("--workload", choices=["coastal", "multi", "ensemble"], default="coastal",
+                    help="coastal: 1024x768x40 with T and S, SUPERBEE; multi: same grid, 256 blob tracers, P2_PDM; "
+                         "ensemble: 96 uneven domains in one launch per kernel kind, hipGraph chunk")
+    ap.add_argument("--domains", type=int, default=96, help="ensemble workload: number of domains")
+    ap.add_argument("--autotune", action="store_true", help="ensemble workload: autotune the launch shape first")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--tracers", type=int, default=0, help="override the workload's tracer count")
     ap.add_argument("--scheme", type=int, default=0, help="override the workload's scheme")
@@ -208,6 +318,11 @@ def main():
     ap.add_argument("--tuning", default="", help="tracers per block h,w,d of the batched kernels (0 = all)[,tile height]")
     ap.add_argument("--solver", type=int, default=0,
                     help="wave-PCR column solve for columns of up to this many layers (0: Thomas only)")
+    ap.add_argument("--rounding", choices=["stochastic", "nearest"], default="stochastic",
+                    help="FP32 tracer stores: stochastic (default) or round-to-nearest")
+    ap.add_argument("--accuracy-steps", type=int, default=100,
+                    help="FP32 runs: steps of the error check against the FP64 method")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -218,6 +333,9 @@ def main():
         print("bench.py measures one GPU: this build runs the step on a single undivided domain; "
               "multi-GPU sharding is described in DESIGN.md and not implemented.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "ensemble":
+        return run_ensemble(args, lambda m: print(m, file=sys.stderr, flush=True))
 
     import numpy as np
     import torch
@@ -252,12 +370,16 @@ def main():
         tc.set_tuning(*[int(x) for x in args.tuning.split(",")])
     if args.solver:
         tc.set_solver(args.solver)
+    tc.set_rounding(args.rounding)
     if kind == "blobs":
         tc.set_values(blob_tracers(ntr, grid, seed=7, dtype=tc.dtype))
     else:
         tc.set_values(dev(d.tracers))
     del d.u, d.v, d.w, d.nuh, d.tracers
     s_bytes = 4 if args.precision == "f32" else 8
+    # initial values of the tracers whose FP32 error is checked (tracers evolve independently)
+    n_check = min(ntr, 16)
+    init_check = tc.values[:n_check].clone() if args.precision == "f32" and not args.no_accuracy else None
 
     def step():
         tc.advance(d.dt, u, v, w, nuh)
@@ -293,6 +415,10 @@ def main():
           ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in ktimes.items()), file=sys.stderr, flush=True)
 
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    accuracy = None
+    if init_check is not None:
+        accuracy = accuracy_check(d, grid, (u, v, w, nuh), init_check, scheme, args.accuracy_steps, args.rounding, log)
+        del init_check
     workload_key = f"{args.workload}-{args.precision}-T{args.tracers}-s{args.scheme}" + (
         "-pertracer" if args.per_tracer else "")
     rec = recorded_traffic(dom, workload_key)
@@ -337,6 +463,7 @@ def main():
             "tracers": args.tracers,
             "scheme": args.scheme,
             "precision": args.precision,
+            "rounding": args.rounding if args.precision == "f32" else None,
             "step_hbm_fraction": b_step / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
             "kernel_ms_per_step": {k: v[0] / args.steps for k, v in ktimes.items()},
             "tuning": args.tuning or "default",
@@ -354,6 +481,7 @@ def main():
             "traffic_source": rec[1] if rec else None,
             "algorithmic_bytes": alg,
         },
+        "accuracy": accuracy,
         "serving": serving,
         "cpu_baseline": cpu,
     }

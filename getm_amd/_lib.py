@@ -47,6 +47,7 @@ _SIGNATURES = {
     "transport_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
     "c_w_momentum_3d": (_i, [_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
     "transport_set_solver": (_i, [_vp, _i]),
+    "transport_set_rounding": (_i, [_vp, _i, _vp]),
     "transport_inventory": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "transport_conserve": (_i, [_vp, _i, _vp, _vp, _vp]),
     "ensemble_create": (_vp, [_i, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
@@ -55,6 +56,7 @@ _SIGNATURES = {
     "ensemble_bind": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _d, _d]),
     "ensemble_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
     "ensemble_set_solver": (_i, [_vp, _i]),
+    "ensemble_set_rounding": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
     "ensemble_commit": (_i, [_vp, _vp]),
     "ensemble_step": (_i, [_vp, _vp]),
     "ensemble_thickness": (_vp, [_vp, _i]),

@@ -282,7 +282,9 @@ void* transport_create(void* tgrid, void* ugrid, void* vgrid, int scheme, double
     tr->tgrid = t;
     tr->ugrid = gu;
     tr->vgrid = gv;
-    if (tr->stats.ensure(2 * sizeof(int)) != GETM_OK || hipMemset(tr->stats.ptr, 0, 2 * sizeof(int)) != hipSuccess) {
+    if (tr->stats.ensure(2 * sizeof(int)) != GETM_OK || hipMemset(tr->stats.ptr, 0, 2 * sizeof(int)) != hipSuccess ||
+        tr->rstep_own.ensure(sizeof(unsigned long long)) != GETM_OK ||
+        hipMemset(tr->rstep_own.ptr, 0, sizeof(unsigned long long)) != hipSuccess) {
         delete tr;
         return nullptr;
     }
@@ -293,7 +295,8 @@ void transport_finalize(void* p)
 {
     Transport* tr = static_cast<Transport*>(p);
     if (!tr) return;
-    for (DeviceBuffer* b : {&tr->scratch, &tr->hA, &tr->hB, &tr->stats, &tr->auxo, &tr->auxn, &tr->ru, &tr->part, &tr->part2, &tr->ratio, &tr->resid})
+    for (DeviceBuffer* b : {&tr->scratch, &tr->hA, &tr->hB, &tr->stats, &tr->auxo, &tr->auxn, &tr->ru, &tr->part,
+                            &tr->part2, &tr->ratio, &tr->resid, &tr->rstep_own})
         b->release();
     delete tr;
 }
@@ -328,6 +331,7 @@ static StepArgs step_args(const Transport* tr, const double* u, const double* v,
     A.hn = t->hn;
     A.ah_u = Ah_u;
     A.ah_v = Ah_v;
+    A.rstep = nullptr;
     return A;
 }
 
@@ -349,15 +353,19 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     const int d1 = tr->ufirst ? 1 : 2;  // first direction of stages 1-2 (operators.py:118-121)
     const int d2 = 3 - d1;
     const int ty = tr->tile_y ? tr->tile_y : auto_tile_y(ntr);
-    if ((rc = launch_hpair<R>(d1, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, s))) return rc;
+    if ((rc = launch_hpair<R>(d1, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, SR_PAIR1, s))) return rc;
     if ((rc = launch_wcol<R>(tr->scheme, A, dt, ntr, tr->tpb_w, S, f, hA, hB, w, w, static_cast<int*>(tr->stats.ptr),
                              s)))
         return rc;
-    if ((rc = launch_hpair<R>(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, s))) return rc;
+    if ((rc = launch_hpair<R>(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, SR_PAIR2, s)))
+        return rc;
     if constexpr (std::is_same<R, float>::value) {
         if (t->nz >= 2 && t->nz <= tr->pcr_max_nz)
-            return launch_vdiff_pcr(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
-        return launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
+            rc = launch_vdiff_pcr(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
+        else
+            rc = launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
+        if (rc) return rc;
+        return launch_count_step(const_cast<unsigned long long*>(A.rstep), s);
     } else {
         // FP64: the reference-order per-tracer diffusion kernels, prepared once (tracer.py:273-276)
         GETM_HIP_OK(hipMemcpyAsync(f, S, n3 * (size_t)ntr * sizeof(R), hipMemcpyDeviceToDevice, s));
@@ -397,6 +405,8 @@ int transport_step(void* p, int ntracers, void* tracers, const double* u, const 
         return GETM_OK;
     }
     StepArgs A = step_args(tr, u, v, Ah_u, Ah_v);
+    if (tr->precision == GETM_F32 && tr->rounding)
+        A.rstep = tr->rstep_ext ? tr->rstep_ext : static_cast<unsigned long long*>(tr->rstep_own.ptr);
     if (tr->precision == GETM_F32)
         return transport_step_t<float>(tr, A, ntracers, static_cast<float*>(tracers), w, nuh, timestep, molecular,
                                        h_out, s);
@@ -469,6 +479,15 @@ int transport_set_solver(void* p, int pcr_max_nz)
     return GETM_OK;
 }
 
+int transport_set_rounding(void* p, int mode, unsigned long long* step_counter)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr || (mode != GETM_ROUND_NEAREST && mode != GETM_ROUND_STOCHASTIC)) return GETM_ERR_ARG;
+    tr->rounding = mode;
+    tr->rstep_ext = step_counter;
+    return GETM_OK;
+}
+
 int transport_get_stats(void* p, int* max_iters, int* capped_columns, int reset, void* stream)
 {
     Transport* tr = static_cast<Transport*>(p);
@@ -505,8 +524,11 @@ void* ensemble_create(int ndomains, void* const* tgrids, void* const* ugrids, vo
     e->tg = new (std::nothrow) const Grid*[ndomains];
     e->ug = new (std::nothrow) const Grid*[ndomains];
     e->vg = new (std::nothrow) const Grid*[ndomains];
-    if (!e->host || !e->tg || !e->ug || !e->vg || e->stats.ensure(2 * sizeof(int)) != GETM_OK ||
-        hipMemset(e->stats.ptr, 0, 2 * sizeof(int)) != hipSuccess) {
+    e->rstep_ext = new (std::nothrow) unsigned long long*[ndomains]();
+    if (!e->host || !e->tg || !e->ug || !e->vg || !e->rstep_ext || e->stats.ensure(2 * sizeof(int)) != GETM_OK ||
+        hipMemset(e->stats.ptr, 0, 2 * sizeof(int)) != hipSuccess ||
+        e->rstep_own.ensure((size_t)ndomains * sizeof(unsigned long long)) != GETM_OK ||
+        hipMemset(e->rstep_own.ptr, 0, (size_t)ndomains * sizeof(unsigned long long)) != hipSuccess) {
         ensemble_finalize(e);
         return nullptr;
     }
@@ -523,8 +545,11 @@ void ensemble_finalize(void* p)
 {
     Ensemble* e = static_cast<Ensemble*>(p);
     if (!e) return;
-    for (DeviceBuffer* b : {&e->doms, &e->work_h, &e->work_w, &e->work_d, &e->scratch, &e->hbuf, &e->stats}) b->release();
+    for (DeviceBuffer* b : {&e->doms, &e->work_h, &e->work_w, &e->work_d, &e->scratch, &e->hbuf, &e->stats,
+                            &e->rstep_own})
+        b->release();
     delete[] e->host;
+    delete[] e->rstep_ext;
     delete[] e->tg;
     delete[] e->ug;
     delete[] e->vg;
@@ -579,6 +604,16 @@ int ensemble_set_solver(void* p, int pcr_max_nz)
     return GETM_OK;
 }
 
+int ensemble_set_rounding(void* p, int mode, unsigned long long* const* step_counters)
+{
+    Ensemble* e = static_cast<Ensemble*>(p);
+    if (!e || (mode != GETM_ROUND_NEAREST && mode != GETM_ROUND_STOCHASTIC)) return GETM_ERR_ARG;
+    e->rounding = mode;
+    for (int d = 0; d < e->ndom; d++) e->rstep_ext[d] = step_counters ? step_counters[d] : nullptr;
+    e->committed = 0;
+    return GETM_OK;
+}
+
 static int upload_work(DeviceBuffer& buf, const std::vector<int4>& w)
 {
     if (w.empty()) return GETM_OK;
@@ -617,6 +652,10 @@ int ensemble_commit(void* p, void* stream)
         D.hB = hb + t->n3();
         hb += 2 * t->n3();
         D.stats = static_cast<int*>(e->stats.ptr);
+        D.rstep = nullptr;
+        if (e->rounding)
+            D.rstep = e->rstep_ext[d] ? e->rstep_ext[d] : static_cast<unsigned long long*>(e->rstep_own.ptr) + d;
+        D.A.rstep = D.rstep;
         D.tpb_h = (e->tpb_h <= 0 || e->tpb_h > D.ntr) ? D.ntr : e->tpb_h;
         D.tpb_w = (e->tpb_w <= 0 || e->tpb_w > D.ntr) ? D.ntr : e->tpb_w;
         D.tpb_d = (e->tpb_d <= 0 || e->tpb_d > D.ntr) ? D.ntr : e->tpb_d;
@@ -707,6 +746,7 @@ int ensemble_step(void* p, void* stream)
         if ((rc = launch_vdiff_pcr_multi(e->cnpar == 1.0, c, doms, wd, e->nwork_p_class[c], s))) return rc;
         wd += e->nwork_p_class[c];
     }
+    if (e->rounding) return launch_count_steps_multi(doms, e->ndom, s);
     return GETM_OK;
 }
 

@@ -103,6 +103,62 @@ __device__ __forceinline__ T& at32(T* base, unsigned byte_off)
     return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
 }
 
+// ---------------------------------------------------------------------------
+// Unbiased (stochastic) rounding of FP32 tracer stores.  In a nearly steady flow
+// the FP32 update of a cell sees nearly the same operands every step, so
+// rounding to nearest errs the same way step after step: an increment below half
+// an ulp is lost every time and the error grows linearly with the step count.
+// Rounding f + d up or down with probability given by the exact residual makes
+// the expected stored value exact, so rounding errors become a random walk.
+// The random bits are a hash of (step, stage, tracer, cell): results are
+// deterministic and do not depend on launch shapes, batching or graph replays.
+// ---------------------------------------------------------------------------
+// 32-bit integer hash (lowbias32 of the hash-prospector family)
+__device__ __host__ __forceinline__ unsigned sr_hash(unsigned x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// Seed of one rounding site of one step: step count and stage within the step.
+__device__ __host__ __forceinline__ unsigned sr_stage_seed(unsigned long long step, unsigned stage)
+{
+    return sr_hash(sr_hash((unsigned)step ^ sr_hash(stage * 0x85ebca6bu + 0x632be5abu)) ^ (unsigned)(step >> 32));
+}
+
+// Seed of one tracer at one rounding site.
+__device__ __forceinline__ unsigned sr_tracer_seed(unsigned stage_seed, int tracer)
+{
+    return sr_hash(stage_seed ^ ((unsigned)tracer * 0x9e3779b9u));
+}
+
+// f + d in FP32, rounded stochastically with the random bits of cell `cell`.
+// The FP64 sum of two floats is exact unless |d| < 2^-29 |f| (then its own
+// rounding is far below an FP32 ulp); adding 29 uniform bits below the FP32
+// mantissa and truncating rounds away from zero with probability equal to the
+// discarded fraction.  FP32-subnormal results take the conversion's rounding.
+__device__ __forceinline__ float sr_add(float f, float d, unsigned tracer_seed, unsigned cell)
+{
+    const double x = (double)f + (double)d;
+    unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    b += sr_hash(cell ^ tracer_seed) >> 3;
+    b &= ~((1ull << 29) - 1ull);
+    return (float)__longlong_as_double((long long)b);
+}
+
+// Stochastic rounding on (seed != 0 tracer seed) or round to nearest.
+struct SRound {
+    unsigned seed;  // 0: round to nearest
+    __device__ __forceinline__ float add(float f, float d, unsigned cell) const
+    {
+        return seed ? sr_add(f, d, seed, cell) : f + d;
+    }
+};
+
 }  // namespace getm
 
 // Instantiate a templated launcher for every scheme; `body` is called with a

@@ -86,7 +86,14 @@ struct StepArgs {
     const double* hn;
     const double* ah_u;  // both NULL or both set
     const double* ah_v;
+    // FP32 tracer stores: NULL rounds to nearest; else stochastic rounding seeded
+    // by this device step counter (advanced by one after every step)
+    const unsigned long long* rstep;
 };
+
+// Rounding sites of one step (stage ids of the stochastic-rounding seeds):
+// stage pair 1 -> 1, 2; stage pair 2 -> 3, 4; column solve -> 5; w sub-step it -> 16 + it.
+enum { SR_PAIR1 = 1, SR_PAIR2 = 3, SR_VDIFF = 5, SR_WCOL = 16 };
 
 // One domain of an ensemble as the ensemble kernels read it (device table).
 template <typename R>
@@ -102,6 +109,7 @@ struct EnsDom {
     double dt, cnpar, molecular;
     int ntr, tpb_h, tpb_w, tpb_d;
     int* stats;
+    unsigned long long* rstep;  // = A.rstep, advanced after every step (NULL: round to nearest)
 };
 
 // Batched T-tracer step object (transport_create).
@@ -119,6 +127,9 @@ struct Transport {
     int tile_y = 0;                         // height of the horizontal-stage tiles (4, 8, 16; 0 = by tracer count)
     int pcr_max_nz = 0;                     // FP32 column solve by wave PCR for 2 <= nz <= this (<= 32; 0: Thomas only)
     DeviceBuffer part, part2, ratio, resid;  // inventory partial sums, fixer ratios and residuals
+    int rounding = 1;                        // FP32 stores: 1 stochastic (default), 0 nearest
+    DeviceBuffer rstep_own;                  // the object's own step counter (zeroed at creation)
+    unsigned long long* rstep_ext = nullptr; // caller's counter (transport_set_rounding), else rstep_own
 };
 
 // Ensemble of independent domains stepped by one launch per kernel kind
@@ -138,6 +149,9 @@ struct Ensemble {
     const Grid** ug = nullptr;
     const Grid** vg = nullptr;
     DeviceBuffer doms, work_h, work_w, work_d, scratch, hbuf, stats;
+    int rounding = 1;                       // as Transport::rounding
+    DeviceBuffer rstep_own;                 // ndom step counters of the ensemble's own (zeroed at creation)
+    unsigned long long** rstep_ext = nullptr;  // per-domain caller counters (NULL entries: own)
 };
 
 // Default tile height of the horizontal stage pair: with few tracers the
@@ -149,7 +163,7 @@ inline int auto_tile_y(int tracers_per_domain) { return tracers_per_domain <= 8 
 // tpb = tracers per block (tracer batches go into the grid); <= 0 means all
 template <typename R>
 int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
-                 const double* hin, double* hout, hipStream_t s);
+                 const double* hin, double* hout, unsigned sr_stage, hipStream_t s);
 int launch_hpair_multi(int stage, int d1, int scheme, int ty, const EnsDom<float>* doms, const int4* work, int nwork,
                        hipStream_t s);
 int launch_wcol_multi(int scheme, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
@@ -164,6 +178,9 @@ int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, cons
                 double* hout, const double* w, const double* wv, int* stats, hipStream_t s);
 int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
                      const float* fin, float* fout, hipStream_t s);
+// advance the step counter of the stochastic rounding (single counter, or every domain's)
+int launch_count_step(unsigned long long* counter, hipStream_t s);
+int launch_count_steps_multi(const EnsDom<float>* doms, int ndom, hipStream_t s);
 
 // ---- inventories (inventory.hip) ----
 template <typename R>

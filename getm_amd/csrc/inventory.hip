@@ -12,17 +12,21 @@
 // D = M - I is shared in proportion to each cell's mass, which in units of
 // the cell's ulp is q_i = (D/I) * c_i / ulp(c_i); with S_i the prefix sum of
 // c/ulp in a fixed cell order, cell i changes by
-//   n_i = rint((D/I) S_i) - rint((D/I) S_{i-1})  ulps   (n_i in {-1, 0, 1} in practice),
-// which is exact in FP32 and touches ~|D/I| * 2^23 cells per tracer.  Flipped
-// cells carry different masses, so a residual of a few cell-ulps remains (of
-// the order of the largest cell's ulp mass when a tracer sits in a few cells,
-// e.g. a fresh point release).  The fix kernel accumulates the fixed
-// inventory and each block's largest-mass cell on the way; a greedy pass (one
-// wave per tracer, cells decided in a fixed order) then walks the wet cells
-// from the column of the tracer's largest-mass cell, changing each by
-// rint(R / ulp mass) ulps (at most 4096, i.e. 5e-4 of the cell's own value), so
-// the residual R never grows and ends below half the ulp mass of the smallest
-// cell it passes.
+//   n_i = floor((D/I) S_i + u) - floor((D/I) S_{i-1} + u)  ulps   (n_i in {-1, 0, 1} in practice),
+// which is exact in FP32 and touches ~|D/I| * 2^23 cells per tracer.  The phase
+// u in [0, 1) is a hash of the bits of D/I, so a defect that repeats from step
+// to step does not flip the same cells every time (expected n_i = (D/I) q_i).
+// Flipped cells carry different masses, so a residual of a few tens of
+// cell-ulps remains (of the order of the largest cell's ulp mass when a tracer
+// sits in a few cells, e.g. a fresh point release).  The fix kernel
+// accumulates the fixed inventory and each block's largest-mass cell on the
+// way; a greedy pass (one wave per tracer, cells decided in a fixed order)
+// then changes cells near the tracer's largest-mass cell: first by at most one
+// ulp each, over the 64-column chunk of that cell in an order rotated by a hash
+// of the residual (so no cell takes the residual step after step), then, if a
+// residual remains, by rint(R / ulp mass) ulps (at most 4096, i.e. 5e-4 of the
+// cell's own value) along a wider walk.  Either way the residual R never grows
+// and ends below half the ulp mass of the smallest cell passed.
 #include <type_traits>
 
 #include "common.h"
@@ -163,6 +167,8 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_f32(const StepArgs A, int n
     const int nblk = gridDim.x * gridDim.y;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x;
     const double r = ratio[t];
+    const unsigned long long rb = (unsigned long long)__double_as_longlong(r);
+    const double u = (double)(sr_hash((unsigned)rb) ^ sr_hash((unsigned)(rb >> 32) + (unsigned)t)) * 0x1p-32;
     bool wet = false;
     size_t p = 0;
     double q = 0.0;
@@ -193,7 +199,7 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_f32(const StepArgs A, int n
         for (int k = 0; k < A.nz; k++) {
             float c = fc[(size_t)k * A.n2];
             const double S1 = S0 + q_of(c);
-            const double n = rint(r * S1) - rint(r * S0);
+            const double n = floor(r * S1 + u) - floor(r * S0 + u);
             if (n != 0.0) {
                 const int e = ilogbf(c);
                 c = c + (float)(n * exp2((double)(e - 23)));
@@ -270,8 +276,10 @@ __global__ void __launch_bounds__(64) k_residual(int ntr, int nblk, const double
     }
 }
 
-// One wave per tracer: greedy changes of up to GREEDY_MAX_ULPS ulps per cell.
-// The walk visits chunks of 64 consecutive columns and, in each, the layers
+// One wave per tracer.  Spread pass: the chunk of the largest-mass cell (row j0,
+// 64 columns), one ulp per cell at most, layers and lanes in an order rotated by
+// a hash of the residual.  Then greedy changes of up to GREEDY_MAX_ULPS ulps per
+// cell if a residual remains.  The walk visits chunks of 64 consecutive columns and, in each, the layers
 // bottom to top; the 64 cells of one layer are loaded together and then decided
 // one lane after the other with the running residual R, which every lane holds.
 // Chunks: first the 64 columns centred on the largest-mass cell in the rows
@@ -298,6 +306,48 @@ __global__ void __launch_bounds__(64) k_greedy_f32(const StepArgs A, int ntr, fl
     const size_t n2 = (size_t)A.n2;
     const size_t p0 = (P >= 0.0 && P < (double)n2) ? (size_t)P : 0;
     const int i0 = (int)(p0 % A.nx_), j0 = (int)(p0 / A.nx_);
+    // ---- spread: one ulp per cell over the chunk of the largest-mass cell (row j0,
+    // columns i0-32 .. i0+31), layers from k_start on and lanes from l_rot on
+    {
+        const unsigned long long rb = (unsigned long long)__double_as_longlong(R);
+        const unsigned hs = sr_hash((unsigned)rb ^ sr_hash((unsigned)(rb >> 32) + (unsigned)t));
+        const int k_start = (int)(hs % (unsigned)A.nz), l_rot = (int)((hs >> 24) & 63u);
+        const int i = i0 - 32 + lane;
+        const bool inr = i >= 0 && i < A.nx_;
+        const size_t p = inr ? (size_t)j0 * A.nx_ + i : 0;
+        const bool col = inr && i >= A.imin && i <= A.imax && j0 >= A.jmin && j0 <= A.jmax && A.tmask[p] == 1;
+        const double a = col ? area[p] : 0.0;
+        for (int kk = 0; kk < A.nz; kk++) {
+            const int k = k_start + kk < A.nz ? k_start + kk : k_start + kk - A.nz;
+            const size_t pk = (size_t)k * n2 + p;
+            float c = col ? ft[pk] : 0.f;
+            const double m = col ? a * A.hn[pk] : 0.0;
+            const int e = ilogbf(c);
+            const bool normal = col && c != 0.f && e >= -126;
+            const double ulp = normal ? exp2((double)(e - 23)) : 0.0;
+            const double wm = m * ulp;
+            unsigned long long cand = __ballot(normal && wm > 0.0 && wm <= 2.0 * fabs(R));
+            // lanes in the order l_rot, l_rot + 1, ..., 63, 0, ..., l_rot - 1
+            cand = l_rot ? (cand >> l_rot) | (cand << (64 - l_rot)) : cand;
+            while (cand) {
+                const int l = (__ffsll((long long)cand) - 1 + l_rot) & 63;
+                cand &= cand - 1;
+                const double wl = __shfl(wm, l);
+                const double n = fmin(fmax(rint(R / wl), -1.0), 1.0);
+                if (n == 0.0) continue;
+                double dR = 0.0;
+                if (lane == l) {
+                    const float cn = c + (float)(n * ulp);
+                    dR = m * ((double)cn - (double)c);
+                    c = cn;
+                    ft[pk] = cn;
+                }
+                R -= __shfl(dR, l);
+                if (fabs(R) <= tol) return;
+            }
+        }
+    }
+    // ---- walk with up to GREEDY_MAX_ULPS per cell
     const size_t nchunks = GREEDY_ROWS + (n2 + 63) / 64;
     size_t visited = 0;
     for (size_t s = 0; s < nchunks && visited < (size_t)max_cells; s++) {

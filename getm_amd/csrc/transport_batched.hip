@@ -235,11 +235,13 @@ struct CellRaw {
 // would accumulate linearly with the step count; in the increment form it is
 // scaled by |hold/hnew - 1| (a Courant number) and the only O(ulp(f)) error is
 // the final, data-dependent rounding.
-// own = LDS offset of the cell, fl1 = offset of its upper face flux.
+// own = LDS offset of the cell, fl1 = offset of its upper face flux, gc = its
+// index in the domain's 3-D array (the cell's stochastic-rounding stream).
 template <typename R>
 struct CellK {
     R a, b, iA;
     int upd, own, fl1;
+    unsigned gc;
 };
 
 template <typename R>
@@ -260,24 +262,39 @@ __device__ __forceinline__ void cell_set(CellK<R>& c, bool upd, double hold, dou
 }
 
 template <typename R>
-__device__ __forceinline__ R cell_update(const CellK<R>& c, double dt, R f, R fl0, R fl1)
+__device__ __forceinline__ R cell_update(const CellK<R>& c, double dt, R f, R fl0, R fl1, const SRound& sr)
 {
     if constexpr (std::is_same<R, double>::value) {
         const double hfo = c.a * f;
         const double advn = (fl1 - fl0) * c.iA;
         return (hfo - dt * advn) / c.b;
     } else {
-        return f + (c.a * f - c.b * (fl1 - fl0));
+        return sr.add(f, c.a * f - c.b * (fl1 - fl0), c.gc);
     }
+}
+
+// Stage seed of a rounding site in this step (0: round to nearest; FP64 never rounds here).
+template <typename R>
+__device__ __forceinline__ unsigned stage_seed(const StepArgs& A, unsigned stage)
+{
+    if constexpr (std::is_same<R, double>::value) return 0u;
+    else return A.rstep ? sr_stage_seed(*A.rstep, stage) : 0u;
+}
+__device__ __forceinline__ SRound tracer_round(unsigned stage_seed, int tracer)
+{
+    return SRound{stage_seed ? (sr_tracer_seed(stage_seed, tracer) | 1u) : 0u};
 }
 
 // Tile-local coordinates: output [0,TX) x [0,TY); input [-2,TX+2) x [-2,TY+2)
 // stored at LDS offset (y+2)*HX + (x+2).  Region A (first stage) cells:
 // D1 = y -> x in [-2,TX+2), y in [0,TY); D1 = x -> x in [0,TX), y in [-2,TY+2).
+// sr_stage: stage id of the first stage's rounding (the second takes sr_stage + 1);
+// tg0: index in the domain of the first tracer of fin/fout.
 template <int S, typename R, int D1, int TY>
 __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0, int y0, int k, int ntr,
                                            const R* __restrict__ fin, R* __restrict__ fout,
-                                           const double* __restrict__ hin, double* __restrict__ hout)
+                                           const double* __restrict__ hin, double* __restrict__ hout,
+                                           unsigned sr_stage, int tg0)
 {
     constexpr int HY = TY + 4;       // input tile height incl. halo 2
     constexpr int NT = HX * HY;      // cells of the input tile
@@ -405,6 +422,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             const int f1 = (yl - FAY0) * FAW + (xl - FAX0);
             ca[m].own = own;
             ca[m].fl1 = f1;
+            ca[m].gc = (unsigned)lay + (unsigned)((y0 + yl) * A.nx_ + x0 + xl);
             double h1 = 0.0;
             if (rca[m].inarr) {
                 const bool upd = wet(x0 + xl, y0 + yl, rca[m].tm);
@@ -438,6 +456,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
         cb[m].upd = 0;
         cb[m].own = own;
         cb[m].fl1 = f1;
+        cb[m].gc = (unsigned)lay + (unsigned)ooff[m];
         if (rcb[m].inarr) {
             const double h1 = s_h[own];
             const bool upd = wet(x0 + xl, y0 + yl, rcb[m].tm);
@@ -450,9 +469,11 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
 
     // ---- tracer loop: tile t+1 was loaded during iteration t-1 and is stored to
     // LDS now, tile t+2 is loaded during this iteration (three LDS tile buffers)
+    const unsigned ss_a = stage_seed<R>(A, sr_stage), ss_b = stage_seed<R>(A, sr_stage + 1);
     if (ntr > 1) load_tile(1);
     for (int t = 0; t < ntr; t++) {
         const R* sin = s_in[t % 3];
+        const SRound sr_a = tracer_round(ss_a, tg0 + t), sr_b = tracer_round(ss_b, tg0 + t);
         if (t + 1 < ntr) store_tile((t + 1) % 3);
         if (t + 2 < ntr) load_tile(t + 2);
         // stage A faces
@@ -477,7 +498,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             if (c < NA) {
                 R out = sin[ca[m].own];
                 if (ca[m].upd)
-                    out = cell_update<R>(ca[m], dt, out, s_fa[ca[m].fl1 - (D1 == 1 ? 1 : FAW)], s_fa[ca[m].fl1]);
+                    out = cell_update<R>(ca[m], dt, out, s_fa[ca[m].fl1 - (D1 == 1 ? 1 : FAW)], s_fa[ca[m].fl1], sr_a);
                 s_a[ca[m].own] = out;
             }
         }
@@ -502,7 +523,8 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
 #pragma unroll
         for (int m = 0; m < NPB; m++) {
             R out = s_a[cb[m].own];
-            if (cb[m].upd) out = cell_update<R>(cb[m], dt, out, s_fb[cb[m].fl1 - (D2 == 1 ? 1 : FBW)], s_fb[cb[m].fl1]);
+            if (cb[m].upd)
+                out = cell_update<R>(cb[m], dt, out, s_fb[cb[m].fl1 - (D2 == 1 ? 1 : FBW)], s_fb[cb[m].fl1], sr_b);
             if (ooff[m] >= 0) at32(fo, (unsigned)ooff[m] * (unsigned)sizeof(R)) = out;
         }
         __syncthreads();
@@ -521,7 +543,7 @@ template <int S, typename R, int D1, int TY>
 __global__ void __launch_bounds__(TX * TY) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
                                                    int nty, int nbatch, const R* __restrict__ fin_all,
                                                    R* __restrict__ fout_all, const double* __restrict__ hin,
-                                                   double* __restrict__ hout)
+                                                   double* __restrict__ hout, unsigned sr_stage)
 {
     const unsigned b = blockIdx.x, per = gridDim.x / 8;
     unsigned v = (b % 8) * per + b / 8;
@@ -535,7 +557,7 @@ __global__ void __launch_bounds__(TX * TY) k_hpair(const StepArgs A, double dt, 
     const int tb0 = tbatch * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
     hpair_body<S, R, D1, TY>(A, dt, tx * TX, ty * TY, k, ntr, fin_all + (size_t)tb0 * A.n3,
-                             fout_all + (size_t)tb0 * A.n3, hin, hout);
+                             fout_all + (size_t)tb0 * A.n3, hin, hout, sr_stage, tb0);
 }
 
 // Ensemble: one work item per block {domain, x0, y0, layer | batch << 16}; the
@@ -553,7 +575,7 @@ __global__ void __launch_bounds__(TX * TY) k_hpair_multi(const EnsDom<R>* __rest
     const int ntr = D.ntr - tb0 < D.tpb_h ? D.ntr - tb0 : D.tpb_h;
     const size_t off = (size_t)tb0 * D.A.n3;
     hpair_body<S, R, D1, TY>(D.A, 0.5 * D.dt, wk.y, wk.z, k, ntr, D.tracers + off, D.scratch + off,
-                             STAGE == 1 ? D.ho : D.hB, D.hA);
+                             STAGE == 1 ? D.ho : D.hB, D.hA, STAGE == 1 ? SR_PAIR1 : SR_PAIR2, tb0);
 }
 
 // ---------------------------------------------------------------------------
@@ -612,6 +634,12 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
         for (int it = 1; it <= iters; it++) {
             const R* sf = (it == 1 ? fin : fout) + (size_t)t0 * n3 + p;
             R* df = fout + (size_t)t0 * n3 + p;
+            SRound sr[TB];
+            {
+                const unsigned ss = stage_seed<R>(A, SR_WCOL + it);
+#pragma unroll
+                for (int b = 0; b < TB; b++) sr[b] = tracer_round(ss, t0 + b);
+            }
             const bool write_h = first && t0 == tb0 && it == iters;
             R fm1[TB], f0[TB], f1[TB], fprev[TB];
             double h0 = h_at(0, it - 1), h1 = nz > 1 ? h_at(1, it - 1) : 0.0;
@@ -675,10 +703,12 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
                     // increment form, as in cell_update: f + ((h0 - hnew)/hnew f - dt/hnew dF)
                     const double rh = rcp_nr(hnew);
                     const R ca = R((h0 - hnew) * rh), cb = R(dtk * rh);
+                    const unsigned gc = (unsigned)((size_t)(k - 1) * n2 + p);
 #pragma unroll
                     for (int b = 0; b < TB; b++)
                         if (b < nb)
-                            df[(size_t)b * n3 + (size_t)(k - 1) * n2] = f0[b] + (ca * f0[b] - cb * (fl[b] - fprev[b]));
+                            df[(size_t)b * n3 + (size_t)(k - 1) * n2] =
+                                sr[b].add(f0[b], ca * f0[b] - cb * (fl[b] - fprev[b]), gc);
                 }
 #pragma unroll
                 for (int b = 0; b < TB; b++) {
@@ -794,10 +824,11 @@ __global__ void __launch_bounds__(256) k_wcol_multi(const EnsDom<R>* __restrict_
 // multiplies the differences only, and a well-mixed column stays exactly as it is.
 // The old column and the eliminated q_k are both kept in registers; columns of
 // nz <= 16 interleave two tracers, deeper ones take one tracer at a time.
+// tg0: index in the domain of the first tracer of fin/fout.
 template <int NZMAX, bool CN1>
 __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double cnpar, double molecular,
                                            const double* __restrict__ nuh, int i, int j, int lane, int ntr,
-                                           const float* __restrict__ fin, float* __restrict__ fout)
+                                           const float* __restrict__ fin, float* __restrict__ fout, int tg0)
 {
     extern __shared__ float s_coef[];  // [ncoef][nz][64]
     constexpr int NT = NZMAX <= 16 ? 2 : 1;
@@ -849,6 +880,7 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
             dz = dz_lo;
         }
     }
+    const unsigned ss = stage_seed<float>(A, SR_VDIFF);
     for (int t = 0; t < ntr; t += NT) {
         const int nt = ntr - t < NT ? ntr - t : NT;
         float c[NT][NZMAX], q[NT][NZMAX];
@@ -894,19 +926,22 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
             // kept in registers from the elimination (nz more live values per lane)
             asm volatile("" ::: "memory");
             float d[NT];
+            SRound sr[NT];
 #pragma unroll
             for (int b = 0; b < NT; b++) {
+                sr[b] = tracer_round(ss, tg0 + t + b);
                 d[b] = q[b][0];
-                c[b][0] += d[b];
+                c[b][0] = sr[b].add(c[b][0], d[b], (unsigned)p);
             }
 #pragma unroll
             for (int k = 1; k < NZMAX; k++) {
                 if (has(k)) {
                     const float r = ru[k * 64 + lane];
+                    const unsigned gc = (unsigned)((size_t)k * n2 + p);
 #pragma unroll
                     for (int b = 0; b < NT; b++) {
                         d[b] = q[b][k] - r * d[b];
-                        c[b][k] += d[b];
+                        c[b][k] = sr[b].add(c[b][k], d[b], gc);
                     }
                 }
             }
@@ -929,7 +964,7 @@ __global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, d
     const int tb0 = blockIdx.z * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
     vdiff_body<NZMAX, CN1>(A, dt, cnpar, molecular, nuh, blockIdx.x * 64 + threadIdx.x, blockIdx.y, threadIdx.x, ntr,
-                           fin_all + (size_t)tb0 * A.n3, fout_all + (size_t)tb0 * A.n3);
+                           fin_all + (size_t)tb0 * A.n3, fout_all + (size_t)tb0 * A.n3, tb0);
 }
 
 // Ensemble: work item {domain, bx, row, batch}; scratch -> tracers.
@@ -943,7 +978,7 @@ __global__ void __launch_bounds__(64) k_vdiff_multi(const EnsDom<float>* __restr
     const int ntr = D.ntr - tb0 < D.tpb_d ? D.ntr - tb0 : D.tpb_d;
     const size_t off = (size_t)tb0 * D.A.n3;
     vdiff_body<NZMAX, CN1>(D.A, D.dt, D.cnpar, D.molecular, D.nuh, wk.y * 64 + threadIdx.x, wk.z, threadIdx.x, ntr,
-                           D.scratch + off, D.tracers + off);
+                           D.scratch + off, D.tracers + off, tb0);
 }
 
 // ---------------------------------------------------------------------------
@@ -971,7 +1006,7 @@ __global__ void __launch_bounds__(64) k_vdiff_multi(const EnsDom<float>* __restr
 template <int NP, bool CN1>
 __device__ __forceinline__ void pcr_body(const StepArgs& A, double dt, double cnpar, double molecular,
                                          const double* __restrict__ nuh, size_t p0, int ntr,
-                                         const float* __restrict__ fin, float* __restrict__ fout)
+                                         const float* __restrict__ fin, float* __restrict__ fout, int tg0)
 {
     constexpr int COLS = 1024 / NP, LDW = COLS + 1;  // padded rows: column reads are conflict-free
     constexpr int LOG = NP == 8 ? 3 : NP == 16 ? 4 : 5;
@@ -1031,6 +1066,8 @@ __device__ __forceinline__ void pcr_body(const StepArgs& A, double dt, double cn
         fg[st] = g;
     }
     const float minv = 1.f / m;
+    const unsigned ss = stage_seed<float>(A, SR_VDIFF);
+    const unsigned gc = (unsigned)((size_t)k * n2 + pc);
     const int ncols = (size_t)COLS < n2 - p0 ? COLS : (int)(n2 - p0);
     for (int t = 0; t < ntr; t++) {
         const float* fi = fin + (size_t)t * n3 + p0;
@@ -1060,7 +1097,7 @@ __device__ __forceinline__ void pcr_body(const StepArgs& A, double dt, double cn
             r = r + (hl ? fa[st] * rl : 0.f) + (hu ? fg[st] * rh : 0.f);
         }
         __syncthreads();  // every lane has read its value before the tile is overwritten
-        if (incol && lev) s_t[k * LDW + cl] = c + r * minv;
+        if (incol && lev) s_t[k * LDW + cl] = tracer_round(ss, tg0 + t).add(c, r * minv, gc);
         __syncthreads();
         // rows out, coalesced
         for (int e = tid; e < nz * COLS; e += 1024) {
@@ -1080,7 +1117,7 @@ __global__ void __launch_bounds__(1024) k_vdiff_pcr(const StepArgs A, double dt,
     const int tb0 = blockIdx.y * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
     pcr_body<NP, CN1>(A, dt, cnpar, molecular, nuh, (size_t)blockIdx.x * (1024 / NP), ntr,
-                      fin_all + (size_t)tb0 * A.n3, fout_all + (size_t)tb0 * A.n3);
+                      fin_all + (size_t)tb0 * A.n3, fout_all + (size_t)tb0 * A.n3, tb0);
 }
 
 // Ensemble: one work item per block {domain, first column, 0, batch}.
@@ -1093,14 +1130,41 @@ __global__ void __launch_bounds__(1024) k_vdiff_pcr_multi(const EnsDom<float>* _
     const int tb0 = wk.w * D.tpb_d;
     const int ntr = D.ntr - tb0 < D.tpb_d ? D.ntr - tb0 : D.tpb_d;
     const size_t off = (size_t)tb0 * D.A.n3;
-    pcr_body<NP, CN1>(D.A, D.dt, D.cnpar, D.molecular, D.nuh, (size_t)wk.y, ntr, D.scratch + off, D.tracers + off);
+    pcr_body<NP, CN1>(D.A, D.dt, D.cnpar, D.molecular, D.nuh, (size_t)wk.y, ntr, D.scratch + off, D.tracers + off,
+                      tb0);
 }
 
 static inline int nbatch(int ntr, int tpb) { return tpb <= 0 ? 1 : (ntr + tpb - 1) / tpb; }
 
+// Step counters of the stochastic rounding, advanced after the step's last
+// kernel (a kernel of its own, so every block of the step read the old count).
+__global__ void __launch_bounds__(64) k_count_step(unsigned long long* counter)
+{
+    if (threadIdx.x == 0) *counter += 1ull;
+}
+__global__ void __launch_bounds__(64) k_count_steps_multi(const EnsDom<float>* __restrict__ doms, int ndom)
+{
+    const int d = blockIdx.x * 64 + threadIdx.x;
+    if (d < ndom && doms[d].rstep) *doms[d].rstep += 1ull;
+}
+int launch_count_step(unsigned long long* counter, hipStream_t s)
+{
+    if (!counter) return GETM_OK;
+    hipLaunchKernelGGL(k_count_step, dim3(1), dim3(64), 0, s, counter);
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+int launch_count_steps_multi(const EnsDom<float>* doms, int ndom, hipStream_t s)
+{
+    if (ndom <= 0) return GETM_OK;
+    hipLaunchKernelGGL(k_count_steps_multi, dim3((ndom + 63) / 64), dim3(64), 0, s, doms, ndom);
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
 template <typename R>
 int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
-                 const double* hin, double* hout, hipStream_t s)
+                 const double* hin, double* hout, unsigned sr_stage, hipStream_t s)
 {
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
@@ -1114,10 +1178,10 @@ int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int n
     GETM_SCHEME_SWITCH(scheme, {                                                                               \
         if (d1 == 2)                                                                                           \
             hipLaunchKernelGGL((k_hpair<S_, R, 2, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, nb, \
-                               fin, fout, hin, hout);                                                          \
+                               fin, fout, hin, hout, sr_stage);                                                \
         else                                                                                                   \
             hipLaunchKernelGGL((k_hpair<S_, R, 1, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, nb, \
-                               fin, fout, hin, hout);                                                          \
+                               fin, fout, hin, hout, sr_stage);                                                \
     })
     if (ty == 16) {
         GETM_HP(16);
@@ -1141,10 +1205,24 @@ int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, cons
     dim3 block(64, 4, 1);
     dim3 grid((A.nx_ + 63) / 64, (A.ny_ + 3) / 4, nb);
     ProfileScope ps(GETM_K_WCOL, s);
-    constexpr int TB = std::is_same<R, double>::value ? 2 : 4;
-    GETM_SCHEME_SWITCH(scheme, {
-        hipLaunchKernelGGL((k_wcol<S_, R, TB>), grid, block, 0, s, A, dt, ntr, tpb, fin, fout, hin, hout, w, wv, stats);
-    });
+    // tracers per sweep: the thickness and w loads of a sweep are shared by TB
+    // tracers; 8 costs occupancy (125 against 96 VGPRs), so only for larger batches
+    if constexpr (std::is_same<R, double>::value) {
+        GETM_SCHEME_SWITCH(scheme, {
+            hipLaunchKernelGGL((k_wcol<S_, R, 2>), grid, block, 0, s, A, dt, ntr, tpb, fin, fout, hin, hout, w, wv,
+                               stats);
+        });
+    } else if (tpb > 4) {
+        GETM_SCHEME_SWITCH(scheme, {
+            hipLaunchKernelGGL((k_wcol<S_, R, 8>), grid, block, 0, s, A, dt, ntr, tpb, fin, fout, hin, hout, w, wv,
+                               stats);
+        });
+    } else {
+        GETM_SCHEME_SWITCH(scheme, {
+            hipLaunchKernelGGL((k_wcol<S_, R, 4>), grid, block, 0, s, A, dt, ntr, tpb, fin, fout, hin, hout, w, wv,
+                               stats);
+        });
+    }
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }
@@ -1221,9 +1299,9 @@ int launch_vdiff_pcr(const StepArgs& A, double dt, double cnpar, double molecula
 }
 
 template int launch_hpair<float>(int, int, int, const StepArgs&, double, int, int, const float*, float*,
-                                 const double*, double*, hipStream_t);
+                                 const double*, double*, unsigned, hipStream_t);
 template int launch_hpair<double>(int, int, int, const StepArgs&, double, int, int, const double*, double*,
-                                  const double*, double*, hipStream_t);
+                                  const double*, double*, unsigned, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // ensemble launchers: one launch per kernel kind covers every domain

@@ -20,6 +20,8 @@ import torch
 from . import _lib
 from .core import Array, Grid
 
+_ROUNDING = {"nearest": 0, "stochastic": 1}
+
 
 class Ensemble:
     def __init__(self, grids: Sequence[Grid], ntracers: Sequence[int], scheme: int = 5, cnpar: float = 1.0,
@@ -41,6 +43,8 @@ class Ensemble:
         self.tracers: List[Optional[torch.Tensor]] = [None] * n
         self.tuning = (0, 0, 16, 16)
         self.pcr_max_nz = 0
+        self.rounding = "stochastic"
+        self._counters = None
 
     def __del__(self):
         p = getattr(self, "p", None)
@@ -79,6 +83,25 @@ class Ensemble:
         """Column solver per domain by depth (TracerCollection.set_solver); takes effect at commit."""
         _lib.check(_lib.load().ensemble_set_solver(self.p, int(pcr_max_nz)), "ensemble_set_solver")
         self.pcr_max_nz = int(pcr_max_nz)
+
+    def set_rounding(self, mode: str = "stochastic", counters: Optional[Sequence[Optional[torch.Tensor]]] = None) -> None:
+        """Rounding of the FP32 tracer updates (TracerCollection.set_rounding), per
+        domain with its own step counter: ``counters[d]`` is a one-element int64
+        device tensor (distinct per domain) or None for the ensemble's own counter
+        (zero at creation).  Takes effect at commit."""
+        code = _ROUNDING[mode]
+        n = len(self.grids)
+        arr = None
+        if counters is not None:
+            if len(counters) != n:
+                raise ValueError("one counter (or None) per domain is needed")
+            for c in counters:
+                if c is not None and (c.dtype != torch.int64 or c.numel() != 1 or not c.is_cuda):
+                    raise ValueError("step counters are one-element int64 device tensors")
+            arr = (ctypes.c_void_p * n)(*[(c.data_ptr() if c is not None else None) for c in counters])
+        _lib.check(_lib.load().ensemble_set_rounding(self.p, code, arr), "ensemble_set_rounding")
+        self._counters = list(counters) if counters is not None else None
+        self.rounding = mode
 
     def commit(self) -> None:
         """Allocate scratch and upload the work tables (synchronous)."""

@@ -350,11 +350,10 @@ def tiny(nx=24, ny=20, nz=8, seed=0, scheme=5, dt=None, land_fraction=0.25, cfl=
                   m["area"], H, z, sigma, ho, hn, hu, hv, u, v, w, nuh, tr, dt, scheme, 1.0, "tiny")
 
 
-def ensemble_domains(ndomains=96, seed=3, size_range=(64, 768), nz_range=(10, 60), tracer_range=(1, 32),
-                     scheme=5, dt=30.0):
-    """Uneven ensemble: per domain nx, ny log-uniform in ``size_range``, nz uniform in
-    ``nz_range``, 1..32 blob tracers, fields generated as by :func:`coastal` from a
-    per-domain seed."""
+def ensemble_specs(ndomains=96, seed=3, size_range=(64, 768), nz_range=(10, 60), tracer_range=(1, 32)):
+    """(nx, ny, nz, tracers, field seed) per domain of an uneven ensemble: nx, ny
+    log-uniform in ``size_range``, nz uniform in ``nz_range``, tracers uniform in
+    ``tracer_range``."""
     rng = np.random.default_rng(seed)
     out = []
     lo, hi = math.log(size_range[0]), math.log(size_range[1])
@@ -363,9 +362,16 @@ def ensemble_domains(ndomains=96, seed=3, size_range=(64, 768), nz_range=(10, 60
         ny = int(round(math.exp(rng.uniform(lo, hi))))
         nz = int(rng.integers(nz_range[0], nz_range[1] + 1))
         nt = int(rng.integers(tracer_range[0], tracer_range[1] + 1))
-        out.append(coastal(nx=nx, ny=ny, nz=nz, dx=2000.0, dt=dt, scheme=scheme, seed=1000 + d, ntracers=nt,
-                           tracer_kind="blobs"))
+        out.append((nx, ny, nz, nt, 1000 + d))
     return out
+
+
+def ensemble_domains(ndomains=96, seed=3, size_range=(64, 768), nz_range=(10, 60), tracer_range=(1, 32),
+                     scheme=5, dt=30.0):
+    """Uneven ensemble (:func:`ensemble_specs`), fields and blob tracers generated as
+    by :func:`coastal` from a per-domain seed."""
+    return [coastal(nx=nx, ny=ny, nz=nz, dx=2000.0, dt=dt, scheme=scheme, seed=s, ntracers=nt, tracer_kind="blobs")
+            for nx, ny, nz, nt, s in ensemble_specs(ndomains, seed, size_range, nz_range, tracer_range)]
 
 
 def steady_release(nx=96, ny=80, nz=20, dx=2000.0, dt=30.0, scheme=5, seed=0, ntracers=1, amount=1e3,

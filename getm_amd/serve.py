@@ -143,6 +143,9 @@ class _Slot:
         self.tc = TracerCollection(g, advection_scheme=req.scheme, precision="f32")
         self.target = self._inventory()
         self.conserve = conserve
+        # step count of the request's stochastic rounding, carried by the request
+        # from ensemble to ensemble, so its results never depend on the mix
+        self.counter = torch.zeros(1, dtype=torch.int64, device=device)
 
     def _inventory(self) -> torch.Tensor:
         out = torch.empty(self.req.ntracers, dtype=torch.float64, device=self.grid.device)
@@ -199,6 +202,7 @@ class Server:
             ens = Ensemble([s.grid for s in slots], [s.req.ntracers for s in slots], scheme=group[0])
             for i, s in enumerate(slots):
                 ens.bind(i, s.tracers, s.u, s.v, s.w, s.nuh, s.dt)
+            ens.set_rounding("stochastic", [s.counter for s in slots])
             if self.tuning:
                 ens.set_tuning(*self.tuning[:4])
                 if len(self.tuning) > 4:

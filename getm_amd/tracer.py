@@ -20,6 +20,7 @@ from . import _lib, operators
 from .core import CENTERS, INTERFACES, Array, Grid
 
 _DTYPES = {"f64": (torch.float64, 1), "f32": (torch.float32, 2)}
+_ROUNDING = {"nearest": 0, "stochastic": 1}
 
 
 class TracerCollection:
@@ -44,6 +45,8 @@ class TracerCollection:
         # FP32 only: restore every tracer's inventory to its value before the step
         self.conserve = bool(conserve) and precision == "f32"
         self._target: Optional[torch.Tensor] = None
+        self.rounding = "stochastic"
+        self._counter: Optional[torch.Tensor] = None
         lib = _lib.load()
         if batched:
             self.p = lib.transport_create(grid.p, grid.ugrid.p, grid.vgrid.p, int(advection_scheme), self.cnpar, code,
@@ -201,6 +204,24 @@ class TracerCollection:
             return
         _lib.check(_lib.load().transport_set_solver(self.p, int(pcr_max_nz)), "transport_set_solver")
         self.pcr_max_nz = int(pcr_max_nz)
+
+    def set_rounding(self, mode: str = "stochastic", counter: Optional[torch.Tensor] = None) -> None:
+        """Rounding of the FP32 tracer updates: "stochastic" (the default) rounds
+        each stored c + dc up or down with probability given by the exact
+        residual, so rounding errors do not accumulate in a steady flow;
+        "nearest" is plain FP32 arithmetic.  The random bits are a hash of (step
+        count, stage, tracer, cell) -- deterministic, independent of launch shape
+        and batching.  ``counter`` (a one-element int64 device tensor) holds the
+        step count and is advanced by every step; None uses the collection's own
+        counter, zero at creation.  No effect on FP64."""
+        if not self.batched:
+            return
+        if counter is not None and (counter.dtype != torch.int64 or counter.numel() != 1 or not counter.is_cuda):
+            raise ValueError("the step counter is a one-element int64 device tensor")
+        rc = _lib.load().transport_set_rounding(self.p, _ROUNDING[mode], _lib.ptr(counter) if counter is not None else None)
+        _lib.check(rc, "transport_set_rounding")
+        self._counter = counter
+        self.rounding = mode
 
     def stats(self, reset: bool = True):
         """(largest z sub-iteration count, columns capped at 200) since the last reset."""

@@ -128,6 +128,9 @@ int c_vertical_diffusion_apply(void* vd, int nx, int ny, int nz, const int* mask
 /* Tracer storage/arithmetic precision of a transport object. */
 enum { GETM_F64 = 1, GETM_F32 = 2 };
 
+/* Rounding of the FP32 tracer stores of the batched step. */
+enum { GETM_ROUND_NEAREST = 0, GETM_ROUND_STOCHASTIC = 1 };
+
 /* No single reference counterpart: replaces the loop of
  * TracerCollection.advance (python/pygetm/tracer.py:193-276), i.e.
  * Advection.apply_3d_batch (operators.py:135-193) followed by
@@ -165,6 +168,19 @@ int transport_set_tuning(void* tr, int tile_y, int tracers_per_block_h, int trac
  * elimination (the reference's algorithm, diffusion.F90:276-313); 0 = Thomas
  * only (default).  The two agree to FP32 rounding, not bit for bit. */
 int transport_set_solver(void* tr, int pcr_max_nz);
+
+/* No reference counterpart (the reference computes in FP64): rounding of the
+ * FP32 tracer updates.  GETM_ROUND_STOCHASTIC (the default) rounds every stored
+ * c + dc up or down with probability given by the exact residual, so the
+ * expected stored value is exact and rounding errors do not accumulate in a
+ * steady flow (with rounding to nearest, an increment below half an ulp is lost
+ * every step).  The random bits are a hash of (step count, stage, tracer index,
+ * cell index): results are deterministic and independent of launch shapes and
+ * batching.  The step count is read from step_counter (a device uint64 that each
+ * transport_step advances by one; NULL: the object's own counter, zero at
+ * creation), so a caller that moves a domain between objects keeps its stream
+ * by passing the same counter.  No effect on GETM_F64. */
+int transport_set_rounding(void* tr, int mode, unsigned long long* step_counter);
 
 /* Tracer inventories (the reference's domain integrals of tracers,
  * simulation.py:1408-1466): inventory[t] = sum over wet cells of
@@ -213,6 +229,12 @@ int ensemble_set_tuning(void* ens, int tile_y, int tracers_per_block_h, int trac
 /* Column solver as transport_set_solver, per domain by its nz; takes effect at
  * the next commit. */
 int ensemble_set_solver(void* ens, int pcr_max_nz);
+
+/* Rounding as transport_set_rounding; step_counters: ndomains device counters
+ * (distinct), or NULL entries / a NULL array for the ensemble's own (zero at
+ * creation).  Results equal transport_step on each domain started from the same
+ * count.  Takes effect at the next commit. */
+int ensemble_set_rounding(void* ens, int mode, unsigned long long* const* step_counters);
 
 /* Allocates scratch, builds and uploads the work tables (synchronous). */
 int ensemble_commit(void* ens, void* stream);

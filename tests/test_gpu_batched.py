@@ -277,3 +277,60 @@ def test_pcr_solver_bound_is_checked():
     with pytest.raises(_lib.GetmError):
         tc.set_solver(33)
     tc.set_solver(32)
+
+
+def _ts_run(d, precision, steps, rounding="stochastic", counter=None, conserve=False):
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.tracer import TracerCollection
+
+    grid = make_grid(d)
+    tc = TracerCollection(grid, advection_scheme=5, precision=precision, conserve=conserve)
+    tc.set_values(dev(d.tracers).float())  # both precisions start from the same FP32 values
+    if precision == "f32":
+        tc.set_rounding(rounding, counter)
+    u, v = grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS)
+    w, nuh = grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES)
+    for _ in range(steps):
+        tc.advance(d.dt, u, v, w, nuh)
+    return host(tc.values).astype(np.float64)
+
+
+def test_stochastic_rounding_keeps_error_from_accumulating():
+    """Temperature and salinity (means 12.5 and 32.5) over 300 steps: rounding to
+    nearest loses the sub-half-ulp part of slow increments the same way every step,
+    so its error grows linearly; stochastic rounding makes it a random walk.  The
+    stochastic error stays below 1e-5 and well below the nearest-rounding error.
+    With the inventory fixer the error is
+    that of the plain FP32 step: the fixer's one-ulp changes are spread over
+    pseudo-randomly chosen cells instead of piling up in the same few cells."""
+    d = inputs.coastal(nx=128, ny=96, nz=30, seed=11, ntracers=2, tracer_kind="TS")
+    wet = np.broadcast_to(d.tmask == 1, d.ho.shape)
+    ref = _ts_run(d, "f64", 300)
+    got = {r: _ts_run(d, "f32", 300, r) for r in ("nearest", "stochastic")}
+    got["fixed"] = _ts_run(d, "f32", 300, "stochastic", conserve=True)
+    for t in range(2):
+        r = ref[t][wet]
+        scale = np.abs(r).max()
+        e = {k: np.abs(v[t][wet] - r).max() / scale for k, v in got.items()}
+        assert e["stochastic"] <= 1e-5, (t, e)
+        assert e["stochastic"] <= 0.6 * e["nearest"], (t, e)
+        assert e["fixed"] <= 1.25 * e["stochastic"], (t, e)
+
+
+def test_stochastic_rounding_follows_the_step_counter():
+    """The random bits are a function of the step count: a caller's counter is
+    advanced once per step; two runs from the same count agree bit for bit; a run
+    from another count differs in its last bits only."""
+    d = inputs.coastal(nx=64, ny=48, nz=12, seed=12, ntracers=2, tracer_kind="TS")
+    c1 = torch.zeros(1, dtype=torch.int64, device="cuda")
+    c2 = torch.zeros(1, dtype=torch.int64, device="cuda")
+    c3 = torch.full((1,), 1000, dtype=torch.int64, device="cuda")
+    a = _ts_run(d, "f32", 5, counter=c1)
+    b = _ts_run(d, "f32", 5, counter=c2)
+    c = _ts_run(d, "f32", 5, counter=c3)
+    own = _ts_run(d, "f32", 5)
+    assert int(c1.item()) == 5 and int(c3.item()) == 1005
+    assert_bitwise(a, b, "same counter start")
+    assert_bitwise(a, own, "own counter starts at zero")
+    assert not np.array_equal(a, c)
+    assert np.abs(a - c).max() <= 1e-5 * np.abs(a).max()
