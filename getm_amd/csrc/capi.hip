@@ -636,7 +636,9 @@ int ensemble_commit(void* p, void* stream)
         nh += 2 * e->tg[d]->n3();
         ntr_all += e->host[d].ntr;
     }
-    e->tile_y_used = e->tile_y ? e->tile_y : auto_tile_y((int)((ntr_all + e->ndom - 1) / e->ndom));
+    const int mean_ntr = (int)((ntr_all + e->ndom - 1) / e->ndom);
+    e->tile_y_used = e->tile_y ? e->tile_y : auto_tile_y(mean_ntr);
+    e->hpair_kg = 1;  // one layer per work item: measured faster than layer groups on uneven ensembles
     int rc;
     if ((rc = e->scratch.ensure(nscr * sizeof(float))) || (rc = e->hbuf.ensure(nh * sizeof(double)))) return rc;
     float* scr = static_cast<float*>(e->scratch.ptr);
@@ -664,11 +666,11 @@ int ensemble_commit(void* p, void* stream)
         const int nbw = (D.ntr + D.tpb_w - 1) / D.tpb_w;
         const int nbd = (D.ntr + D.tpb_d - 1) / D.tpb_d;
         if (nbh > 32767 || t->nz > 65535) return GETM_ERR_CAPACITY;
-        // logical order as k_hpair: tracer batch fastest, then layer, then tile
+        // logical order as k_hpair: tracer batch fastest, then layer group, then tile
         for (int y0 = 0; y0 < t->ny_; y0 += e->tile_y_used)
             for (int x0 = 0; x0 < t->nx_; x0 += 64)
-                for (int k = 0; k < t->nz; k++)
-                    for (int b = 0; b < nbh; b++) wh.push_back(make_int4(d, x0, y0, k | (b << 16)));
+                for (int k0 = 0; k0 < t->nz; k0 += e->hpair_kg)
+                    for (int b = 0; b < nbh; b++) wh.push_back(make_int4(d, x0, y0, k0 | (b << 16)));
         for (int b = 0; b < nbw; b++)
             for (int by = 0; by < (t->ny_ + 3) / 4; by++)
                 for (int bx = 0; bx < (t->nx_ + 63) / 64; bx++) ww.push_back(make_int4(d, bx, by, b));
@@ -728,11 +730,11 @@ int ensemble_step(void* p, void* stream)
     const EnsDom<float>* doms = static_cast<const EnsDom<float>*>(e->doms.ptr);
     const int d1 = e->ufirst ? 1 : 2;
     int rc;
-    if ((rc = launch_hpair_multi(1, d1, e->scheme, e->tile_y_used, doms, static_cast<const int4*>(e->work_h.ptr), e->nwork_h,
+    if ((rc = launch_hpair_multi(1, d1, e->scheme, e->tile_y_used, e->hpair_kg, doms, static_cast<const int4*>(e->work_h.ptr), e->nwork_h,
                                  s)))
         return rc;
     if ((rc = launch_wcol_multi(e->scheme, doms, static_cast<const int4*>(e->work_w.ptr), e->nwork_w, s))) return rc;
-    if ((rc = launch_hpair_multi(2, 3 - d1, e->scheme, e->tile_y_used, doms, static_cast<const int4*>(e->work_h.ptr),
+    if ((rc = launch_hpair_multi(2, 3 - d1, e->scheme, e->tile_y_used, e->hpair_kg, doms, static_cast<const int4*>(e->work_h.ptr),
                                  e->nwork_h, s)))
         return rc;
     const int4* wd = static_cast<const int4*>(e->work_d.ptr);

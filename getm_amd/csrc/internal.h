@@ -91,6 +91,13 @@ struct StepArgs {
     const unsigned long long* rstep;
 };
 
+// Layers per block of the horizontal stage pair: a group of consecutive layers
+// shares the block's layer-independent face and cell terms, which pays when the
+// tracer-independent set-up dominates (few tracers per block); with many tracers
+// one layer per block keeps the tracer loop's registers free.  FP32 only.
+constexpr int HPAIR_KG = 8;
+inline int hpair_group(int tracers_per_block, bool f32) { return f32 && tracers_per_block <= 8 ? HPAIR_KG : 1; }
+
 // Rounding sites of one step (stage ids of the stochastic-rounding seeds):
 // stage pair 1 -> 1, 2; stage pair 2 -> 3, 4; column solve -> 5; w sub-step it -> 16 + it.
 enum { SR_PAIR1 = 1, SR_PAIR2 = 3, SR_VDIFF = 5, SR_WCOL = 16 };
@@ -139,6 +146,7 @@ struct Ensemble {
     double cnpar = 1.0;
     int tile_y = 0, tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tile_y 0 = by tracers per domain
     int tile_y_used = 16;                               // the height of the committed work table
+    int hpair_kg = 1;                                   // layers per horizontal-stage work item (hpair_group)
     int nwork_h = 0, nwork_w = 0, nwork_d = 0, nzmax = 1;
     int nwork_d_class[8] = {};  // diffusion work items per depth class (8 c-7 .. 8 c + 8 layers), in this order
     int pcr_max_nz = 0;         // domains with 2 <= nz <= this use the wave-PCR solve
@@ -164,8 +172,8 @@ inline int auto_tile_y(int tracers_per_domain) { return tracers_per_domain <= 8 
 template <typename R>
 int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
                  const double* hin, double* hout, unsigned sr_stage, hipStream_t s);
-int launch_hpair_multi(int stage, int d1, int scheme, int ty, const EnsDom<float>* doms, const int4* work, int nwork,
-                       hipStream_t s);
+int launch_hpair_multi(int stage, int d1, int scheme, int ty, int kg, const EnsDom<float>* doms, const int4* work,
+                       int nwork, hipStream_t s);
 int launch_wcol_multi(int scheme, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
 // one depth class: every domain of the work items has 8 (cls + 1) - 7 .. 8 (cls + 1) layers
 int launch_vdiff_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);

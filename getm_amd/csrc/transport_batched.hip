@@ -27,9 +27,13 @@ namespace getm {
 // ---------------------------------------------------------------------------
 // Horizontal stage pair on one layer tile (v dt/2 then u dt/2, or u then v).
 //
-// Per block: a 64 x 16 output tile of one layer and its input tile with halo 2.
-// Set-up (tracer-independent, once per block): every global load of the
-// set-up -- face fields of both stages, cell fields and the first tracer's
+// Per block: a 64 x TY output tile of a group of KG (1 or HPAIR_KG) consecutive
+// layers and its input tile with halo 2.  The layer-independent terms (the
+// faces' mask flags and metrics, the cells' wet flags and inverse areas) are
+// loaded once per block into LDS; each layer then loads only its velocities,
+// face and cell thicknesses and tracers.
+// Per-layer set-up (tracer-independent): every global load of the set-up --
+// face fields of both stages, cell thicknesses and the first tracer's
 // tile -- is issued before any is used, so the block waits for HBM once.  Face
 // transports are formed once and shared through LDS for the FP64 thickness
 // update; each face keeps its transport, the cfl-only limiter terms and the
@@ -50,11 +54,6 @@ struct FaceC {
     int flags;    // bit0 open, bit1 vel > 0, bit2 upstream-upstream face open
 };
 
-// Global fields of one face, loaded before use.
-struct FaceRaw {
-    int inloop, m0, mlo, mhi;
-    double vel, hf, mq, mc;
-};
 
 // Face of the reference's face loops?  x faces: i in [imin-1, imax], j in [jmin, jmax];
 // y faces: i in [imin, imax], j in [jmin-1, jmax].  There the neighbour masks at
@@ -66,35 +65,18 @@ __device__ __forceinline__ bool face_in_loop(const StepArgs& A, int gi, int gj)
     return gi >= A.imin && gi <= A.imax && gj >= A.jmin - 1 && gj <= A.jmax;
 }
 
-// Face (gi, gj) of direction DIR (1: U face east of T(gi,gj); 2: V face north of it).
-// In-layer offsets are 32-bit (grids have nx_ ny_ < 2^31) and every base pointer
-// is wave-uniform, so the loads take a scalar base and a 32-bit lane offset.
-template <int DIR>
-__device__ __forceinline__ void face_load(const StepArgs& A, int gi, int gj, size_t lay, FaceRaw& r)
-{
-    r.inloop = face_in_loop<DIR>(A, gi, gj);
-    if (!r.inloop) return;
-    const unsigned p = (unsigned)(gj * A.nx_ + gi);
-    const unsigned s = DIR == 1 ? 1u : (unsigned)A.nx_;  // p - s >= 0 inside the face loops (halo >= 2)
-    const int* mask = DIR == 1 ? A.umask : A.vmask;
-    r.m0 = at32(mask, 4 * p);
-    r.mlo = at32(mask, 4 * (p - s));
-    r.mhi = at32(mask, 4 * (p + s));
-    r.vel = at32(DIR == 1 ? A.u + lay : A.v + lay, 8 * p);
-    r.hf = at32(DIR == 1 ? A.hu + lay : A.hv + lay, 8 * p);
-    r.mq = at32(DIR == 1 ? A.dyu : A.dxv, 8 * p);   // metric of the transport
-    r.mc = at32(DIR == 1 ? A.idxu : A.idyv, 8 * p);  // inverse metric of the CFL number
-}
-
-// QU = u*hu*dyu, cfl = |u*dt*idxu| (template:64-65; QV/cfl likewise :164-165)
-__device__ __forceinline__ FaceC face_make(const FaceRaw& r, double dt)
+// Face transport and CFL number from the face's layer fields (velocity, face
+// thickness), its 2-D metrics and its mask flags fl (bit0 in the face loop and
+// open, bit1 / bit2 the lower / upper neighbour face along the sweep open):
+// QU = u*hu*dyu, cfl = |u*dt*idxu| (template:64-65; QV/cfl likewise :164-165).
+__device__ __forceinline__ FaceC face_make2(unsigned fl, double vel, double hf, double mq, double mc, double dt)
 {
     FaceC fc{0.0, 0.0, 0};
-    if (!r.inloop || !open_face(r.m0)) return fc;
-    fc.q = (r.vel * r.hf) * r.mq;
-    fc.cfl = fabs((r.vel * dt) * r.mc);
-    const bool pos = r.vel > 0.0;
-    const bool fuu_ok = pos ? open_face(r.mlo) : open_face(r.mhi);
+    if (!(fl & 1u)) return fc;
+    fc.q = (vel * hf) * mq;
+    fc.cfl = fabs((vel * dt) * mc);
+    const bool pos = vel > 0.0;
+    const bool fuu_ok = pos ? (fl & 2u) : (fl & 4u);
     fc.flags = 1 | (pos ? 2 : 0) | (fuu_ok ? 4 : 0);
     return fc;
 }
@@ -220,11 +202,6 @@ __device__ __forceinline__ R ah_flux(const StepArgs& A, int gi, int gj, int k, R
     return (((R(ah[p] * hf)) * (f1 - f0)) * R(m1)) * R(m2);
 }
 
-// Global fields of one cell, loaded before use.
-struct CellRaw {
-    int inarr, tm;
-    double h0, iA;
-};
 
 // Per-cell terms of one stage; the cell is updated iff interior and wet.
 //   FP64: hold, hnew, iA for the reference order (h f - dt (dF iA)) / hnew
@@ -290,8 +267,10 @@ __device__ __forceinline__ SRound tracer_round(unsigned stage_seed, int tracer)
 // D1 = y -> x in [-2,TX+2), y in [0,TY); D1 = x -> x in [0,TX), y in [-2,TY+2).
 // sr_stage: stage id of the first stage's rounding (the second takes sr_stage + 1);
 // tg0: index in the domain of the first tracer of fin/fout.
-template <int S, typename R, int D1, int TY>
-__device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0, int y0, int k, int ntr,
+// GROUP: the block walks several layers and keeps the layer-independent terms
+// in LDS; otherwise (one layer) they are loaded straight into registers.
+template <int S, typename R, int D1, int TY, bool GROUP>
+__device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0, int y0, int k0, int k1, int ntr,
                                            const R* __restrict__ fin, R* __restrict__ fout,
                                            const double* __restrict__ hin, double* __restrict__ hout,
                                            unsigned sr_stage, int tg0)
@@ -322,259 +301,401 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
     __shared__ R s_fb[NFB];
     __shared__ double s_q[NFA > NFB ? NFA : NFB];  // FP64 face transports of the stage being set up
     __shared__ double s_h[NT];                     // FP64 thickness after stage A
+    // layer-independent terms, formed once per block: face metrics (transport and
+    // inverse CFL metric) and mask flags, cell inverse areas and wet flags
+    constexpr int GFA = GROUP ? NFA : 1, GFB = GROUP ? NFB : 1, GT = GROUP ? NT : 1;
+    __shared__ double s_mqa[GFA], s_mca[GFA], s_mqb[GFB], s_mcb[GFB];
+    __shared__ double s_ia[GT];
+    __shared__ unsigned char s_ffa[GFA], s_ffb[GFB], s_cf[GT];
 
     const int tid = threadIdx.x;
-    const size_t lay = (size_t)k * A.n2;
     auto inarr = [&](int gi, int gj) { return gi >= 0 && gi < A.nx_ && gj >= 0 && gj < A.ny_; };
-    auto cell_load = [&](int gi, int gj, CellRaw& r) {
-        r.inarr = inarr(gi, gj);
-        if (!r.inarr) return;
-        const unsigned p = (unsigned)(gj * A.nx_ + gi);
-        r.h0 = at32(hin + lay, 8 * p);
-        r.iA = at32(A.iarea, 8 * p);
-        r.tm = at32(A.tmask, 4 * p);
-    };
     auto wet = [&](int gi, int gj, int tm) {
         return gi >= A.imin && gi <= A.imax && gj >= A.jmin && gj <= A.jmax && tm == 1;
     };
 
-    // ---- every set-up load in flight at once
-    FaceRaw rfa[NPFA], rfb[NPFB];
-    CellRaw rca[NPA], rcb[NPB];
-    R pf[NPIN];
-    // in-layer offsets of this lane's input-tile cells (-1 outside the array) and
-    // output cells (-1 outside), formed once: the tracer loop only adds them to
-    // the tracer's layer base, a wave-uniform pointer
-    int toff[NPIN], ooff[NPB];
-#pragma unroll
-    for (int m = 0; m < NPIN; m++) {
-        const int c = tid + m * NTHR;
-        toff[m] = -1;
-        if (c < NT) {
-            const int gi = x0 - 2 + c % HX, gj = y0 - 2 + c / HX;
-            if (inarr(gi, gj)) toff[m] = gj * A.nx_ + gi;
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < NPB; m++) {
-        const int c = tid + m * NTHR;
-        const int gi = x0 + c % TX, gj = y0 + c / TX;
-        ooff[m] = (gi < A.nx_ && gj < A.ny_) ? gj * A.nx_ + gi : -1;
-    }
-    auto load_tile = [&](int t) {
-        const R* fi = fin + (size_t)t * A.n3 + lay;
-#pragma unroll
-        for (int m = 0; m < NPIN; m++) pf[m] = toff[m] >= 0 ? at32(fi, (unsigned)toff[m] * (unsigned)sizeof(R)) : R(0);
+    // ---- layer-independent set-up (into LDS for a layer group).
+    // faces: 2-D loads (masks of the face and of its two neighbours along the sweep,
+    // transport and CFL metrics) -> flags bit0 in the face loop and open, bit1 the
+    // lower neighbour face open, bit2 the upper one (the upstream-upstream faces)
+    auto face2d = [&](auto dirc, int gi, int gj, unsigned char& fl, double& mq, double& mc) -> int {
+        constexpr int DIR = decltype(dirc)::value;
+        fl = 0;
+        mq = 0.0;
+        mc = 0.0;
+        if (!face_in_loop<DIR>(A, gi, gj)) return -1;
+        const unsigned p = (unsigned)(gj * A.nx_ + gi);
+        const unsigned s = DIR == 1 ? 1u : (unsigned)A.nx_;  // p - s >= 0 inside the face loops (halo >= 2)
+        const int* mask = DIR == 1 ? A.umask : A.vmask;
+        // every load issued before any is used (no load waits for the mask)
+        const int m0 = at32(mask, 4 * p), mlo = at32(mask, 4 * (p - s)), mhi = at32(mask, 4 * (p + s));
+        mq = at32(DIR == 1 ? A.dyu : A.dxv, 8 * p);   // metric of the transport
+        mc = at32(DIR == 1 ? A.idxu : A.idyv, 8 * p);  // inverse metric of the CFL number
+        fl = open_face(m0) ? (1 | (open_face(mlo) ? 2 : 0) | (open_face(mhi) ? 4 : 0)) : 0;
+        return (int)p;  // in the face loop (open or closed)
     };
-    auto store_tile = [&](int buf) {
-#pragma unroll
-        for (int m = 0; m < NPIN; m++) {
-            const int c = tid + m * NTHR;
-            if (c < NT) s_in[buf][c] = pf[m];
-        }
+    // cells: bit0 in the array, bit1 updated (interior and wet); returns the in-layer offset or -1
+    auto cell2d = [&](int gi, int gj, unsigned char& cf, double& ia) -> int {
+        cf = 0;
+        ia = 0.0;
+        if (!inarr(gi, gj)) return -1;
+        const unsigned p = (unsigned)(gj * A.nx_ + gi);
+        cf = 1 | (wet(gi, gj, at32(A.tmask, 4 * p)) ? 2 : 0);
+        ia = at32(A.iarea, 8 * p);
+        return (int)p;
     };
-#pragma unroll
-    for (int m = 0; m < NPFA; m++) {
-        const int c = tid + m * NTHR;
-        rfa[m].inloop = 0;
-        if (c < NFA) face_load<D1>(A, x0 + FAX0 + c % FAW, y0 + FAY0 + c / FAW, lay, rfa[m]);
-    }
-#pragma unroll
-    for (int m = 0; m < NPFB; m++) {
-        const int c = tid + m * NTHR;
-        rfb[m].inloop = 0;
-        if (c < NFB) face_load<D2>(A, x0 + FBX0 + c % FBW, y0 + FBY0 + c / FBW, lay, rfb[m]);
-    }
-#pragma unroll
-    for (int m = 0; m < NPA; m++) {
-        const int c = tid + m * NTHR;
-        rca[m].inarr = 0;
-        if (c < NA) cell_load(x0 + AX0 + c % AW, y0 + AY0 + c / AW, rca[m]);
-    }
-#pragma unroll
-    for (int m = 0; m < NPB; m++) {
-        const int c = tid + m * NTHR;
-        cell_load(x0 + c % TX, y0 + c / TX, rcb[m]);
-    }
-    if (ntr > 0) load_tile(0);
-
-    // ---- stage A faces: transports to LDS for the thickness update
-    FaceK<S, R> fa[NPFA];
-#pragma unroll
-    for (int m = 0; m < NPFA; m++) {
-        const int c = tid + m * NTHR;
-        const FaceC fc = face_make(rfa[m], dt);
-        if (c < NFA) s_q[c] = fc.q;
-        const int base = (FAY0 + c / FAW + 2) * HX + (FAX0 + c % FAW + 2);
-        fa[m].set(fc, c < NFA ? base : 0, SD1);
-    }
-    if (ntr > 0) store_tile(0);
-    __syncthreads();
-    // ---- stage A cells: h1 = h0 - (dt (Q1 - Q0)) iA
-    CellK<R> ca[NPA];
-#pragma unroll
-    for (int m = 0; m < NPA; m++) {
-        const int c = tid + m * NTHR;
-        ca[m].upd = 0;
-        if (c < NA) {
-            const int xl = AX0 + c % AW, yl = AY0 + c / AW;
-            const int own = (yl + 2) * HX + (xl + 2);
-            const int f1 = (yl - FAY0) * FAW + (xl - FAX0);
-            ca[m].own = own;
-            ca[m].fl1 = f1;
-            ca[m].gc = (unsigned)lay + (unsigned)((y0 + yl) * A.nx_ + x0 + xl);
-            double h1 = 0.0;
-            if (rca[m].inarr) {
-                const bool upd = wet(x0 + xl, y0 + yl, rca[m].tm);
-                h1 = rca[m].h0;
-                if (upd) h1 = rca[m].h0 - (dt * (s_q[f1] - s_q[f1 - (D1 == 1 ? 1 : FAW)])) * rca[m].iA;
-                cell_set<R>(ca[m], upd, rca[m].h0, h1, rca[m].iA, dt);
-            }
-            s_h[own] = h1;
-        }
-    }
-    __syncthreads();
-    // ---- stage B faces
-    FaceK<S, R> fb[NPFB];
-#pragma unroll
-    for (int m = 0; m < NPFB; m++) {
-        const int c = tid + m * NTHR;
-        const FaceC fc = face_make(rfb[m], dt);
-        if (c < NFB) s_q[c] = fc.q;
-        const int base = (FBY0 + c / FBW + 2) * HX + (FBX0 + c % FBW + 2);
-        fb[m].set(fc, c < NFB ? base : 0, SD2);
-    }
-    __syncthreads();
-    // ---- stage B cells: h2 = h1 - (dt (Q1 - Q0)) iA, written as the pair's thickness
-    CellK<R> cb[NPB];
-#pragma unroll
-    for (int m = 0; m < NPB; m++) {
-        const int c = tid + m * NTHR;
-        const int xl = c % TX, yl = c / TX;
-        const int own = (yl + 2) * HX + (xl + 2);
-        const int f1 = (yl - FBY0) * FBW + (xl - FBX0);
-        cb[m].upd = 0;
-        cb[m].own = own;
-        cb[m].fl1 = f1;
-        cb[m].gc = (unsigned)lay + (unsigned)ooff[m];
-        if (rcb[m].inarr) {
-            const double h1 = s_h[own];
-            const bool upd = wet(x0 + xl, y0 + yl, rcb[m].tm);
-            double h2 = h1;
-            if (upd) h2 = h1 - (dt * (s_q[f1] - s_q[f1 - (D2 == 1 ? 1 : FBW)])) * rcb[m].iA;
-            cell_set<R>(cb[m], upd, h1, h2, rcb[m].iA, dt);
-            at32(hout + lay, 8u * (unsigned)((y0 + yl) * A.nx_ + x0 + xl)) = h2;
-        }
-    }
-
-    // ---- tracer loop: tile t+1 was loaded during iteration t-1 and is stored to
-    // LDS now, tile t+2 is loaded during this iteration (three LDS tile buffers)
-    const unsigned ss_a = stage_seed<R>(A, sr_stage), ss_b = stage_seed<R>(A, sr_stage + 1);
-    if (ntr > 1) load_tile(1);
-    for (int t = 0; t < ntr; t++) {
-        const R* sin = s_in[t % 3];
-        const SRound sr_a = tracer_round(ss_a, tg0 + t), sr_b = tracer_round(ss_b, tg0 + t);
-        if (t + 1 < ntr) store_tile((t + 1) % 3);
-        if (t + 2 < ntr) load_tile(t + 2);
-        // stage A faces
+    if constexpr (GROUP) {
 #pragma unroll
         for (int m = 0; m < NPFA; m++) {
             const int c = tid + m * NTHR;
             if (c < NFA) {
-                R fl = fa[m].flux(sin);
-                if (A.ah_u && fa[m].open) {
-                    const int xs = FAX0 + c % FAW, ys = FAY0 + c / FAW;
-                    const int base = (ys + 2) * HX + xs + 2;
-                    fl = fl - ah_flux<D1, R>(A, x0 + xs, y0 + ys, k, sin[base], sin[base + SD1]);
-                }
-                s_fa[c] = fl;
+                unsigned char fl;
+                double mq, mc;
+                face2d(std::integral_constant<int, D1>(), x0 + FAX0 + c % FAW, y0 + FAY0 + c / FAW, fl, mq, mc);
+                s_ffa[c] = fl;
+                s_mqa[c] = mq;
+                s_mca[c] = mc;
             }
         }
-        __syncthreads();
-        // stage A cells
-#pragma unroll
-        for (int m = 0; m < NPA; m++) {
-            const int c = tid + m * NTHR;
-            if (c < NA) {
-                R out = sin[ca[m].own];
-                if (ca[m].upd)
-                    out = cell_update<R>(ca[m], dt, out, s_fa[ca[m].fl1 - (D1 == 1 ? 1 : FAW)], s_fa[ca[m].fl1], sr_a);
-                s_a[ca[m].own] = out;
-            }
-        }
-        __syncthreads();
-        // stage B faces
 #pragma unroll
         for (int m = 0; m < NPFB; m++) {
             const int c = tid + m * NTHR;
             if (c < NFB) {
-                R fl = fb[m].flux(s_a);
-                if (A.ah_u && fb[m].open) {
-                    const int xs = FBX0 + c % FBW, ys = FBY0 + c / FBW;
-                    const int base = (ys + 2) * HX + xs + 2;
-                    fl = fl - ah_flux<D2, R>(A, x0 + xs, y0 + ys, k, s_a[base], s_a[base + SD2]);
-                }
-                s_fb[c] = fl;
+                unsigned char fl;
+                double mq, mc;
+                face2d(std::integral_constant<int, D2>(), x0 + FBX0 + c % FBW, y0 + FBY0 + c / FBW, fl, mq, mc);
+                s_ffb[c] = fl;
+                s_mqb[c] = mq;
+                s_mcb[c] = mc;
+            }
+        }
+        // region A cells (region B's are a subset)
+#pragma unroll
+        for (int m = 0; m < NPA; m++) {
+            const int c = tid + m * NTHR;
+            if (c < NA) {
+                const int xl = AX0 + c % AW, yl = AY0 + c / AW;
+                const int own = (yl + 2) * HX + (xl + 2);
+                unsigned char cf;
+                double ia;
+                cell2d(x0 + xl, y0 + yl, cf, ia);
+                s_cf[own] = cf;
+                s_ia[own] = ia;
             }
         }
         __syncthreads();
-        // stage B cells -> HBM
-        R* fo = fout + (size_t)t * A.n3 + lay;
+    }
+
+    const unsigned ss_a = stage_seed<R>(A, sr_stage), ss_b = stage_seed<R>(A, sr_stage + 1);
+    for (int k = k0; k < k1; k++) {
+        // the lane index through an empty asm: the layer-invariant index arithmetic
+        // below is recomputed per layer instead of being hoisted out of the loop
+        // and held in registers across it
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        // in-layer offsets of this lane's input-tile cells (-1 outside the array),
+        // output cells (-1 outside), open faces and region A cells (-1: nothing to load)
+        int toff[NPIN], ooff[NPB], foa[NPFA], fob[NPFB], coa[NPA];
+#pragma unroll
+        for (int m = 0; m < NPIN; m++) {
+            const int c = tid + m * NTHR;
+            toff[m] = -1;
+            if (c < NT) {
+                const int gi = x0 - 2 + c % HX, gj = y0 - 2 + c / HX;
+                if (inarr(gi, gj)) toff[m] = gj * A.nx_ + gi;
+            }
+        }
 #pragma unroll
         for (int m = 0; m < NPB; m++) {
-            R out = s_a[cb[m].own];
-            if (cb[m].upd)
-                out = cell_update<R>(cb[m], dt, out, s_fb[cb[m].fl1 - (D2 == 1 ? 1 : FBW)], s_fb[cb[m].fl1], sr_b);
-            if (ooff[m] >= 0) at32(fo, (unsigned)ooff[m] * (unsigned)sizeof(R)) = out;
+            const int c = tid + m * NTHR;
+            const int gi = x0 + c % TX, gj = y0 + c / TX;
+            ooff[m] = (gi < A.nx_ && gj < A.ny_) ? gj * A.nx_ + gi : -1;
+        }
+        // one layer: the layer-independent terms of this lane's faces and cells in registers
+        unsigned char rffa[NPFA], rffb[NPFB], rcfa[NPA], rcfb[NPB];
+        double rmqa[NPFA], rmca[NPFA], rmqb[NPFB], rmcb[NPFB], ria_a[NPA], ria_b[NPB];
+#pragma unroll
+        for (int m = 0; m < NPFA; m++) {
+            const int c = tid + m * NTHR;
+            if constexpr (GROUP) {
+                foa[m] = (c < NFA && (s_ffa[c] & 1)) ? (y0 + FAY0 + c / FAW) * A.nx_ + x0 + FAX0 + c % FAW : -1;
+            } else {
+                rffa[m] = 0;
+                rmqa[m] = rmca[m] = 0.0;
+                foa[m] = c < NFA ? face2d(std::integral_constant<int, D1>(), x0 + FAX0 + c % FAW, y0 + FAY0 + c / FAW,
+                                          rffa[m], rmqa[m], rmca[m])
+                                 : -1;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NPFB; m++) {
+            const int c = tid + m * NTHR;
+            if constexpr (GROUP) {
+                fob[m] = (c < NFB && (s_ffb[c] & 1)) ? (y0 + FBY0 + c / FBW) * A.nx_ + x0 + FBX0 + c % FBW : -1;
+            } else {
+                rffb[m] = 0;
+                rmqb[m] = rmcb[m] = 0.0;
+                fob[m] = c < NFB ? face2d(std::integral_constant<int, D2>(), x0 + FBX0 + c % FBW, y0 + FBY0 + c / FBW,
+                                          rffb[m], rmqb[m], rmcb[m])
+                                 : -1;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NPA; m++) {
+            const int c = tid + m * NTHR;
+            coa[m] = -1;
+            rcfa[m] = 0;
+            ria_a[m] = 0.0;
+            if (c < NA) {
+                const int xl = AX0 + c % AW, yl = AY0 + c / AW;
+                if constexpr (GROUP) {
+                    if (s_cf[(yl + 2) * HX + (xl + 2)] & 1) coa[m] = (y0 + yl) * A.nx_ + x0 + xl;
+                } else {
+                    coa[m] = cell2d(x0 + xl, y0 + yl, rcfa[m], ria_a[m]);
+                }
+            }
+        }
+        if constexpr (!GROUP) {
+#pragma unroll
+            for (int m = 0; m < NPB; m++) {
+                const int c = tid + m * NTHR;
+                cell2d(x0 + c % TX, y0 + c / TX, rcfb[m], ria_b[m]);
+            }
+        }
+        const size_t lay = (size_t)k * A.n2;
+        R pf[NPIN];
+        auto load_tile = [&](int t) {
+            const R* fi = fin + (size_t)t * A.n3 + lay;
+#pragma unroll
+            for (int m = 0; m < NPIN; m++)
+                pf[m] = toff[m] >= 0 ? at32(fi, (unsigned)toff[m] * (unsigned)sizeof(R)) : R(0);
+        };
+        auto store_tile = [&](int buf) {
+#pragma unroll
+            for (int m = 0; m < NPIN; m++) {
+                const int c = tid + m * NTHR;
+                if (c < NT) s_in[buf][c] = pf[m];
+            }
+        };
+        // ---- every per-layer load of the set-up in flight at once
+        double va[NPFA], ha[NPFA], vb[NPFB], hb[NPFB], h0a[NPA];
+#pragma unroll
+        for (int m = 0; m < NPFA; m++) {
+            va[m] = 0.0;
+            ha[m] = 0.0;
+            if (foa[m] >= 0) {
+                va[m] = at32(D1 == 1 ? A.u + lay : A.v + lay, 8u * (unsigned)foa[m]);
+                ha[m] = at32(D1 == 1 ? A.hu + lay : A.hv + lay, 8u * (unsigned)foa[m]);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NPFB; m++) {
+            vb[m] = 0.0;
+            hb[m] = 0.0;
+            if (fob[m] >= 0) {
+                vb[m] = at32(D2 == 1 ? A.u + lay : A.v + lay, 8u * (unsigned)fob[m]);
+                hb[m] = at32(D2 == 1 ? A.hu + lay : A.hv + lay, 8u * (unsigned)fob[m]);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NPA; m++) h0a[m] = coa[m] >= 0 ? at32(hin + lay, 8u * (unsigned)coa[m]) : 0.0;
+        if (ntr > 0) load_tile(0);
+
+        // ---- stage A faces: transports to LDS for the thickness update
+        FaceK<S, R> fa[NPFA];
+#pragma unroll
+        for (int m = 0; m < NPFA; m++) {
+            const int c = tid + m * NTHR;
+            FaceC fc{0.0, 0.0, 0};
+            if (c < NFA) {
+                if constexpr (GROUP) fc = face_make2(s_ffa[c], va[m], ha[m], s_mqa[c], s_mca[c], dt);
+                else fc = face_make2(rffa[m], va[m], ha[m], rmqa[m], rmca[m], dt);
+            }
+            if (c < NFA) s_q[c] = fc.q;
+            const int base = (FAY0 + c / FAW + 2) * HX + (FAX0 + c % FAW + 2);
+            fa[m].set(fc, c < NFA ? base : 0, SD1);
+        }
+        if (ntr > 0) store_tile(0);
+        __syncthreads();
+        // ---- stage A cells: h1 = h0 - (dt (Q1 - Q0)) iA
+        CellK<R> ca[NPA];
+#pragma unroll
+        for (int m = 0; m < NPA; m++) {
+            const int c = tid + m * NTHR;
+            ca[m].upd = 0;
+            if (c < NA) {
+                const int xl = AX0 + c % AW, yl = AY0 + c / AW;
+                const int own = (yl + 2) * HX + (xl + 2);
+                const int f1 = (yl - FAY0) * FAW + (xl - FAX0);
+                ca[m].own = own;
+                ca[m].fl1 = f1;
+                ca[m].gc = (unsigned)lay + (unsigned)((y0 + yl) * A.nx_ + x0 + xl);
+                double h1 = 0.0;
+                if (coa[m] >= 0) {
+                    const bool upd = GROUP ? (s_cf[own] & 2) : (rcfa[m] & 2);
+                    const double ia = GROUP ? s_ia[own] : ria_a[m];
+                    h1 = h0a[m];
+                    if (upd) h1 = h0a[m] - (dt * (s_q[f1] - s_q[f1 - (D1 == 1 ? 1 : FAW)])) * ia;
+                    cell_set<R>(ca[m], upd, h0a[m], h1, ia, dt);
+                }
+                s_h[own] = h1;
+            }
         }
         __syncthreads();
+        // ---- stage B faces
+        FaceK<S, R> fb[NPFB];
+#pragma unroll
+        for (int m = 0; m < NPFB; m++) {
+            const int c = tid + m * NTHR;
+            FaceC fc{0.0, 0.0, 0};
+            if (c < NFB) {
+                if constexpr (GROUP) fc = face_make2(s_ffb[c], vb[m], hb[m], s_mqb[c], s_mcb[c], dt);
+                else fc = face_make2(rffb[m], vb[m], hb[m], rmqb[m], rmcb[m], dt);
+            }
+            if (c < NFB) s_q[c] = fc.q;
+            const int base = (FBY0 + c / FBW + 2) * HX + (FBX0 + c % FBW + 2);
+            fb[m].set(fc, c < NFB ? base : 0, SD2);
+        }
+        __syncthreads();
+        // ---- stage B cells: h2 = h1 - (dt (Q1 - Q0)) iA, written as the pair's thickness
+        CellK<R> cb[NPB];
+#pragma unroll
+        for (int m = 0; m < NPB; m++) {
+            const int c = tid + m * NTHR;
+            const int xl = c % TX, yl = c / TX;
+            const int own = (yl + 2) * HX + (xl + 2);
+            const int f1 = (yl - FBY0) * FBW + (xl - FBX0);
+            cb[m].upd = 0;
+            cb[m].own = own;
+            cb[m].fl1 = f1;
+            cb[m].gc = (unsigned)lay + (unsigned)ooff[m];
+            if (ooff[m] >= 0) {
+                const double h1 = s_h[own];
+                const bool upd = GROUP ? (s_cf[own] & 2) : (rcfb[m] & 2);
+                const double ia = GROUP ? s_ia[own] : ria_b[m];
+                double h2 = h1;
+                if (upd) h2 = h1 - (dt * (s_q[f1] - s_q[f1 - (D2 == 1 ? 1 : FBW)])) * ia;
+                cell_set<R>(cb[m], upd, h1, h2, ia, dt);
+                at32(hout + lay, 8u * (unsigned)ooff[m]) = h2;
+            }
+        }
+
+        // ---- tracer loop: tile t+1 was loaded during iteration t-1 and is stored to
+        // LDS now, tile t+2 is loaded during this iteration (three LDS tile buffers)
+        if (ntr > 1) load_tile(1);
+        for (int t = 0; t < ntr; t++) {
+            const R* sin = s_in[t % 3];
+            const SRound sr_a = tracer_round(ss_a, tg0 + t), sr_b = tracer_round(ss_b, tg0 + t);
+            if (t + 1 < ntr) store_tile((t + 1) % 3);
+            if (t + 2 < ntr) load_tile(t + 2);
+            // stage A faces
+#pragma unroll
+            for (int m = 0; m < NPFA; m++) {
+                const int c = tid + m * NTHR;
+                if (c < NFA) {
+                    R fl = fa[m].flux(sin);
+                    if (A.ah_u && fa[m].open) {
+                        const int xs = FAX0 + c % FAW, ys = FAY0 + c / FAW;
+                        const int base = (ys + 2) * HX + xs + 2;
+                        fl = fl - ah_flux<D1, R>(A, x0 + xs, y0 + ys, k, sin[base], sin[base + SD1]);
+                    }
+                    s_fa[c] = fl;
+                }
+            }
+            __syncthreads();
+            // stage A cells
+#pragma unroll
+            for (int m = 0; m < NPA; m++) {
+                const int c = tid + m * NTHR;
+                if (c < NA) {
+                    R out = sin[ca[m].own];
+                    if (ca[m].upd)
+                        out = cell_update<R>(ca[m], dt, out, s_fa[ca[m].fl1 - (D1 == 1 ? 1 : FAW)], s_fa[ca[m].fl1],
+                                             sr_a);
+                    s_a[ca[m].own] = out;
+                }
+            }
+            __syncthreads();
+            // stage B faces
+#pragma unroll
+            for (int m = 0; m < NPFB; m++) {
+                const int c = tid + m * NTHR;
+                if (c < NFB) {
+                    R fl = fb[m].flux(s_a);
+                    if (A.ah_u && fb[m].open) {
+                        const int xs = FBX0 + c % FBW, ys = FBY0 + c / FBW;
+                        const int base = (ys + 2) * HX + xs + 2;
+                        fl = fl - ah_flux<D2, R>(A, x0 + xs, y0 + ys, k, s_a[base], s_a[base + SD2]);
+                    }
+                    s_fb[c] = fl;
+                }
+            }
+            __syncthreads();
+            // stage B cells -> HBM
+            R* fo = fout + (size_t)t * A.n3 + lay;
+#pragma unroll
+            for (int m = 0; m < NPB; m++) {
+                R out = s_a[cb[m].own];
+                if (cb[m].upd)
+                    out = cell_update<R>(cb[m], dt, out, s_fb[cb[m].fl1 - (D2 == 1 ? 1 : FBW)], s_fb[cb[m].fl1],
+                                         sr_b);
+                if (ooff[m] >= 0) at32(fo, (unsigned)ooff[m] * (unsigned)sizeof(R)) = out;
+            }
+            __syncthreads();
+        }
+        if (ntr == 0) __syncthreads();  // s_q and s_h are rewritten by the next layer
     }
 }
 
 // Single domain, 1-D grid in XCD-aware order.  Work-groups go to the 8 XCDs
 // round-robin (block b to XCD b % 8), and each XCD has its own L2, so block b
 // takes logical item v = (b % 8) * (gridDim.x / 8) + b / 8: every XCD walks a
-// contiguous range of the logical order (tracer batch fastest, then layer, then
-// tile).  The blocks resident on one XCD are then the same tile at consecutive
-// layers, and the tile's 2-D fields (3 masks, 5 metrics) come from that XCD's
-// L2 instead of being fetched again from HBM for every layer.  Every
-// tracer-batch block writes the same thickness.
-template <int S, typename R, int D1, int TY>
-__global__ void __launch_bounds__(TX * TY) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
+// contiguous range of the logical order (tracer batch fastest, then layer
+// group, then tile).  The blocks resident on one XCD are then the same tile at
+// consecutive layer groups, and the tile's 2-D fields (3 masks, 5 metrics) come
+// from that XCD's L2 instead of being fetched again from HBM for every group.
+// Every tracer-batch block writes the same thickness.
+// FP32: at most 128 VGPRs (four waves per SIMD), two 512-lane blocks per CU at TY = 8.
+// KG: layers per block (1 or HPAIR_KG).
+template <int S, typename R, int D1, int TY, int KG>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
                                                    int nty, int nbatch, const R* __restrict__ fin_all,
                                                    R* __restrict__ fout_all, const double* __restrict__ hin,
                                                    double* __restrict__ hout, unsigned sr_stage)
 {
     const unsigned b = blockIdx.x, per = gridDim.x / 8;
     unsigned v = (b % 8) * per + b / 8;
-    const unsigned nitems = (unsigned)nbatch * A.nz * ntx * nty;
+    const unsigned ngroups = (unsigned)(A.nz + KG - 1) / KG;
+    const unsigned nitems = (unsigned)nbatch * ngroups * ntx * nty;
     if (v >= nitems) return;
     const int tbatch = v % nbatch;
     v /= nbatch;
-    const int k = v % A.nz;
-    v /= A.nz;
+    const int k0 = (int)(v % ngroups) * KG;
+    v /= ngroups;
+    const int k1 = k0 + KG < A.nz ? k0 + KG : A.nz;
     const int tx = v % ntx, ty = v / ntx;
     const int tb0 = tbatch * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
-    hpair_body<S, R, D1, TY>(A, dt, tx * TX, ty * TY, k, ntr, fin_all + (size_t)tb0 * A.n3,
+    hpair_body<S, R, D1, TY, (KG > 1)>(A, dt, tx * TX, ty * TY, k0, k1, ntr, fin_all + (size_t)tb0 * A.n3,
                              fout_all + (size_t)tb0 * A.n3, hin, hout, sr_stage, tb0);
 }
 
-// Ensemble: one work item per block {domain, x0, y0, layer | batch << 16}; the
+// Ensemble: one work item per block {domain, x0, y0, first layer of the group | batch << 16}; the
 // table is built in the XCD-aware order of k_hpair (domain -1 = padding).
 // STAGE 1: stages 1-2 (tracers -> scratch, ho -> hA); STAGE 2: stages 4-5
 // (tracers -> scratch, hB -> hA).
-template <int S, typename R, int D1, int TY, int STAGE>
-__global__ void __launch_bounds__(TX * TY) k_hpair_multi(const EnsDom<R>* __restrict__ doms,
+template <int S, typename R, int D1, int TY, int STAGE, int KG>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair_multi(const EnsDom<R>* __restrict__ doms,
                                                          const int4* __restrict__ work)
 {
     const int4 wk = work[blockIdx.x];
     if (wk.x < 0) return;  // padding of the XCD-ordered table
     const EnsDom<R>& D = doms[wk.x];
-    const int k = wk.w & 0xffff, tb0 = (wk.w >> 16) * D.tpb_h;
+    const int k0 = wk.w & 0xffff, tb0 = (wk.w >> 16) * D.tpb_h;
+    const int k1 = k0 + KG < D.A.nz ? k0 + KG : D.A.nz;
     const int ntr = D.ntr - tb0 < D.tpb_h ? D.ntr - tb0 : D.tpb_h;
     const size_t off = (size_t)tb0 * D.A.n3;
-    hpair_body<S, R, D1, TY>(D.A, 0.5 * D.dt, wk.y, wk.z, k, ntr, D.tracers + off, D.scratch + off,
+    hpair_body<S, R, D1, TY, (KG > 1)>(D.A, 0.5 * D.dt, wk.y, wk.z, k0, k1, ntr, D.tracers + off, D.scratch + off,
                              STAGE == 1 ? D.ho : D.hB, D.hA, STAGE == 1 ? SR_PAIR1 : SR_PAIR2, tb0);
 }
 
@@ -1170,19 +1291,30 @@ int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int n
     const int nb = nbatch(ntr, tpb);
     if (ty != 4 && ty != 8 && ty != 16) return GETM_ERR_ARG;
     const int ntx = (A.nx_ + TX - 1) / TX, nty = (A.ny_ + ty - 1) / ty;
-    const long nitems = (long)nb * A.nz * ntx * nty;
+    const int kg = hpair_group(tpb, std::is_same<R, float>::value);
+    const long nitems = (long)nb * ((A.nz + kg - 1) / kg) * ntx * nty;
     if (nitems > 0x7fffff00L) return GETM_ERR_CAPACITY;
     dim3 grid((unsigned)(8 * ((nitems + 7) / 8)));
     ProfileScope ps(d1 == 2 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
-#define GETM_HP(TYV)                                                                                           \
-    GETM_SCHEME_SWITCH(scheme, {                                                                               \
-        if (d1 == 2)                                                                                           \
-            hipLaunchKernelGGL((k_hpair<S_, R, 2, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, nb, \
-                               fin, fout, hin, hout, sr_stage);                                                \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_hpair<S_, R, 1, TYV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, nb, \
-                               fin, fout, hin, hout, sr_stage);                                                \
+#define GETM_HPK(TYV, KGV)                                                                                        \
+    GETM_SCHEME_SWITCH(scheme, {                                                                                  \
+        if (d1 == 2)                                                                                              \
+            hipLaunchKernelGGL((k_hpair<S_, R, 2, TYV, KGV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, \
+                               nb, fin, fout, hin, hout, sr_stage);                                               \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_hpair<S_, R, 1, TYV, KGV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, \
+                               nb, fin, fout, hin, hout, sr_stage);                                               \
     })
+#define GETM_HP(TYV)                                                     \
+    if constexpr (std::is_same<R, float>::value) {                       \
+        if (kg == HPAIR_KG) {                                            \
+            GETM_HPK(TYV, HPAIR_KG);                                     \
+        } else {                                                         \
+            GETM_HPK(TYV, 1);                                            \
+        }                                                                \
+    } else {                                                             \
+        GETM_HPK(TYV, 1);                                                \
+    }
     if (ty == 16) {
         GETM_HP(16);
     } else if (ty == 8) {
@@ -1191,6 +1323,7 @@ int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int n
         GETM_HP(4);
     }
 #undef GETM_HP
+#undef GETM_HPK
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }
@@ -1306,22 +1439,28 @@ template int launch_hpair<double>(int, int, int, const StepArgs&, double, int, i
 // ---------------------------------------------------------------------------
 // ensemble launchers: one launch per kernel kind covers every domain
 // ---------------------------------------------------------------------------
-int launch_hpair_multi(int stage, int d1, int scheme, int ty, const EnsDom<float>* doms, const int4* work, int nwork,
-                       hipStream_t s)
+int launch_hpair_multi(int stage, int d1, int scheme, int ty, int kg, const EnsDom<float>* doms, const int4* work,
+                       int nwork, hipStream_t s)
 {
     if (nwork <= 0) return GETM_OK;
+    if (kg != 1) return GETM_ERR_ARG;  // ensembles take one layer per work item
     ProfileScope ps(stage == 1 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
-#define GETM_HPM(TYV)                                                                                              \
-    GETM_SCHEME_SWITCH(scheme, {                                                                                   \
-        if (stage == 1 && d1 == 2)                                                                                 \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 1>), dim3(nwork), dim3(TX * TYV), 0, s, doms, work); \
-        else if (stage == 1)                                                                                       \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 1>), dim3(nwork), dim3(TX * TYV), 0, s, doms, work); \
-        else if (d1 == 2)                                                                                          \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 2>), dim3(nwork), dim3(TX * TYV), 0, s, doms, work); \
-        else                                                                                                       \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 2>), dim3(nwork), dim3(TX * TYV), 0, s, doms, work); \
+#define GETM_HPMK(TYV, KGV)                                                                                     \
+    GETM_SCHEME_SWITCH(scheme, {                                                                                \
+        if (stage == 1 && d1 == 2)                                                                              \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 1, KGV>), dim3(nwork), dim3(TX * TYV), 0, s, doms, \
+                               work);                                                                           \
+        else if (stage == 1)                                                                                    \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 1, KGV>), dim3(nwork), dim3(TX * TYV), 0, s, doms, \
+                               work);                                                                           \
+        else if (d1 == 2)                                                                                       \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 2, KGV>), dim3(nwork), dim3(TX * TYV), 0, s, doms, \
+                               work);                                                                           \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 2, KGV>), dim3(nwork), dim3(TX * TYV), 0, s, doms, \
+                               work);                                                                           \
     })
+#define GETM_HPM(TYV) GETM_HPMK(TYV, 1)
     if (ty == 16) {
         GETM_HPM(16);
     } else if (ty == 8) {
@@ -1332,6 +1471,7 @@ int launch_hpair_multi(int stage, int d1, int scheme, int ty, const EnsDom<float
         return GETM_ERR_ARG;
     }
 #undef GETM_HPM
+#undef GETM_HPMK
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }
