@@ -269,7 +269,9 @@ __device__ __forceinline__ SRound tracer_round(unsigned stage_seed, int tracer)
 // tg0: index in the domain of the first tracer of fin/fout.
 // GROUP: the block walks several layers and keeps the layer-independent terms
 // in LDS; otherwise (one layer) they are loaded straight into registers.
-template <int S, typename R, int D1, int TY, bool GROUP>
+// HDIFF: horizontal diffusion fluxes (Ah given).  A separate instantiation: the
+// branch alone, even never taken, holds ~45 VGPRs of its operands.
+template <int S, typename R, int D1, int TY, bool GROUP, bool HDIFF>
 __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0, int y0, int k0, int k1, int ntr,
                                            const R* __restrict__ fin, R* __restrict__ fout,
                                            const double* __restrict__ hin, double* __restrict__ hout,
@@ -594,10 +596,12 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
                 const int c = tid + m * NTHR;
                 if (c < NFA) {
                     R fl = fa[m].flux(sin);
-                    if (A.ah_u && fa[m].open) {
-                        const int xs = FAX0 + c % FAW, ys = FAY0 + c / FAW;
-                        const int base = (ys + 2) * HX + xs + 2;
-                        fl = fl - ah_flux<D1, R>(A, x0 + xs, y0 + ys, k, sin[base], sin[base + SD1]);
+                    if constexpr (HDIFF) {
+                        if (fa[m].open) {
+                            const int xs = FAX0 + c % FAW, ys = FAY0 + c / FAW;
+                            const int base = (ys + 2) * HX + xs + 2;
+                            fl = fl - ah_flux<D1, R>(A, x0 + xs, y0 + ys, k, sin[base], sin[base + SD1]);
+                        }
                     }
                     s_fa[c] = fl;
                 }
@@ -622,10 +626,12 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
                 const int c = tid + m * NTHR;
                 if (c < NFB) {
                     R fl = fb[m].flux(s_a);
-                    if (A.ah_u && fb[m].open) {
-                        const int xs = FBX0 + c % FBW, ys = FBY0 + c / FBW;
-                        const int base = (ys + 2) * HX + xs + 2;
-                        fl = fl - ah_flux<D2, R>(A, x0 + xs, y0 + ys, k, s_a[base], s_a[base + SD2]);
+                    if constexpr (HDIFF) {
+                        if (fb[m].open) {
+                            const int xs = FBX0 + c % FBW, ys = FBY0 + c / FBW;
+                            const int base = (ys + 2) * HX + xs + 2;
+                            fl = fl - ah_flux<D2, R>(A, x0 + xs, y0 + ys, k, s_a[base], s_a[base + SD2]);
+                        }
                     }
                     s_fb[c] = fl;
                 }
@@ -656,8 +662,8 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
 // from that XCD's L2 instead of being fetched again from HBM for every group.
 // Every tracer-batch block writes the same thickness.
 // FP32: at most 128 VGPRs (four waves per SIMD), two 512-lane blocks per CU at TY = 8.
-// KG: layers per block (1 or HPAIR_KG).
-template <int S, typename R, int D1, int TY, int KG>
+// KG: layers per block (1 or HPAIR_KG); HDIFF: horizontal diffusion (A.ah_u set).
+template <int S, typename R, int D1, int TY, int KG, bool HDIFF>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
                                                    int nty, int nbatch, const R* __restrict__ fin_all,
                                                    R* __restrict__ fout_all, const double* __restrict__ hin,
@@ -676,7 +682,7 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(si
     const int tx = v % ntx, ty = v / ntx;
     const int tb0 = tbatch * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
-    hpair_body<S, R, D1, TY, (KG > 1)>(A, dt, tx * TX, ty * TY, k0, k1, ntr, fin_all + (size_t)tb0 * A.n3,
+    hpair_body<S, R, D1, TY, (KG > 1), HDIFF>(A, dt, tx * TX, ty * TY, k0, k1, ntr, fin_all + (size_t)tb0 * A.n3,
                              fout_all + (size_t)tb0 * A.n3, hin, hout, sr_stage, tb0);
 }
 
@@ -695,7 +701,8 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(si
     const int k1 = k0 + KG < D.A.nz ? k0 + KG : D.A.nz;
     const int ntr = D.ntr - tb0 < D.tpb_h ? D.ntr - tb0 : D.tpb_h;
     const size_t off = (size_t)tb0 * D.A.n3;
-    hpair_body<S, R, D1, TY, (KG > 1)>(D.A, 0.5 * D.dt, wk.y, wk.z, k0, k1, ntr, D.tracers + off, D.scratch + off,
+    hpair_body<S, R, D1, TY, (KG > 1), false>(D.A, 0.5 * D.dt, wk.y, wk.z, k0, k1, ntr, D.tracers + off,
+                                              D.scratch + off,
                              STAGE == 1 ? D.ho : D.hB, D.hA, STAGE == 1 ? SR_PAIR1 : SR_PAIR2, tb0);
 }
 
@@ -1291,29 +1298,32 @@ int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int n
     const int nb = nbatch(ntr, tpb);
     if (ty != 4 && ty != 8 && ty != 16) return GETM_ERR_ARG;
     const int ntx = (A.nx_ + TX - 1) / TX, nty = (A.ny_ + ty - 1) / ty;
-    const int kg = hpair_group(tpb, std::is_same<R, float>::value);
+    const bool ah = A.ah_u != nullptr;
+    const int kg = ah ? 1 : hpair_group(tpb, std::is_same<R, float>::value);
     const long nitems = (long)nb * ((A.nz + kg - 1) / kg) * ntx * nty;
     if (nitems > 0x7fffff00L) return GETM_ERR_CAPACITY;
     dim3 grid((unsigned)(8 * ((nitems + 7) / 8)));
     ProfileScope ps(d1 == 2 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
-#define GETM_HPK(TYV, KGV)                                                                                        \
-    GETM_SCHEME_SWITCH(scheme, {                                                                                  \
-        if (d1 == 2)                                                                                              \
-            hipLaunchKernelGGL((k_hpair<S_, R, 2, TYV, KGV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, \
-                               nb, fin, fout, hin, hout, sr_stage);                                               \
-        else                                                                                                      \
-            hipLaunchKernelGGL((k_hpair<S_, R, 1, TYV, KGV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, nty, \
-                               nb, fin, fout, hin, hout, sr_stage);                                               \
+#define GETM_HPK(TYV, KGV, AHV)                                                                                  \
+    GETM_SCHEME_SWITCH(scheme, {                                                                                    \
+        if (d1 == 2)                                                                                                \
+            hipLaunchKernelGGL((k_hpair<S_, R, 2, TYV, KGV, AHV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, \
+                               nty, nb, fin, fout, hin, hout, sr_stage);                                            \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_hpair<S_, R, 1, TYV, KGV, AHV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, \
+                               nty, nb, fin, fout, hin, hout, sr_stage);                                            \
     })
-#define GETM_HP(TYV)                                                     \
-    if constexpr (std::is_same<R, float>::value) {                       \
-        if (kg == HPAIR_KG) {                                            \
-            GETM_HPK(TYV, HPAIR_KG);                                     \
-        } else {                                                         \
-            GETM_HPK(TYV, 1);                                            \
-        }                                                                \
-    } else {                                                             \
-        GETM_HPK(TYV, 1);                                                \
+#define GETM_HP(TYV)                                  \
+    if (ah) {                                         \
+        GETM_HPK(TYV, 1, true);                       \
+    } else if constexpr (std::is_same<R, float>::value) { \
+        if (kg == HPAIR_KG) {                         \
+            GETM_HPK(TYV, HPAIR_KG, false);           \
+        } else {                                      \
+            GETM_HPK(TYV, 1, false);                  \
+        }                                             \
+    } else {                                          \
+        GETM_HPK(TYV, 1, false);                      \
     }
     if (ty == 16) {
         GETM_HP(16);
