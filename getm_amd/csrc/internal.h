@@ -162,10 +162,15 @@ struct Ensemble {
     unsigned long long** rstep_ext = nullptr;  // per-domain caller counters (NULL entries: own)
 };
 
-// Default tile height of the horizontal stage pair: with few tracers the
-// per-tile set-up dominates and two 512-lane blocks per CU overlap each other's
-// barrier phases (8 rows); with many, the smaller halo share of 16-row tiles wins.
-inline int auto_tile_y(int tracers_per_domain) { return tracers_per_domain <= 8 ? 8 : 16; }
+// Default tile height of the horizontal stage pair: 8 rows.  Three 512-lane
+// blocks per CU overlap each other's barrier phases; 16-row tiles have a smaller
+// halo share but fit one or two 1024-lane blocks per CU, and were slower at
+// 2 and at 256 tracers on the coastal grid.
+inline int auto_tile_y(int tracers_per_domain)
+{
+    (void)tracers_per_domain;
+    return 8;
+}
 
 // ---- batched kernels (transport_batched.hip) ----
 // tpb = tracers per block (tracer batches go into the grid); <= 0 means all

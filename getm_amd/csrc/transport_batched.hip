@@ -297,12 +297,24 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
     constexpr int NB = TX * TY, NPB = NB / NTHR;
     constexpr int NPIN = (NT + NTHR - 1) / NTHR;
 
-    __shared__ R s_in[3][NT];
-    __shared__ R s_a[NT];
-    __shared__ R s_fa[NFA];
-    __shared__ R s_fb[NFB];
-    __shared__ double s_q[NFA > NFB ? NFA : NFB];  // FP64 face transports of the stage being set up
-    __shared__ double s_h[NT];                     // FP64 thickness after stage A
+    // LDS: the first tile buffer, and one pool that holds the other two tile
+    // buffers, the stage-A tile and the flux rows while tracers are processed,
+    // and the FP64 face transports (s_q) and stage-A thickness (s_h) during each
+    // layer's set-up (a smaller footprint leaves room for a third block per CU)
+    constexpr int NQ = NFA > NFB ? NFA : NFB;
+    constexpr int POOL_T = (int)sizeof(R) * (3 * NT + NFA + NFB);
+    constexpr int POOL_S = (int)sizeof(double) * (NQ + NT);
+    static_assert(POOL_S <= POOL_T, "set-up arrays must fit in the tracer-loop pool");
+    __shared__ R s_in0[NT];
+    __shared__ __attribute__((aligned(16))) unsigned char s_pool[POOL_T];
+    R* const s_in1 = reinterpret_cast<R*>(s_pool);
+    R* const s_in2 = s_in1 + NT;
+    R* const s_a = s_in2 + NT;
+    R* const s_fa = s_a + NT;
+    R* const s_fb = s_fa + NFA;
+    double* const s_q = reinterpret_cast<double*>(s_pool);  // FP64 face transports of the stage being set up
+    double* const s_h = s_q + NQ;                           // FP64 thickness after stage A
+    auto s_in = [&](int b) -> R* { return b == 0 ? s_in0 : (b == 1 ? s_in1 : s_in2); };
     // layer-independent terms, formed once per block: face metrics (transport and
     // inverse CFL metric) and mask flags, cell inverse areas and wet flags
     constexpr int GFA = GROUP ? NFA : 1, GFB = GROUP ? NFB : 1, GT = GROUP ? NT : 1;
@@ -473,10 +485,11 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
                 pf[m] = toff[m] >= 0 ? at32(fi, (unsigned)toff[m] * (unsigned)sizeof(R)) : R(0);
         };
         auto store_tile = [&](int buf) {
+            R* const dst = s_in(buf);
 #pragma unroll
             for (int m = 0; m < NPIN; m++) {
                 const int c = tid + m * NTHR;
-                if (c < NT) s_in[buf][c] = pf[m];
+                if (c < NT) dst[c] = pf[m];
             }
         };
         // ---- every per-layer load of the set-up in flight at once
@@ -585,8 +598,9 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
         // ---- tracer loop: tile t+1 was loaded during iteration t-1 and is stored to
         // LDS now, tile t+2 is loaded during this iteration (three LDS tile buffers)
         if (ntr > 1) load_tile(1);
+        __syncthreads();  // s_q and s_h (read above) share the pool with the tile buffers
         for (int t = 0; t < ntr; t++) {
-            const R* sin = s_in[t % 3];
+            const R* sin = s_in(t % 3);
             const SRound sr_a = tracer_round(ss_a, tg0 + t), sr_b = tracer_round(ss_b, tg0 + t);
             if (t + 1 < ntr) store_tile((t + 1) % 3);
             if (t + 2 < ntr) load_tile(t + 2);

@@ -187,9 +187,8 @@ class TracerCollection:
     def set_tuning(self, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
                    tracers_per_block_d: int = 16, tile_y: int = 0) -> None:
         """Launch shape of the batched kernels: tracers per block of the three kernel
-        kinds (0 = all) and the height of the horizontal-stage tiles (4, 8, 16; 0 = 8
-        rows for up to 8 tracers, else 16).  Does
-        not change results."""
+        kinds (0 = all) and the height of the horizontal-stage tiles (4, 8, 16; 0 = the
+        default, 8 rows).  Does not change results."""
         if not self.batched:
             return
         _lib.check(_lib.load().transport_set_tuning(self.p, int(tile_y), int(tracers_per_block_h),

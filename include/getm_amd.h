@@ -154,8 +154,8 @@ int transport_step(void* tr, int ntracers, void* tracers, const double* u, const
                    double* h_out, void* stream);
 
 /* No reference counterpart: launch shape of the batched kernels -- the height
- * of the horizontal-stage tiles (4, 8 or 16 rows of 64 columns; 0 = 8 rows for
- * up to 8 tracers, else 16) and, for the
+ * of the horizontal-stage tiles (4, 8 or 16 rows of 64 columns; 0 = the default,
+ * 8 rows) and, for the
  * three kernel kinds (horizontal stage pairs, w stage, FP32 diffusion), the
  * tracers per block (the tracer batches of one step are spread over the grid;
  * 0 = all tracers in every block).  Results do not depend on these values. */
@@ -221,8 +221,8 @@ void ensemble_finalize(void* ens);
 int ensemble_bind(void* ens, int domain, float* tracers, const double* u, const double* v, const double* w,
                   const double* nuh, double timestep, double molecular);
 
-/* Launch shape as transport_set_tuning (tile_y 0: by the mean tracer count per
- * domain); takes effect at the next commit. */
+/* Launch shape as transport_set_tuning (tile_y 0: the default, 8 rows); takes
+ * effect at the next commit. */
 int ensemble_set_tuning(void* ens, int tile_y, int tracers_per_block_h, int tracers_per_block_w,
                         int tracers_per_block_d);
 
