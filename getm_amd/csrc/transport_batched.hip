@@ -659,7 +659,11 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
                 if (cb[m].upd)
                     out = cell_update<R>(cb[m], dt, out, s_fb[cb[m].fl1 - (D2 == 1 ? 1 : FBW)], s_fb[cb[m].fl1],
                                          sr_b);
-                if (ooff[m] >= 0) at32(fo, (unsigned)ooff[m] * (unsigned)sizeof(R)) = out;
+                // FP32: only updated cells are stored (the scratch buffer's other cells are
+                // never read, and the tracers' own keep their values); FP64 stores every
+                // cell, as its step ends with a copy of the whole scratch buffer
+                if (std::is_same<R, float>::value ? cb[m].upd : ooff[m] >= 0)
+                    at32(fo, (unsigned)ooff[m] * (unsigned)sizeof(R)) = out;
             }
             __syncthreads();
         }
@@ -749,8 +753,12 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
     if (!inside || A.tmask[p] != 1) {
         if (first)
             for (int k = 0; k < nz; k++) hout[(size_t)k * n2 + p] = hin[(size_t)k * n2 + p];
-        for (int t = tb0; t < tb1; t++)
-            for (int k = 0; k < nz; k++) fout[(size_t)t * n3 + (size_t)k * n2 + p] = fin[(size_t)t * n3 + (size_t)k * n2 + p];
+        // FP32: the tracers of columns that are not updated keep their values in fout
+        // (no stage writes them); FP64 copies them (its step ends with a full copy)
+        if constexpr (std::is_same<R, double>::value)
+            for (int t = tb0; t < tb1; t++)
+                for (int k = 0; k < nz; k++)
+                    fout[(size_t)t * n3 + (size_t)k * n2 + p] = fin[(size_t)t * n3 + (size_t)k * n2 + p];
         return;
     }
     const double* hi = hin + p;
@@ -988,7 +996,11 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
     if (i >= A.nx_) return;
     const size_t p = (size_t)j * A.nx_ + i;
     const bool inside = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax;
-    const bool solve = inside && A.tmask[p] == 1 && nz > 1;
+    const bool wetcol = inside && A.tmask[p] == 1;
+    const bool solve = wetcol && nz > 1;
+    // columns that are not solved keep their values in fout (no stage writes them),
+    // except that with nz = 1 every point takes the reference's (dz c)/dz
+    if (!solve && nz > 1) return;
     if (solve) {
         // factorisation from the surface down, FP64.  The coefficients of interface
         // k (an, ao) serve level k as its lower and level k - 1 as its upper
@@ -1028,7 +1040,8 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
         float c[NT][NZMAX], q[NT][NZMAX];
 #pragma unroll
         for (int b = 0; b < NT; b++) {
-            const float* fi = fin + (size_t)(t + (b < nt ? b : 0)) * n3 + p;
+            // nz = 1: the column's value is in fin if the stages updated it, else in fout
+            const float* fi = (wetcol ? fin : fout) + (size_t)(t + (b < nt ? b : 0)) * n3 + p;
 #pragma unroll
             for (int k = 0; k < NZMAX; k++)
                 if (has(k)) c[b][k] = fi[(size_t)k * n2];
@@ -1154,6 +1167,7 @@ __device__ __forceinline__ void pcr_body(const StepArgs& A, double dt, double cn
     constexpr int LOG = NP == 8 ? 3 : NP == 16 ? 4 : 5;
     static_assert(NP == 8 || NP == 16 || NP == 32, "wave PCR covers columns of up to 32 levels");
     __shared__ float s_t[NP * LDW];
+    __shared__ unsigned char s_solve[COLS];  // columns that are solved (the others keep their values in fout)
     const int tid = threadIdx.x, lane = tid % 64;
     const int k = tid % NP, cl = tid / NP;  // this lane's level and column in the tile
     const int nz = A.nz;
@@ -1165,6 +1179,7 @@ __device__ __forceinline__ void pcr_body(const StepArgs& A, double dt, double cn
         const int i = (int)(pc % A.nx_), j = (int)(pc / A.nx_);
         solve = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[pc] == 1;
     }
+    if (k == 0) s_solve[cl] = solve;  // visible after the first barrier of the tracer loop
     // row k in FP64, rounded once
     float l = 0.f, m = 1.f, u = 0.f, eu = 0.f, el = 0.f, ou = 0.f, ol = 0.f;
     if (solve && lev) {
@@ -1241,10 +1256,10 @@ __device__ __forceinline__ void pcr_body(const StepArgs& A, double dt, double cn
         __syncthreads();  // every lane has read its value before the tile is overwritten
         if (incol && lev) s_t[k * LDW + cl] = tracer_round(ss, tg0 + t).add(c, r * minv, gc);
         __syncthreads();
-        // rows out, coalesced
+        // rows out, coalesced (solved columns only)
         for (int e = tid; e < nz * COLS; e += 1024) {
             const int kk = e / COLS, cc = e % COLS;
-            if (cc < ncols) fo[(size_t)kk * n2 + cc] = s_t[kk * LDW + cc];
+            if (cc < ncols && s_solve[cc]) fo[(size_t)kk * n2 + cc] = s_t[kk * LDW + cc];
         }
         __syncthreads();
     }
