@@ -130,32 +130,41 @@ __device__ __host__ __forceinline__ unsigned sr_stage_seed(unsigned long long st
     return sr_hash(sr_hash((unsigned)step ^ sr_hash(stage * 0x85ebca6bu + 0x632be5abu)) ^ (unsigned)(step >> 32));
 }
 
-// Seed of one tracer at one rounding site.
-__device__ __forceinline__ unsigned sr_tracer_seed(unsigned stage_seed, int tracer)
+// Random bits of one cell at one rounding site (stage seed != 0).  Every tracer
+// of the cell uses them, each XORed with its own key (sr_tracer_key): for one
+// tracer the bits stay uniform and fresh at every step and stage, and the hash
+// is computed once per cell rather than once per tracer.
+__device__ __forceinline__ unsigned sr_cell_bits(unsigned stage_seed, unsigned cell)
 {
-    return sr_hash(stage_seed ^ ((unsigned)tracer * 0x9e3779b9u));
+    return sr_hash(cell ^ stage_seed);
 }
 
-// f + d in FP32, rounded stochastically with the random bits of cell `cell`.
-// The FP64 sum of two floats is exact unless |d| < 2^-29 |f| (then its own
-// rounding is far below an FP32 ulp); adding 29 uniform bits below the FP32
-// mantissa and truncating rounds away from zero with probability equal to the
-// discarded fraction.  FP32-subnormal results take the conversion's rounding.
-__device__ __forceinline__ float sr_add(float f, float d, unsigned tracer_seed, unsigned cell)
+// Key of a tracer (odd, so never 0).
+__device__ __forceinline__ unsigned sr_tracer_key(int tracer)
+{
+    return sr_hash((unsigned)tracer * 0x9e3779b9u + 0x7f4a7c15u) | 1u;
+}
+
+// f + d in FP32, rounded stochastically with the 32 random bits r.  The FP64 sum
+// of two floats is exact unless |d| < 2^-29 |f| (then its own rounding is far
+// below an FP32 ulp); adding 29 uniform bits below the FP32 mantissa and
+// truncating rounds away from zero with probability equal to the discarded
+// fraction.  FP32-subnormal results take the conversion's rounding.
+__device__ __forceinline__ float sr_add(float f, float d, unsigned r)
 {
     const double x = (double)f + (double)d;
     unsigned long long b = (unsigned long long)__double_as_longlong(x);
-    b += sr_hash(cell ^ tracer_seed) >> 3;
+    b += r >> 3;
     b &= ~((1ull << 29) - 1ull);
     return (float)__longlong_as_double((long long)b);
 }
 
-// Stochastic rounding on (seed != 0 tracer seed) or round to nearest.
+// Stochastic rounding of one tracer (key = its sr_tracer_key) or round to nearest (key 0).
 struct SRound {
-    unsigned seed;  // 0: round to nearest
-    __device__ __forceinline__ float add(float f, float d, unsigned cell) const
+    unsigned key;
+    __device__ __forceinline__ float add(float f, float d, unsigned cell_bits) const
     {
-        return seed ? sr_add(f, d, seed, cell) : f + d;
+        return key ? sr_add(f, d, cell_bits ^ key) : f + d;
     }
 };
 

@@ -212,13 +212,14 @@ __device__ __forceinline__ R ah_flux(const StepArgs& A, int gi, int gj, int k, R
 // would accumulate linearly with the step count; in the increment form it is
 // scaled by |hold/hnew - 1| (a Courant number) and the only O(ulp(f)) error is
 // the final, data-dependent rounding.
-// own = LDS offset of the cell, fl1 = offset of its upper face flux, gc = its
-// index in the domain's 3-D array (the cell's stochastic-rounding stream).
+// own = LDS offset of the cell, fl1 = offset of its upper face flux, rb = the
+// cell's random bits of this stage (stochastic rounding; sr_cell_bits of its
+// index in the domain's 3-D array).
 template <typename R>
 struct CellK {
     R a, b, iA;
     int upd, own, fl1;
-    unsigned gc;
+    unsigned rb;
 };
 
 template <typename R>
@@ -246,7 +247,7 @@ __device__ __forceinline__ R cell_update(const CellK<R>& c, double dt, R f, R fl
         const double advn = (fl1 - fl0) * c.iA;
         return (hfo - dt * advn) / c.b;
     } else {
-        return sr.add(f, c.a * f - c.b * (fl1 - fl0), c.gc);
+        return sr.add(f, c.a * f - c.b * (fl1 - fl0), c.rb);
     }
 }
 
@@ -259,7 +260,7 @@ __device__ __forceinline__ unsigned stage_seed(const StepArgs& A, unsigned stage
 }
 __device__ __forceinline__ SRound tracer_round(unsigned stage_seed, int tracer)
 {
-    return SRound{stage_seed ? (sr_tracer_seed(stage_seed, tracer) | 1u) : 0u};
+    return SRound{stage_seed ? sr_tracer_key(tracer) : 0u};
 }
 
 // Tile-local coordinates: output [0,TX) x [0,TY); input [-2,TX+2) x [-2,TY+2)
@@ -544,7 +545,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
                 const int f1 = (yl - FAY0) * FAW + (xl - FAX0);
                 ca[m].own = own;
                 ca[m].fl1 = f1;
-                ca[m].gc = (unsigned)lay + (unsigned)((y0 + yl) * A.nx_ + x0 + xl);
+                ca[m].rb = sr_cell_bits(ss_a, (unsigned)lay + (unsigned)((y0 + yl) * A.nx_ + x0 + xl));
                 double h1 = 0.0;
                 if (coa[m] >= 0) {
                     const bool upd = GROUP ? (s_cf[own] & 2) : (rcfa[m] & 2);
@@ -583,7 +584,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             cb[m].upd = 0;
             cb[m].own = own;
             cb[m].fl1 = f1;
-            cb[m].gc = (unsigned)lay + (unsigned)ooff[m];
+            cb[m].rb = sr_cell_bits(ss_b, (unsigned)lay + (unsigned)ooff[m]);
             if (ooff[m] >= 0) {
                 const double h1 = s_h[own];
                 const bool upd = GROUP ? (s_cf[own] & 2) : (rcfb[m] & 2);
@@ -785,11 +786,9 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
             const R* sf = (it == 1 ? fin : fout) + (size_t)t0 * n3 + p;
             R* df = fout + (size_t)t0 * n3 + p;
             SRound sr[TB];
-            {
-                const unsigned ss = stage_seed<R>(A, SR_WCOL + it);
+            const unsigned ss = stage_seed<R>(A, SR_WCOL + it);
 #pragma unroll
-                for (int b = 0; b < TB; b++) sr[b] = tracer_round(ss, t0 + b);
-            }
+            for (int b = 0; b < TB; b++) sr[b] = tracer_round(ss, t0 + b);
             const bool write_h = first && t0 == tb0 && it == iters;
             R fm1[TB], f0[TB], f1[TB], fprev[TB];
             double h0 = h_at(0, it - 1), h1 = nz > 1 ? h_at(1, it - 1) : 0.0;
@@ -853,12 +852,12 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
                     // increment form, as in cell_update: f + ((h0 - hnew)/hnew f - dt/hnew dF)
                     const double rh = rcp_nr(hnew);
                     const R ca = R((h0 - hnew) * rh), cb = R(dtk * rh);
-                    const unsigned gc = (unsigned)((size_t)(k - 1) * n2 + p);
+                    const unsigned rb = sr_cell_bits(ss, (unsigned)((size_t)(k - 1) * n2 + p));
 #pragma unroll
                     for (int b = 0; b < TB; b++)
                         if (b < nb)
                             df[(size_t)b * n3 + (size_t)(k - 1) * n2] =
-                                sr[b].add(f0[b], ca * f0[b] - cb * (fl[b] - fprev[b]), gc);
+                                sr[b].add(f0[b], ca * f0[b] - cb * (fl[b] - fprev[b]), rb);
                 }
 #pragma unroll
                 for (int b = 0; b < TB; b++) {
@@ -1082,21 +1081,22 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
             asm volatile("" ::: "memory");
             float d[NT];
             SRound sr[NT];
+            const unsigned rb0 = sr_cell_bits(ss, (unsigned)p);
 #pragma unroll
             for (int b = 0; b < NT; b++) {
                 sr[b] = tracer_round(ss, tg0 + t + b);
                 d[b] = q[b][0];
-                c[b][0] = sr[b].add(c[b][0], d[b], (unsigned)p);
+                c[b][0] = sr[b].add(c[b][0], d[b], rb0);
             }
 #pragma unroll
             for (int k = 1; k < NZMAX; k++) {
                 if (has(k)) {
                     const float r = ru[k * 64 + lane];
-                    const unsigned gc = (unsigned)((size_t)k * n2 + p);
+                    const unsigned rb = sr_cell_bits(ss, (unsigned)((size_t)k * n2 + p));
 #pragma unroll
                     for (int b = 0; b < NT; b++) {
                         d[b] = q[b][k] - r * d[b];
-                        c[b][k] = sr[b].add(c[b][k], d[b], gc);
+                        c[b][k] = sr[b].add(c[b][k], d[b], rb);
                     }
                 }
             }
@@ -1224,7 +1224,7 @@ __device__ __forceinline__ void pcr_body(const StepArgs& A, double dt, double cn
     }
     const float minv = 1.f / m;
     const unsigned ss = stage_seed<float>(A, SR_VDIFF);
-    const unsigned gc = (unsigned)((size_t)k * n2 + pc);
+    const unsigned rb = sr_cell_bits(ss, (unsigned)((size_t)k * n2 + pc));
     const int ncols = (size_t)COLS < n2 - p0 ? COLS : (int)(n2 - p0);
     for (int t = 0; t < ntr; t++) {
         const float* fi = fin + (size_t)t * n3 + p0;
@@ -1254,7 +1254,7 @@ __device__ __forceinline__ void pcr_body(const StepArgs& A, double dt, double cn
             r = r + (hl ? fa[st] * rl : 0.f) + (hu ? fg[st] * rh : 0.f);
         }
         __syncthreads();  // every lane has read its value before the tile is overwritten
-        if (incol && lev) s_t[k * LDW + cl] = tracer_round(ss, tg0 + t).add(c, r * minv, gc);
+        if (incol && lev) s_t[k * LDW + cl] = tracer_round(ss, tg0 + t).add(c, r * minv, rb);
         __syncthreads();
         // rows out, coalesced (solved columns only)
         for (int e = tid; e < nz * COLS; e += 1024) {
