@@ -334,3 +334,81 @@ def test_stochastic_rounding_follows_the_step_counter():
     assert_bitwise(a, own, "own counter starts at zero")
     assert not np.array_equal(a, c)
     assert np.abs(a - c).max() <= 1e-5 * np.abs(a).max()
+
+
+def _interior_wet(d):
+    wet2 = d.tmask == 1
+    inner = np.zeros_like(wet2)
+    h = 2
+    inner[h:-h, h:-h] = True
+    return np.broadcast_to(wet2 & inner, d.ho.shape)
+
+
+@pytest.mark.parametrize("nz,solver", [(1, 0), (12, 0), (12, 16)])
+def test_f32_cells_not_updated_keep_their_values(nz, solver):
+    """The FP32 stages store only the cells they update: land and halo cells keep
+    their values bit for bit (NaN included), and nothing read from them (the
+    scratch buffer's copies are never read) reaches a wet cell: wet results are
+    the same whether the land holds numbers or NaN.  With nz = 1 the diffusion is
+    the reference's (dz c)/dz over the whole array (diffusion.F90:224-231), which
+    every step applies to the cells that are not updated as well (0/0 = NaN on
+    dry land)."""
+    d = inputs.coastal(nx=80, ny=64, nz=nz, seed=21, ntracers=2, tracer_kind="TS")
+    upd = _interior_wet(d)
+    outs = {}
+    for fill in ("finite", "nan"):
+        dd = d.copy()
+        tr = dd.tracers.astype(np.float32).astype(np.float64)
+        if fill == "nan":
+            tr = np.where(upd[None], tr, np.nan)
+        dd.tracers = tr
+        got, _, _ = run_batched(dd, "f32", nsteps=4, solver=solver)
+        outs[fill] = (got, tr)
+    for fill, (got, tr) in outs.items():
+        # cells that are not updated keep their initial (FP32) values, NaN positions included
+        want = tr.astype(np.float32)
+        if nz == 1:
+            dz = d.hn.astype(np.float64)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                for _ in range(4):
+                    want = ((dz[None] * want.astype(np.float64)) / dz[None]).astype(np.float32)
+        assert_bitwise(got[:, ~upd].astype(np.float32), want[:, ~upd], f"{fill}: not updated")
+        assert np.isfinite(got[:, upd]).all(), fill
+    assert_bitwise(outs["finite"][0][:, upd], outs["nan"][0][:, upd], "wet cells with finite vs NaN land")
+
+
+def test_ensemble_cells_not_updated_keep_their_values():
+    """As above for an ensemble of two domains: NaN land stays NaN, wet cells are finite
+    and equal to the run with finite land."""
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.ensemble import Ensemble
+
+    doms = [inputs.coastal(nx=70, ny=50, nz=10, seed=31, ntracers=2, tracer_kind="TS"),
+            inputs.coastal(nx=90, ny=40, nz=14, seed=32, ntracers=3, tracer_kind="TS")]
+    res = {}
+    for fill in ("finite", "nan"):
+        grids, keep, tracers = [], [], []
+        for d in doms:
+            g = make_grid(d)
+            upd = _interior_wet(d)
+            tr = d.tracers.astype(np.float32)
+            if fill == "nan":
+                tr = np.where(upd[None], tr, np.float32(np.nan))
+            grids.append(g)
+            tracers.append(dev(np.ascontiguousarray(tr)))
+            keep.append((d.dt, g.ugrid.wrap(dev(d.u), CENTERS), g.vgrid.wrap(dev(d.v), CENTERS),
+                         g.wrap(dev(d.w), INTERFACES), g.wrap(dev(d.nuh), INTERFACES)))
+        ens = Ensemble(grids, [d.ntracers for d in doms], scheme=5)
+        for i, (dt, u, v, w, nuh) in enumerate(keep):
+            ens.bind(i, tracers[i], u, v, w, nuh, dt)
+        ens.commit()
+        for _ in range(4):
+            ens.step()
+        res[fill] = [host(t) for t in tracers]
+    for i, d in enumerate(doms):
+        upd = _interior_wet(d)
+        init = d.tracers.astype(np.float32)
+        assert_bitwise(res["finite"][i][:, ~upd], init[:, ~upd], f"domain {i}: not updated")
+        assert np.isnan(res["nan"][i][:, ~upd]).all(), i
+        assert np.isfinite(res["nan"][i][:, upd]).all(), i
+        assert_bitwise(res["finite"][i][:, upd], res["nan"][i][:, upd], f"domain {i}: wet cells")
