@@ -85,20 +85,22 @@ def accuracy_check(d, grid, fields, init, scheme, steps, rounding, log):
     return out
 
 
-def serving_run(nreq: int, log):
+def serving_run(nreq: int, log, policy: str = "uniform"):
     """Serving loop on a mix of dye-release requests (sizes 64-384, 10-60 layers,
-    1-64 tracers, 50-200 steps, SUPERBEE or HSIMT): saturation, FP64 check, 80% load."""
+    1-64 tracers, 50-200 steps, SUPERBEE or HSIMT): saturation, FP64 check, 80% load.
+    Policy "uniform" (every request completes in 8..16 ticks whatever its size)."""
     from getm_amd import serve
 
     reqs = serve.draw_requests(nreq, seed=0, size_range=(64, 384), nz_range=(10, 60), tracer_range=(1, 64),
                                step_range=(50, 200))
     domains = {r.rid: r.domain() for r in reqs}
-    server = serve.Server(hbm_budget=250e9, max_slots=64)
+    server = serve.Server(hbm_budget=250e9, max_slots=64, policy=policy)
     rep = serve.sweep(reqs, domains, [0.8], server, verify_results=True, log=log)
     sat, run = rep["saturation"], rep["sweep"][0]
     return {
         "requests": f"{nreq} dye releases: nx, ny logU[64,384], nz U[10,60], 1-64 tracers, 50-200 steps, "
                     "SUPERBEE/HSIMT, FP32 with FP64 h and per-request inventory fixer",
+        "policy": policy,
         "saturation_updates_per_s": sat["tracer_cell_updates_per_s"],
         "load": run["fraction_of_saturation"],
         "arrival_rate_per_s": run["arrival_rate_per_s"],

@@ -118,16 +118,21 @@ def draw_requests(n: int, seed: int = 0, size_range=(32, 192), nz_range=(10, 60)
 class _Slot:
     _serial = 0
 
-    def __init__(self, req: Request, dom: inputs.Domain, device, conserve: bool, quantum: Optional[float]):
+    def __init__(self, req: Request, dom: inputs.Domain, device, conserve: bool, quantum: Optional[float],
+                 horizon: Optional[int] = None):
         _Slot._serial += 1
         self.serial = _Slot._serial
         self.req = req
-        # steps per loop tick: 1 (lockstep), or with a work quantum Q the power of two
-        # nearest Q / (updates per step), 1 .. 64 -- a property of the request alone
+        # steps per loop tick, a property of the request alone: 1 (lockstep); with a
+        # work quantum Q the power of two nearest Q / (updates per step), 1 .. 64
+        # (fair); with a horizon H the power of two >= steps / H, so that every
+        # request completes in H/2 .. H ticks whatever its size (uniform)
         self.cost = dom.n_wet_columns * dom.nz * req.ntracers
         self.k = 1
         if quantum:
             self.k = int(min(64, max(1, 2 ** round(math.log2(max(quantum / max(self.cost, 1), 1e-9))))))
+        if horizon:
+            self.k = int(2 ** math.ceil(math.log2(max(1, math.ceil(req.nsteps / horizon)))))
         self.grid = Grid.from_domain(dom, device=device)
         g = self.grid
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
@@ -171,20 +176,27 @@ class _Slot:
 
 
 class Server:
-    """policy "fair": every tick each request advances about ``quantum`` tracer
-    cell-updates (k = 1 .. 64 steps, by its size), so a small request is not held
-    to the pace of the largest one in the mix; its inventory is restored once per
-    tick.  policy "lockstep": every request advances one step per tick and is
-    fixed after every step.  Either way a request's result depends only on the
-    request itself, never on the mix."""
+    """policy "lockstep": every request advances one step per tick and is fixed
+    after every step.  policy "fair": every tick each request advances about
+    ``quantum`` tracer cell-updates (k = 1 .. 64 steps, by its size), so a small
+    request is not held to the pace of the largest one in the mix.  policy
+    "uniform": every request advances k >= steps / ``horizon`` steps per tick (a
+    power of two), so each completes in horizon/2 .. horizon ticks whatever its
+    size: latency no longer follows request size (the tail narrows), at the cost
+    of small requests' latency.  Under "fair" and "uniform" a request's inventory
+    is restored once per tick.  Under every policy a request's result depends only
+    on the request itself, never on the mix."""
 
     def __init__(self, hbm_budget: float = 250e9, max_slots: int = 64, conserve: bool = True,
                  keep_results: bool = False, tuning: Optional[Sequence[int]] = None, device=None,
-                 policy: str = "lockstep", quantum: float = 2.0 ** 25):
-        if policy not in ("fair", "lockstep"):
-            raise ValueError("policy must be 'fair' or 'lockstep'")
+                 policy: str = "lockstep", quantum: float = 2.0 ** 25, horizon: int = 16):
+        if policy not in ("fair", "lockstep", "uniform"):
+            raise ValueError("policy must be 'fair', 'lockstep' or 'uniform'")
+        if policy == "uniform" and horizon < 1:
+            raise ValueError("horizon must be at least one tick")
         self.policy = policy
         self.quantum = float(quantum) if policy == "fair" else None
+        self.horizon = int(horizon) if policy == "uniform" else None
         self.hbm_budget = float(hbm_budget)
         self.max_slots = int(max_slots)
         self.conserve = conserve
@@ -248,7 +260,7 @@ class Server:
                 if used + need > self.hbm_budget:
                     break
                 pending.pop(0)
-                active.append(_Slot(req, domains[req.rid], self.device, self.conserve, self.quantum))
+                active.append(_Slot(req, domains[req.rid], self.device, self.conserve, self.quantum, self.horizon))
                 used += need
                 admitted_at[req.rid] = now()
             self.peak_active = max(self.peak_active, len(active))
@@ -423,7 +435,8 @@ def main(argv=None):
     ap.add_argument("--steps", default="50,2000")
     ap.add_argument("--no-verify", action="store_true", help="skip the FP64 check of the saturation run")
     ap.add_argument("--no-conserve", action="store_true", help="no inventory fixer (raw FP32 steps)")
-    ap.add_argument("--policy", choices=["fair", "lockstep"], default="lockstep")
+    ap.add_argument("--policy", choices=["fair", "lockstep", "uniform"], default="lockstep")
+    ap.add_argument("--horizon", type=int, default=16, help="ticks per request under the uniform policy")
     ap.add_argument("--quantum", type=float, default=2.0 ** 25,
                     help="tracer cell-updates per request per tick under the fair policy")
     ap.add_argument("--seed", type=int, default=0)
@@ -437,11 +450,12 @@ def main(argv=None):
     log(f"[serve] generating {len(reqs)} scenarios")
     domains = {r.rid: r.domain() for r in reqs}
     server = Server(hbm_budget=args.budget_gb * 1e9, max_slots=args.max_slots, conserve=not args.no_conserve,
-                    policy=args.policy, quantum=args.quantum)
+                    policy=args.policy, quantum=args.quantum, horizon=args.horizon)
     report = sweep(reqs, domains, [float(x) for x in args.fractions.split(",") if x], server,
                    verify_results=not args.no_verify, seed=args.seed, log=log)
     report["requests"].update(size=args.size, layers=args.layers, tracers=args.tracers, steps=args.steps,
-                              conserve=not args.no_conserve, policy=args.policy, quantum=args.quantum)
+                              conserve=not args.no_conserve, policy=args.policy, quantum=args.quantum,
+                              horizon=args.horizon if args.policy == "uniform" else None)
     print(json.dumps(report))
     if args.out:
         with open(args.out, "w") as fh:

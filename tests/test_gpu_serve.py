@@ -125,3 +125,19 @@ def test_fair_policy_mix_equals_alone_bitwise(domains):
         assert res.inventory_drift <= 1e-12
     # at least one small request advanced more than one step per tick
     assert any(r.finished - r.admitted < r.steps for r in mix.values())
+
+
+def test_uniform_policy_mix_equals_alone_bitwise(domains):
+    """Uniform completion time: every request takes a power of two >= steps/horizon
+    steps per tick, so each completes within horizon/2 .. horizon ticks of its
+    admission whatever its step count; results still do not depend on the mix."""
+    horizon = 4
+    kw = dict(policy="uniform", horizon=horizon)
+    solo = run_alone(domains, **kw)
+    server, mix = run_mix(domains, **kw)
+    assert server.peak_active >= 2
+    for rid, res in mix.items():
+        assert not res.error
+        assert torch.equal(res.tracers, solo[rid].tracers), f"request {rid} differs in the mix"
+        assert res.inventory_drift <= 1e-12
+        assert res.finished - res.admitted <= horizon, (rid, res.admitted, res.finished)
