@@ -40,8 +40,10 @@ extern "C" {
 void* create_grid(int imin, int imax, int jmin, int jmax, int kmin, int kmax, int halox, int haloy)
 {
     if (imax < imin || jmax < jmin || kmax < kmin || halox < 0 || haloy < 0) return nullptr;
-    // in-layer byte offsets of FP64 fields are 32-bit in the kernels
-    if ((long long)(imax - imin + 1 + 2 * halox) * (jmax - jmin + 1 + 2 * haloy) * 8 >= (1LL << 32)) return nullptr;
+    // in-layer byte offsets of FP64 fields are 32-bit in the kernels, and row and
+    // column indices take 24-bit multiplies
+    const long long nx_ = imax - imin + 1 + 2LL * halox, ny_ = jmax - jmin + 1 + 2LL * haloy;
+    if (nx_ * ny_ * 8 >= (1LL << 32) || nx_ >= (1LL << 24) || ny_ >= (1LL << 24)) return nullptr;
     Grid* g = new (std::nothrow) Grid();
     if (!g) return nullptr;
     g->imin = imin;

@@ -46,6 +46,10 @@ namespace getm {
 // read by an updated cell and are left closed.
 // ---------------------------------------------------------------------------
 constexpr int TX = 64;      // output tile width (one wave); the height TY is a template parameter
+
+// Row offset j * nx_ (and other products of two array indices): both are below
+// 2^24 (create_grid), so the full-rate 24-bit multiply gives the exact product.
+__device__ __forceinline__ int mul24(int a, int b) { return (int)__umul24((unsigned)a, (unsigned)b); }
 constexpr int HX = TX + 4;  // input tile width incl. halo 2
 
 struct FaceC {
@@ -339,7 +343,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
         mq = 0.0;
         mc = 0.0;
         if (!face_in_loop<DIR>(A, gi, gj)) return -1;
-        const unsigned p = (unsigned)(gj * A.nx_ + gi);
+        const unsigned p = (unsigned)(mul24(gj, A.nx_) + gi);
         const unsigned s = DIR == 1 ? 1u : (unsigned)A.nx_;  // p - s >= 0 inside the face loops (halo >= 2)
         const int* mask = DIR == 1 ? A.umask : A.vmask;
         // every load issued before any is used (no load waits for the mask)
@@ -354,7 +358,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
         cf = 0;
         ia = 0.0;
         if (!inarr(gi, gj)) return -1;
-        const unsigned p = (unsigned)(gj * A.nx_ + gi);
+        const unsigned p = (unsigned)(mul24(gj, A.nx_) + gi);
         cf = 1 | (wet(gi, gj, at32(A.tmask, 4 * p)) ? 2 : 0);
         ia = at32(A.iarea, 8 * p);
         return (int)p;
@@ -417,14 +421,14 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             toff[m] = -1;
             if (c < NT) {
                 const int gi = x0 - 2 + c % HX, gj = y0 - 2 + c / HX;
-                if (inarr(gi, gj)) toff[m] = gj * A.nx_ + gi;
+                if (inarr(gi, gj)) toff[m] = mul24(gj, A.nx_) + gi;
             }
         }
 #pragma unroll
         for (int m = 0; m < NPB; m++) {
             const int c = tid + m * NTHR;
             const int gi = x0 + c % TX, gj = y0 + c / TX;
-            ooff[m] = (gi < A.nx_ && gj < A.ny_) ? gj * A.nx_ + gi : -1;
+            ooff[m] = (gi < A.nx_ && gj < A.ny_) ? mul24(gj, A.nx_) + gi : -1;
         }
         // one layer: the layer-independent terms of this lane's faces and cells in registers
         unsigned char rffa[NPFA], rffb[NPFB], rcfa[NPA], rcfb[NPB];
@@ -433,7 +437,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
         for (int m = 0; m < NPFA; m++) {
             const int c = tid + m * NTHR;
             if constexpr (GROUP) {
-                foa[m] = (c < NFA && (s_ffa[c] & 1)) ? (y0 + FAY0 + c / FAW) * A.nx_ + x0 + FAX0 + c % FAW : -1;
+                foa[m] = (c < NFA && (s_ffa[c] & 1)) ? mul24(y0 + FAY0 + c / FAW, A.nx_) + x0 + FAX0 + c % FAW : -1;
             } else {
                 rffa[m] = 0;
                 rmqa[m] = rmca[m] = 0.0;
@@ -446,7 +450,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
         for (int m = 0; m < NPFB; m++) {
             const int c = tid + m * NTHR;
             if constexpr (GROUP) {
-                fob[m] = (c < NFB && (s_ffb[c] & 1)) ? (y0 + FBY0 + c / FBW) * A.nx_ + x0 + FBX0 + c % FBW : -1;
+                fob[m] = (c < NFB && (s_ffb[c] & 1)) ? mul24(y0 + FBY0 + c / FBW, A.nx_) + x0 + FBX0 + c % FBW : -1;
             } else {
                 rffb[m] = 0;
                 rmqb[m] = rmcb[m] = 0.0;
@@ -464,7 +468,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             if (c < NA) {
                 const int xl = AX0 + c % AW, yl = AY0 + c / AW;
                 if constexpr (GROUP) {
-                    if (s_cf[(yl + 2) * HX + (xl + 2)] & 1) coa[m] = (y0 + yl) * A.nx_ + x0 + xl;
+                    if (s_cf[(yl + 2) * HX + (xl + 2)] & 1) coa[m] = mul24(y0 + yl, A.nx_) + x0 + xl;
                 } else {
                     coa[m] = cell2d(x0 + xl, y0 + yl, rcfa[m], ria_a[m]);
                 }
@@ -545,7 +549,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
                 const int f1 = (yl - FAY0) * FAW + (xl - FAX0);
                 ca[m].own = own;
                 ca[m].fl1 = f1;
-                ca[m].rb = sr_cell_bits(ss_a, (unsigned)lay + (unsigned)((y0 + yl) * A.nx_ + x0 + xl));
+                ca[m].rb = sr_cell_bits(ss_a, (unsigned)lay + (unsigned)(mul24(y0 + yl, A.nx_) + x0 + xl));
                 double h1 = 0.0;
                 if (coa[m] >= 0) {
                     const bool upd = GROUP ? (s_cf[own] & 2) : (rcfa[m] & 2);
