@@ -257,3 +257,24 @@ class VerticalDiffusion:
                  ea2: Optional[Array] = None, ea4: Optional[Array] = None, use_ho: bool = False):
         self.prepare(nuh, timestep, molecular, use_ho)
         self.apply(var, ea2, ea4, use_ho)
+
+
+def vertical_advection_to_sources(halox: int, haloy: int, mask: torch.Tensor, c: torch.Tensor, w: torch.Tensor,
+                                  h: torch.Tensor, s: torch.Tensor) -> None:
+    """pygetm._pygetm.vertical_advection_to_sources (python/_pygetm.pyx:574-575,
+    python/src/wrapper.F90:447-486): first-order upstream transfer of the vertical
+    velocity ``w`` (sinking < 0, floating > 0) of concentration ``c`` into the
+    sources ``s`` of the layers it moves material between, in place.  All arrays
+    are contiguous (nz, ny_, nx_) device tensors with layer 0 at the surface (the
+    order pygetm's FABM coupling uses), ``mask`` int32, the others float64."""
+    shape = tuple(c.shape)
+    if len(shape) != 3:
+        raise ValueError("arrays must be (nz, ny_, nx_)")
+    for name, a, dt in (("mask", mask, torch.int32), ("c", c, torch.float64), ("w", w, torch.float64),
+                        ("h", h, torch.float64), ("s", s, torch.float64)):
+        if tuple(a.shape) != shape or a.dtype != dt or not a.is_contiguous() or not a.is_cuda:
+            raise ValueError(f"{name}: expected a contiguous {dt} device tensor of shape {shape}")
+    rc = _lib.load().c_vertical_advection_to_sources(shape[2], shape[1], shape[0], int(halox), int(haloy),
+                                                     _lib.ptr(mask), _lib.ptr(c), _lib.ptr(w), _lib.ptr(h),
+                                                     _lib.ptr(s), _stream())
+    _lib.check(rc, "c_vertical_advection_to_sources")

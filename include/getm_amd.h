@@ -274,6 +274,19 @@ int c_w_momentum_3d(int nx, int ny, int nz, int imin, int imax, int jmin, int jm
                     const double* dyu, const double* dxv, const double* iarea, const double* ho, const double* hn,
                     const double* pk, const double* qk, double timestep, double* w, void* stream);
 
+/* Replaces c_vertical_advection_to_sources (python/src/wrapper.F90:447-486,
+ * declared at python/_pygetm.pyx:65, called by python/pygetm/fabm.py:368):
+ * first-order upstream transfer of a vertical velocity w (sinking < 0, floating
+ * > 0) of concentration c into the sources s of the layers it moves material
+ * between, in place.  Arrays are the Fortran (nx, ny, nz) -- C (nz, ny, nx), k = 0
+ * at the surface -- with nx, ny the extents including the halos halox, haloy, as
+ * the reference passes them.  If w is 0 at every interior point, s is not
+ * touched.  Bit-identical to the reference.  The activity test uses a flag that
+ * the library holds per device: calls on one device must be ordered (one stream,
+ * or synchronised). */
+int c_vertical_advection_to_sources(int nx, int ny, int nz, int halox, int haloy, const int* mask, const double* c,
+                                    const double* w, const double* h, double* s, void* stream);
+
 /* No reference counterpart (the reference accumulates cpu_time around its
  * loops, diffusion.F90:161-311).  When enabled, every kernel launch of this
  * library is bracketed by HIP events recorded on the launch stream;

@@ -169,6 +169,17 @@ def w_momentum_3d(mask, dyu, dxv, iarea, ho, hn, pk, qk, dt, w, halo=2, extra=1)
     _check(rc, "w_momentum_3d")
 
 
+def vertical_advection_to_sources(halox, haloy, mask, c, w, h, s):
+    """In-place s (nz, ny_, nx_): upstream sinking/floating into sources
+    (wrapper.F90:447-486), with the reference's argument order (_pygetm.pyx:574-575)."""
+    nz, ny_, nx_ = c.shape
+    for a in (mask, c, w, h, s):
+        assert a.shape == (nz, ny_, nx_) and a.flags["C_CONTIGUOUS"]
+    rc = lib().oracle_vertical_advection_to_sources(nx_, ny_, nz, int(halox), int(haloy), _i(mask), _d(c), _d(w),
+                                                    _d(h), _d(s))
+    _check(rc, "vertical_advection_to_sources")
+
+
 def transport_step(d, tracers: np.ndarray, scheme=None, dt=None, cnpar=None, molecular=0.0,
                    ah_u=None, ah_v=None, return_h=False):
     """One adv+diff step on stacked FP64 tracers (T, nz, ny_, nx_), in place.

@@ -444,3 +444,36 @@ int oracle_w_momentum_3d(int nx, int ny, int nz, int imin, int imax, int jmin, i
     }
     return 0;
 }
+
+/* python/src/wrapper.F90:447-486: the activity test over the interior of every
+ * layer (:461-467), then the k loop (outermost in the reference; the order of
+ * the (i, j) loops does not matter, each column's updates only touch itself). */
+int oracle_vertical_advection_to_sources(int nx, int ny, int nz, int halox, int haloy, const int* mask,
+                                         const double* c, const double* w, const double* h, double* s)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || halox < 0 || haloy < 0) return 1;
+    const size_t n2 = (size_t)nx * ny;
+    int active = 0;
+    for (int k = 0; k < nz && !active; k++)
+        for (int j = haloy; j < ny - haloy && !active; j++)
+            for (int i = halox; i < nx - halox; i++)
+                if (w[(size_t)k * n2 + (size_t)j * nx + i] != 0.0) {
+                    active = 1;
+                    break;
+                }
+    if (!active) return 0;
+    const double upward = -1.0;
+    for (int k = 0; k < nz - 1; k++) {
+        for (int j = haloy; j < ny - haloy; j++) {
+            for (int i = halox; i < nx - halox; i++) {
+                const size_t p0 = (size_t)k * n2 + (size_t)j * nx + i, p1 = p0 + n2;
+                if (mask[p0] != 1 || mask[p1] != 1) continue;
+                const double lw = upward * 0.5 * (w[p0] + w[p1]);
+                const double flux = lw > 0.0 ? lw * c[p0] : lw * c[p1];
+                s[p0] = s[p0] - flux / h[p0];
+                s[p1] = s[p1] + flux / h[p1];
+            }
+        }
+    }
+    return 0;
+}

@@ -72,6 +72,18 @@ int oracle_w_momentum_3d(int nx, int ny, int nz, int imin, int imax, int jmin, i
                          const double* dyu, const double* dxv, const double* iarea, const double* ho,
                          const double* hn, const double* pk, const double* qk, double dt, double* w);
 
+/* First-order upstream transfer of a vertical velocity (sinking/floating) into
+ * source terms, python/src/wrapper.F90:447-486 (c_vertical_advection_to_sources).
+ * Arrays are (nz, ny, nx) in C order, i.e. the Fortran (nx, ny, nz), with k = 0 at
+ * the surface (the reference's surface-to-bottom ordering, upward = -1).  If w is
+ * 0 at every interior point (halos excluded) of every layer, s is left untouched.
+ * Else, for every interior (i, j) and k = 0 .. nz-2 with mask == 1 at k and k+1,
+ * in increasing k:
+ *   lw = (-1 * 0.5) (w_k + w_{k+1});  flux = lw (lw > 0 ? c_k : c_{k+1});
+ *   s_k -= flux / h_k;  s_{k+1} += flux / h_{k+1}. */
+int oracle_vertical_advection_to_sources(int nx, int ny, int nz, int halox, int haloy, const int* mask,
+                                         const double* c, const double* w, const double* h, double* s);
+
 /* Limiter value for a given scheme, cfl and slope ratio (limiters.h:1-44);
  * exported for the closed-form tests. */
 double oracle_limiter(int scheme, double cfl, double ratio);

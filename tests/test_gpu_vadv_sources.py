@@ -1,0 +1,66 @@
+"""vertical_advection_to_sources on the GPU (wrapper.F90:447-486) against the oracle, bit for bit."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(seed, nx, ny, nz, halo=2, nan_behind_mask=True):
+    rng = np.random.default_rng(seed)
+    shape = (nz, ny + 2 * halo, nx + 2 * halo)
+    mask = (rng.random(shape) > 0.2).astype(np.int32)
+    c = rng.uniform(0.0, 5.0, shape)
+    if nan_behind_mask:
+        c[mask == 0] = np.nan  # never read: a masked layer stops the transfer
+    w = rng.uniform(-2e-3, 2e-3, shape)
+    w[rng.random(shape) < 0.1] = 0.0
+    h = rng.uniform(0.2, 4.0, shape)
+    s = rng.uniform(-1e-3, 1e-3, shape)
+    return halo, mask, c, w, h, s
+
+
+def _gpu(halo, mask, c, w, h, s):
+    from getm_amd.operators import vertical_advection_to_sources
+
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    sd = d(s)
+    vertical_advection_to_sources(halo, halo, d(mask), d(c), d(w), d(h), sd)
+    torch.cuda.synchronize()
+    return sd.cpu().numpy()
+
+
+@pytest.mark.parametrize("seed,nx,ny,nz", [(1, 37, 29, 12), (2, 64, 8, 40), (3, 5, 70, 2), (4, 20, 20, 1)])
+def test_vadv_sources_bitwise(oracle_mod, seed, nx, ny, nz):
+    halo, mask, c, w, h, s = _case(seed, nx, ny, nz)
+    want = s.copy()
+    oracle_mod.vertical_advection_to_sources(halo, halo, mask, c, w, h, want)
+    got = _gpu(halo, mask, c, w, h, s)
+    assert_bitwise(got, want, f"seed {seed}")
+    if nz > 1:
+        assert not np.array_equal(got, s), "the case must move something"
+
+
+def test_vadv_sources_inactive_is_a_no_op(oracle_mod):
+    """w = 0 in the interior (nonzero only in the halo): s comes back bit for bit,
+    including -0.0 and cells next to NaN concentrations."""
+    halo, mask, c, w, h, s = _case(5, 30, 24, 10, nan_behind_mask=False)
+    inner = np.zeros(s.shape[1:], bool)
+    inner[halo:-halo, halo:-halo] = True
+    w[:, inner] = 0.0
+    c[3, 10, 10] = np.nan
+    s[4, 10, 10] = -0.0
+    got = _gpu(halo, mask, c, w, h, s)
+    assert got.tobytes() == s.tobytes()
+
+
+def test_vadv_sources_flag_is_cleared_between_calls(oracle_mod):
+    """An active call, then an inactive one on the same device: the second leaves s as it was."""
+    halo, mask, c, w, h, s = _case(6, 25, 19, 8)
+    want = s.copy()
+    oracle_mod.vertical_advection_to_sources(halo, halo, mask, c, w, h, want)
+    assert_bitwise(_gpu(halo, mask, c, w, h, s), want, "active call")
+    w0 = np.zeros_like(w)
+    assert _gpu(halo, mask, c, w0, h, s).tobytes() == s.tobytes()

@@ -48,3 +48,11 @@ def test_w_momentum_rejects_bad_range_without_gpu():
     # imin = 1 (Fortran) would read column -1; checked before anything is launched
     assert lib.c_w_momentum_3d(10, 8, 3, 1, 9, 2, 7, None, None, None, None, None, None, None, None, 1.0, None,
                                None) == 1
+
+
+def test_vertical_advection_to_sources_rejects_bad_arguments_without_gpu():
+    lib = _lib.load()
+    # checked before any device call
+    assert lib.c_vertical_advection_to_sources(0, 8, 3, 2, 2, None, None, None, None, None, None) == 1
+    assert lib.c_vertical_advection_to_sources(10, 8, 3, -1, 2, None, None, None, None, None, None) == 1
+    assert lib.c_vertical_advection_to_sources(10, 8, 3, 2, 2, None, None, None, None, None, None) == 1

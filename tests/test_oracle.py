@@ -340,3 +340,81 @@ def test_w_momentum_column_closes_at_surface(oracle_mod, seed):
     scale = np.abs(w[1:nz, h:-h, h:-h][:, wet]).max()
     assert scale > 0
     assert np.abs(surf).max() <= 1e-12 * max(scale, np.abs((d.hn - d.ho) / d.dt).max())
+
+
+# ---- vertical_advection_to_sources (wrapper.F90:447-486), pinned by physics ----
+
+def _vadv_case(seed, nx=9, ny=7, nz=6, halo=2):
+    rng = np.random.default_rng(seed)
+    shape = (nz, ny + 2 * halo, nx + 2 * halo)
+    mask = (rng.random(shape) > 0.15).astype(np.int32)
+    c = rng.uniform(0.0, 5.0, shape)
+    w = rng.uniform(-1e-3, 1e-3, shape)
+    h = rng.uniform(0.5, 3.0, shape)
+    s = rng.uniform(-1e-3, 1e-3, shape)
+    return halo, mask, c, w, h, s
+
+
+def test_vadv_sources_conserves_each_column(oracle_mod):
+    """Whatever leaves one layer enters its neighbour: sum_k h_k (s_k - s0_k) = 0 per column."""
+    halo, mask, c, w, h, s = _vadv_case(1)
+    s0 = s.copy()
+    oracle_mod.vertical_advection_to_sources(halo, halo, mask, c, w, h, s)
+    mass = ((s - s0) * h).sum(axis=0)
+    scale = (np.abs(s - s0) * h).sum(axis=0).max()
+    assert scale > 0
+    assert np.abs(mass).max() <= 1e-14 * scale
+
+
+@pytest.mark.parametrize("sign", [-1.0, 1.0])
+def test_vadv_sources_moves_material_upstream(oracle_mod, sign):
+    """A tracer in one layer of one column: sinking (w < 0, k = 0 at the surface)
+    moves it to the layer below (k + 1), floating (w > 0) to the one above, at the
+    rate |w| c / h, and nothing else changes."""
+    halo, nz = 2, 6
+    shape = (nz, 5 + 2 * halo, 4 + 2 * halo)
+    mask = np.ones(shape, np.int32)
+    c = np.zeros(shape)
+    w = np.full(shape, sign * 2e-3)
+    h = np.full(shape, 1.5)
+    s = np.zeros(shape)
+    k0, j0, i0 = 3, 4, 3
+    c[k0, j0, i0] = 7.0
+    oracle_mod.vertical_advection_to_sources(halo, halo, mask, c, w, h, s)
+    rate = 2e-3 * 7.0 / 1.5
+    dst = k0 + 1 if sign < 0 else k0 - 1
+    want = np.zeros(shape)
+    want[k0, j0, i0] = -rate
+    want[dst, j0, i0] = rate
+    np.testing.assert_allclose(s, want, rtol=1e-15, atol=0)
+
+
+def test_vadv_sources_respects_mask_and_halo(oracle_mod):
+    """No transfer between layers of which one is masked, and none in halo columns."""
+    halo, mask, c, w, h, s = _vadv_case(2)
+    mask[:, :, :] = 1
+    mask[2, 5, 5] = 0  # cuts column (5, 5) between layers 1-2 and 2-3
+    s0 = s.copy()
+    oracle_mod.vertical_advection_to_sources(halo, halo, mask, c, w, h, s)
+    assert np.array_equal(s[2, 5, 5], s0[2, 5, 5])
+    inner = np.zeros(s.shape[1:], bool)
+    inner[halo:-halo, halo:-halo] = True
+    assert np.array_equal(s[:, ~inner], s0[:, ~inner])
+    # column (5, 5) still conserves above and below the cut, separately
+    for ks in (slice(0, 2), slice(3, None)):
+        d = ((s - s0) * h)[ks, 5, 5].sum()
+        assert abs(d) <= 1e-15 * np.abs((s - s0) * h)[ks, 5, 5].sum() + 1e-300
+
+
+def test_vadv_sources_inactive_leaves_sources_untouched(oracle_mod):
+    """w = 0 at every interior point (nonzero only in the halo): s is not touched at
+    all (no 0 * NaN, no -0 - 0 = +0), as the reference's activity test returns early."""
+    halo, mask, c, w, h, s = _vadv_case(3)
+    inner = np.zeros(s.shape[1:], bool)
+    inner[halo:-halo, halo:-halo] = True
+    w[:, inner] = 0.0
+    c[1, 4, 4] = np.nan
+    s[2, 4, 4] = -0.0
+    s0 = s.copy()
+    oracle_mod.vertical_advection_to_sources(halo, halo, mask, c, w, h, s)
+    assert s.tobytes() == s0.tobytes()
