@@ -12,8 +12,10 @@ update kept in FP64.
 from __future__ import annotations
 
 import ctypes
+import json
 from typing import List, Optional
 
+import numpy as np
 import torch
 
 from . import _lib, operators
@@ -53,6 +55,12 @@ class TracerCollection:
                                           int(ufirst))
             if not self.p:
                 raise ValueError(f"transport_create failed (scheme {advection_scheme!r})")
+            # the collection's own step counter of the stochastic rounding, visible to
+            # state_dict (a restart continues the same stream of random bits)
+            self._own_counter = torch.zeros(1, dtype=torch.int64, device=grid.device)
+            if precision == "f32":
+                _lib.check(lib.transport_set_rounding(self.p, _ROUNDING["stochastic"], _lib.ptr(self._own_counter)),
+                           "transport_set_rounding")
         else:
             self.p = None
             self._advection = operators.Advection(grid, scheme=advection_scheme)
@@ -212,15 +220,56 @@ class TracerCollection:
         count, stage, tracer, cell) -- deterministic, independent of launch shape
         and batching.  ``counter`` (a one-element int64 device tensor) holds the
         step count and is advanced by every step; None uses the collection's own
-        counter, zero at creation.  No effect on FP64."""
+        counter (zero at creation, saved by state_dict).  No effect on FP64."""
         if not self.batched:
             return
         if counter is not None and (counter.dtype != torch.int64 or counter.numel() != 1 or not counter.is_cuda):
             raise ValueError("the step counter is a one-element int64 device tensor")
-        rc = _lib.load().transport_set_rounding(self.p, _ROUNDING[mode], _lib.ptr(counter) if counter is not None else None)
+        ctr = counter if counter is not None else self._own_counter
+        rc = _lib.load().transport_set_rounding(self.p, _ROUNDING[mode], _lib.ptr(ctr))
         _lib.check(rc, "transport_set_rounding")
         self._counter = counter
         self.rounding = mode
+
+    # ---- restart: snapshot and restore of the state the step depends on ----
+    def state_dict(self) -> dict:
+        """Host copy of the collection's state: tracer values (in their storage
+        precision), per-tracer names and molecular diffusivities, and for the FP32
+        form the rounding mode and step count.  A collection restored from it
+        continues bit for bit as the original would (the grid fields and the
+        velocities are inputs of every step, not state of the collection)."""
+        torch.cuda.synchronize(self.grid.device)
+        ctr = self._counter if self._counter is not None else getattr(self, "_own_counter", None)
+        return {"values": self.values.detach().cpu().numpy().copy(), "precision": self.precision,
+                "names": list(self._names), "molecular": list(self._molecular),
+                "rounding": self.rounding, "step": int(ctr.item()) if ctr is not None else 0}
+
+    def load_state_dict(self, state: dict) -> None:
+        """Restore a state_dict; the collection must have the same grid shape and
+        precision.  Tracer sources and vertical velocities are not part of it
+        (re-add them with ``add`` semantics if the run uses them)."""
+        if state["precision"] != self.precision:
+            raise ValueError(f"state is {state['precision']}, collection is {self.precision}")
+        self.set_values(torch.from_numpy(np.ascontiguousarray(state["values"])))
+        self._names = list(state["names"])
+        self._molecular = [float(m) for m in state["molecular"]]
+        if self.batched and self.precision == "f32":
+            self.set_rounding(state["rounding"], self._counter)
+            ctr = self._counter if self._counter is not None else self._own_counter
+            ctr.fill_(int(state["step"]))
+
+    def save(self, path: str) -> None:
+        """Write state_dict to an .npz file (arrays and a JSON header; no pickles)."""
+        st = self.state_dict()
+        meta = {k: v for k, v in st.items() if k != "values"}
+        np.savez(path, values=st["values"], meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+
+    def load(self, path: str) -> None:
+        """Restore a file written by ``save``."""
+        with np.load(path, allow_pickle=False) as f:
+            st = json.loads(bytes(f["meta"]).decode())
+            st["values"] = f["values"]
+        self.load_state_dict(st)
 
     def stats(self, reset: bool = True):
         """(largest z sub-iteration count, columns capped at 200) since the last reset."""
