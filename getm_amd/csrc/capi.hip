@@ -369,20 +369,9 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
         if (rc) return rc;
         return launch_count_step(const_cast<unsigned long long*>(A.rstep), s);
     } else {
-        // FP64: the reference-order per-tracer diffusion kernels, prepared once (tracer.py:273-276)
-        GETM_HIP_OK(hipMemcpyAsync(f, S, n3 * (size_t)ntr * sizeof(R), hipMemcpyDeviceToDevice, s));
-        if ((rc = tr->auxo.ensure(n3 * sizeof(double))) || (rc = tr->auxn.ensure(n3 * sizeof(double))) ||
-            (rc = tr->ru.ensure(n3 * sizeof(double))))
-            return rc;
-        double* auxo = static_cast<double*>(tr->auxo.ptr);
-        double* auxn = static_cast<double*>(tr->auxn.ptr);
-        if ((rc = launch_vdiff_prepare_f64(*t, t->nz, molecular, nuh, dt, tr->cnpar, t->hn, t->hn, auxo, auxn, s)))
-            return rc;
-        for (int i = 0; i < ntr; i++)
-            if ((rc = launch_vdiff_apply_f64(*t, t->nz, t->mask, t->hn, t->hn, auxo, auxn, f + (size_t)i * n3, nullptr,
-                                             nullptr, static_cast<double*>(tr->ru.ptr), s)))
-                return rc;
-        return GETM_OK;
+        // FP64: the reference's prepare + per-tracer apply (tracer.py:273-276) in one
+        // launch, scratch -> tracers, bit for bit
+        return launch_vdiff_batch_f64(*t, t->nz, t->mask, t->hn, nuh, dt, tr->cnpar, molecular, ntr, S, f, s);
     }
 }
 

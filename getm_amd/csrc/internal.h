@@ -216,6 +216,10 @@ int launch_vdiff_prepare_f64(const Grid& t, int nz, double molecular, const doub
 int launch_vdiff_apply_f64(const Grid& t, int nz, const int* mask, const double* dzo, const double* dzn,
                            const double* auxo, const double* auxn, double* var, const double* ea2,
                            const double* ea4, double* ru, hipStream_t s);
+// the transport step's FP64 diffusion of all tracers, fin -> fout (dzo = dzn = hn);
+// fin is scratch: the solve overwrites it
+int launch_vdiff_batch_f64(const Grid& t, int nz, const int* mask, const double* hn, const double* nuh, double dt,
+                           double cnpar, double molecular, int ntr, double* fin, double* fout, hipStream_t s);
 
 }  // namespace getm
 
