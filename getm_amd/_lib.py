@@ -47,6 +47,8 @@ _SIGNATURES = {
     "transport_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
     "c_w_momentum_3d": (_i, [_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
     "c_vertical_advection_to_sources": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "c_elevation2depth": (_i, [_i, _vp, _vp, _d, _vp, _vp, _vp]),
+    "c_thickness2vertical_coordinates": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "transport_set_solver": (_i, [_vp, _i]),
     "transport_set_rounding": (_i, [_vp, _i, _vp]),
     "transport_inventory": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),

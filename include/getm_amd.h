@@ -287,6 +287,19 @@ int c_w_momentum_3d(int nx, int ny, int nz, int imin, int imax, int jmin, int jm
 int c_vertical_advection_to_sources(int nx, int ny, int nz, int halox, int haloy, const int* mask, const double* c,
                                     const double* w, const double* h, double* s, void* stream);
 
+/* Replaces c_elevation2depth (python/src/wrapper.F90, declared at
+ * python/_pygetm.pyx:49, called by update_depth, simulation.py:1575-1577):
+ * D = max(H + z, Dmin) where mask != 0, over n points.  Bit-identical. */
+int c_elevation2depth(int n, const double* z, const double* H, double Dmin, const int* mask, double* D, void* stream);
+
+/* Replaces c_thickness2vertical_coordinates (python/src/wrapper.F90, declared at
+ * python/_pygetm.pyx:47, called at simulation.py:1596-1599): layer-centre depths
+ * zc (nz levels) and interface depths zf (nz + 1 levels) of the layer
+ * thicknesses h above the bottom at -bottom_depth, where mask != 0; nx, ny the
+ * extents including halos.  Bit-identical. */
+int c_thickness2vertical_coordinates(int nx, int ny, int nz, const int* mask, const double* bottom_depth,
+                                     const double* h, double* zc, double* zf, void* stream);
+
 /* No reference counterpart (the reference accumulates cpu_time around its
  * loops, diffusion.F90:161-311).  When enabled, every kernel launch of this
  * library is bracketed by HIP events recorded on the launch stream;

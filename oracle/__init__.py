@@ -180,6 +180,22 @@ def vertical_advection_to_sources(halox, haloy, mask, c, w, h, s):
     _check(rc, "vertical_advection_to_sources")
 
 
+def elevation2depth(z, H, Dmin, mask, D):
+    """In-place D = max(H + z, Dmin) where mask != 0 (wrapper.F90 c_elevation2depth)."""
+    n = z.size
+    for a in (z, H, mask, D):
+        assert a.size == n and a.flags["C_CONTIGUOUS"]
+    _check(lib().oracle_elevation2depth(n, _d(z), _d(H), ctypes.c_double(Dmin), _i(mask), _d(D)), "elevation2depth")
+
+
+def thickness2vertical_coordinates(mask, H, h, zc, zf):
+    """In-place zc (nz, ny_, nx_) and zf (nz+1, ny_, nx_) (wrapper.F90 c_thickness2vertical_coordinates)."""
+    nz, ny_, nx_ = h.shape
+    assert zc.shape == h.shape and zf.shape == (nz + 1, ny_, nx_) and mask.shape == H.shape == (ny_, nx_)
+    _check(lib().oracle_thickness2vertical_coordinates(nx_, ny_, nz, _i(mask), _d(H), _d(h), _d(zc), _d(zf)),
+           "thickness2vertical_coordinates")
+
+
 def transport_step(d, tracers: np.ndarray, scheme=None, dt=None, cnpar=None, molecular=0.0,
                    ah_u=None, ah_v=None, return_h=False):
     """One adv+diff step on stacked FP64 tracers (T, nz, ny_, nx_), in place.

@@ -477,3 +477,30 @@ int oracle_vertical_advection_to_sources(int nx, int ny, int nz, int halox, int 
     }
     return 0;
 }
+
+/* python/src/wrapper.F90 c_elevation2depth: where (mask /= 0) D = max(H + z, Dmin) */
+int oracle_elevation2depth(int n, const double* z, const double* H, double Dmin, const int* mask, double* D)
+{
+    if (n < 0) return 1;
+    for (int p = 0; p < n; p++)
+        if (mask[p] != 0) D[p] = fmax2(H[p] + z[p], Dmin);
+    return 0;
+}
+
+/* python/src/wrapper.F90 c_thickness2vertical_coordinates: the Fortran's whole-array
+ * where statements per k, restated per point (each point only reads itself). */
+int oracle_thickness2vertical_coordinates(int nx, int ny, int nz, const int* mask, const double* H, const double* h,
+                                          double* zc, double* zf)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return 1;
+    const size_t n2 = (size_t)nx * ny;
+    for (size_t p = 0; p < n2; p++) {
+        if (mask[p] == 0) continue;
+        zc[p] = -H[p] + 0.5 * h[p];
+        for (int k = 1; k < nz; k++)
+            zc[(size_t)k * n2 + p] = zc[(size_t)(k - 1) * n2 + p] + 0.5 * (h[(size_t)(k - 1) * n2 + p] + h[(size_t)k * n2 + p]);
+        zf[p] = -H[p];
+        for (int k = 1; k <= nz; k++) zf[(size_t)k * n2 + p] = zf[(size_t)(k - 1) * n2 + p] + h[(size_t)(k - 1) * n2 + p];
+    }
+    return 0;
+}

@@ -84,6 +84,17 @@ int oracle_w_momentum_3d(int nx, int ny, int nz, int imin, int imax, int jmin, i
 int oracle_vertical_advection_to_sources(int nx, int ny, int nz, int halox, int haloy, const int* mask,
                                          const double* c, const double* w, const double* h, double* s);
 
+/* Water depth from elevation, python/src/wrapper.F90 (c_elevation2depth):
+ * D = max(H + z, Dmin) where mask != 0 (flang's MAX: a > b ? a : b), n points. */
+int oracle_elevation2depth(int n, const double* z, const double* H, double Dmin, const int* mask, double* D);
+
+/* Layer-centre and interface depths from layer thicknesses, python/src/wrapper.F90
+ * (c_thickness2vertical_coordinates), where mask != 0: zc_0 = -H + 0.5 h_0,
+ * zc_k = zc_{k-1} + 0.5 (h_{k-1} + h_k); zf_0 = -H, zf_k = zf_{k-1} + h_{k-1}
+ * (zf has nz + 1 levels).  Arrays (nz, ny, nx) C order, k = 0 at the bottom. */
+int oracle_thickness2vertical_coordinates(int nx, int ny, int nz, const int* mask, const double* H, const double* h,
+                                          double* zc, double* zf);
+
 /* Limiter value for a given scheme, cfl and slope ratio (limiters.h:1-44);
  * exported for the closed-form tests. */
 double oracle_limiter(int scheme, double cfl, double ratio);

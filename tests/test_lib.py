@@ -56,3 +56,13 @@ def test_vertical_advection_to_sources_rejects_bad_arguments_without_gpu():
     assert lib.c_vertical_advection_to_sources(0, 8, 3, 2, 2, None, None, None, None, None, None) == 1
     assert lib.c_vertical_advection_to_sources(10, 8, 3, -1, 2, None, None, None, None, None, None) == 1
     assert lib.c_vertical_advection_to_sources(10, 8, 3, 2, 2, None, None, None, None, None, None) == 1
+
+
+def test_depth_helpers_reject_bad_arguments_without_gpu():
+    lib = _lib.load()
+    # checked before any device call; n = 0 launches nothing
+    assert lib.c_elevation2depth(-1, None, None, 0.0, None, None, None) == 1
+    assert lib.c_elevation2depth(5, None, None, 0.0, None, None, None) == 1
+    assert lib.c_elevation2depth(0, None, None, 0.0, None, None, None) == 0
+    assert lib.c_thickness2vertical_coordinates(0, 4, 3, None, None, None, None, None, None) == 1
+    assert lib.c_thickness2vertical_coordinates(4, 4, 3, None, None, None, None, None, None) == 1

@@ -418,3 +418,47 @@ def test_vadv_sources_inactive_leaves_sources_untouched(oracle_mod):
     s0 = s.copy()
     oracle_mod.vertical_advection_to_sources(halo, halo, mask, c, w, h, s)
     assert s.tobytes() == s0.tobytes()
+
+
+# ---- depth from elevation and vertical coordinates (wrapper.F90), pinned by invariants ----
+
+def test_elevation2depth_clips_at_dmin_and_respects_mask(oracle_mod):
+    rng = np.random.default_rng(7)
+    n = 500
+    H = rng.uniform(-2.0, 50.0, n)
+    z = rng.uniform(-3.0, 3.0, n)
+    mask = (rng.random(n) > 0.3).astype(np.int32)
+    D = np.full(n, -9.0)
+    oracle_mod.elevation2depth(z, H, 0.5, mask, D)
+    wet = mask != 0
+    assert np.array_equal(D[~wet], np.full((~wet).sum(), -9.0))
+    deep = wet & (H + z > 0.5)
+    assert np.array_equal(D[deep], (H + z)[deep])
+    assert np.array_equal(D[wet & ~deep], np.full((wet & ~deep).sum(), 0.5))
+    # flang's MAX is a compare-select: a NaN depth gives Dmin
+    Dn = np.zeros(1)
+    oracle_mod.elevation2depth(np.array([np.nan]), np.array([1.0]), 0.5, np.ones(1, np.int32), Dn)
+    assert Dn[0] == 0.5
+
+
+def test_vertical_coordinates_close_at_the_surface(oracle_mod):
+    """zf climbs from -H by the layer thicknesses, so with layers summing to
+    D = H + z the top interface is the surface elevation z; every centre zc is the
+    mid-point of its two interfaces; masked points are not touched."""
+    rng = np.random.default_rng(8)
+    nz, ny_, nx_ = 12, 9, 11
+    H = rng.uniform(5.0, 80.0, (ny_, nx_))
+    zs = rng.uniform(-1.0, 1.0, (ny_, nx_))
+    frac = rng.uniform(0.5, 1.5, nz)
+    frac /= frac.sum()
+    h = np.ascontiguousarray(frac[:, None, None] * (H + zs)[None])
+    mask = (rng.random((ny_, nx_)) > 0.2).astype(np.int32)
+    zc = np.full((nz, ny_, nx_), 7.0)
+    zf = np.full((nz + 1, ny_, nx_), 7.0)
+    oracle_mod.thickness2vertical_coordinates(mask, H, h, zc, zf)
+    wet = mask != 0
+    np.testing.assert_allclose(zf[-1][wet], zs[wet], atol=1e-12 * 80)
+    np.testing.assert_array_equal(zf[0][wet], -H[wet])
+    np.testing.assert_allclose(zc[:, wet], 0.5 * (zf[:-1, wet] + zf[1:, wet]), rtol=0, atol=1e-12 * 80)
+    assert (np.diff(zf[:, wet], axis=0) > 0).all()
+    assert (zc[:, ~wet] == 7.0).all() and (zf[:, ~wet] == 7.0).all()
