@@ -48,6 +48,11 @@ _SIGNATURES = {
     "c_w_momentum_3d": (_i, [_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
     "c_vertical_advection_to_sources": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "c_elevation2depth": (_i, [_i, _vp, _vp, _d, _vp, _vp, _vp]),
+    "grid_interp_x": (_i, [_i, _i, _i, _vp, _vp, _i, _vp]),
+    "grid_interp_y": (_i, [_i, _i, _i, _vp, _vp, _i, _vp]),
+    "grid_interp_z": (_i, [_i, _i, _i, _i, _vp, _vp, _i, _vp]),
+    "grid_interp_xy": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp]),
+    "c_reconstruct_transport_change": (_i, [_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _d, _vp]),
     "c_thickness2vertical_coordinates": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "transport_set_solver": (_i, [_vp, _i]),
     "transport_set_rounding": (_i, [_vp, _i, _vp]),

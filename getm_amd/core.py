@@ -57,13 +57,82 @@ class Array:
     def numpy(self) -> np.ndarray:
         return self.all_values.detach().cpu().numpy()
 
+    def interp(self, target, z=None) -> "Array":
+        """Interpolate to another grid, or between layer centres and interfaces of
+        this grid (core.py:940-972); ``target`` is the receiving Array or a Grid
+        (then a new Array is made).  The averages run on the device
+        (``grid_interp_*``, bit-identical to python/src/interpolation.F90)."""
+        if not isinstance(target, Array):
+            target = target.array(z=z if z is not None else self.z, dtype=self.all_values.dtype)
+        src, dst = self.all_values, target.all_values
+        if self.grid is target.grid:
+            if self.z == INTERFACES and target.z == CENTERS:
+                _interp_z(src, dst, 0)
+            elif self.z == CENTERS and target.z == INTERFACES:
+                _interp_z(src, dst, 1)  # top and bottom interfaces are left as they are
+        else:
+            if src.dim() == 2:
+                src, dst = src[None], dst[None]
+            self.grid.interpolator(target.grid)(src, dst)
+        return target
+
+
+def _check_f64(*ts):
+    for t in ts:
+        if t.dtype != torch.float64 or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("interpolation works on contiguous float64 device tensors")
+
+
+def _interp_z(src: torch.Tensor, dst: torch.Tensor, koffset: int) -> None:
+    _check_f64(src, dst)
+    if src.dim() != 3 or dst.dim() != 3 or src.shape[1:] != dst.shape[1:]:
+        raise ValueError("vertical interpolation needs (nz, ny, nx) fields of one grid")
+    nz1, ny, nx = src.shape
+    _check(_lib.load().grid_interp_z(nx, ny, nz1, dst.shape[0], _lib.ptr(src), _lib.ptr(dst), koffset,
+                                     _lib.stream_handle()), "grid_interp_z")
+
+
+def _interp_1(axis: str, offset: int):
+    def run(src: torch.Tensor, dst: torch.Tensor) -> None:
+        _check_f64(src, dst)
+        if src.shape != dst.shape:
+            raise ValueError(f"interp_{axis}: source and target shapes differ")
+        nz, ny, nx = src.shape
+        fn = _lib.load().grid_interp_x if axis == "x" else _lib.load().grid_interp_y
+        _check(fn(nx, ny, nz, _lib.ptr(src), _lib.ptr(dst), offset, _lib.stream_handle()), f"grid_interp_{axis}")
+    return run
+
+
+def _interp_xy(ioffset: int, joffset: int):
+    def run(src: torch.Tensor, dst: torch.Tensor) -> None:
+        _check_f64(src, dst)
+        if src.shape[0] != dst.shape[0]:
+            raise ValueError("interp_xy: source and target have different numbers of levels")
+        nz, ny1, nx1 = src.shape
+        _check(_lib.load().grid_interp_xy(nx1, ny1, dst.shape[2], dst.shape[1], nz, _lib.ptr(src), _lib.ptr(dst),
+                                          ioffset, joffset, _lib.stream_handle()), "grid_interp_xy")
+    return run
+
+
+def _check(rc: int, what: str) -> None:
+    _lib.check(rc, what)
+
+
+# (ioffset, joffset) in half cells of the grid kinds (python/pygetm/domain.py:725-745)
+OFFSETS = {"T": (1, 1), "U": (2, 1), "V": (1, 2), "X": (0, 0), "UU": (3, 1), "UV": (2, 2), "VU": (2, 2),
+           "VV": (1, 3)}
+
 
 class Grid:
-    """One staggered grid (T, U or V points).  ``ugrid``/``vgrid`` of a T grid
-    are its U and V grids (core.py:271-308)."""
+    """One staggered grid (T, U, V, X points or the velocity advection grids
+    UU, UV, VU, VV).  ``ugrid``/``vgrid`` are the grids of the faces that an
+    advection on this grid uses (core.py:271-308): U and V for T, UU and UV for U,
+    VU and VV for V.  ``ioffset``/``joffset`` place the grid's points in half
+    cells, as in domain.py:725-745."""
 
     def __init__(self, nx: int, ny: int, nz: int, halox: int = 2, haloy: int = 2,
                  device: Optional[torch.device] = None, name: str = "T"):
+        self.ioffset, self.joffset = OFFSETS.get(name, (1, 1))
         self.nx, self.ny, self.nz = nx, ny, nz
         self.halox, self.haloy = halox, haloy
         self.nx_, self.ny_, self.nz_ = nx + 2 * halox, ny + 2 * haloy, nz
@@ -128,6 +197,24 @@ class Grid:
             raise ValueError(f"{name}: tensor must be contiguous")
         self._check_shape(values, name)
         return Array(self, values, z, name)
+
+    def interpolator(self, target: "Grid"):
+        """The routine that moves values from this grid's points to ``target``'s
+        (core.py:407-462), chosen by the grids' offsets."""
+        di, dj = self.ioffset - target.ioffset, self.joffset - target.joffset
+        table = {(1, 0): _interp_1("x", 1), (-1, 0): _interp_1("x", 0), (0, 1): _interp_1("y", 1),
+                 (0, -1): _interp_1("y", 0), (-1, -1): _interp_xy(0, 0), (1, 1): _interp_xy(1, 1),
+                 (-1, 1): _interp_xy(0, 1), (1, -1): _interp_xy(1, 0)}
+        if (di, dj) in table:
+            return table[(di, dj)]
+        if (di, dj) in ((-2, 0), (0, -2)):
+            # T to UU / VV: the same points shifted by one, a copy (core.py:440-455)
+            if (self.nx, self.ny) != (target.nx, target.ny):
+                raise ValueError("grids of different extents")
+            if di == -2:
+                return lambda src, dst: dst[..., :-1].copy_(src[..., 1:])
+            return lambda src, dst: dst[..., :-1, :].copy_(src[..., 1:, :])
+        raise NotImplementedError(f"cannot interpolate from grid {self.name} to grid {target.name}")
 
     # -- construction from generated inputs ----------------------------------
     @classmethod

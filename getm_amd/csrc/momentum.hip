@@ -34,6 +34,21 @@ __global__ void __launch_bounds__(256) k_w_momentum_3d(int nx, int ny, int nz, i
     }
 }
 
+// un = (un * hn - Uo) / dt as python/src/momentum.F90:323-340 computes it (idt
+// = 1/dt once, then a product), over all i and rows j0 .. j0 + nj - 1 of every
+// layer; blockIdx.x = (layer, row) pair, blockIdx.y a 256-wide chunk of i.
+__global__ void __launch_bounds__(256) k_reconstruct_transport_change(int nx, int ny, int j0, int nj,
+                                                                      double* __restrict__ un,
+                                                                      const double* __restrict__ hn,
+                                                                      const double* __restrict__ Uo, double idt)
+{
+    const int i = blockIdx.y * 256 + threadIdx.x;
+    if (i >= nx) return;
+    const unsigned r = blockIdx.x, k = r / (unsigned)nj, j = j0 + (r - k * (unsigned)nj);
+    const size_t p = ((size_t)k * ny + j) * nx + i;
+    un[p] = (un[p] * hn[p] - Uo[p]) * idt;
+}
+
 }  // namespace getm
 
 using namespace getm;
@@ -54,6 +69,25 @@ extern "C" int c_w_momentum_3d(int nx, int ny, int nz, int imin, int imax, int j
     dim3 grid((i1 - i0 + 1 + 63) / 64, (j1 - j0 + 1 + 3) / 4);
     hipLaunchKernelGGL(k_w_momentum_3d, grid, block, 0, reinterpret_cast<hipStream_t>(stream), nx, ny, nz, i0, i1,
                        j0, j1, mask, dyu, dxv, iarea, ho, hn, pk, qk, idt, w);
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+extern "C" int c_reconstruct_transport_change(int nx, int ny, int nz, int imin, int imax, int jmin, int jmax,
+                                              double* un, const double* hn, const double* Uo, double timestep,
+                                              void* stream)
+{
+    // the reference takes imin/imax but loops i over 1 .. nx (momentum.F90:335); so does this
+    (void)imin;
+    (void)imax;
+    const int j0 = jmin - 1, j1 = jmax - 1;
+    if (nx < 1 || ny < 1 || nz < 1 || j0 < 0 || j1 >= ny || !un || !hn || !Uo) return GETM_ERR_ARG;
+    if (j1 < j0) return GETM_OK;
+    const long long rows = (long long)nz * (j1 - j0 + 1);
+    if (rows > 0x7fffffffLL) return GETM_ERR_ARG;
+    const double idt = 1.0 / timestep;
+    hipLaunchKernelGGL(k_reconstruct_transport_change, dim3((unsigned)rows, (nx + 255) / 256), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), nx, ny, j0, j1 - j0 + 1, un, hn, Uo, idt);
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }

@@ -300,6 +300,28 @@ int c_elevation2depth(int n, const double* z, const double* H, double Dmin, cons
 int c_thickness2vertical_coordinates(int nx, int ny, int nz, const int* mask, const double* bottom_depth,
                                      const double* h, double* zc, double* zf, void* stream);
 
+/* Replace grid_interp_x / grid_interp_y / grid_interp_z / grid_interp_xy
+ * (python/src/interpolation.F90:9-72, declared at python/_pygetm.pyx:28-31, bound
+ * at _pygetm.pyx:180-190 and chosen by Grid.interpolator, core.py:407-462):
+ * target(i + off) = 0.5 (source(i) + source(i + 1)) along x, y or z, and the
+ * four-point average for xy, over i-fastest arrays (nx, ny, nz); other target
+ * points are not written.  Bit-identical.  source and target must be distinct
+ * arrays; offsets that would write outside the target are rejected. */
+int grid_interp_x(int nx, int ny, int nz, const double* source, double* target, int ioffset, void* stream);
+int grid_interp_y(int nx, int ny, int nz, const double* source, double* target, int joffset, void* stream);
+int grid_interp_z(int nx, int ny, int nz1, int nz2, const double* source, double* target, int koffset,
+                  void* stream);
+int grid_interp_xy(int nx1, int ny1, int nx2, int ny2, int nz, const double* source, double* target, int ioffset,
+                   int joffset, void* stream);
+
+/* Replaces c_reconstruct_transport_change (python/src/momentum.F90:323-340,
+ * declared at python/_pygetm.pyx:59, called by momentum.py:941-942, 959-960,
+ * 1088, 1097): un = (un * hn - Uo) / timestep for every i and Fortran rows
+ * jmin..jmax of nz layers (imin/imax are accepted and, as in the reference, not
+ * used).  Bit-identical. */
+int c_reconstruct_transport_change(int nx, int ny, int nz, int imin, int imax, int jmin, int jmax, double* un,
+                                   const double* hn, const double* Uo, double timestep, void* stream);
+
 /* No reference counterpart (the reference accumulates cpu_time around its
  * loops, diffusion.F90:161-311).  When enabled, every kernel launch of this
  * library is bracketed by HIP events recorded on the launch stream;

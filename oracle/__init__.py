@@ -13,7 +13,8 @@ Two implementations live here:
   (src/operators/advection_schemes/) compiled from /root/reference with
   amdflang by ``oracle/Makefile`` plus the bind(c) driver ``ref_shim.F90``.
   It pins the C restatement of the x/y/z sweeps and the six limiters bit for
-  bit.  The diffusion Fortran (src/operators/diffusion.F90) cannot be built
+  bit; python/src/interpolation.F90 (self-contained bind(c) routines) is
+  compiled into the same library and pins the grid interpolations.  The diffusion Fortran (src/operators/diffusion.F90) cannot be built
   here without stand-ins for modules the image lacks (it needs getm_domain ->
   flexout), so its restatement is pinned by closed forms and invariants in
   tests/test_oracle.py instead.
@@ -194,6 +195,63 @@ def thickness2vertical_coordinates(mask, H, h, zc, zf):
     assert zc.shape == h.shape and zf.shape == (nz + 1, ny_, nx_) and mask.shape == H.shape == (ny_, nx_)
     _check(lib().oracle_thickness2vertical_coordinates(nx_, ny_, nz, _i(mask), _d(H), _d(h), _d(zc), _d(zf)),
            "thickness2vertical_coordinates")
+
+
+def _interp_args(source, target):
+    assert source.dtype == target.dtype == np.float64 and source.flags["C_CONTIGUOUS"] and target.flags["C_CONTIGUOUS"]
+    assert source.ndim == target.ndim == 3
+
+
+def interp_x(source, target, offset, impl="c"):
+    """In-place target[..., i + offset] = 0.5 (source[..., i] + source[..., i + 1])
+    (interpolation.F90 grid_interp_x); impl="ref" runs the reference Fortran."""
+    _interp_args(source, target)
+    assert source.shape == target.shape
+    nz, ny, nx = source.shape
+    if impl == "ref":
+        ref_lib().grid_interp_x(nx, ny, nz, _d(source), _d(target), int(offset))
+        return
+    _check(lib().oracle_interp_x(nx, ny, nz, _d(source), _d(target), int(offset)), "interp_x")
+
+
+def interp_y(source, target, offset, impl="c"):
+    _interp_args(source, target)
+    assert source.shape == target.shape
+    nz, ny, nx = source.shape
+    if impl == "ref":
+        ref_lib().grid_interp_y(nx, ny, nz, _d(source), _d(target), int(offset))
+        return
+    _check(lib().oracle_interp_y(nx, ny, nz, _d(source), _d(target), int(offset)), "interp_y")
+
+
+def interp_z(source, target, offset, impl="c"):
+    _interp_args(source, target)
+    nz1, ny, nx = source.shape
+    assert target.shape[1:] == (ny, nx)
+    if impl == "ref":
+        ref_lib().grid_interp_z(nx, ny, nz1, target.shape[0], _d(source), _d(target), int(offset))
+        return
+    _check(lib().oracle_interp_z(nx, ny, nz1, target.shape[0], _d(source), _d(target), int(offset)), "interp_z")
+
+
+def interp_xy(source, target, ioffset, joffset, impl="c"):
+    _interp_args(source, target)
+    nz, ny1, nx1 = source.shape
+    assert target.shape[0] == nz
+    ny2, nx2 = target.shape[1:]
+    if impl == "ref":
+        ref_lib().grid_interp_xy(nx1, ny1, nx2, ny2, nz, _d(source), _d(target), int(ioffset), int(joffset))
+        return
+    _check(lib().oracle_interp_xy(nx1, ny1, nx2, ny2, nz, _d(source), _d(target), int(ioffset), int(joffset)),
+           "interp_xy")
+
+
+def reconstruct_transport_change(un, hn, Uo, timestep, jmin, jmax):
+    """In-place un = (un hn - Uo) / timestep over Fortran rows jmin..jmax (momentum.F90:323-340)."""
+    nz, ny, nx = un.shape
+    assert hn.shape == Uo.shape == un.shape
+    _check(lib().oracle_reconstruct_transport_change(nx, ny, nz, int(jmin), int(jmax), _d(un), _d(hn), _d(Uo),
+                                                     ctypes.c_double(timestep)), "reconstruct_transport_change")
 
 
 def transport_step(d, tracers: np.ndarray, scheme=None, dt=None, cnpar=None, molecular=0.0,

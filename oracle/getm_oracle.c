@@ -504,3 +504,72 @@ int oracle_thickness2vertical_coordinates(int nx, int ny, int nz, const int* mas
     }
     return 0;
 }
+
+/* python/src/interpolation.F90:9-23 grid_interp_x */
+int oracle_interp_x(int nx, int ny, int nz, const double* source, double* target, int ioffset)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || ioffset < 0 || ioffset > 1) return 1;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++) {
+            const size_t row = ((size_t)k * ny + j) * nx;
+            for (int i = 0; i < nx - 1; i++)
+                target[row + i + ioffset] = 0.5 * (source[row + i] + source[row + i + 1]);
+        }
+    return 0;
+}
+
+/* python/src/interpolation.F90:25-39 grid_interp_y */
+int oracle_interp_y(int nx, int ny, int nz, const double* source, double* target, int joffset)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || joffset < 0 || joffset > 1) return 1;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny - 1; j++)
+            for (int i = 0; i < nx; i++) {
+                const size_t p = ((size_t)k * ny + j) * nx + i;
+                target[p + (size_t)joffset * nx] = 0.5 * (source[p] + source[p + nx]);
+            }
+    return 0;
+}
+
+/* python/src/interpolation.F90:41-55 grid_interp_z */
+int oracle_interp_z(int nx, int ny, int nz1, int nz2, const double* source, double* target, int koffset)
+{
+    if (nx < 1 || ny < 1 || nz1 < 1 || nz2 < 1 || koffset < 0 || nz1 - 1 + koffset > nz2) return 1;
+    const size_t n2 = (size_t)nx * ny;
+    for (int k = 0; k < nz1 - 1; k++)
+        for (size_t p = 0; p < n2; p++)
+            target[(size_t)(k + koffset) * n2 + p] = 0.5 * (source[(size_t)k * n2 + p] + source[(size_t)(k + 1) * n2 + p]);
+    return 0;
+}
+
+/* python/src/interpolation.F90:57-72 grid_interp_xy */
+int oracle_interp_xy(int nx1, int ny1, int nx2, int ny2, int nz, const double* source, double* target, int ioffset,
+                     int joffset)
+{
+    if (nx1 < 1 || ny1 < 1 || nx2 < 1 || ny2 < 1 || nz < 1 || ioffset < 0 || joffset < 0 ||
+        nx1 - 1 + ioffset > nx2 || ny1 - 1 + joffset > ny2)
+        return 1;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny1 - 1; j++)
+            for (int i = 0; i < nx1 - 1; i++) {
+                const size_t p = ((size_t)k * ny1 + j) * nx1 + i;
+                target[((size_t)k * ny2 + j + joffset) * nx2 + i + ioffset] =
+                    0.25 * (source[p] + source[p + 1] + source[p + nx1] + source[p + nx1 + 1]);
+            }
+    return 0;
+}
+
+/* python/src/momentum.F90:323-340 c_reconstruct_transport_change */
+int oracle_reconstruct_transport_change(int nx, int ny, int nz, int jmin, int jmax, double* un, const double* hn,
+                                        const double* Uo, double timestep)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || jmin < 1 || jmax > ny) return 1;
+    const double idt = 1.0 / timestep;
+    for (int k = 0; k < nz; k++)
+        for (int j = jmin - 1; j <= jmax - 1; j++)
+            for (int i = 0; i < nx; i++) {
+                const size_t p = ((size_t)k * ny + j) * nx + i;
+                un[p] = (un[p] * hn[p] - Uo[p]) * idt;
+            }
+    return 0;
+}

@@ -95,6 +95,22 @@ int oracle_elevation2depth(int n, const double* z, const double* H, double Dmin,
 int oracle_thickness2vertical_coordinates(int nx, int ny, int nz, const int* mask, const double* H, const double* h,
                                           double* zc, double* zf);
 
+/* Grid-to-grid averages, python/src/interpolation.F90 (grid_interp_x/_y/_z/_xy):
+ * target(i + off) = 0.5 (source(i) + source(i + 1)) along x, y or z, and the
+ * four-point average 0.25 (s(i,j) + s(i+1,j) + s(i,j+1) + s(i+1,j+1)) for xy, in
+ * that order.  Arrays (nz, ny, nx) C order; other target points are untouched. */
+int oracle_interp_x(int nx, int ny, int nz, const double* source, double* target, int ioffset);
+int oracle_interp_y(int nx, int ny, int nz, const double* source, double* target, int joffset);
+int oracle_interp_z(int nx, int ny, int nz1, int nz2, const double* source, double* target, int koffset);
+int oracle_interp_xy(int nx1, int ny1, int nx2, int ny2, int nz, const double* source, double* target, int ioffset,
+                     int joffset);
+
+/* Change of transport from advected velocity, python/src/momentum.F90:323-340:
+ * un = (un hn - Uo) idt with idt = 1/timestep, for all i and Fortran rows
+ * jmin..jmax of each of the nz layers (imin/imax are not used there either). */
+int oracle_reconstruct_transport_change(int nx, int ny, int nz, int jmin, int jmax, double* un, const double* hn,
+                                        const double* Uo, double timestep);
+
 /* Limiter value for a given scheme, cfl and slope ratio (limiters.h:1-44);
  * exported for the closed-form tests. */
 double oracle_limiter(int scheme, double cfl, double ratio);

@@ -66,3 +66,21 @@ def test_depth_helpers_reject_bad_arguments_without_gpu():
     assert lib.c_elevation2depth(0, None, None, 0.0, None, None, None) == 0
     assert lib.c_thickness2vertical_coordinates(0, 4, 3, None, None, None, None, None, None) == 1
     assert lib.c_thickness2vertical_coordinates(4, 4, 3, None, None, None, None, None, None) == 1
+
+
+def test_interpolation_and_transport_change_reject_bad_arguments_without_gpu():
+    import ctypes
+
+    lib = _lib.load()
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)  # never dereferenced: checked first
+    assert lib.grid_interp_x(0, 4, 3, a, b, 0, None) == 1
+    assert lib.grid_interp_x(4, 4, 3, a, a, 0, None) == 1  # in place would differ from the sequential loop
+    assert lib.grid_interp_x(4, 4, 3, a, b, 2, None) == 1
+    assert lib.grid_interp_y(4, 4, 3, None, b, 0, None) == 1
+    assert lib.grid_interp_z(4, 4, 6, 4, a, b, 0, None) == 1  # level 5 of 4 would be written
+    assert lib.grid_interp_xy(8, 6, 7, 6, 2, a, b, 1, 0, None) == 1  # column 8 of 7
+    assert lib.grid_interp_xy(8, 6, 8, 5, 2, a, b, 0, 1, None) == 1  # row 6 of 5
+    assert lib.grid_interp_x(1, 4, 3, a, b, 0, None) == 0  # nothing to average, nothing launched
+    assert lib.c_reconstruct_transport_change(5, 4, 2, 1, 5, 0, 4, a, a, a, 1.0, None) == 1
+    assert lib.c_reconstruct_transport_change(5, 4, 2, 1, 5, 1, 5, a, a, a, 1.0, None) == 1
+    assert lib.c_reconstruct_transport_change(5, 4, 2, 1, 5, 1, 4, None, a, a, 1.0, None) == 1

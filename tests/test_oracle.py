@@ -11,6 +11,7 @@
   (python/tests/test_3d_barotropic.py:121-134).
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -462,3 +463,97 @@ def test_vertical_coordinates_close_at_the_surface(oracle_mod):
     np.testing.assert_allclose(zc[:, wet], 0.5 * (zf[:-1, wet] + zf[1:, wet]), rtol=0, atol=1e-12 * 80)
     assert (np.diff(zf[:, wet], axis=0) > 0).all()
     assert (zc[:, ~wet] == 7.0).all() and (zf[:, ~wet] == 7.0).all()
+
+
+def _interp_golden():
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "interp_golden.npz")
+    return np.load(path)
+
+
+def _run_interp(oracle_mod, kind, src, dst, offs, impl="c"):
+    if kind == "x":
+        oracle_mod.interp_x(src, dst, offs[0], impl=impl)
+    elif kind == "y":
+        oracle_mod.interp_y(src, dst, offs[0], impl=impl)
+    elif kind == "z":
+        oracle_mod.interp_z(src, dst, offs[0], impl=impl)
+    else:
+        oracle_mod.interp_xy(src, dst, offs[0], offs[1], impl=impl)
+
+
+def test_interp_matches_reference_golden_vectors(oracle_mod):
+    """The C restatement reproduces the reference Fortran's grid_interp_* outputs
+    (tests/golden/interp_golden.npz, written through oracle/_ref) bit for bit,
+    including the target points each routine leaves alone."""
+    g = _interp_golden()
+    names = sorted({k.split("/")[0] for k in g.files})
+    assert len(names) == 11
+    for name in names:
+        dst = g[f"{name}/before"].copy()
+        _run_interp(oracle_mod, str(g[f"{name}/kind"]), g[f"{name}/source"], dst, tuple(g[f"{name}/offsets"]))
+        assert np.array_equal(dst.view(np.int64), g[f"{name}/after"].view(np.int64)), name
+
+
+def test_interp_matches_reference_live(oracle_mod):
+    if not oracle_mod.ref_available():
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(31)
+    for kind, sshape, tshape, offs in (("x", (3, 11, 17), (3, 11, 17), (1,)), ("y", (2, 13, 6), (2, 13, 6), (0,)),
+                                       ("z", (9, 4, 5), (8, 4, 5), (0,)), ("z", (8, 4, 5), (9, 4, 5), (1,)),
+                                       ("xy", (4, 9, 12), (4, 9, 12), (1, 0)),
+                                       ("xy", (1, 30, 2), (1, 30, 2), (0, 1))):
+        src = rng.standard_normal(sshape) * 1e3
+        a = np.full(tshape, np.pi)
+        b = a.copy()
+        _run_interp(oracle_mod, kind, src, a, offs)
+        _run_interp(oracle_mod, kind, src, b, offs, impl="ref")
+        assert np.array_equal(a.view(np.int64), b.view(np.int64)), (kind, offs)
+
+
+def test_interp_closed_forms(oracle_mod):
+    """Averages of a field linear in i, j and k land exactly on the mid-points
+    (the values are small integers and halves, so every sum is exact); the points
+    a routine does not reach keep their values."""
+    nz, ny, nx = 4, 6, 7
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    f = (3.0 * i - 2.0 * j + 5.0 * k + 1.0).astype(np.float64)
+    t = np.full(f.shape, -1.0)
+    oracle_mod.interp_x(f, t, 1)
+    assert np.array_equal(t[..., 1:], f[..., 1:] - 1.5) and (t[..., 0] == -1.0).all()
+    t = np.full(f.shape, -1.0)
+    oracle_mod.interp_x(f, t, 0)
+    assert np.array_equal(t[..., :-1], f[..., :-1] + 1.5) and (t[..., -1] == -1.0).all()
+    t = np.full(f.shape, -1.0)
+    oracle_mod.interp_y(f, t, 1)
+    assert np.array_equal(t[:, 1:], f[:, 1:] + 1.0) and (t[:, 0] == -1.0).all()
+    t = np.full((nz + 1, ny, nx), -1.0)
+    oracle_mod.interp_z(f, t, 1)  # centres -> interior interfaces
+    assert np.array_equal(t[1:nz], f[1:] - 2.5) and (t[0] == -1.0).all() and (t[nz] == -1.0).all()
+    t = np.full(f.shape, -1.0)
+    oracle_mod.interp_xy(f, t, 1, 1)
+    assert np.array_equal(t[:, 1:, 1:], f[:, 1:, 1:] - 1.5 + 1.0)
+    assert (t[:, 0, :] == -1.0).all() and (t[:, :, 0] == -1.0).all()
+    t = np.full(f.shape, -1.0)
+    oracle_mod.interp_xy(f, t, 0, 1)  # V to U: i = -1 and j = 0 undefined
+    assert np.array_equal(t[:, 1:, :-1], f[:, 1:, :-1] + 1.5 + 1.0)
+    with pytest.raises(ValueError):
+        oracle_mod.interp_xy(f, np.zeros((nz, ny - 2, nx)), 0, 0)
+
+
+def test_reconstruct_transport_change_closed_form(oracle_mod):
+    """(un hn - Uo) / dt is the transport change over the step: with dyadic
+    values and dt a power of two it is exact; rows outside jmin..jmax keep un."""
+    rng = np.random.default_rng(9)
+    nz, ny, nx = 3, 8, 5
+    un = rng.integers(-64, 64, (nz, ny, nx)) / 8.0
+    hn = rng.integers(1, 32, (nz, ny, nx)) / 4.0
+    Uo = rng.integers(-256, 256, (nz, ny, nx)) / 16.0
+    got = un.copy()
+    oracle_mod.reconstruct_transport_change(got, hn, Uo, 0.25, 3, 6)
+    rows = slice(2, 6)
+    assert np.array_equal(got[:, rows], (un[:, rows] * hn[:, rows] - Uo[:, rows]) * 4.0)
+    assert np.array_equal(got[:, :2], un[:, :2]) and np.array_equal(got[:, 6:], un[:, 6:])
+    # an unchanged transport gives exactly zero
+    same = un.copy()
+    oracle_mod.reconstruct_transport_change(same, hn, un * hn, 30.0, 1, ny)
+    assert not same.any()
