@@ -48,6 +48,9 @@ _SIGNATURES = {
     "c_w_momentum_3d": (_i, [_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),
     "c_vertical_advection_to_sources": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "c_elevation2depth": (_i, [_i, _vp, _vp, _d, _vp, _vp, _vp]),
+    "c_alpha": (_i, [_i, _vp, _d, _d, _vp, _vp, _vp]),
+    "c_update_gvc": (_i, [_i, _i, _i, _d, _vp, _d, _i, _vp, _vp, _vp, _vp]),
+    "c_update_sigma": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "grid_interp_x": (_i, [_i, _i, _i, _vp, _vp, _i, _vp]),
     "grid_interp_y": (_i, [_i, _i, _i, _vp, _vp, _i, _vp]),
     "grid_interp_z": (_i, [_i, _i, _i, _i, _vp, _vp, _i, _vp]),

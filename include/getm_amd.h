@@ -300,6 +300,28 @@ int c_elevation2depth(int n, const double* z, const double* H, double Dmin, cons
 int c_thickness2vertical_coordinates(int nx, int ny, int nz, const int* mask, const double* bottom_depth,
                                      const double* h, double* zc, double* zf, void* stream);
 
+/* Replaces c_alpha (python/src/wrapper.F90:428-435, declared at
+ * python/_pygetm.pyx:48, called at simulation.py:1569-1570): the shallow-water
+ * damping alpha = max(0, min(1, (D - Dmin) / (Dcrit - Dmin))) where mask != 0,
+ * over n points.  Bit-identical. */
+int c_alpha(int n, const double* D, double Dmin, double Dcrit, const int* mask, double* alpha, void* stream);
+
+/* Replaces c_update_gvc (python/src/wrapper.F90:567-601, declared at
+ * python/_pygetm.pyx:66, called by vertical_coordinates.GVC.__call__): layer
+ * thicknesses h (nz levels) of generalized vertical coordinates from the water
+ * depth D where mask != 0; kk is the 1-based reference layer as the reference's
+ * wrapper passes it (_pygetm.pyx:583-585) and dbeta is a device array of nz
+ * zoomed sigma fractions.  Bit-identical. */
+int c_update_gvc(int nx, int ny, int nz, double dsigma, const double* dbeta, double Dgamma, int kk, const double* D,
+                 const int* mask, double* h, void* stream);
+
+/* Layer thicknesses of sigma coordinates, h_k = dga[k] * D where mask != 0: the
+ * numpy product of vertical_coordinates.Sigma.__call__
+ * (python/pygetm/vertical_coordinates.py:106-113), which has no compiled
+ * counterpart in the reference.  dga is a device array of nz fractions. */
+int c_update_sigma(int nx, int ny, int nz, const double* dga, const double* D, const int* mask, double* h,
+                   void* stream);
+
 /* Replace grid_interp_x / grid_interp_y / grid_interp_z / grid_interp_xy
  * (python/src/interpolation.F90:9-72, declared at python/_pygetm.pyx:28-31, bound
  * at _pygetm.pyx:180-190 and chosen by Grid.interpolator, core.py:407-462):

@@ -197,6 +197,34 @@ def thickness2vertical_coordinates(mask, H, h, zc, zf):
            "thickness2vertical_coordinates")
 
 
+def alpha(D, Dmin, Dcrit, mask, out):
+    """In-place out = max(0, min(1, (D - Dmin) / (Dcrit - Dmin))) where mask != 0 (wrapper.F90 c_alpha)."""
+    n = D.size
+    for a in (D, mask, out):
+        assert a.size == n and a.flags["C_CONTIGUOUS"]
+    _check(lib().oracle_alpha(n, _d(D), ctypes.c_double(Dmin), ctypes.c_double(Dcrit), _i(mask), _d(out)), "alpha")
+
+
+def update_gvc(dsigma, dbeta, Dgamma, kk, D, mask, h):
+    """In-place h (nz, ny, nx) of generalized vertical coordinates (wrapper.F90 c_update_gvc);
+    kk is the reference layer as _pygetm.update_gvc takes it (negative counts from the top)."""
+    nz = dbeta.size
+    if kk < 0:
+        kk = nz + kk
+    ny, nx = D.shape
+    assert h.shape == (nz, ny, nx) and mask.shape == D.shape
+    _check(lib().oracle_update_gvc(nx, ny, nz, ctypes.c_double(dsigma), _d(np.ascontiguousarray(dbeta)),
+                                   ctypes.c_double(Dgamma), 1 + kk, _d(D), _i(mask), _d(h)), "update_gvc")
+
+
+def update_sigma(dga, D, mask, h):
+    nz = dga.size
+    ny, nx = D.shape
+    assert h.shape == (nz, ny, nx) and mask.shape == D.shape
+    _check(lib().oracle_update_sigma(nx, ny, nz, _d(np.ascontiguousarray(dga)), _d(D), _i(mask), _d(h)),
+           "update_sigma")
+
+
 def _interp_args(source, target):
     assert source.dtype == target.dtype == np.float64 and source.flags["C_CONTIGUOUS"] and target.flags["C_CONTIGUOUS"]
     assert source.ndim == target.ndim == 3

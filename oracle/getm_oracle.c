@@ -573,3 +573,38 @@ int oracle_reconstruct_transport_change(int nx, int ny, int nz, int jmin, int jm
             }
     return 0;
 }
+
+/* python/src/wrapper.F90:428-435 c_alpha */
+int oracle_alpha(int n, const double* D, double Dmin, double Dcrit, const int* mask, double* alpha)
+{
+    if (n < 0) return 1;
+    for (int p = 0; p < n; p++)
+        if (mask[p] != 0) alpha[p] = fmax2(0.0, fmin2(1.0, (D[p] - Dmin) / (Dcrit - Dmin)));
+    return 0;
+}
+
+/* python/src/wrapper.F90:567-601 c_update_gvc */
+int oracle_update_gvc(int nx, int ny, int nz, double dsigma, const double* dbeta, double Dgamma, int kk,
+                      const double* D, const int* mask, double* h)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || kk < 1 || kk > nz) return 1;
+    const size_t n2 = (size_t)nx * ny;
+    const double bref = dbeta[kk - 1];
+    for (size_t p = 0; p < n2; p++) {
+        if (mask[p] == 0) continue;
+        const double alpha = (fmin2(Dgamma / D[p], 1.0) * dsigma - bref) / (dsigma - bref);
+        for (int k = 0; k < nz; k++) h[(size_t)k * n2 + p] = D[p] * (dbeta[k] + alpha * (dsigma - dbeta[k]));
+    }
+    return 0;
+}
+
+/* python/pygetm/vertical_coordinates.py:106-113 Sigma.__call__ */
+int oracle_update_sigma(int nx, int ny, int nz, const double* dga, const double* D, const int* mask, double* h)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return 1;
+    const size_t n2 = (size_t)nx * ny;
+    for (int k = 0; k < nz; k++)
+        for (size_t p = 0; p < n2; p++)
+            if (mask[p] != 0) h[(size_t)k * n2 + p] = dga[k] * D[p];
+    return 0;
+}

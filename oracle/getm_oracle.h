@@ -111,6 +111,22 @@ int oracle_interp_xy(int nx1, int ny1, int nx2, int ny2, int nz, const double* s
 int oracle_reconstruct_transport_change(int nx, int ny, int nz, int jmin, int jmax, double* un, const double* hn,
                                         const double* Uo, double timestep);
 
+/* Shallow-water damping, python/src/wrapper.F90:428-435 (c_alpha): where
+ * mask != 0, alpha = max(0, min(1, (D - Dmin) / (Dcrit - Dmin))), flang's
+ * MAX/MIN being the compare-selects a > b ? a : b and a < b ? a : b. */
+int oracle_alpha(int n, const double* D, double Dmin, double Dcrit, const int* mask, double* alpha);
+
+/* Generalized vertical coordinates, python/src/wrapper.F90:567-601 (c_update_gvc):
+ * per point with mask != 0, alpha = (min(Dgamma / D, 1) dsigma - dbeta[kk-1]) /
+ * (dsigma - dbeta[kk-1]) and h_k = D (dbeta[k] + alpha (dsigma - dbeta[k])); kk is
+ * the 1-based reference layer.  D, mask (ny, nx); h (nz, ny, nx). */
+int oracle_update_gvc(int nx, int ny, int nz, double dsigma, const double* dbeta, double Dgamma, int kk,
+                      const double* D, const int* mask, double* h);
+
+/* Sigma coordinates, python/pygetm/vertical_coordinates.py:106-113: h_k = dga[k] D
+ * where mask != 0 (numpy's multiply(dga, D, where=mask)). */
+int oracle_update_sigma(int nx, int ny, int nz, const double* dga, const double* D, const int* mask, double* h);
+
 /* Limiter value for a given scheme, cfl and slope ratio (limiters.h:1-44);
  * exported for the closed-form tests. */
 double oracle_limiter(int scheme, double cfl, double ratio);

@@ -76,3 +76,162 @@ def test_depth_helpers_reject_fields_of_the_wrong_kind():
     h = g.array(CENTERS, fill=1.0)
     with pytest.raises(ValueError):
         thickness2vertical_coordinates(g.mask, g.array(fill=5.0), h, g.array(CENTERS), g.array(CENTERS))
+
+
+def test_alpha_bitwise(oracle_mod):
+    from getm_amd.depth import alpha
+
+    g, rng, mask = _grid(50, 31, 4, 2, 11)
+    D = rng.uniform(-0.1, 1.2, mask.shape)
+    D[0, :4] = [0.04, 0.08, 0.5, 0.7]  # exactly at and beyond the two limits
+    out0 = np.full(mask.shape, 3.5)
+    want = out0.copy()
+    oracle_mod.alpha(D, 0.04, 0.5, mask, want)
+    d = lambda a: g.wrap(torch.from_numpy(np.ascontiguousarray(a)).cuda())
+    out = d(out0)
+    alpha(d(D), 0.04, 0.5, out)
+    torch.cuda.synchronize()
+    assert_bitwise(out.numpy(), want, "alpha")
+    got = out.numpy()
+    assert np.any(got[mask != 0] == 0.0) and np.any(got[mask != 0] == 1.0)
+
+
+@pytest.mark.parametrize("kind", ["sigma", "gvc_surface", "gvc_bottom"])
+def test_vertical_coordinates_bitwise(oracle_mod, kind):
+    from getm_amd.vertical_coordinates import GVC, Sigma, calculate_sigma
+
+    nz = 25
+    g, rng, mask = _grid(70, 40, nz, 2, 12)
+    if kind == "sigma":
+        vc, dga = Sigma(nz, ddl=0.7, ddu=1.3), calculate_sigma(nz, 0.7, 1.3)
+    elif kind == "gvc_surface":
+        vc, dga = GVC(nz, ddl=0.5, ddu=1.5, gamma_surf=True, Dgamma=15.0), calculate_sigma(nz, 0.5, 1.5)
+    else:
+        vc, dga = GVC(nz, ddl=1.5, ddu=0.5, gamma_surf=False, Dgamma=15.0), calculate_sigma(nz, 1.5, 0.5)
+    Dmax = 200.0 if kind == "sigma" else 0.9 * min(vc.D_max, 200.0)
+    D = rng.uniform(0.5, Dmax, mask.shape)
+    h0 = np.full((nz,) + mask.shape, -2.0)
+    want = h0.copy()
+    if kind == "sigma":
+        oracle_mod.update_sigma(np.ascontiguousarray(dga), D, mask, want)
+    else:
+        oracle_mod.update_gvc(1.0 / nz, np.ascontiguousarray(dga), 15.0, vc.k_ref, D, mask, want)
+    h = torch.from_numpy(h0.copy()).cuda()
+    vc(torch.from_numpy(D).cuda(), h, torch.from_numpy(mask).cuda())
+    torch.cuda.synchronize()
+    assert_bitwise(h.cpu().numpy(), want, kind)
+
+
+def _update_depth_case(seed, vc_kind):
+    from getm_amd.core import CENTERS, Grid
+    from getm_amd.vertical_coordinates import GVC, Sigma
+
+    nx, ny, nz, halo = 41, 27, 10, 2
+    dev0 = torch.device("cuda", 0)
+    rng = np.random.default_rng(seed)
+    mk = lambda name, ex=0: Grid(nx + ex, ny + ex, nz, halo, halo, dev0, name)
+    T, U, V, X = mk("T"), mk("U"), mk("V"), mk("X", 1)
+    UU, UV, VU, VV = mk("UU"), mk("UV"), mk("VU"), mk("VV")
+    T.ugrid, T.vgrid, U.ugrid, U.vgrid, V.ugrid, V.vgrid = U, V, UU, UV, VU, VV
+    shape2 = (T.ny_, T.nx_)
+    host = {}
+    for name, g in (("T", T), ("U", U), ("V", V), ("X", X)):
+        s2 = (g.ny_, g.nx_)
+        f = {"mask": (rng.random(s2) > 0.2).astype(np.int32), "H": rng.uniform(0.5, 60.0, s2),
+             "D": np.full(s2, 0.25)}
+        if name != "X":
+            f["hn"] = rng.uniform(0.1, 6.0, (nz,) + s2)
+        for k, v in f.items():
+            g.set_field(k, v)
+        host[name] = f
+    for name, g in (("UU", UU), ("UV", UV), ("VU", VU), ("VV", VV)):
+        g.set_field("D", np.full(shape2, 0.75))
+        host[name] = {"D": np.full(shape2, 0.75)}
+    for k in ("z", "zo", "zin", "zio"):
+        v = rng.uniform(-1.5, 0.8, shape2)
+        T.set_field(k, v)
+        host["T"][k] = v
+    if vc_kind == "sigma":
+        vc = Sigma(nz, ddl=1.0, ddu=1.0)
+    else:
+        vc = GVC(nz, ddl=0.5, ddu=2.0, gamma_surf=True, Dgamma=12.0)
+    return (T, U, V, X, UU, UV, VU, VV), host, vc
+
+
+def _oracle_update_depth(oracle_mod, host, vc, Dmin, Dcrit, _3d):
+    """simulation.py:1498-1600 composed from the oracle's routines and numpy."""
+    T = host["T"]
+    out = {}
+    if _3d:
+        zio, zin = T["zin"].copy(), T["z"].copy()
+        zT, zoT = zin, zio
+        out["zio"], out["zin"] = zio, zin
+    else:
+        zT, zoT = T["z"], T["zo"]
+    TD = T["D"].copy()
+    oracle_mod.elevation2depth(zT, T["H"], -1000.0, T["mask"], TD)
+    out["T.D"], out["T.Dclip"] = TD, np.maximum(TD, Dmin)
+    zh = zoT + zT
+    zh *= 0.5
+    ny_, nx_ = zh.shape
+    zU, zV, zX = (np.full((1, ny_, nx_), np.nan), np.full((1, ny_, nx_), np.nan),
+                  np.full((1, ny_ + 1, nx_ + 1), np.nan))
+    oracle_mod.interp_x(zh[None].copy(), zU, 0)
+    oracle_mod.interp_y(zh[None].copy(), zV, 0)
+    oracle_mod.interp_xy(zh[None].copy(), zX, 1, 1)
+    for g, z in (("U", zU[0]), ("V", zV[0]), ("X", zX[0])):
+        D = host[g]["D"].copy()
+        oracle_mod.elevation2depth(np.ascontiguousarray(z), host[g]["H"], Dmin, host[g]["mask"], D)
+        out[f"{g}.D"] = D
+    for g in ("U", "V"):
+        a = np.full(host[g]["D"].shape, np.nan)
+        oracle_mod.alpha(out[f"{g}.D"], 2 * Dmin, Dcrit, host[g]["mask"], a)
+        out[f"{g}.alpha"] = a
+    oracle_mod.elevation2depth(zh, T["H"], Dmin, T["mask"], zh)
+    UU, VV = host["UU"]["D"].copy(), host["VV"]["D"].copy()
+    UU[:, :-1] = zh[:, 1:]
+    VV[:-1, :] = zh[1:, :]
+    out["UU.D"], out["VV.D"] = UU, VV
+    out["UV.D"] = out["X.D"][1:, 1:].copy()
+    out["VU.D"] = out["UV.D"].copy()
+    if _3d:
+        from getm_amd.vertical_coordinates import GVC
+
+        for g, D in (("T", out["T.Dclip"]), ("U", out["U.D"]), ("V", out["V.D"])):
+            G = host[g]
+            out[f"{g}.ho"] = G["hn"].copy()
+            hn = G["hn"].copy()
+            if isinstance(vc, GVC):
+                oracle_mod.update_gvc(vc.dsigma, vc.dbeta, vc.Dgamma, vc.k_ref, np.ascontiguousarray(D), G["mask"], hn)
+            else:
+                oracle_mod.update_sigma(vc.dga_host, np.ascontiguousarray(D), G["mask"], hn)
+            out[f"{g}.hn"] = hn
+            zc, zf = np.full(hn.shape, np.nan), np.full((hn.shape[0] + 1,) + hn.shape[1:], np.nan)
+            oracle_mod.thickness2vertical_coordinates(G["mask"], G["H"], hn, zc, zf)
+            out[f"{g}.zc"], out[f"{g}.zf"] = zc, zf
+    return out
+
+
+@pytest.mark.parametrize("vc_kind,_3d", [("gvc", True), ("sigma", True), ("gvc", False)])
+def test_update_depth_bitwise(oracle_mod, vc_kind, _3d):
+    from getm_amd.depth import DepthUpdate
+
+    grids, host, vc = _update_depth_case(21, vc_kind)
+    T, U, V, X, UU, UV, VU, VV = grids
+    Dmin, Dcrit = 0.02, 0.1
+    upd = DepthUpdate(T, U, V, X, Dmin, Dcrit, vc)
+    upd(_3d=_3d, timestep=600.0)
+    torch.cuda.synchronize()
+    want = _oracle_update_depth(oracle_mod, host, vc, Dmin, Dcrit, _3d)
+    byname = {"T": T, "U": U, "V": V, "X": X, "UU": UU, "UV": UV, "VU": VU, "VV": VV}
+    for key, w in want.items():
+        if key in ("zio", "zin"):
+            got = getattr(T, key).numpy()
+        else:
+            g, f = key.split(".")
+            got = getattr(byname[g], f).numpy()
+        assert_bitwise(got, w, key)
+    assert len(want) == (25 if _3d else 11)
+    if _3d:
+        dry = host["T"]["mask"] == 0
+        assert np.all(np.isnan(T.zc.numpy()[:, dry]))

@@ -84,3 +84,16 @@ def test_interpolation_and_transport_change_reject_bad_arguments_without_gpu():
     assert lib.c_reconstruct_transport_change(5, 4, 2, 1, 5, 0, 4, a, a, a, 1.0, None) == 1
     assert lib.c_reconstruct_transport_change(5, 4, 2, 1, 5, 1, 5, a, a, a, 1.0, None) == 1
     assert lib.c_reconstruct_transport_change(5, 4, 2, 1, 5, 1, 4, None, a, a, 1.0, None) == 1
+
+
+def test_depth_update_entries_reject_bad_arguments_without_gpu():
+    import ctypes
+
+    lib = _lib.load()
+    a = ctypes.c_void_p(0x1000)  # never dereferenced: checked first
+    assert lib.c_alpha(-1, a, 0.1, 0.5, a, a, None) == 1
+    assert lib.c_alpha(0, None, 0.1, 0.5, None, None, None) == 0
+    assert lib.c_update_gvc(5, 4, 3, 1 / 3, a, 10.0, 0, a, a, a, None) == 1  # kk is 1-based
+    assert lib.c_update_gvc(5, 4, 3, 1 / 3, a, 10.0, 4, a, a, a, None) == 1
+    assert lib.c_update_gvc(5, 4, 3, 1 / 3, None, 10.0, 3, a, a, a, None) == 1
+    assert lib.c_update_sigma(5, 0, 3, a, a, a, a, None) == 1

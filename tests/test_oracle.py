@@ -557,3 +557,55 @@ def test_reconstruct_transport_change_closed_form(oracle_mod):
     same = un.copy()
     oracle_mod.reconstruct_transport_change(same, hn, un * hn, 30.0, 1, ny)
     assert not same.any()
+
+
+def test_alpha_closed_form(oracle_mod):
+    """(D - Dmin) / (Dcrit - Dmin) between the two depths (exact for dyadic
+    values), 0 below Dmin, 1 above Dcrit, masked points untouched."""
+    D = np.array([0.05, 0.1, 0.15, 0.25, 0.5, 3.0, 0.2])
+    mask = np.array([1, 1, 1, 1, 1, 1, 0], np.int32)
+    out = np.full(D.size, -5.0)
+    oracle_mod.alpha(D, 0.1, 0.5, mask, out)
+    assert out[0] == 0.0 and out[1] == 0.0 and out[4] == 1.0 and out[5] == 1.0 and out[6] == -5.0
+    np.testing.assert_allclose(out[2:4], [0.125, 0.375], rtol=1e-15)
+
+
+def test_gvc_layers_sum_to_depth_and_keep_the_reference_layer(oracle_mod):
+    """Each column's layers sum to D (both fraction sets sum to 1); in water
+    shallower than Dgamma every layer is D/nz; deeper, the reference layer (the
+    surface one, or the bottom one) is Dgamma/nz thick; masked points untouched."""
+    from getm_amd.vertical_coordinates import GVC, calculate_sigma
+
+    nz, Dgamma = 12, 20.0
+    for gamma_surf, ddl, ddu in ((True, 0.5, 1.5), (False, 1.5, 0.5)):
+        g = GVC(nz, ddl=ddl, ddu=ddu, gamma_surf=gamma_surf, Dgamma=Dgamma)
+        dbeta = calculate_sigma(nz, ddl, ddu)
+        D = np.array([[3.0, 19.0, 20.0, 35.0, 0.9 * min(g.D_max, 400.0), 7.0]])
+        mask = np.array([[1, 1, 1, 1, 1, 0]], np.int32)
+        h = np.full((nz,) + D.shape, -1.0)
+        oracle_mod.update_gvc(1.0 / nz, np.ascontiguousarray(dbeta), Dgamma, g.k_ref, D, mask, h)
+        wet = mask[0] == 1
+        np.testing.assert_allclose(h.sum(axis=0)[0, wet], D[0, wet], rtol=1e-14)
+        for c in (0, 1, 2):
+            np.testing.assert_allclose(h[:, 0, c], D[0, c] / nz, rtol=1e-14)
+        for c in (3, 4):
+            np.testing.assert_allclose(h[g.k_ref, 0, c], Dgamma / nz, rtol=1e-13)
+            assert (h[:, 0, c] > 0).all()
+        assert (h[:, 0, 5] == -1.0).all()
+
+
+def test_sigma_fractions_and_thicknesses(oracle_mod):
+    from getm_amd.vertical_coordinates import calculate_sigma
+
+    assert np.array_equal(calculate_sigma(8), np.full(8, 0.125))
+    sym = calculate_sigma(20, 1.0, 1.0)
+    np.testing.assert_allclose(sym, sym[::-1], rtol=1e-12)
+    surf = calculate_sigma(20, 0.0, 2.0)  # thin layers towards the surface (the top, last index)
+    assert (np.diff(surf) < 0).all() and abs(surf.sum() - 1.0) < 1e-14
+    bot = calculate_sigma(20, 2.0, 0.0)
+    assert (np.diff(bot) > 0).all()
+    D = np.array([[10.0, 33.0], [2.5, 1.0]])
+    mask = np.array([[1, 1], [0, 1]], np.int32)
+    h = np.full((20, 2, 2), -1.0)
+    oracle_mod.update_sigma(np.ascontiguousarray(surf), D, mask, h)
+    assert np.array_equal(h[:, 0, 1], surf * 33.0) and (h[:, 1, 0] == -1.0).all()
