@@ -24,6 +24,13 @@ defect |I_after - I_before| / |I_before| (FP64 compensated sums).
 cpu_baseline: the reference's Fortran sweeps (oracle/_ref) with the C restatement
 of the diffusion, FP64, on bounded slabs of the same workload, one process per
 host core (the C restatement alone where oracle/_ref is not built).
+
+--workload producers times the kernels on either side of the step at the same
+coastal size (one launch each, HIP events on the launch stream, median of
+--steps launches): w from continuity, sinking into sources, GVC layer
+thicknesses, layer coordinates, a grid interpolation and the transport change
+of the velocity self-advection.  Each is reported against its algorithmic bytes
+and the HBM peak, beside the C restatement timed on one host core on a slab.
 """
 from __future__ import annotations
 
@@ -303,12 +310,179 @@ def run_ensemble(args, log):
     print(json.dumps(out), flush=True)
 
 
+def _producer_cases(d, grid, torch, np):
+    """(name, callable launching the kernel once, algorithmic bytes per launch,
+    points per launch, oracle callable on host copies of a slab) for the kernels
+    on either side of the step."""
+    from getm_amd import depth, momentum, operators
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.vertical_coordinates import GVC
+
+    nz, ny_, nx_ = d.hn.shape
+    n2, n3 = ny_ * nx_, nz * ny_ * nx_
+    wet2 = int((d.tmask != 0).sum())
+    wetc = int((d.tmask == 1).sum())
+    nxi, nyi = nx_ - 2 * d.halo, ny_ - 2 * d.halo
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    rng = np.random.default_rng(5)
+    pk = grid.ugrid.wrap(dev(d.u * d.hu), CENTERS)
+    qk = grid.vgrid.wrap(dev(d.v * d.hv), CENTERS)
+    w = grid.wrap(dev(np.zeros((nz + 1, ny_, nx_))), INTERFACES)
+    mask3 = dev(np.broadcast_to(d.tmask, (nz, ny_, nx_)).astype(np.int32))
+    conc = dev(rng.uniform(0.0, 5.0, (nz, ny_, nx_)))
+    wsink = dev(np.full((nz, ny_, nx_), -1e-4))
+    hn = dev(d.hn)
+    src = dev(np.zeros((nz, ny_, nx_)))
+    gvc = GVC(nz, ddl=0.5, ddu=1.5, gamma_surf=True, Dgamma=20.0)
+    D = grid.D.all_values
+    H = grid.wrap(dev(d.H))
+    zc = grid.array(CENTERS)
+    zf = grid.array(INTERFACES)
+    hgvc = torch.empty((nz, ny_, nx_), dtype=torch.float64, device=D.device)
+    f3 = dev(rng.standard_normal((nz, ny_, nx_)))
+    g3 = torch.empty_like(f3)
+    un = grid.ugrid.wrap(dev(d.u), CENTERS)
+    Uo = grid.ugrid.wrap(dev(d.u * d.hu), CENTERS)
+    hu = dev(d.hu)
+    interp = lambda: grid.wrap(f3, CENTERS).interp(grid.ugrid.wrap(g3, CENTERS))
+    return [
+        ("w_momentum_3d", lambda: momentum.w_momentum_3d(pk, qk, d.dt, w),
+         wetc * (28 + 40 * (nz - 1)), wetc * nz),
+        # activity pass: w of the top layer of each interior column (it stops at the
+        # first non-zero value); column pass: mask and s (read and written) of every
+        # interior cell, and w, the upstream c and h of wet cells
+        ("vertical_advection_to_sources",
+         lambda: operators.vertical_advection_to_sources(d.halo, d.halo, mask3, conc, wsink, hn, src),
+         nxi * nyi * (8 + nz * 20) + wetc * nz * 24, nxi * nyi * nz),
+        ("update_gvc", lambda: gvc(D, hgvc, grid.mask.all_values), n2 * 12 + wet2 * nz * 8, n2 * nz),
+        ("thickness2vertical_coordinates",
+         lambda: depth.thickness2vertical_coordinates(grid.mask, H, grid.hn, zc, zf),
+         n2 * 4 + wet2 * (8 + 8 * nz + 8 * nz + 8 * (nz + 1)), n2 * nz),
+        ("grid_interp_x", interp, 16 * n3, n3),
+        ("reconstruct_transport_change", lambda: momentum.reconstruct_transport_change(un, hu, Uo, d.dt),
+         32 * nx_ * nyi * nz, nx_ * nyi * nz),
+    ]
+
+
+def producers_cpu_baseline(seconds_each: float = 1.5):
+    """The C restatement of each producer kernel on one host core, on a 256 x 192 x 40
+    slab of the coastal workload: points per second."""
+    import numpy as np
+
+    import oracle
+    from getm_amd import inputs
+    from getm_amd.vertical_coordinates import calculate_sigma
+
+    d = inputs.coastal(nx=256, ny=192, nz=NZ, ntracers=0)
+    nz, ny_, nx_ = d.hn.shape
+    rng = np.random.default_rng(5)
+    n2, n3 = ny_ * nx_, nz * ny_ * nx_
+    mask3 = np.ascontiguousarray(np.broadcast_to(d.tmask, (nz, ny_, nx_)).astype(np.int32))
+    conc = rng.uniform(0.0, 5.0, (nz, ny_, nx_))
+    wsink = np.full((nz, ny_, nx_), -1e-4)
+    s = np.zeros((nz, ny_, nx_))
+    dbeta = np.ascontiguousarray(calculate_sigma(nz, 0.5, 1.5))
+    D = np.ascontiguousarray(d.hn.sum(axis=0))
+    h = np.empty((nz, ny_, nx_))
+    zc, zf = np.empty((nz, ny_, nx_)), np.empty((nz + 1, ny_, nx_))
+    f3 = rng.standard_normal((nz, ny_, nx_))
+    g3 = np.empty_like(f3)
+    w = np.zeros((nz + 1, ny_, nx_))
+    pk, qk = np.ascontiguousarray(d.u * d.hu), np.ascontiguousarray(d.v * d.hv)
+    un = np.ascontiguousarray(d.u.copy())
+    wetc = int((d.tmask == 1).sum())
+    nxi, nyi = nx_ - 2 * d.halo, ny_ - 2 * d.halo
+    cases = {
+        "w_momentum_3d": (lambda: oracle.w_momentum_3d(d.tmask, d.dyu, d.dxv, d.iarea, d.ho, d.hn, pk, qk, d.dt, w),
+                          wetc * nz),
+        "vertical_advection_to_sources": (
+            lambda: oracle.vertical_advection_to_sources(d.halo, d.halo, mask3, conc, wsink, d.hn, s), nxi * nyi * nz),
+        "update_gvc": (lambda: oracle.update_gvc(1.0 / nz, dbeta, 20.0, -1, D, d.tmask, h), n2 * nz),
+        "thickness2vertical_coordinates": (
+            lambda: oracle.thickness2vertical_coordinates(d.tmask, d.H, d.hn, zc, zf), n2 * nz),
+        "grid_interp_x": (lambda: oracle.interp_x(f3, g3, 0), n3),
+        "reconstruct_transport_change": (
+            lambda: oracle.reconstruct_transport_change(un, d.hu, pk, d.dt, d.halo + 1, d.halo + nyi), nx_ * nyi * nz),
+    }
+    out = {}
+    for name, (fn, pts) in cases.items():
+        fn()
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds_each:
+            fn()
+            n += 1
+        out[name] = pts * n / (time.perf_counter() - t0)
+    return out
+
+
+def run_producers(args, log):
+    import numpy as np
+    import torch
+
+    from getm_amd import inputs
+    from getm_amd.core import Grid
+
+    torch.cuda.set_device(0)
+    d = inputs.coastal(nx=NX, ny=NY, nz=NZ, ntracers=0)
+    grid = Grid.from_domain(d, device="cuda:0")
+    grid.set_field("H", d.H)
+    cases = _producer_cases(d, grid, torch, np)
+    rows = {}
+    total_bytes = total_s = 0.0
+    for name, fn, nbytes, pts in cases:
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(args.steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1) * 1e-3)
+        t = float(np.median(times))
+        rows[name] = {"ms": t * 1e3, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+                      "frac": nbytes / t / 1e9 / HBM_PEAK_GBS, "points_per_s": pts / t}
+        total_bytes += nbytes
+        total_s += t
+        log(f"[bench] {name}: {t * 1e3:.3f} ms, {nbytes / t / 1e9:.0f} GB/s of algorithmic bytes")
+    cpu = None
+    if not args.no_cpu:
+        rates = producers_cpu_baseline()
+        for name, r in rates.items():
+            rows[name]["cpu_points_per_s"] = r
+            rows[name]["gpu_over_cpu"] = rows[name]["points_per_s"] / r
+        cpu = {"value": None, "unit": "points/s per kernel (config.kernels.*.cpu_points_per_s)", "cores": 1,
+               "kind": "port", "sample": "the C restatement of each kernel on a 256x192x40 slab of the coastal "
+                                         "workload, ~1.5 s each, one host core"}
+    out = {
+        "metric": "producer kernels: algorithmic bytes / launch time",
+        "value": total_bytes / total_s / 1e9,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded coastal generator)",
+        "config": {"workload": "kernels either side of the transport step, coastal 1024x768x40, one launch each "
+                               "(median of the timed launches)", "kernels": rows},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["coastal", "multi", "ensemble"], default="coastal",
+    ap.add_argument("--workload", choices=["coastal", "multi", "ensemble", "producers"], default="coastal",
                     help="coastal: 1024x768x40 with T and S, SUPERBEE; multi: same grid, 256 blob tracers, P2_PDM; "
                          "ensemble: 96 uneven domains in one launch per kernel kind, hipGraph chunk")
     ap.add_argument("--domains", type=int, default=96, help="ensemble workload: number of domains")
@@ -338,6 +512,8 @@ def main():
 
     if args.workload == "ensemble":
         return run_ensemble(args, lambda m: print(m, file=sys.stderr, flush=True))
+    if args.workload == "producers":
+        return run_producers(args, lambda m: print(m, file=sys.stderr, flush=True))
 
     import numpy as np
     import torch
