@@ -18,9 +18,11 @@ requests: saturation throughput, then Poisson arrivals at 80% of it, with
 latency percentiles, inventory drift and the error against the FP64 method.
 accuracy (FP32 runs): the first (up to 16) tracers, which evolve independently
 of the others, rerun from the same initial values for --accuracy-steps steps in
-FP64 (the plain method), FP32 as benchmarked, and FP32 with the inventory fixer:
-max |c32 - c64| / max |c64| over wet cells, and the largest per-step inventory
-defect |I_after - I_before| / |I_before| (FP64 compensated sums).
+FP64 (the plain method), FP32 as benchmarked, FP32 with the inventory fixer, and
+FP32 storing anomalies about per-tracer offsets (the same kernels; offsets
+"auto"): max |c32 - c64| / max |c64| over wet cells, and the largest per-step
+inventory defect |I_after - I_before| / |I_before| (FP64 compensated sums, of the
+absolute values).
 cpu_baseline: the reference's Fortran sweeps (oracle/_ref) with the C restatement
 of the diffusion, FP64, on bounded slabs of the same workload, one process per
 host core (the C restatement alone where oracle/_ref is not built).
@@ -80,12 +82,14 @@ def accuracy_check(d, grid, fields, init, scheme, steps, rounding, log):
         e = 0.0
         for t in range(init.shape[0]):
             r = ref.values[t][wet]
-            e = max(e, float((tc.values[t][wet].double() - r).abs().max() / r.abs().max()))
+            e = max(e, float((tc.absolute(t)[wet] - r).abs().max() / r.abs().max()))
         return e
 
-    for name, conserve in (("plain", False), ("with_fixer", True)):
-        tc, dd, _ = precision.run(d, grid, fields, init, scheme, steps, "f32", conserve, rounding)
+    for name, conserve, offsets in (("plain", False, None), ("with_fixer", True, None),
+                                    ("anomalies", False, "auto")):
+        tc, dd, _ = precision.run(d, grid, fields, init, scheme, steps, "f32", conserve, rounding, offsets)
         out[name] = {"max_error_vs_fp64": err(tc), "max_step_inventory_defect": dd}
+    out["anomaly_offsets"] = [float(o) for o in tc.offsets]
         del tc
         torch.cuda.empty_cache()
     log(f"[bench] accuracy over {steps} steps: {out}")

@@ -6,8 +6,10 @@
 Runs the batched step from identical inputs four ways on the GPU: FP64 (the
 plain method, bit-identical to the reference algorithm -- see the GPU parity
 tests); FP32 with FP64 thickness and tracer stores rounded to nearest
-("f32_nearest"), or stochastically ("f32", the default form); and the latter with
-the inventory fixer ("f32_fixed").  Reports,
+("f32_nearest"), or stochastically ("f32", the default form); the latter with
+the inventory fixer ("f32_fixed"); and stochastically rounded anomalies about
+per-tracer offsets (``offsets="auto"``, "f32_offsets"), compared on their
+absolute values and absolute inventories.  Reports,
 per limiter, the error max |c32 - c64| / max |c64| over wet cells (max and
 median over tracers) after the run, and the largest per-step conservation
 defect |sum(area hn c_after) - sum(area ho c_before)| / |sum(area ho c_before)|
@@ -53,21 +55,21 @@ def _blob_tracers(ntr, grid, seed, dtype):
     return out
 
 
-def run(d, grid, fields, init, scheme, steps, precision, conserve, rounding="stochastic"):
+def run(d, grid, fields, init, scheme, steps, precision, conserve, rounding="stochastic", offsets=None):
     """Advance ``steps`` steps; return the collection, the largest per-step
     conservation defect |sum(area hn c_after) - sum(area ho c_before)| / |..|
-    over tracers and steps, and the wall time."""
+    over tracers and steps (of the absolute values), and the wall time."""
     u, v, w, nuh = fields
     tc = TracerCollection(grid, advection_scheme=scheme, cnpar=1.0, precision=precision, conserve=conserve)
     tc.set_rounding(rounding)
-    tc.set_values(init)
+    tc.set_values(init, offsets=offsets)
     ho = grid.ho.all_values
     worst = torch.zeros((), dtype=torch.float64, device=grid.device)
     t0 = time.perf_counter()
     for _ in range(steps):
-        before = tc.inventory_device(ho)
+        before = tc.inventory_device(ho, absolute=True)
         tc.advance(d.dt, u, v, w, nuh)
-        after = tc.inventory_device()
+        after = tc.inventory_device(absolute=True)
         worst = torch.maximum(worst, ((after - before).abs() / before.abs()).max())
     torch.cuda.synchronize()
     return tc, float(worst.item()), time.perf_counter() - t0
@@ -100,13 +102,15 @@ def main(argv=None):
         ref, d64, el64 = run(d, grid, fields, init, scheme, args.steps, "f64", False)
         log(f"[precision] scheme {scheme} f64: {args.steps} steps in {el64:.1f} s, per-step defect {d64:.1e}")
         rows = {"f64": {"max_step_conservation_defect": d64, "seconds": el64}}
-        for name, conserve, rounding in (("f32_nearest", False, "nearest"), ("f32", False, "stochastic"),
-                                         ("f32_fixed", True, "stochastic")):
-            tc, dd, el = run(d, grid, fields, init, scheme, args.steps, "f32", conserve, rounding)
+        for name, conserve, rounding, offsets in (("f32_nearest", False, "nearest", None),
+                                                  ("f32", False, "stochastic", None),
+                                                  ("f32_fixed", True, "stochastic", None),
+                                                  ("f32_offsets", False, "stochastic", "auto")):
+            tc, dd, el = run(d, grid, fields, init, scheme, args.steps, "f32", conserve, rounding, offsets)
             err = []
             for t in range(ntr):
                 r = ref.values[t][wet]
-                err.append(float((tc.values[t][wet].double() - r).abs().max() / r.abs().max()))
+                err.append(float((tc.absolute(t)[wet] - r).abs().max() / r.abs().max()))
             rows[name] = {"max_error": max(err), "median_error": float(np.median(err)),
                           "max_step_conservation_defect": dd, "seconds": el}
             log(f"[precision] scheme {scheme} {name}: {args.steps} steps in {el:.1f} s, max error {max(err):.2e}, "

@@ -8,6 +8,16 @@ C-ABI, which streams all tracers through shared stage coefficients; with
 ``batched=False`` it is the reference's call sequence of per-tracer operators
 (FP64 only).  ``precision="f32"`` stores tracers in FP32 with the thickness
 update kept in FP64.
+
+FP32 anomalies (``set_values(..., offsets=...)``): every FP32 kernel updates a
+tracer by increments formed from differences of its values (limited face values
+shift with the field, the implicit column solve acts on increments), so a step
+of c - o equals the step of c minus o up to rounding.  Storing each tracer as
+its anomaly about an offset o_t near its smallest value makes the stored
+magnitudes, and with them the ulp that every rounding of the stored values
+scales with, several times smaller (salinity 30-35 stored about 30: 8x).  The
+stored array then holds c - o_t; ``absolute`` adds the offsets back in FP64 and
+``inventory_device(absolute=True)`` adds o_t times the water volume.
 """
 from __future__ import annotations
 
@@ -41,6 +51,8 @@ class TracerCollection:
         self._names: List[str] = []
         self._extra: List[tuple] = []  # (source, surface_flux, source_scale, vertical_velocity) per tracer
         self.values = torch.empty((0, grid.nz_, grid.ny_, grid.nx_), dtype=self.dtype, device=grid.device)
+        self.offsets = np.zeros(0)  # FP32 anomalies: stored value = c - offsets[t]
+        self._ones: Optional[torch.Tensor] = None
         self.h = torch.empty((grid.nz_, grid.ny_, grid.nx_), dtype=torch.float64, device=grid.device)
         self.Ah_u: Optional[Array] = None
         self.Ah_v: Optional[Array] = None
@@ -103,17 +115,44 @@ class TracerCollection:
         if values is not None:
             new[-1].copy_(torch.as_tensor(values, device=g.device))
         self.values = new
+        self.offsets = np.append(self.offsets, 0.0)
         self._molecular.append(float(molecular_diffusivity))
         self._names.append(name)
         self._extra.append((source, surface_flux, float(source_scale), vertical_velocity))
         return self[len(self) - 1]
 
-    def set_values(self, values: torch.Tensor, molecular_diffusivity: float = 0.0) -> None:
-        """Replace all tracers by a stacked (T, nz, ny_, nx_) tensor (copied)."""
+    def set_values(self, values: torch.Tensor, molecular_diffusivity: float = 0.0, offsets=None) -> None:
+        """Replace all tracers by a stacked (T, nz, ny_, nx_) tensor (copied).
+
+        ``offsets`` (FP32 only) stores each tracer as its anomaly c - o_t: None
+        stores c itself; "auto" takes o_t = floor of the tracer's smallest value
+        over wet cells (integer offsets keep c - o_t exact for values above o_t);
+        a sequence of T numbers gives them."""
         g = self.grid
         if tuple(values.shape[1:]) != (g.nz_, g.ny_, g.nx_):
             raise ValueError(f"expected (T, {g.nz_}, {g.ny_}, {g.nx_}), got {tuple(values.shape)}")
-        self.values = values.to(device=g.device, dtype=self.dtype).contiguous().clone()
+        ntr = values.shape[0]
+        if offsets is None:
+            off = np.zeros(ntr)
+        else:
+            if self.precision != "f32":
+                raise ValueError("offsets apply to the FP32 form only (the FP64 form is the reference's method)")
+            if isinstance(offsets, str):
+                if offsets != "auto":
+                    raise ValueError('offsets: None, "auto" or one number per tracer')
+                wet = (g.mask.all_values == 1).to(g.device)
+                off = np.array([float(np.floor(values[t].to(g.device)[..., wet].min().item())) if bool(wet.any())
+                                else 0.0 for t in range(ntr)])
+            else:
+                off = np.asarray(offsets, dtype=np.float64).reshape(-1)
+                if off.size != ntr:
+                    raise ValueError(f"expected {ntr} offsets, got {off.size}")
+        src = values.to(device=g.device)
+        if off.any():
+            o = torch.as_tensor(off, dtype=torch.float64, device=g.device).view(-1, 1, 1, 1)
+            src = src.double() - o
+        self.values = src.to(dtype=self.dtype).contiguous().clone()
+        self.offsets = off
         self._molecular = [float(molecular_diffusivity)] * values.shape[0]
         self._names = [""] * values.shape[0]
         self._extra = [(None, None, 1.0, None)] * values.shape[0]
@@ -177,16 +216,35 @@ class TracerCollection:
             _lib.check(_lib.load().transport_conserve(self.p, len(self), _lib.ptr(self.values), _lib.ptr(self._target),
                                                       _lib.stream_handle()), "transport_conserve")
 
-    def inventory_device(self, h: Optional[torch.Tensor] = None) -> torch.Tensor:
+    def inventory_device(self, h: Optional[torch.Tensor] = None, absolute: bool = False) -> torch.Tensor:
         """Per-tracer sum of area * h * c over wet cells (FP64, compensated,
-        deterministic) as a device tensor; h defaults to the T grid's hn."""
+        deterministic) as a device tensor; h defaults to the T grid's hn.  Of the
+        stored values; with ``absolute`` and anomalies stored, of c itself (the
+        stored inventory plus o_t times the water volume sum(area * h))."""
         if not self.batched:
             raise ValueError("inventories are computed by the batched transport object")
         out = torch.empty(len(self), dtype=torch.float64, device=self.grid.device)
         _lib.check(_lib.load().transport_inventory(self.p, len(self), _lib.ptr(self.values),
                                                    _lib.ptr(h) if h is not None else None, _lib.ptr(out),
                                                    _lib.stream_handle()), "transport_inventory")
+        if absolute and self.offsets.any():
+            g = self.grid
+            if self._ones is None:
+                self._ones = torch.ones((1, g.nz_, g.ny_, g.nx_), dtype=self.dtype, device=g.device)
+            vol = torch.empty(1, dtype=torch.float64, device=g.device)
+            _lib.check(_lib.load().transport_inventory(self.p, 1, _lib.ptr(self._ones),
+                                                       _lib.ptr(h) if h is not None else None, _lib.ptr(vol),
+                                                       _lib.stream_handle()), "transport_inventory")
+            out = out + torch.as_tensor(self.offsets, dtype=torch.float64, device=g.device) * vol
         return out
+
+    def absolute(self, i: Optional[int] = None) -> torch.Tensor:
+        """Tracer values c in FP64 (stored value plus offset): tracer i, or all
+        stacked."""
+        if i is None:
+            o = torch.as_tensor(self.offsets, dtype=torch.float64, device=self.grid.device).view(-1, 1, 1, 1)
+            return self.values.double() + o
+        return self.values[i].double() + float(self.offsets[i])
 
     def inventory(self, h: Optional[torch.Tensor] = None):
         """Host copy of inventory_device (simulation.py:1408-1466 report these)."""
@@ -242,7 +300,8 @@ class TracerCollection:
         ctr = self._counter if self._counter is not None else getattr(self, "_own_counter", None)
         return {"values": self.values.detach().cpu().numpy().copy(), "precision": self.precision,
                 "names": list(self._names), "molecular": list(self._molecular),
-                "rounding": self.rounding, "step": int(ctr.item()) if ctr is not None else 0}
+                "rounding": self.rounding, "step": int(ctr.item()) if ctr is not None else 0,
+                "offsets": [float(o) for o in self.offsets]}
 
     def load_state_dict(self, state: dict) -> None:
         """Restore a state_dict; the collection must have the same grid shape and
@@ -251,6 +310,7 @@ class TracerCollection:
         if state["precision"] != self.precision:
             raise ValueError(f"state is {state['precision']}, collection is {self.precision}")
         self.set_values(torch.from_numpy(np.ascontiguousarray(state["values"])))
+        self.offsets = np.asarray(state.get("offsets", np.zeros(len(self))), dtype=np.float64)
         self._names = list(state["names"])
         self._molecular = [float(m) for m in state["molecular"]]
         if self.batched and self.precision == "f32":

@@ -279,20 +279,23 @@ def test_pcr_solver_bound_is_checked():
     tc.set_solver(32)
 
 
-def _ts_run(d, precision, steps, rounding="stochastic", counter=None, conserve=False):
+def _ts_run(d, precision, steps, rounding="stochastic", counter=None, conserve=False, offsets=None,
+            collection=False):
     from getm_amd.core import CENTERS, INTERFACES
     from getm_amd.tracer import TracerCollection
 
     grid = make_grid(d)
     tc = TracerCollection(grid, advection_scheme=5, precision=precision, conserve=conserve)
-    tc.set_values(dev(d.tracers).float())  # both precisions start from the same FP32 values
+    tc.set_values(dev(d.tracers).float(), offsets=offsets)  # both precisions start from the same FP32 values
     if precision == "f32":
         tc.set_rounding(rounding, counter)
     u, v = grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS)
     w, nuh = grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES)
     for _ in range(steps):
         tc.advance(d.dt, u, v, w, nuh)
-    return host(tc.values).astype(np.float64)
+    if collection:
+        return tc, (u, v, w, nuh)
+    return host(tc.absolute())
 
 
 def test_stochastic_rounding_keeps_error_from_accumulating():
@@ -448,3 +451,62 @@ def test_restart_continues_bit_for_bit(tmp_path, precision):
     assert_bitwise(host(second.values), host(ref.values), "restarted run")
     if precision == "f32":
         assert second.state_dict()["step"] == 7
+
+
+def test_f32_anomalies_start_exact_and_track_fp64_more_closely():
+    """Tracers stored as anomalies about integer offsets below their minima start
+    from exactly the given values (c - o is exact in FP32 there) and, over 300
+    steps, stay closer to the FP64 method than the plain FP32 form: the stored
+    magnitudes, and with them the size of every stochastic rounding, are smaller
+    (salinity 30-35 about 30: 8x; temperature 5-20 about 5: up to 2x)."""
+    d = inputs.coastal(nx=128, ny=96, nz=30, seed=11, ntracers=2, tracer_kind="TS")
+    wet = np.broadcast_to(d.tmask == 1, d.ho.shape)
+    tc0, _ = _ts_run(d, "f32", 0, offsets="auto", collection=True)
+    assert tc0.offsets[1] >= 29.0 and tc0.offsets[0] >= 4.0, tc0.offsets
+    assert (host(tc0.values)[:, wet] >= 0).all()
+    start = d.tracers.astype(np.float32).astype(np.float64)
+    assert_bitwise(host(tc0.absolute())[:, wet], start[:, wet], "anomalies at step 0")
+    ref = _ts_run(d, "f64", 300)
+    plain = _ts_run(d, "f32", 300)
+    anom = _ts_run(d, "f32", 300, offsets="auto")
+    for t in range(2):
+        r = ref[t][wet]
+        scale = np.abs(r).max()
+        e_plain = np.abs(plain[t][wet] - r).max() / scale
+        e_anom = np.abs(anom[t][wet] - r).max() / scale
+        assert e_anom <= 1e-5, (t, e_anom, e_plain)
+        print(f"tracer {t}: error plain {e_plain:.2e}, anomalies {e_anom:.2e}")
+        if t == 1:  # salinity: 8x smaller ulp
+            assert e_anom <= 0.5 * e_plain, (t, e_anom, e_plain)
+
+
+def test_f32_anomalies_with_fixer_conserve_absolute_inventories():
+    """With the fixer, the absolute inventories (stored anomalies plus offset
+    times water volume) are kept to 1e-12 per step; a restart keeps the offsets."""
+    d = inputs.coastal(nx=96, ny=80, nz=20, seed=13, ntracers=2, tracer_kind="TS")
+    tc, (u, v, w, nuh) = _ts_run(d, "f32", 0, conserve=True, offsets="auto", collection=True)
+    ho = tc.grid.ho.all_values
+    for _ in range(20):
+        before = tc.inventory_device(ho, absolute=True)
+        tc.advance(d.dt, u, v, w, nuh)
+        after = tc.inventory_device(absolute=True)
+        assert float(((after - before).abs() / before.abs()).max()) <= 1e-12
+    stored = tc.inventory_device()
+    assert float((stored - tc.inventory_device(absolute=True)).abs().min()) > 0  # offsets do count
+    st = tc.state_dict()
+    tc2, _ = _ts_run(d, "f32", 0, conserve=True, collection=True)
+    tc2.load_state_dict(st)
+    assert list(tc2.offsets) == list(tc.offsets)
+    assert_bitwise(host(tc2.absolute()), host(tc.absolute()), "restored anomalies")
+
+
+def test_offsets_are_for_the_f32_form_only():
+    from getm_amd.tracer import TracerCollection
+
+    d = inputs.tiny(nx=12, ny=10, nz=4, seed=3, ntracers=1)
+    tc = TracerCollection(make_grid(d), precision="f64")
+    with pytest.raises(ValueError):
+        tc.set_values(dev(d.tracers), offsets="auto")
+    tc32 = TracerCollection(make_grid(d), precision="f32")
+    with pytest.raises(ValueError):
+        tc32.set_values(dev(d.tracers), offsets=[1.0, 2.0])
