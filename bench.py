@@ -89,7 +89,8 @@ def accuracy_check(d, grid, fields, init, scheme, steps, rounding, log):
                                     ("anomalies", False, "auto")):
         tc, dd, _ = precision.run(d, grid, fields, init, scheme, steps, "f32", conserve, rounding, offsets)
         out[name] = {"max_error_vs_fp64": err(tc), "max_step_inventory_defect": dd}
-    out["anomaly_offsets"] = [float(o) for o in tc.offsets]
+        if offsets is not None:
+            out[name]["offsets"] = [float(o) for o in tc.offsets]
         del tc
         torch.cuda.empty_cache()
     log(f"[bench] accuracy over {steps} steps: {out}")

@@ -97,3 +97,18 @@ def test_depth_update_entries_reject_bad_arguments_without_gpu():
     assert lib.c_update_gvc(5, 4, 3, 1 / 3, a, 10.0, 4, a, a, a, None) == 1
     assert lib.c_update_gvc(5, 4, 3, 1 / 3, None, 10.0, 3, a, a, a, None) == 1
     assert lib.c_update_sigma(5, 0, 3, a, a, a, a, None) == 1
+
+
+def test_scripts_and_modules_compile():
+    """bench.py, the entry points and every package module are valid Python
+    (the driver runs them on the GPU box without a prior import here)."""
+    import glob
+    import os
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    files = [os.path.join(root, "bench.py"), os.path.join(root, "__graft_entry__.py")]
+    files += sorted(glob.glob(os.path.join(root, "getm_amd", "*.py")))
+    files += sorted(glob.glob(os.path.join(root, "oracle", "*.py")))
+    for f in files:
+        with open(f) as fh:
+            compile(fh.read(), f, "exec")
