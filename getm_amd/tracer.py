@@ -75,11 +75,23 @@ class TracerCollection:
                            "transport_set_rounding")
         else:
             self.p = None
-            self._advection = operators.Advection(grid, scheme=advection_scheme)
-            self._advection.ufirst = ufirst
-            self._vertical_diffusion = operators.VerticalDiffusion(grid, cnpar=cnpar)
-            self._wcomb = grid.array(z=INTERFACES, fill=0.0)  # tracer.py:143 _w
-            self._source = grid.array(z=CENTERS, fill=0.0)  # tracer.py:140 _source
+        self._scheme, self._ufirst = advection_scheme, ufirst
+        self._advection = None
+        if not batched:
+            self._sequence_operators()
+
+    def _sequence_operators(self) -> None:
+        """Operators of the reference's per-tracer call sequence (made on first use
+        by an FP64 batched collection whose tracers have sources, surface fluxes
+        or vertical velocities)."""
+        if self._advection is not None:
+            return
+        grid = self.grid
+        self._advection = operators.Advection(grid, scheme=self._scheme)
+        self._advection.ufirst = self._ufirst
+        self._vertical_diffusion = operators.VerticalDiffusion(grid, cnpar=self.cnpar)
+        self._wcomb = grid.array(z=INTERFACES, fill=0.0)  # tracer.py:143 _w
+        self._source = grid.array(z=CENTERS, fill=0.0)  # tracer.py:140 _source
 
     def __del__(self):
         p = getattr(self, "p", None)
@@ -159,8 +171,15 @@ class TracerCollection:
 
     def advance(self, timestep: float, u: Array, v: Array, w: Array, diffusivity: Array) -> None:
         """tracer.py:193-246: advection of all tracers with velocities at t + dt/2,
-        then vertical diffusion with dzo = dzn = hn (tracer.py:248-276)."""
-        if not self.batched:
+        then vertical diffusion with dzo = dzn = hn (tracer.py:248-276).  An FP64
+        collection with tracer sources, surface fluxes or vertical velocities runs
+        the per-tracer sequence (bit-identical to the batched step for the other
+        tracers); the FP32 form has no source terms and rejects them."""
+        extras = any(e[0] is not None or e[1] is not None or e[3] is not None for e in self._extra)
+        if extras and self.precision != "f64":
+            raise ValueError("tracer sources, surface fluxes and vertical velocities need the FP64 form")
+        if not self.batched or extras:
+            self._sequence_operators()
             tracers = [self[i] for i in range(len(self))]
             extra = dict(zip(map(id, tracers), self._extra))
 
@@ -196,8 +215,6 @@ class TracerCollection:
                     self._vertical_diffusion.prepare(diffusivity, timestep, avmol)
                 self._vertical_diffusion.apply(tracer, ea4=self._source if ea4 is not None else None)
             return
-        if any(e[0] is not None or e[1] is not None or e[3] is not None for e in self._extra):
-            raise ValueError("tracer sources, surface fluxes and vertical velocities need batched=False")
         mols = set(self._molecular)
         if len(mols) > 1:
             raise ValueError("the batched step needs one molecular diffusivity for all tracers")

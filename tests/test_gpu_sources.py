@@ -44,8 +44,12 @@ def oracle_advance(oracle_mod, d, tracers, mols, extras):
     return current
 
 
+@pytest.mark.parametrize("batched", [False, True])
 @pytest.mark.parametrize("scheme", [5, 1])
-def test_sources_fluxes_and_sinking_bitwise(oracle_mod, scheme):
+def test_sources_fluxes_and_sinking_bitwise(oracle_mod, scheme, batched):
+    """Per-tracer sources, surface fluxes and vertical velocities against the
+    oracle; an FP64 collection created batched runs the per-tracer sequence for
+    them, with the same bits."""
     from getm_amd.core import CENTERS, INTERFACES
     from getm_amd.tracer import TracerCollection
 
@@ -58,7 +62,7 @@ def test_sources_fluxes_and_sinking_bitwise(oracle_mod, scheme):
     mols = [0.0, 1e-6, 1e-6]
 
     grid = make_grid(d)
-    tc = TracerCollection(grid, advection_scheme=scheme, cnpar=d.cnpar, precision="f64", batched=False)
+    tc = TracerCollection(grid, advection_scheme=scheme, cnpar=d.cnpar, precision="f64", batched=batched)
     tc.add(dev(d.tracers[0]), mols[0], source=grid.wrap(dev(src), CENTERS), surface_flux=grid.wrap(dev(flux)),
            source_scale=0.5)
     tc.add(dev(d.tracers[1]), mols[1], vertical_velocity=grid.wrap(dev(vv), CENTERS))
@@ -73,7 +77,7 @@ def test_sources_fluxes_and_sinking_bitwise(oracle_mod, scheme):
     assert_bitwise(host(tc.h), h, "thickness")
 
 
-def test_batched_step_rejects_sources():
+def test_f32_form_rejects_sources():
     from getm_amd.core import CENTERS
     from getm_amd.tracer import TracerCollection
 
