@@ -512,7 +512,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the step on a single undivided domain; "
-              "multi-GPU sharding is described in DESIGN.md and not implemented.", file=sys.stderr)
+              "sharding over several GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "ensemble":

@@ -101,7 +101,7 @@ def test_depth_update_entries_reject_bad_arguments_without_gpu():
 
 def test_scripts_and_modules_compile():
     """bench.py, the entry points and every package module are valid Python
-    (the driver runs them on the GPU box without a prior import here)."""
+    (GPU runs start them without importing them here first)."""
     import glob
     import os
 

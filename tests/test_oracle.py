@@ -58,7 +58,7 @@ def test_golden_cases_exercise_substepping(oracle_mod, golden):
 @pytest.mark.parametrize("scheme", SCHEMES)
 def test_sweeps_match_reference_live(oracle_mod, seed, scheme):
     if not oracle_mod.ref_available():
-        pytest.skip("oracle/_ref not built (needs /root/reference and amdflang)")
+        pytest.skip("oracle/_ref not built (make -C oracle ref; needs the reference sources and amdflang)")
     d = inputs.tiny(nx=30, ny=22, nz=9, seed=seed, cfl=0.5 if seed == 21 else 0.1, wscale=3.0,
                     mask_codes=seed == 22, ntracers=1)
     ah = np.full_like(d.idxu, 25.0) if seed == 22 else None
