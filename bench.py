@@ -511,7 +511,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build runs the step on a single undivided domain; "
+        print("bench.py measures one GPU: the step runs on a single undivided domain; "
               "sharding over several GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
