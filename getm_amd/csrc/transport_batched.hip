@@ -164,10 +164,12 @@ struct LimF {
             float lim;
             if constexpr (S == SUPERBEE) lim = fmax2(fmin2(2.f * r, 1.f), fmin2(r, 2.f));
             else if constexpr (S == MUSCL) lim = fmin3(2.f * r, 0.5f * (1.f + r), 2.f);
-            else if constexpr (S == SPLMAX13) lim = fmin3(2.f * r, (1.f / 3.f) * fmax2(1.f + 2.f * r, 2.f + r), 2.f);
-            else if constexpr (S == HSIMT) lim = fmin3(2.f * r, fmax2(0.f, l[0] + l[1] * r), 2.f);
-            else lim = fmin3(r * l[2], l[0] + l[1] * r, l[3]);  // P2_PDM
-            const float lim_fu = fu + (lim * ch) * deltaf;
+            else if constexpr (S == SPLMAX13)
+                lim = fmin3(2.f * r, (1.f / 3.f) * fmax2(__builtin_fmaf(2.f, r, 1.f), 2.f + r), 2.f);
+            else if constexpr (S == HSIMT) lim = fmin3(2.f * r, fmax2(0.f, __builtin_fmaf(l[1], r, l[0])), 2.f);
+            else lim = fmin3(r * l[2], __builtin_fmaf(l[1], r, l[0]), l[3]);  // P2_PDM
+            // FP32 only: the contractions are written out (the build keeps FP64 uncontracted)
+            const float lim_fu = __builtin_fmaf(lim * ch, deltaf, fu);
             return deltaf * deltafu > 0.f ? lim_fu : fu;
         }
     }
@@ -251,7 +253,7 @@ __device__ __forceinline__ R cell_update(const CellK<R>& c, double dt, R f, R fl
         const double advn = (fl1 - fl0) * c.iA;
         return (hfo - dt * advn) / c.b;
     } else {
-        return sr.add(f, c.a * f - c.b * (fl1 - fl0), c.rb);
+        return sr.add(f, __builtin_fmaf(c.a, f, -(c.b * (fl1 - fl0))), c.rb);
     }
 }
 
@@ -861,7 +863,7 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
                     for (int b = 0; b < TB; b++)
                         if (b < nb)
                             df[(size_t)b * n3 + (size_t)(k - 1) * n2] =
-                                sr[b].add(f0[b], ca * f0[b] - cb * (fl[b] - fprev[b]), rb);
+                                sr[b].add(f0[b], __builtin_fmaf(ca, f0[b], -(cb * (fl[b] - fprev[b]))), rb);
                 }
 #pragma unroll
                 for (int b = 0; b < TB; b++) {
@@ -1073,7 +1075,7 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
                         const float ck = c[b][k];
                         const float dup = has_up(k) ? c[b][k + 1 < NZMAX ? k + 1 : k] - ck : 0.f;
                         const float ddn = k > 0 ? c[b][k > 0 ? k - 1 : 0] - ck : 0.f;
-                        float qk = ek * (dup + q_up[b]) - rk * ddn;
+                        float qk = __builtin_fmaf(ek, dup + q_up[b], -(rk * ddn));
                         if (!CN1) qk += e3 * dup + e4 * ddn;
                         q[b][k] = qk;
                         q_up[b] = qk;
@@ -1099,7 +1101,7 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
                     const unsigned rb = sr_cell_bits(ss, (unsigned)((size_t)k * n2 + p));
 #pragma unroll
                     for (int b = 0; b < NT; b++) {
-                        d[b] = q[b][k] - r * d[b];
+                        d[b] = __builtin_fmaf(-r, d[b], q[b][k]);
                         c[b][k] = sr[b].add(c[b][k], d[b], rb);
                     }
                 }
