@@ -278,12 +278,14 @@ __device__ __forceinline__ SRound tracer_round(unsigned stage_seed, int tracer)
 // in LDS; otherwise (one layer) they are loaded straight into registers.
 // HDIFF: horizontal diffusion fluxes (Ah given).  A separate instantiation: the
 // branch alone, even never taken, holds ~45 VGPRs of its operands.
-template <int S, typename R, int D1, int TY, bool GROUP, bool HDIFF>
+template <int S, typename R, int D1, int TY, bool GROUP, bool HDIFF, int TXW = TX>
 __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0, int y0, int k0, int k1, int ntr,
                                            const R* __restrict__ fin, R* __restrict__ fout,
                                            const double* __restrict__ hin, double* __restrict__ hout,
                                            unsigned sr_stage, int tg0)
 {
+    // tile width of this instantiation (the file-wide 64 unless given)
+    constexpr int TX = TXW, HX = TXW + 4;
     constexpr int HY = TY + 4;       // input tile height incl. halo 2
     constexpr int NT = HX * HY;      // cells of the input tile
     constexpr int NTHR = TX * TY;    // one lane per output cell
@@ -688,8 +690,8 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
 // Every tracer-batch block writes the same thickness.
 // FP32: at most 128 VGPRs (four waves per SIMD), two 512-lane blocks per CU at TY = 8.
 // KG: layers per block (1 or HPAIR_KG); HDIFF: horizontal diffusion (A.ah_u set).
-template <int S, typename R, int D1, int TY, int KG, bool HDIFF>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
+template <int S, typename R, int D1, int TY, int KG, bool HDIFF, int TXW>
+__global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
                                                    int nty, int nbatch, const R* __restrict__ fin_all,
                                                    R* __restrict__ fout_all, const double* __restrict__ hin,
                                                    double* __restrict__ hout, unsigned sr_stage)
@@ -707,7 +709,7 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(si
     const int tx = v % ntx, ty = v / ntx;
     const int tb0 = tbatch * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
-    hpair_body<S, R, D1, TY, (KG > 1), HDIFF>(A, dt, tx * TX, ty * TY, k0, k1, ntr, fin_all + (size_t)tb0 * A.n3,
+    hpair_body<S, R, D1, TY, (KG > 1), HDIFF, TXW>(A, dt, tx * TXW, ty * TY, k0, k1, ntr, fin_all + (size_t)tb0 * A.n3,
                              fout_all + (size_t)tb0 * A.n3, hin, hout, sr_stage, tb0);
 }
 
@@ -1332,40 +1334,63 @@ int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int n
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
     if (ty != 4 && ty != 8 && ty != 16) return GETM_ERR_ARG;
-    const int ntx = (A.nx_ + TX - 1) / TX, nty = (A.ny_ + ty - 1) / ty;
+    // Tile shape.  Stages 4-5 sweep x first, so their first stage covers the
+    // tile's rows plus two on each side: 32-wide tiles of twice the height
+    // (same lane count) cut that overhead from (ty + 4)/ty to (2 ty + 4)/(2 ty).
+    // That pays where the per-layer set-up dominates, i.e. for few tracers per
+    // block (coastal, 2 tracers: 0.768 -> 0.732 ms); with many, the narrower
+    // rows cost more in memory access than they save (256 tracers: 35.5 -> 40.3
+    // ms).  Stages 1-2 sweep y first and keep 64-wide rows; so does the FP64 form
+    // (measured for FP32 only).
+    const bool narrow = std::is_same<R, float>::value && d1 == 1 && ty <= 8 && tpb <= HPAIR_KG;
+    const int txw = narrow ? 32 : TX;
+    const int tye = narrow ? 2 * ty : ty;
+    const int ntx = (A.nx_ + txw - 1) / txw, nty = (A.ny_ + tye - 1) / tye;
     const bool ah = A.ah_u != nullptr;
     const int kg = ah ? 1 : hpair_group(tpb, std::is_same<R, float>::value);
     const long nitems = (long)nb * ((A.nz + kg - 1) / kg) * ntx * nty;
     if (nitems > 0x7fffff00L) return GETM_ERR_CAPACITY;
     dim3 grid((unsigned)(8 * ((nitems + 7) / 8)));
     ProfileScope ps(d1 == 2 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
-#define GETM_HPK(TYV, KGV, AHV)                                                                                  \
+#define GETM_HPK(D1V, TYV, TXV, KGV, AHV)                                                                         \
     GETM_SCHEME_SWITCH(scheme, {                                                                                    \
-        if (d1 == 2)                                                                                                \
-            hipLaunchKernelGGL((k_hpair<S_, R, 2, TYV, KGV, AHV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, \
-                               nty, nb, fin, fout, hin, hout, sr_stage);                                            \
-        else                                                                                                        \
-            hipLaunchKernelGGL((k_hpair<S_, R, 1, TYV, KGV, AHV>), grid, dim3(TX * TYV), 0, s, A, dt, ntr, tpb, ntx, \
-                               nty, nb, fin, fout, hin, hout, sr_stage);                                            \
+        hipLaunchKernelGGL((k_hpair<S_, R, D1V, TYV, KGV, AHV, TXV>), grid, dim3(TXV * TYV), 0, s, A, dt, ntr, tpb,  \
+                           ntx, nty, nb, fin, fout, hin, hout, sr_stage);                                           \
     })
-#define GETM_HP(TYV)                                  \
-    if (ah) {                                         \
-        GETM_HPK(TYV, 1, true);                       \
+#define GETM_HP(D1V, TYV, TXV)                            \
+    if (ah) {                                             \
+        GETM_HPK(D1V, TYV, TXV, 1, true);                 \
     } else if constexpr (std::is_same<R, float>::value) { \
-        if (kg == HPAIR_KG) {                         \
-            GETM_HPK(TYV, HPAIR_KG, false);           \
-        } else {                                      \
-            GETM_HPK(TYV, 1, false);                  \
-        }                                             \
-    } else {                                          \
-        GETM_HPK(TYV, 1, false);                      \
+        if (kg == HPAIR_KG) {                             \
+            GETM_HPK(D1V, TYV, TXV, HPAIR_KG, false);     \
+        } else {                                          \
+            GETM_HPK(D1V, TYV, TXV, 1, false);            \
+        }                                                 \
+    } else {                                              \
+        GETM_HPK(D1V, TYV, TXV, 1, false);                \
     }
-    if (ty == 16) {
-        GETM_HP(16);
-    } else if (ty == 8) {
-        GETM_HP(8);
+    if (d1 == 2) {
+        if (ty == 16) {
+            GETM_HP(2, 16, TX);
+        } else if (ty == 8) {
+            GETM_HP(2, 8, TX);
+        } else {
+            GETM_HP(2, 4, TX);
+        }
+    } else if (narrow) {
+        if (ty == 8) {
+            GETM_HP(1, 16, 32);
+        } else {
+            GETM_HP(1, 8, 32);
+        }
     } else {
-        GETM_HP(4);
+        if (ty == 16) {
+            GETM_HP(1, 16, TX);
+        } else if (ty == 8) {
+            GETM_HP(1, 8, TX);
+        } else {
+            GETM_HP(1, 4, TX);
+        }
     }
 #undef GETM_HP
 #undef GETM_HPK
