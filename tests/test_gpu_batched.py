@@ -138,12 +138,15 @@ def test_batched_f64_ufirst(oracle_mod):
     assert_bitwise(h, hh, "thickness")
 
 
+@pytest.mark.parametrize("ntracers", [7, 10])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_batched_launch_shape_does_not_change_results(precision):
-    """Tracer batches spread over the grid (transport_set_tuning) give identical bits."""
-    d = inputs.tiny(nx=37, ny=30, nz=9, seed=91, ntracers=7, cfl=0.5, wscale=2.0)
+def test_batched_launch_shape_does_not_change_results(precision, ntracers):
+    """Tracer batches spread over the grid (transport_set_tuning) give identical
+    bits, and so do the tile shapes they select (FP32 stages 4-5: 32-wide tiles
+    for up to 8 tracers per block, 64-wide beyond; tile heights 4, 8, 16)."""
+    d = inputs.tiny(nx=37, ny=30, nz=9, seed=91, ntracers=ntracers, cfl=0.5, wscale=2.0)
     outs = []
-    for tuning in [(0, 0, 0, 16), (1, 1, 1, 8), (2, 3, 5, 4), (4, 16, 16, 16)]:
+    for tuning in [(0, 0, 0, 16), (1, 1, 1, 8), (2, 3, 5, 4), (4, 16, 16, 16), (0, 0, 0, 8), (10, 4, 4, 4)]:
         from getm_amd.core import CENTERS, INTERFACES
         from getm_amd.tracer import TracerCollection
 
