@@ -142,8 +142,7 @@ class DepthUpdate:
 
         # total water depth on T, raw and clipped (simulation.py:1530-1535)
         elevation2depth(z_T, T.H, -1000.0, T.D)
-        torch.maximum(T.D.all_values, torch.tensor(self.Dmin, dtype=torch.float64, device=T.device),
-                      out=T.Dclip.all_values)
+        torch.clamp(T.D.all_values, min=self.Dmin, out=T.Dclip.all_values)  # np.maximum(D, Dmin)
         # elevations half a step behind T, then depths on U, V and X (:1537-1555)
         torch.add(zo_T.all_values, z_T.all_values, out=z_T_half.all_values)
         z_T_half.all_values.mul_(0.5)
