@@ -355,11 +355,16 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     const int d1 = tr->ufirst ? 1 : 2;  // first direction of stages 1-2 (operators.py:118-121)
     const int d2 = 3 - d1;
     const int ty = tr->tile_y ? tr->tile_y : auto_tile_y(ntr);
+    // FP32: the w stage leaves the thickness of the columns it does not update in hA
+    // (stages 1-2 wrote every cell there), and stages 4-5 read those cells from hA
+    // instead of a copy in hB; into hA itself they store updated cells only.
+    constexpr bool f32 = std::is_same<R, float>::value;
     if ((rc = launch_hpair<R>(d1, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, SR_PAIR1, s))) return rc;
     if ((rc = launch_wcol<R>(tr->scheme, A, dt, ntr, tr->tpb_w, S, f, hA, hB, w, w, static_cast<int*>(tr->stats.ptr),
-                             s)))
+                             s, !f32)))
         return rc;
-    if ((rc = launch_hpair<R>(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, SR_PAIR2, s)))
+    if ((rc = launch_hpair<R>(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, SR_PAIR2, s,
+                              f32 ? hA : hB, !(f32 && hfinal == hA))))
         return rc;
     if constexpr (std::is_same<R, float>::value) {
         if (t->nz >= 2 && t->nz <= tr->pcr_max_nz)

@@ -175,8 +175,11 @@ inline int auto_tile_y(int tracers_per_domain)
 // ---- batched kernels (transport_batched.hip) ----
 // tpb = tracers per block (tracer batches go into the grid); <= 0 means all
 template <typename R>
+// hin_dry: thickness of the cells the stages do not update (NULL: hin); write_dry:
+// also store those cells' thickness into hout (false only where hout already holds it)
 int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
-                 const double* hin, double* hout, unsigned sr_stage, hipStream_t s);
+                 const double* hin, double* hout, unsigned sr_stage, hipStream_t s, const double* hin_dry = nullptr,
+                 bool write_dry = true);
 int launch_hpair_multi(int stage, int d1, int scheme, int ty, int kg, const EnsDom<float>* doms, const int4* work,
                        int nwork, hipStream_t s);
 int launch_wcol_multi(int scheme, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
@@ -187,8 +190,9 @@ int launch_vdiff_pcr(const StepArgs& A, double dt, double cnpar, double molecula
                      const float* fin, float* fout, hipStream_t s);
 int launch_vdiff_pcr_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
 template <typename R>
+// copy_dry: copy the thickness of columns that are not updated from hin to hout
 int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout, const double* hin,
-                double* hout, const double* w, const double* wv, int* stats, hipStream_t s);
+                double* hout, const double* w, const double* wv, int* stats, hipStream_t s, bool copy_dry = true);
 int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
                      const float* fin, float* fout, hipStream_t s);
 // advance the step counter of the stochastic rounding (single counter, or every domain's)

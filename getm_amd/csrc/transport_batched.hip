@@ -282,7 +282,7 @@ template <int S, typename R, int D1, int TY, bool GROUP, bool HDIFF, int TXW = T
 __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0, int y0, int k0, int k1, int ntr,
                                            const R* __restrict__ fin, R* __restrict__ fout,
                                            const double* __restrict__ hin, double* __restrict__ hout,
-                                           unsigned sr_stage, int tg0)
+                                           unsigned sr_stage, int tg0, const double* hin_dry, bool write_dry)
 {
     // tile width of this instantiation (the file-wide 64 unless given)
     constexpr int TX = TXW, HX = TXW + 4;
@@ -463,6 +463,9 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
                                  : -1;
             }
         }
+        // bit m: region A cell m is updated (its thickness comes from hin; the
+        // others' from hin_dry, which holds the same values where nothing updates them)
+        unsigned wet_a = 0;
 #pragma unroll
         for (int m = 0; m < NPA; m++) {
             const int c = tid + m * NTHR;
@@ -472,9 +475,12 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             if (c < NA) {
                 const int xl = AX0 + c % AW, yl = AY0 + c / AW;
                 if constexpr (GROUP) {
-                    if (s_cf[(yl + 2) * HX + (xl + 2)] & 1) coa[m] = mul24(y0 + yl, A.nx_) + x0 + xl;
+                    const unsigned char cf = s_cf[(yl + 2) * HX + (xl + 2)];
+                    if (cf & 1) coa[m] = mul24(y0 + yl, A.nx_) + x0 + xl;
+                    if (cf & 2) wet_a |= 1u << m;
                 } else {
                     coa[m] = cell2d(x0 + xl, y0 + yl, rcfa[m], ria_a[m]);
+                    if (rcfa[m] & 2) wet_a |= 1u << m;
                 }
             }
         }
@@ -522,7 +528,8 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             }
         }
 #pragma unroll
-        for (int m = 0; m < NPA; m++) h0a[m] = coa[m] >= 0 ? at32(hin + lay, 8u * (unsigned)coa[m]) : 0.0;
+        for (int m = 0; m < NPA; m++)
+            h0a[m] = coa[m] >= 0 ? at32(((wet_a >> m) & 1u ? hin : hin_dry) + lay, 8u * (unsigned)coa[m]) : 0.0;
         if (ntr > 0) load_tile(0);
 
         // ---- stage A faces: transports to LDS for the thickness update
@@ -600,7 +607,7 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
                 double h2 = h1;
                 if (upd) h2 = h1 - (dt * (s_q[f1] - s_q[f1 - (D2 == 1 ? 1 : FBW)])) * ia;
                 cell_set<R>(cb[m], upd, h1, h2, ia, dt);
-                at32(hout + lay, 8u * (unsigned)ooff[m]) = h2;
+                if (upd || write_dry) at32(hout + lay, 8u * (unsigned)ooff[m]) = h2;
             }
         }
 
@@ -694,7 +701,8 @@ template <int S, typename R, int D1, int TY, int KG, bool HDIFF, int TXW>
 __global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
                                                    int nty, int nbatch, const R* __restrict__ fin_all,
                                                    R* __restrict__ fout_all, const double* __restrict__ hin,
-                                                   double* __restrict__ hout, unsigned sr_stage)
+                                                   double* __restrict__ hout, unsigned sr_stage,
+                                                   const double* hin_dry, int write_dry)
 {
     const unsigned b = blockIdx.x, per = gridDim.x / 8;
     unsigned v = (b % 8) * per + b / 8;
@@ -710,7 +718,7 @@ __global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(s
     const int tb0 = tbatch * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
     hpair_body<S, R, D1, TY, (KG > 1), HDIFF, TXW>(A, dt, tx * TXW, ty * TY, k0, k1, ntr, fin_all + (size_t)tb0 * A.n3,
-                             fout_all + (size_t)tb0 * A.n3, hin, hout, sr_stage, tb0);
+                             fout_all + (size_t)tb0 * A.n3, hin, hout, sr_stage, tb0, hin_dry, write_dry != 0);
 }
 
 // Ensemble: one work item per block {domain, x0, y0, first layer of the group | batch << 16}; the
@@ -730,7 +738,8 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(si
     const size_t off = (size_t)tb0 * D.A.n3;
     hpair_body<S, R, D1, TY, (KG > 1), false>(D.A, 0.5 * D.dt, wk.y, wk.z, k0, k1, ntr, D.tracers + off,
                                               D.scratch + off,
-                             STAGE == 1 ? D.ho : D.hB, D.hA, STAGE == 1 ? SR_PAIR1 : SR_PAIR2, tb0);
+                             STAGE == 1 ? D.ho : D.hB, D.hA, STAGE == 1 ? SR_PAIR1 : SR_PAIR2, tb0,
+                             STAGE == 1 ? D.ho : D.hB, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -749,6 +758,7 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(si
 // ---------------------------------------------------------------------------
 template <int S, typename R, int TB>
 __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, int j, int tb0, int tb1, bool first,
+                                          bool copy_dry,
                                           const R* __restrict__ fin, R* __restrict__ fout,
                                           const double* __restrict__ hin, double* __restrict__ hout,
                                           const double* __restrict__ w, const double* __restrict__ wv,
@@ -760,7 +770,8 @@ __device__ __forceinline__ void wcol_body(const StepArgs& A, double dt, int i, i
     const size_t p = (size_t)j * A.nx_ + i;
     const bool inside = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax;
     if (!inside || A.tmask[p] != 1) {
-        if (first)
+        // copy_dry = false: the next stage reads these columns' thickness from hin
+        if (first && copy_dry)
             for (int k = 0; k < nz; k++) hout[(size_t)k * n2 + p] = hin[(size_t)k * n2 + p];
         // FP32: the tracers of columns that are not updated keep their values in fout
         // (no stage writes them); FP64 copies them (its step ends with a full copy)
@@ -936,12 +947,12 @@ template <int S, typename R, int TB>
 __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int ntr, int tpb, const R* __restrict__ fin,
                                               R* __restrict__ fout, const double* __restrict__ hin,
                                               double* __restrict__ hout, const double* __restrict__ w,
-                                              const double* __restrict__ wv, int* __restrict__ stats)
+                                              const double* __restrict__ wv, int* __restrict__ stats, int copy_dry)
 {
     const int tb0 = blockIdx.z * tpb;
     const int tb1 = tb0 + tpb < ntr ? tb0 + tpb : ntr;
     wcol_body<S, R, TB>(A, dt, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y, tb0, tb1,
-                        blockIdx.z == 0, fin, fout, hin, hout, w, wv, stats);
+                        blockIdx.z == 0, copy_dry != 0, fin, fout, hin, hout, w, wv, stats);
 }
 
 // Ensemble: work item {domain, bx, by, batch}; scratch -> tracers, hA -> hB.
@@ -952,7 +963,7 @@ __global__ void __launch_bounds__(256) k_wcol_multi(const EnsDom<R>* __restrict_
     const EnsDom<R>& D = doms[wk.x];
     const int tb0 = wk.w * D.tpb_w;
     const int tb1 = tb0 + D.tpb_w < D.ntr ? tb0 + D.tpb_w : D.ntr;
-    wcol_body<S, R, TB>(D.A, D.dt, wk.y * 64 + threadIdx.x, wk.z * 4 + threadIdx.y, tb0, tb1, wk.w == 0, D.scratch,
+    wcol_body<S, R, TB>(D.A, D.dt, wk.y * 64 + threadIdx.x, wk.z * 4 + threadIdx.y, tb0, tb1, wk.w == 0, true, D.scratch,
                         D.tracers, D.hA, D.hB, D.w, D.w, D.stats);
 }
 
@@ -1329,8 +1340,10 @@ int launch_count_steps_multi(const EnsDom<float>* doms, int ndom, hipStream_t s)
 
 template <typename R>
 int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
-                 const double* hin, double* hout, unsigned sr_stage, hipStream_t s)
+                 const double* hin, double* hout, unsigned sr_stage, hipStream_t s, const double* hin_dry,
+                 bool write_dry)
 {
+    if (!hin_dry) hin_dry = hin;
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
     if (ty != 4 && ty != 8 && ty != 16) return GETM_ERR_ARG;
@@ -1355,7 +1368,7 @@ int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int n
 #define GETM_HPK(D1V, TYV, TXV, KGV, AHV)                                                                         \
     GETM_SCHEME_SWITCH(scheme, {                                                                                    \
         hipLaunchKernelGGL((k_hpair<S_, R, D1V, TYV, KGV, AHV, TXV>), grid, dim3(TXV * TYV), 0, s, A, dt, ntr, tpb,  \
-                           ntx, nty, nb, fin, fout, hin, hout, sr_stage);                                           \
+                           ntx, nty, nb, fin, fout, hin, hout, sr_stage, hin_dry, (int)write_dry);                  \
     })
 #define GETM_HP(D1V, TYV, TXV)                            \
     if (ah) {                                             \
@@ -1400,7 +1413,7 @@ int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int n
 
 template <typename R>
 int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout, const double* hin,
-                double* hout, const double* w, const double* wv, int* stats, hipStream_t s)
+                double* hout, const double* w, const double* wv, int* stats, hipStream_t s, bool copy_dry)
 {
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
@@ -1413,17 +1426,17 @@ int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, cons
     if constexpr (std::is_same<R, double>::value) {
         GETM_SCHEME_SWITCH(scheme, {
             hipLaunchKernelGGL((k_wcol<S_, R, 2>), grid, block, 0, s, A, dt, ntr, tpb, fin, fout, hin, hout, w, wv,
-                               stats);
+                               stats, (int)copy_dry);
         });
     } else if (tpb > 4) {
         GETM_SCHEME_SWITCH(scheme, {
             hipLaunchKernelGGL((k_wcol<S_, R, 8>), grid, block, 0, s, A, dt, ntr, tpb, fin, fout, hin, hout, w, wv,
-                               stats);
+                               stats, (int)copy_dry);
         });
     } else {
         GETM_SCHEME_SWITCH(scheme, {
             hipLaunchKernelGGL((k_wcol<S_, R, 4>), grid, block, 0, s, A, dt, ntr, tpb, fin, fout, hin, hout, w, wv,
-                               stats);
+                               stats, (int)copy_dry);
         });
     }
     GETM_HIP_OK(hipGetLastError());
@@ -1502,9 +1515,9 @@ int launch_vdiff_pcr(const StepArgs& A, double dt, double cnpar, double molecula
 }
 
 template int launch_hpair<float>(int, int, int, const StepArgs&, double, int, int, const float*, float*,
-                                 const double*, double*, unsigned, hipStream_t);
+                                 const double*, double*, unsigned, hipStream_t, const double*, bool);
 template int launch_hpair<double>(int, int, int, const StepArgs&, double, int, int, const double*, double*,
-                                  const double*, double*, unsigned, hipStream_t);
+                                  const double*, double*, unsigned, hipStream_t, const double*, bool);
 
 // ---------------------------------------------------------------------------
 // ensemble launchers: one launch per kernel kind covers every domain
@@ -1585,9 +1598,9 @@ int launch_vdiff_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4*
     return GETM_OK;
 }
 template int launch_wcol<float>(int, const StepArgs&, double, int, int, const float*, float*, const double*, double*,
-                                const double*, const double*, int*, hipStream_t);
+                                const double*, const double*, int*, hipStream_t, bool);
 template int launch_wcol<double>(int, const StepArgs&, double, int, int, const double*, double*, const double*,
-                                 double*, const double*, const double*, int*, hipStream_t);
+                                 double*, const double*, const double*, int*, hipStream_t, bool);
 
 int launch_vdiff_pcr_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s)
 {

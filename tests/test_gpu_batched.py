@@ -513,3 +513,32 @@ def test_offsets_are_for_the_f32_form_only():
     tc32 = TracerCollection(make_grid(d), precision="f32")
     with pytest.raises(ValueError):
         tc32.set_values(dev(d.tracers), offsets=[1.0, 2.0])
+
+
+@pytest.mark.parametrize("tpb", [0, 1, 16])
+def test_f32_step_without_thickness_output_equals_with_it(tpb):
+    """With h_out NULL the FP32 step keeps the final thickness in its own buffer,
+    where the cells that no stage updates already hold theirs (they are neither
+    copied by the w stage nor stored again); the tracers equal those of a step
+    that writes the thickness out, bit for bit, over several steps."""
+    from getm_amd import _lib
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.tracer import TracerCollection
+
+    d = inputs.tiny(nx=45, ny=38, nz=12, seed=71, ntracers=5, cfl=0.6, wscale=2.0)
+    outs = []
+    for with_h in (True, False):
+        grid = make_grid(d)
+        tc = TracerCollection(grid, advection_scheme=d.scheme, cnpar=d.cnpar, precision="f32")
+        tc.set_tuning(tpb, tpb, tpb, 0)
+        tc.set_values(dev(d.tracers))
+        u, v = grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS)
+        w, nuh = grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES)
+        for _ in range(3):
+            rc = _lib.load().transport_step(tc.p, len(tc), _lib.ptr(tc.values), _lib.ptr(u.all_values),
+                                            _lib.ptr(v.all_values), _lib.ptr(w.all_values),
+                                            _lib.ptr(nuh.all_values), None, None, float(d.dt), 0.0,
+                                            _lib.ptr(tc.h) if with_h else None, _lib.stream_handle())
+            _lib.check(rc, "transport_step")
+        outs.append(host(tc.values))
+    assert_bitwise(outs[1], outs[0], "tracers without h_out")
