@@ -739,7 +739,9 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(si
     hpair_body<S, R, D1, TY, (KG > 1), false>(D.A, 0.5 * D.dt, wk.y, wk.z, k0, k1, ntr, D.tracers + off,
                                               D.scratch + off,
                              STAGE == 1 ? D.ho : D.hB, D.hA, STAGE == 1 ? SR_PAIR1 : SR_PAIR2, tb0,
-                             STAGE == 1 ? D.ho : D.hB, true);
+                             // stages 4-5 end in hA, where stages 1-2 left the cells
+                             // that nothing updates (the w stage does not copy them)
+                             STAGE == 1 ? D.ho : D.hA, STAGE == 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -955,7 +957,7 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
                         blockIdx.z == 0, copy_dry != 0, fin, fout, hin, hout, w, wv, stats);
 }
 
-// Ensemble: work item {domain, bx, by, batch}; scratch -> tracers, hA -> hB.
+// Ensemble: work item {domain, bx, by, batch}; scratch -> tracers, hA -> hB (updated columns only).
 template <int S, typename R, int TB>
 __global__ void __launch_bounds__(256) k_wcol_multi(const EnsDom<R>* __restrict__ doms, const int4* __restrict__ work)
 {
@@ -963,7 +965,7 @@ __global__ void __launch_bounds__(256) k_wcol_multi(const EnsDom<R>* __restrict_
     const EnsDom<R>& D = doms[wk.x];
     const int tb0 = wk.w * D.tpb_w;
     const int tb1 = tb0 + D.tpb_w < D.ntr ? tb0 + D.tpb_w : D.ntr;
-    wcol_body<S, R, TB>(D.A, D.dt, wk.y * 64 + threadIdx.x, wk.z * 4 + threadIdx.y, tb0, tb1, wk.w == 0, true, D.scratch,
+    wcol_body<S, R, TB>(D.A, D.dt, wk.y * 64 + threadIdx.x, wk.z * 4 + threadIdx.y, tb0, tb1, wk.w == 0, false, D.scratch,
                         D.tracers, D.hA, D.hB, D.w, D.w, D.stats);
 }
 
