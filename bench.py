@@ -100,7 +100,7 @@ def accuracy_check(d, grid, fields, init, scheme, steps, rounding, log):
 def serving_run(nreq: int, log, policy: str = "uniform"):
     """Serving loop on a mix of dye-release requests (sizes 64-384, 10-60 layers,
     1-64 tracers, 50-200 steps, SUPERBEE or HSIMT): saturation, FP64 check, 80% load.
-    Policy "uniform" (every request completes in 8..16 ticks whatever its size)."""
+    Policy "uniform" (every request completes in at most 8 ticks, about 8, whatever its size)."""
     from getm_amd import serve
 
     reqs = serve.draw_requests(nreq, seed=0, size_range=(64, 384), nz_range=(10, 60), tracer_range=(1, 64),

@@ -125,14 +125,15 @@ class _Slot:
         self.req = req
         # steps per loop tick, a property of the request alone: 1 (lockstep); with a
         # work quantum Q the power of two nearest Q / (updates per step), 1 .. 64
-        # (fair); with a horizon H the power of two >= steps / H, so that every
-        # request completes in H/2 .. H ticks whatever its size (uniform)
+        # (fair); with a horizon H, ceil(steps / H), so that every request completes
+        # in at most H ticks, and in about H, whatever its size (uniform)
         self.cost = dom.n_wet_columns * dom.nz * req.ntracers
         self.k = 1
         if quantum:
             self.k = int(min(64, max(1, 2 ** round(math.log2(max(quantum / max(self.cost, 1), 1e-9))))))
         if horizon:
-            self.k = int(2 ** math.ceil(math.log2(max(1, math.ceil(req.nsteps / horizon)))))
+            self.k = max(1, math.ceil(req.nsteps / horizon))
+        self.exact = bool(horizon)
         self.grid = Grid.from_domain(dom, device=device)
         g = self.grid
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
@@ -167,8 +168,10 @@ class _Slot:
         return float(((now - self.target).abs() / self.target.abs()).max().item())
 
     def steps_this_tick(self) -> int:
-        """k, or the largest power of two not above the steps left."""
+        """k, or (uniform) the steps left, or the largest power of two not above them."""
         left = self.req.nsteps - self.steps
+        if self.exact:
+            return max(1, min(self.k, left))
         kk = self.k
         while kk > left:
             kk //= 2
@@ -180,16 +183,16 @@ class Server:
     after every step.  policy "fair": every tick each request advances about
     ``quantum`` tracer cell-updates (k = 1 .. 64 steps, by its size), so a small
     request is not held to the pace of the largest one in the mix.  policy
-    "uniform": every request advances k >= steps / ``horizon`` steps per tick (a
-    power of two), so each completes in horizon/2 .. horizon ticks whatever its
-    size: latency no longer follows request size (the tail narrows), at the cost
-    of small requests' latency.  Under "fair" and "uniform" a request's inventory
+    "uniform": every request advances k = ceil(steps / ``horizon``) steps per tick
+    (its last tick the steps left), so each completes in at most ``horizon`` ticks,
+    and in about that many, whatever its size: latency no longer follows request
+    size (the tail narrows), at the cost of small requests' latency.  Under "fair" and "uniform" a request's inventory
     is restored once per tick.  Under every policy a request's result depends only
     on the request itself, never on the mix."""
 
     def __init__(self, hbm_budget: float = 250e9, max_slots: int = 64, conserve: bool = True,
                  keep_results: bool = False, tuning: Optional[Sequence[int]] = None, device=None,
-                 policy: str = "lockstep", quantum: float = 2.0 ** 25, horizon: int = 16):
+                 policy: str = "lockstep", quantum: float = 2.0 ** 25, horizon: int = 8):
         if policy not in ("fair", "lockstep", "uniform"):
             raise ValueError("policy must be 'fair', 'lockstep' or 'uniform'")
         if policy == "uniform" and horizon < 1:
@@ -436,7 +439,7 @@ def main(argv=None):
     ap.add_argument("--no-verify", action="store_true", help="skip the FP64 check of the saturation run")
     ap.add_argument("--no-conserve", action="store_true", help="no inventory fixer (raw FP32 steps)")
     ap.add_argument("--policy", choices=["fair", "lockstep", "uniform"], default="lockstep")
-    ap.add_argument("--horizon", type=int, default=16, help="ticks per request under the uniform policy")
+    ap.add_argument("--horizon", type=int, default=8, help="ticks per request under the uniform policy")
     ap.add_argument("--quantum", type=float, default=2.0 ** 25,
                     help="tracer cell-updates per request per tick under the fair policy")
     ap.add_argument("--seed", type=int, default=0)

@@ -128,9 +128,9 @@ def test_fair_policy_mix_equals_alone_bitwise(domains):
 
 
 def test_uniform_policy_mix_equals_alone_bitwise(domains):
-    """Uniform completion time: every request takes a power of two >= steps/horizon
-    steps per tick, so each completes within horizon/2 .. horizon ticks of its
-    admission whatever its step count; results still do not depend on the mix."""
+    """Uniform completion time: every request takes ceil(steps/horizon) steps per
+    tick, so each completes within horizon ticks of its admission whatever its
+    step count; results still do not depend on the mix."""
     horizon = 4
     kw = dict(policy="uniform", horizon=horizon)
     solo = run_alone(domains, **kw)
