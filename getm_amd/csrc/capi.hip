@@ -367,10 +367,10 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
                               f32 ? hA : hB, !(f32 && hfinal == hA))))
         return rc;
     if constexpr (std::is_same<R, float>::value) {
-        if (t->nz >= 2 && t->nz <= tr->pcr_max_nz)
+        if (t->nz >= 2 && t->nz <= tr->pcr_max_nz && !tr->ea4)
             rc = launch_vdiff_pcr(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
         else
-            rc = launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
+            rc = launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s, tr->ea4);
         if (rc) return rc;
         return launch_count_step(const_cast<unsigned long long*>(A.rstep), s);
     } else {
@@ -481,6 +481,15 @@ int transport_set_rounding(void* p, int mode, unsigned long long* step_counter)
     if (!tr || (mode != GETM_ROUND_NEAREST && mode != GETM_ROUND_STOCHASTIC)) return GETM_ERR_ARG;
     tr->rounding = mode;
     tr->rstep_ext = step_counter;
+    return GETM_OK;
+}
+
+int transport_set_sources(void* p, const double* ea4)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr) return GETM_ERR_ARG;
+    if (ea4 && (tr->precision != GETM_F32 || tr->cnpar != 1.0)) return GETM_ERR_ARG;
+    tr->ea4 = ea4;
     return GETM_OK;
 }
 

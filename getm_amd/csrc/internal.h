@@ -137,6 +137,7 @@ struct Transport {
     int rounding = 1;                        // FP32 stores: 1 stochastic (default), 0 nearest
     DeviceBuffer rstep_own;                  // the object's own step counter (zeroed at creation)
     unsigned long long* rstep_ext = nullptr; // caller's counter (transport_set_rounding), else rstep_own
+    const double* ea4 = nullptr;             // FP32 tracer sources (transport_set_sources), NULL: none
 };
 
 // Ensemble of independent domains stepped by one launch per kernel kind
@@ -194,7 +195,8 @@ template <typename R>
 int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout, const double* hin,
                 double* hout, const double* w, const double* wv, int* stats, hipStream_t s, bool copy_dry = true);
 int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
-                     const float* fin, float* fout, hipStream_t s);
+                     const float* fin, float* fout, hipStream_t s,
+                     const double* ea4 = nullptr);
 // advance the step counter of the stochastic rounding (single counter, or every domain's)
 int launch_count_step(unsigned long long* counter, hipStream_t s);
 int launch_count_steps_multi(const EnsDom<float>* doms, int ndom, hipStream_t s);

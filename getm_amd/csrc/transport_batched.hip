@@ -995,10 +995,16 @@ __global__ void __launch_bounds__(256) k_wcol_multi(const EnsDom<R>* __restrict_
 // The old column and the eliminated q_k are both kept in registers; columns of
 // nz <= 16 interleave two tracers, deeper ones take one tracer at a time.
 // tg0: index in the domain of the first tracer of fin/fout.
-template <int NZMAX, bool CN1>
+// SRC: tracer sources ea4 (ntr x nz levels, FP64, the reference's ea4 = source *
+// dt * scale with the surface flux in the top layer; cnpar = 1 only).  The
+// reference solves (dz + K) c_new = dz c + ea4 (diffusion.F90:233-313); with
+// s = ea4 / dz that is the solve from c + s, so the differences are taken of
+// c + s and the stored value is c + (s + d), rounded once.
+template <int NZMAX, bool CN1, bool SRC = false>
 __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double cnpar, double molecular,
                                            const double* __restrict__ nuh, int i, int j, int lane, int ntr,
-                                           const float* __restrict__ fin, float* __restrict__ fout, int tg0)
+                                           const float* __restrict__ fin, float* __restrict__ fout, int tg0,
+                                           const double* __restrict__ ea4 = nullptr)
 {
     extern __shared__ float s_coef[];  // [ncoef][nz][64]
     constexpr int NT = NZMAX <= 16 ? 2 : 1;
@@ -1058,6 +1064,7 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
     for (int t = 0; t < ntr; t += NT) {
         const int nt = ntr - t < NT ? ntr - t : NT;
         float c[NT][NZMAX], q[NT][NZMAX];
+        float s[SRC ? NT : 1][SRC ? NZMAX : 1];
 #pragma unroll
         for (int b = 0; b < NT; b++) {
             // nz = 1: the column's value is in fin if the stages updated it, else in fout
@@ -1065,12 +1072,28 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
 #pragma unroll
             for (int k = 0; k < NZMAX; k++)
                 if (has(k)) c[b][k] = fi[(size_t)k * n2];
+            if constexpr (SRC) {
+                const double* e = ea4 + (size_t)(t + (b < nt ? b : 0)) * n3 + p;
+#pragma unroll
+                for (int k = 0; k < NZMAX; k++)
+                    if (has(k)) s[b][k] = float(e[(size_t)k * n2] / A.hn[(size_t)k * n2 + p]);
+            }
         }
+        // value whose differences drive the solve: c, or c + s with sources
+        auto cv = [&](int b, int k) -> float {
+            if constexpr (SRC) return c[b][k] + s[b][k];
+            else return c[b][k];
+        };
         if (nz == 1) {
             // diffusion.F90:224-231: every point, halos and land included
             const double dz = A.hn[p];
 #pragma unroll
-            for (int b = 0; b < NT; b++) c[b][0] = float((dz * (double)c[b][0]) / dz);
+            for (int b = 0; b < NT; b++) {
+                if constexpr (SRC)  // diffusion.F90:224-231 with ea4
+                    c[b][0] = float((dz * (double)c[b][0] + ea4[(size_t)(t + (b < nt ? b : 0)) * n3 + p]) / dz);
+                else
+                    c[b][0] = float((dz * (double)c[b][0]) / dz);
+            }
         } else if (solve) {
             float q_up[NT];
 #pragma unroll
@@ -1087,9 +1110,9 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
                     }
 #pragma unroll
                     for (int b = 0; b < NT; b++) {
-                        const float ck = c[b][k];
-                        const float dup = has_up(k) ? c[b][k + 1 < NZMAX ? k + 1 : k] - ck : 0.f;
-                        const float ddn = k > 0 ? c[b][k > 0 ? k - 1 : 0] - ck : 0.f;
+                        const float ck = cv(b, k);
+                        const float dup = has_up(k) ? cv(b, k + 1 < NZMAX ? k + 1 : k) - ck : 0.f;
+                        const float ddn = k > 0 ? cv(b, k > 0 ? k - 1 : 0) - ck : 0.f;
                         float qk = __builtin_fmaf(ek, dup + q_up[b], -(rk * ddn));
                         if (!CN1) qk += e3 * dup + e4 * ddn;
                         q[b][k] = qk;
@@ -1107,7 +1130,8 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
             for (int b = 0; b < NT; b++) {
                 sr[b] = tracer_round(ss, tg0 + t + b);
                 d[b] = q[b][0];
-                c[b][0] = sr[b].add(c[b][0], d[b], rb0);
+                if constexpr (SRC) c[b][0] = sr[b].add(c[b][0], s[b][0] + d[b], rb0);
+                else c[b][0] = sr[b].add(c[b][0], d[b], rb0);
             }
 #pragma unroll
             for (int k = 1; k < NZMAX; k++) {
@@ -1117,7 +1141,8 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
 #pragma unroll
                     for (int b = 0; b < NT; b++) {
                         d[b] = __builtin_fmaf(-r, d[b], q[b][k]);
-                        c[b][k] = sr[b].add(c[b][k], d[b], rb);
+                        if constexpr (SRC) c[b][k] = sr[b].add(c[b][k], s[b][k] + d[b], rb);
+                        else c[b][k] = sr[b].add(c[b][k], d[b], rb);
                     }
                 }
             }
@@ -1132,15 +1157,17 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
             }
     }
 }
-template <int NZMAX, bool CN1>
+template <int NZMAX, bool CN1, bool SRC>
 __global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, double cnpar, double molecular,
                                                   const double* __restrict__ nuh, int ntr_all, int tpb,
-                                                  const float* __restrict__ fin_all, float* __restrict__ fout_all)
+                                                  const float* __restrict__ fin_all, float* __restrict__ fout_all,
+                                                  const double* __restrict__ ea4_all)
 {
     const int tb0 = blockIdx.z * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
-    vdiff_body<NZMAX, CN1>(A, dt, cnpar, molecular, nuh, blockIdx.x * 64 + threadIdx.x, blockIdx.y, threadIdx.x, ntr,
-                           fin_all + (size_t)tb0 * A.n3, fout_all + (size_t)tb0 * A.n3, tb0);
+    vdiff_body<NZMAX, CN1, SRC>(A, dt, cnpar, molecular, nuh, blockIdx.x * 64 + threadIdx.x, blockIdx.y, threadIdx.x,
+                                ntr, fin_all + (size_t)tb0 * A.n3, fout_all + (size_t)tb0 * A.n3, tb0,
+                                SRC ? ea4_all + (size_t)tb0 * A.n3 : nullptr);
 }
 
 // Ensemble: work item {domain, bx, row, batch}; scratch -> tracers.
@@ -1446,9 +1473,10 @@ int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, cons
 }
 
 int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
-                     const float* fin, float* fout, hipStream_t s)
+                     const float* fin, float* fout, hipStream_t s, const double* ea4)
 {
     if (A.nz > 64 || A.nz < 1) return GETM_ERR_CAPACITY;
+    if (ea4 && cnpar != 1.0) return GETM_ERR_ARG;  // sources: the cnpar = 1 form only
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
     if (nb > 65535) return GETM_ERR_CAPACITY;
@@ -1458,12 +1486,15 @@ int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecula
     ProfileScope ps(GETM_K_VDIFF_F32, s);
 #define GETM_VD(NZM)                                                                                             \
     do {                                                                                                         \
-        if (cn1)                                                                                                 \
-            hipLaunchKernelGGL((k_vdiff_f32<NZM, true>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, ntr, \
-                               tpb, fin, fout);                                                                  \
+        if (ea4)                                                                                                 \
+            hipLaunchKernelGGL((k_vdiff_f32<NZM, true, true>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, \
+                               ntr, tpb, fin, fout, ea4);                                                        \
+        else if (cn1)                                                                                            \
+            hipLaunchKernelGGL((k_vdiff_f32<NZM, true, false>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, \
+                               ntr, tpb, fin, fout, nullptr);                                                    \
         else                                                                                                     \
-            hipLaunchKernelGGL((k_vdiff_f32<NZM, false>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, ntr, \
-                               tpb, fin, fout);                                                                  \
+            hipLaunchKernelGGL((k_vdiff_f32<NZM, false, false>), grid, dim3(64), shm, s, A, dt, cnpar, molecular,   \
+                               nuh, ntr, tpb, fin, fout, nullptr);                                               \
     } while (0)
     // depth class: the kernel is unrolled for 8 (cls + 1) layers and needs nz > 8 cls
     switch ((A.nz - 1) / 8) {

@@ -113,8 +113,9 @@ class TracerCollection:
         ``source`` (layers), ``surface_flux`` (2-D), ``source_scale`` and
         ``vertical_velocity`` (layers; movement through the water, e.g. sinking)
         have the meaning of pygetm.tracer.Tracer's (tracer.py:35-95) and are
-        applied by :meth:`advance` as TracerCollection.advance does; they are
-        supported by the per-tracer operator sequence (``batched=False``)."""
+        applied by :meth:`advance` as TracerCollection.advance does: in FP64 by the
+        per-tracer operator sequence, in FP32 (sources and surface fluxes, cnpar =
+        1) by the batched step's column solve."""
         g = self.grid
         if source is not None and (source.grid is not g or source.z != CENTERS):
             raise ValueError("source must be a layer field of the tracer's grid")
@@ -174,11 +175,22 @@ class TracerCollection:
         then vertical diffusion with dzo = dzn = hn (tracer.py:248-276).  An FP64
         collection with tracer sources, surface fluxes or vertical velocities runs
         the per-tracer sequence (bit-identical to the batched step for the other
-        tracers); the FP32 form has no source terms and rejects them."""
-        extras = any(e[0] is not None or e[1] is not None or e[3] is not None for e in self._extra)
-        if extras and self.precision != "f64":
-            raise ValueError("tracer sources, surface fluxes and vertical velocities need the FP64 form")
-        if not self.batched or extras:
+        tracers); the FP32 form takes sources and surface fluxes in its column
+        solve (cnpar = 1) and rejects per-tracer vertical velocities."""
+        vv_extra = any(e[3] is not None for e in self._extra)
+        src_extra = any(e[0] is not None or e[1] is not None for e in self._extra)
+        if self.precision == "f32":
+            if vv_extra:
+                raise ValueError("per-tracer vertical velocities need the FP64 form")
+            if src_extra and self.cnpar != 1.0:
+                raise ValueError("tracer sources in the FP32 form need cnpar = 1")
+            if src_extra and self.conserve:
+                raise ValueError("the inventory fixer restores pre-step inventories; with sources call "
+                                 "transport_conserve with targets that include them")
+            sequence = False
+        else:
+            sequence = vv_extra or src_extra
+        if not self.batched or sequence:
             self._sequence_operators()
             tracers = [self[i] for i in range(len(self))]
             extra = dict(zip(map(id, tracers), self._extra))
@@ -222,16 +234,44 @@ class TracerCollection:
         if self.conserve and len(self):
             # closed domain without sources: sum(area ho c) before = sum(area hn c) after
             self._target = self.inventory_device(self.grid.ho.all_values)
-        rc = _lib.load().transport_step(
-            self.p, len(self), _lib.ptr(self.values), _lib.ptr(u.all_values), _lib.ptr(v.all_values),
-            _lib.ptr(w.all_values), _lib.ptr(diffusivity.all_values),
-            _lib.ptr(self.Ah_u.all_values) if self.Ah_u is not None else None,
-            _lib.ptr(self.Ah_v.all_values) if self.Ah_v is not None else None, float(timestep), mol,
-            _lib.ptr(self.h), _lib.stream_handle())
-        _lib.check(rc, "transport_step")
+        ea4 = self._fp32_sources(timestep) if src_extra else None
+        if ea4 is not None:
+            _lib.check(_lib.load().transport_set_sources(self.p, _lib.ptr(ea4)), "transport_set_sources")
+        try:
+            rc = _lib.load().transport_step(
+                self.p, len(self), _lib.ptr(self.values), _lib.ptr(u.all_values), _lib.ptr(v.all_values),
+                _lib.ptr(w.all_values), _lib.ptr(diffusivity.all_values),
+                _lib.ptr(self.Ah_u.all_values) if self.Ah_u is not None else None,
+                _lib.ptr(self.Ah_v.all_values) if self.Ah_v is not None else None, float(timestep), mol,
+                _lib.ptr(self.h), _lib.stream_handle())
+            _lib.check(rc, "transport_step")
+        finally:
+            if ea4 is not None:
+                _lib.load().transport_set_sources(self.p, None)
         if self.conserve and len(self):
             _lib.check(_lib.load().transport_conserve(self.p, len(self), _lib.ptr(self.values), _lib.ptr(self._target),
                                                       _lib.stream_handle()), "transport_conserve")
+
+    def _fp32_sources(self, timestep: float) -> torch.Tensor:
+        """The ea4 terms of every tracer stacked (FP64; zero for tracers without
+        sources), formed as the reference's _diffuse does (tracer.py:263-272)."""
+        g = self.grid
+        shape = (len(self), g.nz_, g.ny_, g.nx_)
+        if getattr(self, "_ea4", None) is None or tuple(self._ea4.shape) != shape:
+            self._ea4 = torch.zeros(shape, dtype=torch.float64, device=g.device)
+        for t, (source, flux, scale, _) in enumerate(self._extra):
+            e = self._ea4[t]
+            if source is None and flux is None:
+                e.zero_()
+                continue
+            if source is not None:
+                e.copy_(source.all_values)
+            else:
+                e.zero_()
+            if flux is not None:
+                e[-1] += flux.all_values
+            e *= timestep * scale
+        return self._ea4
 
     def inventory_device(self, h: Optional[torch.Tensor] = None, absolute: bool = False) -> torch.Tensor:
         """Per-tracer sum of area * h * c over wet cells (FP64, compensated,

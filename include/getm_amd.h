@@ -182,6 +182,15 @@ int transport_set_solver(void* tr, int pcr_max_nz);
  * by passing the same counter.  No effect on GETM_F64. */
 int transport_set_rounding(void* tr, int mode, unsigned long long* step_counter);
 
+/* Tracer sources of the FP32 form's next steps, the counterpart of the ea4 term
+ * that TracerCollection._diffuse passes to the diffusion (python/pygetm/tracer.py:
+ * 263-272, diffusion.F90:269-270): a device array of ntracers x nz x ny_ x nx_
+ * FP64 values, (source + surface flux in the top layer) * dt * source_scale per
+ * tracer, read by every transport_step until replaced; NULL clears it.  FP32
+ * transports with cnpar = 1 only (GETM_ERR_ARG otherwise); the column solve is
+ * then the per-lane elimination whatever transport_set_solver chose. */
+int transport_set_sources(void* tr, const double* ea4);
+
 /* Tracer inventories (the reference's domain integrals of tracers,
  * simulation.py:1408-1466): inventory[t] = sum over wet cells of
  * area * h * c for each of ntracers stacked tracers, accumulated in FP64

@@ -77,13 +77,77 @@ def test_sources_fluxes_and_sinking_bitwise(oracle_mod, scheme, batched):
     assert_bitwise(host(tc.h), h, "thickness")
 
 
-def test_f32_form_rejects_sources():
+def test_f32_form_rejects_vertical_velocities():
     from getm_amd.core import CENTERS
     from getm_amd.tracer import TracerCollection
 
     d = inputs.tiny(nx=12, ny=10, nz=4, seed=3, ntracers=1)
     grid = make_grid(d)
     tc = TracerCollection(grid, precision="f32")
-    tc.add(dev(d.tracers[0]), source=grid.wrap(dev(np.zeros_like(d.ho)), CENTERS))
+    tc.add(dev(d.tracers[0]), vertical_velocity=grid.wrap(dev(np.zeros_like(d.ho)), CENTERS))
     with pytest.raises(ValueError):
         tc.advance(d.dt, None, None, None, None)
+
+
+@pytest.mark.parametrize("nz", [1, 12, 30])
+def test_f32_sources_and_fluxes_follow_fp64(oracle_mod, nz):
+    """The FP32 form's sources and surface fluxes (solved as the diffusion of
+    c + ea4/dz, cnpar = 1) against the FP64 per-tracer sequence on the smooth
+    coastal fields (T- and S-like values): within 1e-5 of the tracer's range
+    over 5 steps, for a tracer with sources and a surface flux, one with a flux
+    only and one with neither."""
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.tracer import TracerCollection
+
+    d = inputs.coastal(nx=96, ny=80, nz=nz, seed=13, ntracers=3, tracer_kind="TS")
+    rng = np.random.default_rng(nz)
+    src = rng.standard_normal(d.ho.shape) * 1e-5
+    flux = rng.standard_normal(d.ho.shape[1:]) * 1e-4
+    wet = np.broadcast_to(d.tmask == 1, d.ho.shape)
+    out = {}
+    for precision in ("f64", "f32"):
+        grid = make_grid(d)
+        tc = TracerCollection(grid, advection_scheme=5, cnpar=1.0, precision=precision)
+        tc.add(dev(d.tracers[0]), 1e-6, source=grid.wrap(dev(src), CENTERS), surface_flux=grid.wrap(dev(flux)),
+               source_scale=0.5)
+        tc.add(dev(d.tracers[1]), 1e-6, surface_flux=grid.wrap(dev(flux)))
+        tc.add(dev(d.tracers[2]), 1e-6)
+        args = (grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS),
+                grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES))
+        for _ in range(5):
+            tc.advance(d.dt, *args)
+        out[precision] = host(tc.absolute())
+    for t in range(3):
+        r = out["f64"][t][wet]
+        err = np.abs(out["f32"][t][wet] - r).max() / np.abs(r).max()
+        assert err <= 1e-5, (t, err)
+    # the sources did change the first two tracers
+    plain = single_f64_without_sources(d)
+    assert np.abs(out["f64"][0][wet] - plain[0][wet]).max() > 1e-6 * np.abs(plain[0][wet]).max()
+
+
+def single_f64_without_sources(d):
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.tracer import TracerCollection
+
+    grid = make_grid(d)
+    tc = TracerCollection(grid, advection_scheme=5, cnpar=1.0, precision="f64")
+    tc.set_values(dev(d.tracers), molecular_diffusivity=1e-6)
+    args = (grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS),
+            grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES))
+    for _ in range(5):
+        tc.advance(d.dt, *args)
+    return host(tc.values)
+
+
+def test_f32_sources_need_cnpar_one_and_no_fixer():
+    from getm_amd.core import CENTERS
+    from getm_amd.tracer import TracerCollection
+
+    d = inputs.tiny(nx=12, ny=10, nz=4, seed=3, ntracers=1)
+    grid = make_grid(d)
+    for kw in (dict(cnpar=0.5), dict(cnpar=1.0, conserve=True)):
+        tc = TracerCollection(grid, precision="f32", **kw)
+        tc.add(dev(d.tracers[0]), source=grid.wrap(dev(np.zeros_like(d.ho)), CENTERS))
+        with pytest.raises(ValueError):
+            tc.advance(d.dt, None, None, None, None)

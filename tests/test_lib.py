@@ -112,3 +112,8 @@ def test_scripts_and_modules_compile():
     for f in files:
         with open(f) as fh:
             compile(fh.read(), f, "exec")
+
+
+def test_transport_set_sources_rejects_without_gpu():
+    lib = _lib.load()
+    assert lib.transport_set_sources(None, None) == 1
