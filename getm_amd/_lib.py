@@ -49,6 +49,7 @@ _SIGNATURES = {
     "c_vertical_advection_to_sources": (_i, [_i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "c_elevation2depth": (_i, [_i, _vp, _vp, _d, _vp, _vp, _vp]),
     "transport_set_sources": (_i, [_vp, _vp]),
+    "c_velocities_from_transports": (_i, [ctypes.c_longlong, _vp, _vp, _vp, _vp]),
     "c_alpha": (_i, [_i, _vp, _d, _d, _vp, _vp, _vp]),
     "c_update_gvc": (_i, [_i, _i, _i, _d, _vp, _d, _i, _vp, _vp, _vp, _vp]),
     "c_update_sigma": (_i, [_i, _i, _i, _vp, _vp, _vp, _vp, _vp]),

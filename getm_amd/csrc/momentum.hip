@@ -34,6 +34,16 @@ __global__ void __launch_bounds__(256) k_w_momentum_3d(int nx, int ny, int nz, i
     }
 }
 
+// uk = pk / hn over every point, numpy's divide in Momentum.update_diagnostics
+// (python/pygetm/momentum.py:885-892); IEEE division, so the same bits as numpy's,
+// zero thicknesses giving inf or NaN alike.
+__global__ void __launch_bounds__(256) k_divide(long long n, const double* __restrict__ a,
+                                                const double* __restrict__ b, double* __restrict__ out)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = a[p] / b[p];
+}
+
 // un = (un * hn - Uo) / dt as python/src/momentum.F90:323-340 computes it (idt
 // = 1/dt once, then a product), over all i and rows j0 .. j0 + nj - 1 of every
 // layer; blockIdx.x = (layer, row) pair, blockIdx.y a 256-wide chunk of i.
@@ -88,6 +98,18 @@ extern "C" int c_reconstruct_transport_change(int nx, int ny, int nz, int imin, 
     const double idt = 1.0 / timestep;
     hipLaunchKernelGGL(k_reconstruct_transport_change, dim3((unsigned)rows, (nx + 255) / 256), dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), nx, ny, j0, j1 - j0 + 1, un, hn, Uo, idt);
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+extern "C" int c_velocities_from_transports(long long n, const double* pk, const double* hn, double* uk, void* stream)
+{
+    if (n < 0 || (n > 0 && (!pk || !hn || !uk))) return GETM_ERR_ARG;
+    if (n == 0) return GETM_OK;
+    const long long nblk = (n + 255) / 256;
+    if (nblk > 0x7fffffffLL) return GETM_ERR_ARG;
+    hipLaunchKernelGGL(k_divide, dim3((unsigned)nblk), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), n, pk, hn,
+                       uk);
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }

@@ -47,9 +47,18 @@ def w_momentum_3d(pk: Array, qk: Array, timestep: float, w: Array) -> None:
 
 
 def velocities_from_transports(pk: Array, qk: Array, uk: Array, vk: Array) -> None:
-    """uk = pk / hn (U grid), vk = qk / hn (V grid), momentum.py:885-892."""
-    torch.div(pk.all_values, pk.grid.hn.all_values, out=uk.all_values)
-    torch.div(qk.all_values, qk.grid.hn.all_values, out=vk.all_values)
+    """uk = pk / hn (U grid), vk = qk / hn (V grid) over every point,
+    momentum.py:885-892 (C-ABI ``c_velocities_from_transports``)."""
+    for p, u in ((pk, uk), (qk, vk)):
+        hn = p.grid.hn.all_values
+        for name, t in (("transport", p.all_values), ("hn", hn), ("velocity", u.all_values)):
+            if t.dtype != torch.float64 or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError(f"{name}: expected a contiguous float64 device tensor")
+        if not (p.all_values.shape == hn.shape == u.all_values.shape):
+            raise ValueError("transport, thickness and velocity must have one shape")
+        rc = _lib.load().c_velocities_from_transports(p.all_values.numel(), _lib.ptr(p.all_values), _lib.ptr(hn),
+                                                      _lib.ptr(u.all_values), _lib.stream_handle())
+        _lib.check(rc, "c_velocities_from_transports")
 
 
 def reconstruct_transport_change(un: Array, hn: torch.Tensor, Uo: Array, timestep: float) -> None:

@@ -353,6 +353,12 @@ int grid_interp_xy(int nx1, int ny1, int nx2, int ny2, int nz, const double* sou
 int c_reconstruct_transport_change(int nx, int ny, int nz, int imin, int imax, int jmin, int jmax, double* un,
                                    const double* hn, const double* Uo, double timestep, void* stream);
 
+/* Velocities from layer transports, uk = pk / hn over n points: the numpy
+ * division of Momentum.update_diagnostics (python/pygetm/momentum.py:885-892),
+ * which has no compiled counterpart in the reference.  IEEE division: the bits
+ * of numpy's result, zero thicknesses included. */
+int c_velocities_from_transports(long long n, const double* pk, const double* hn, double* uk, void* stream);
+
 /* No reference counterpart (the reference accumulates cpu_time around its
  * loops, diffusion.F90:161-311).  When enabled, every kernel launch of this
  * library is bracketed by HIP events recorded on the launch stream;

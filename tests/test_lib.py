@@ -117,3 +117,10 @@ def test_scripts_and_modules_compile():
 def test_transport_set_sources_rejects_without_gpu():
     lib = _lib.load()
     assert lib.transport_set_sources(None, None) == 1
+
+
+def test_velocities_from_transports_rejects_without_gpu():
+    lib = _lib.load()
+    assert lib.c_velocities_from_transports(-1, None, None, None, None) == 1
+    assert lib.c_velocities_from_transports(8, None, None, None, None) == 1
+    assert lib.c_velocities_from_transports(0, None, None, None, None) == 0
