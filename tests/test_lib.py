@@ -124,3 +124,23 @@ def test_velocities_from_transports_rejects_without_gpu():
     assert lib.c_velocities_from_transports(-1, None, None, None, None) == 1
     assert lib.c_velocities_from_transports(8, None, None, None, None) == 1
     assert lib.c_velocities_from_transports(0, None, None, None, None) == 0
+
+
+def test_header_and_ctypes_signatures_agree():
+    """Every entry point declared in include/getm_amd.h is bound in getm_amd._lib
+    with as many arguments as the declaration has parameters."""
+    import os
+    import re
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(root, "include", "getm_amd.h")) as fh:
+        text = re.sub(r"/\*.*?\*/", "", fh.read(), flags=re.S)
+    decls = {}
+    for m in re.finditer(r"(?:^|\n)\s*(?:const\s+)?(?:int|void|double|long)\s*\**\s*([a-zA-Z_]\w*)\s*\(([^)]*)\)\s*;",
+                         text):
+        params = m.group(2).strip()
+        decls[m.group(1)] = 0 if params in ("", "void") else len(params.split(","))
+    sigs = _lib._SIGNATURES
+    assert set(decls) == set(sigs), (sorted(set(decls) ^ set(sigs)))
+    for name, (_, args) in sigs.items():
+        assert decls[name] == len(args), (name, decls[name], len(args))
