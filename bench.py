@@ -29,9 +29,9 @@ host core (the C restatement alone where oracle/_ref is not built).
 
 --workload producers times the kernels on either side of the step at the same
 coastal size (one launch each, HIP events on the launch stream, median of
---steps launches): w from continuity, sinking into sources, GVC layer
-thicknesses, layer coordinates, a grid interpolation and the transport change
-of the velocity self-advection.  Each is reported against its algorithmic bytes
+--steps launches): w from continuity, velocities from transports, sinking into
+sources, GVC layer thicknesses, layer coordinates, a grid interpolation and the
+transport change of the velocity self-advection.  Each is reported against its algorithmic bytes
 and the HBM peak, beside the C restatement timed on one host core on a slab.
 """
 from __future__ import annotations
@@ -349,6 +349,7 @@ def _producer_cases(d, grid, torch, np):
     un = grid.ugrid.wrap(dev(d.u), CENTERS)
     Uo = grid.ugrid.wrap(dev(d.u * d.hu), CENTERS)
     hu = dev(d.hu)
+    uk, vk = grid.ugrid.array(CENTERS), grid.vgrid.array(CENTERS)
     interp = lambda: grid.wrap(f3, CENTERS).interp(grid.ugrid.wrap(g3, CENTERS))
     return [
         ("w_momentum_3d", lambda: momentum.w_momentum_3d(pk, qk, d.dt, w),
@@ -366,6 +367,8 @@ def _producer_cases(d, grid, torch, np):
         ("grid_interp_x", interp, 16 * n3, n3),
         ("reconstruct_transport_change", lambda: momentum.reconstruct_transport_change(un, hu, Uo, d.dt),
          32 * nx_ * nyi * nz, nx_ * nyi * nz),
+        ("velocities_from_transports", lambda: momentum.velocities_from_transports(pk, qk, uk, vk),
+         2 * 24 * n3, 2 * n3),
     ]
 
 
@@ -408,6 +411,8 @@ def producers_cpu_baseline(seconds_each: float = 1.5):
         "grid_interp_x": (lambda: oracle.interp_x(f3, g3, 0), n3),
         "reconstruct_transport_change": (
             lambda: oracle.reconstruct_transport_change(un, d.hu, pk, d.dt, d.halo + 1, d.halo + nyi), nx_ * nyi * nz),
+        # numpy's divide, as the reference itself computes it
+        "velocities_from_transports": (lambda: (np.divide(pk, d.hu, out=g3), np.divide(qk, d.hv, out=g3)), 2 * n3),
     }
     out = {}
     for name, (fn, pts) in cases.items():
