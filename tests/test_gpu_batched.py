@@ -69,6 +69,44 @@ def test_batched_f64_bitwise(oracle_mod, case, scheme):
         assert mi > 1
 
 
+# degenerate and ragged domains: no water at all, no land at all, one row, one
+# column, one cell of one layer (diffusion.F90:224-231 then divides the whole
+# array, land included, so land cells turn to NaN), and a row wider than one
+# 64-cell tile with a one-cell tail
+EDGE_CASES = {
+    "all_land": dict(nx=20, ny=17, nz=5, land_fraction=1.0),
+    "all_water": dict(nx=19, ny=16, nz=6, land_fraction=-1.0, cfl=0.3),
+    "one_row": dict(nx=40, ny=1, nz=3, cfl=0.5),
+    "one_column": dict(nx=1, ny=37, nz=4, cfl=0.5),
+    "one_cell": dict(nx=1, ny=1, nz=1, land_fraction=-1.0),
+    "ragged": dict(nx=65, ny=3, nz=2, cfl=0.7, wscale=2.0),
+}
+
+
+@pytest.mark.parametrize("case", list(EDGE_CASES))
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_batched_f64_edge_domains_bitwise(oracle_mod, case, scheme):
+    d = inputs.tiny(seed=60 + scheme, scheme=scheme, ntracers=3, **EDGE_CASES[case])
+    got, h, _ = run_batched(d, "f64", nsteps=2)
+    ref, href = run_oracle(oracle_mod, d, nsteps=2)
+    assert_bitwise(got, ref, "tracers")
+    assert_bitwise(h, href, "advected thickness")
+    if case == "all_land":
+        assert_bitwise(got, d.tracers, "tracers of a domain without water")
+
+
+@pytest.mark.parametrize("case", list(EDGE_CASES))
+def test_batched_f32_edge_domains(oracle_mod, case):
+    kw = dict(EDGE_CASES[case], cfl=0.05, wscale=0.2)
+    d = inputs.tiny(seed=70, scheme=5, ntracers=3, **kw)
+    got, h, _ = run_batched(d, "f32", nsteps=2)
+    ref, href = run_oracle(oracle_mod, d, nsteps=2)
+    assert_bitwise(h, href, "FP64 thickness in the FP32 form")
+    fin = np.isfinite(ref)
+    np.testing.assert_array_equal(np.isfinite(got), fin)
+    assert np.abs(got[fin] - ref[fin]).max() / np.abs(ref[fin]).max() <= F32_TOL
+
+
 def test_batched_f64_matches_per_tracer_operators():
     """The batched step equals the reference's per-tracer call sequence (batched=False)."""
     d = inputs.seamount(nx=48, ny=40, nz=12, ntracers=2)
