@@ -59,6 +59,20 @@ def test_ensemble_equals_single_domains(scheme):
         assert_bitwise(host(ens.tracers[i]), single_domain(d, 3), f"domain {i}")
 
 
+def test_ensemble_with_degenerate_domains_equals_single_domains():
+    """Domains without water, without land, of one row, one column or one cell of
+    one layer, mixed with an ordinary one in a single ensemble."""
+    shapes = [dict(nx=20, ny=17, nz=5, land_fraction=1.0), dict(nx=19, ny=16, nz=6, land_fraction=-1.0),
+              dict(nx=40, ny=1, nz=3), dict(nx=1, ny=37, nz=4), dict(nx=1, ny=1, nz=1, land_fraction=-1.0),
+              dict(nx=65, ny=3, nz=2), dict(nx=48, ny=40, nz=12)]
+    doms = [inputs.tiny(seed=80 + i, scheme=5, ntracers=1 + i % 3, cfl=0.3, **kw) for i, kw in enumerate(shapes)]
+    ens = make_ensemble(doms)
+    for _ in range(2):
+        ens.step()
+    for i, d in enumerate(doms):
+        assert_bitwise(host(ens.tracers[i]), single_domain(d, 2), f"domain {i}")
+
+
 def test_ensemble_launch_shape_independent():
     doms = small_ensemble(5, seed=9)
     outs = []
