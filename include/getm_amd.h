@@ -52,7 +52,9 @@ enum {
 /* ---- grids ------------------------------------------------------------- */
 
 /* Replaces create_grid (python/src/wrapper.F90:48-59).  Fortran index bounds;
- * padded extents are nx_ = imax-imin+1+2*halox, ny_ likewise, nz = kmax-kmin+1. */
+ * padded extents are nx_ = imax-imin+1+2*halox, ny_ likewise, nz = kmax-kmin+1.
+ * Returns NULL for empty ranges, negative halos, nx_ or ny_ >= 2^24, or a layer
+ * of FP64 values (nx_*ny_*8 bytes) of 4 GiB or more. */
 void* create_grid(int imin, int imax, int jmin, int jmax, int kmin, int kmax, int halox, int haloy);
 
 /* Replaces grid_finalize (wrapper.F90:61-68). */

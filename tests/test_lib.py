@@ -43,6 +43,25 @@ def test_create_rejects_bad_scheme_without_gpu():
     lib.grid_finalize(g)
 
 
+def test_create_grid_size_limits_without_gpu():
+    """A layer's FP64 field must stay below 4 GiB (32-bit in-layer byte offsets)
+    and each padded extent below 2^24 (24-bit index multiplies); sizes up to the
+    limit are accepted."""
+    lib = _lib.load()
+    # 23170^2 * 8 B < 2^32 <= 23171^2 * 8 B (padded extents, halos of 2 included)
+    g = lib.create_grid(1, 23166, 1, 23166, 1, 40, 2, 2)
+    assert g
+    lib.grid_finalize(g)
+    assert lib.create_grid(1, 23167, 1, 23166, 1, 40, 2, 2) is None
+    # long thin domains: padded extent 2^24 - 1 is accepted, 2^24 is not
+    g = lib.create_grid(1, (1 << 24) - 5, 1, 1, 1, 1, 2, 2)
+    assert g
+    lib.grid_finalize(g)
+    assert lib.create_grid(1, (1 << 24) - 4, 1, 1, 1, 1, 2, 2) is None
+    assert lib.create_grid(1, 1, 1, (1 << 24) - 4, 1, 1, 2, 2) is None
+    assert lib.create_grid(1, 4, 1, 4, 1, 3, -1, 2) is None
+
+
 def test_w_momentum_rejects_bad_range_without_gpu():
     lib = _lib.load()
     # imin = 1 (Fortran) would read column -1; checked before anything is launched
