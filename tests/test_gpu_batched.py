@@ -96,6 +96,27 @@ def test_batched_f64_edge_domains_bitwise(oracle_mod, case, scheme):
 
 
 @pytest.mark.parametrize("case", list(EDGE_CASES))
+@pytest.mark.parametrize("scheme", [1, 3, 5])
+def test_per_tracer_operators_edge_domains_bitwise(oracle_mod, case, scheme):
+    """The reference's per-tracer operator sequence (Advection.apply_3d_batch and
+    VerticalDiffusion on the single-sweep kernels) on the same domains."""
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.tracer import TracerCollection
+
+    d = inputs.tiny(seed=60 + scheme, scheme=scheme, ntracers=2, **EDGE_CASES[case])
+    grid = make_grid(d)
+    tc = TracerCollection(grid, advection_scheme=d.scheme, cnpar=d.cnpar, precision="f64", batched=False)
+    tc.set_values(dev(d.tracers))
+    args = (grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS), grid.wrap(dev(d.w), INTERFACES),
+            grid.wrap(dev(d.nuh), INTERFACES))
+    for _ in range(2):
+        tc.advance(d.dt, *args)
+    ref, href = run_oracle(oracle_mod, d, nsteps=2)
+    assert_bitwise(host(tc.values), ref, "tracers")
+    assert_bitwise(host(tc.h), href, "advected thickness")
+
+
+@pytest.mark.parametrize("case", list(EDGE_CASES))
 def test_batched_f32_edge_domains(oracle_mod, case):
     kw = dict(EDGE_CASES[case], cfl=0.05, wscale=0.2)
     d = inputs.tiny(seed=70, scheme=5, ntracers=3, **kw)
