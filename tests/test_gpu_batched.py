@@ -117,10 +117,11 @@ def test_per_tracer_operators_edge_domains_bitwise(oracle_mod, case, scheme):
 
 
 @pytest.mark.parametrize("case", list(EDGE_CASES))
-def test_batched_f32_edge_domains(oracle_mod, case):
+@pytest.mark.parametrize("solver", [0, 16])
+def test_batched_f32_edge_domains(oracle_mod, case, solver):
     kw = dict(EDGE_CASES[case], cfl=0.05, wscale=0.2)
     d = inputs.tiny(seed=70, scheme=5, ntracers=3, **kw)
-    got, h, _ = run_batched(d, "f32", nsteps=2)
+    got, h, _ = run_batched(d, "f32", nsteps=2, solver=solver)
     ref, href = run_oracle(oracle_mod, d, nsteps=2)
     assert_bitwise(h, href, "FP64 thickness in the FP32 form")
     fin = np.isfinite(ref)
