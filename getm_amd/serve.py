@@ -115,6 +115,13 @@ def draw_requests(n: int, seed: int = 0, size_range=(32, 192), nz_range=(10, 60)
     return out
 
 
+def relative_drift(now: torch.Tensor, target: torch.Tensor) -> float:
+    """max over tracers of |I - I0| / |I0|; a tracer whose inventory at admission
+    is zero (no mass, or no water) contributes its absolute change instead."""
+    scale = torch.where(target != 0, target.abs(), torch.ones_like(target))
+    return float(((now - target).abs() / scale).max().item())
+
+
 class _Slot:
     _serial = 0
 
@@ -164,8 +171,7 @@ class _Slot:
                                                   _lib.ptr(self.target), _lib.stream_handle()), "transport_conserve")
 
     def drift(self) -> float:
-        now = self._inventory()
-        return float(((now - self.target).abs() / self.target.abs()).max().item())
+        return relative_drift(self._inventory(), self.target)
 
     def steps_this_tick(self) -> int:
         """k, or (uniform) the steps left, or the largest power of two not above them."""

@@ -163,3 +163,17 @@ def test_header_and_ctypes_signatures_agree():
     assert set(decls) == set(sigs), (sorted(set(decls) ^ set(sigs)))
     for name, (_, args) in sigs.items():
         assert decls[name] == len(args), (name, decls[name], len(args))
+
+
+def test_serving_drift_of_tracers_without_mass_is_finite():
+    """Host logic of the serving loop: relative inventory drift, with the absolute
+    change for a tracer whose admission inventory is zero (no NaN from 0/0)."""
+    import torch
+
+    from getm_amd.serve import relative_drift
+
+    target = torch.tensor([1000.0, 0.0, -4.0], dtype=torch.float64)
+    assert relative_drift(target.clone(), target) == 0.0
+    assert relative_drift(torch.tensor([1000.5, 0.0, -4.0], dtype=torch.float64), target) == 5e-4
+    assert relative_drift(torch.tensor([1000.0, 1e-9, -4.0], dtype=torch.float64), target) == 1e-9
+    assert relative_drift(torch.tensor([1000.0, 0.0, -4.4], dtype=torch.float64), target) == abs(-4.4 + 4.0) / 4.0
