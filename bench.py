@@ -497,7 +497,7 @@ def main():
                          "ensemble: 96 uneven domains in one launch per kernel kind, hipGraph chunk")
     ap.add_argument("--domains", type=int, default=96, help="ensemble workload: number of domains")
     ap.add_argument("--autotune", action="store_true", help="ensemble workload: autotune the launch shape first")
-    ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--precision", choices=["f32", "f32-sr", "f64"], default="f32")
     ap.add_argument("--tracers", type=int, default=0, help="override the workload's tracer count")
     ap.add_argument("--scheme", type=int, default=0, help="override the workload's scheme")
     ap.add_argument("--per-tracer", action="store_true", help="FP64 per-tracer operator sequence (reference order)")

@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgetm_amd.so")
 
 GETM_OK = 0
+# tracer precision of a transport object (include/getm_amd.h)
+GETM_F64, GETM_F32, GETM_F32C = 1, 2, 3
 ERRORS = {
     1: "invalid argument or shape mismatch",
     2: "invalid advection scheme",
@@ -43,6 +45,8 @@ _SIGNATURES = {
     "transport_create": (_vp, [_vp, _vp, _vp, _i, _d, _i, _i]),
     "transport_finalize": (None, [_vp]),
     "transport_step": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp]),
+    "transport_step_mixed": (_i, [_vp, _i, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp), _vp, _vp, _vp, _d,
+                                  ctypes.POINTER(_d), _vp, _vp]),
     "transport_get_stats": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _i, _vp]),
     "transport_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
     "c_w_momentum_3d": (_i, [_i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp]),

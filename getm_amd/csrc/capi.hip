@@ -270,7 +270,7 @@ int c_vertical_diffusion_apply(void* vd, int nx, int ny, int nz, const int* mask
 void* transport_create(void* tgrid, void* ugrid, void* vgrid, int scheme, double cnpar, int precision, int ufirst)
 {
     if (scheme < GETM_HSIMT || scheme > GETM_UPSTREAM || !tgrid || !ugrid || !vgrid) return nullptr;
-    if (precision != GETM_F64 && precision != GETM_F32) return nullptr;
+    if (precision != GETM_F64 && precision != GETM_F32 && precision != GETM_F32C) return nullptr;
     const Grid* t = static_cast<const Grid*>(tgrid);
     const Grid* gu = static_cast<const Grid*>(ugrid);
     const Grid* gv = static_cast<const Grid*>(vgrid);
@@ -337,17 +337,63 @@ static StepArgs step_args(const Transport* tr, const double* u, const double* v,
     return A;
 }
 
-template <typename R>
-static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, const double* w, const double* nuh,
-                            double dt, double molecular, double* h_out, hipStream_t s)
+// Every grid-dimension limit of the step's launches, checked before the first
+// of them, so that a call that cannot run leaves the tracers untouched: the
+// tracer batches of the w stage and of the diffusion (grid.z <= 65535; the FP64
+// diffusion takes one tracer per z slice), the work items of the horizontal
+// stage pairs (a 1-D grid of at most 0x7fffff00 blocks, at least 4-row tiles of
+// 32 columns and one layer each) and the column solve's depth.
+static int step_capacity(const Transport* tr, int ntr)
 {
     const Grid* t = tr->tgrid;
-    const size_t n3 = t->n3();
+    auto batches = [&](int tpb) -> long { return tpb <= 0 || tpb >= ntr ? 1 : (ntr + tpb - 1) / tpb; };
+    if (batches(tr->tpb_w) > 65535) return GETM_ERR_CAPACITY;
+    if (tr->precision == GETM_F64 ? ntr > 65535 : batches(tr->tpb_d) > 65535) return GETM_ERR_CAPACITY;
+    if (tr->precision != GETM_F64 && (t->nz > 64 || t->nz < 1)) return GETM_ERR_CAPACITY;
+    const long tiles = (long)((t->nx_ + 31) / 32) * ((t->ny_ + 3) / 4);
+    if ((double)batches(tr->tpb_h) * t->nz * tiles > (double)0x7fffff00L) return GETM_ERR_CAPACITY;
+    return GETM_OK;
+}
+
+// Consecutive tracers [t0, t1) that share a value: the step's per-tracer
+// w stages and diffusion run once per such run of tracers.
+template <typename V, typename F>
+static int for_runs(int ntr, const V* vals, V fallback, F&& fn)
+{
+    int t0 = 0;
+    while (t0 < ntr) {
+        const V v0 = vals ? vals[t0] : fallback;
+        int t1 = t0 + 1;
+        while (t1 < ntr && (vals ? vals[t1] : fallback) == v0) t1++;
+        const int rc = fn(t0, t1, v0);
+        if (rc) return rc;
+        t0 = t1;
+    }
+    return GETM_OK;
+}
+
+// One step of ntr tracers, tracer t advected in the w stage by wv[t] (NULL: w)
+// and diffused with molecular diffusivity mol[t].  Stages 1-2 and 4-5 take all
+// tracers at once; the w stage and the diffusion run once per run of tracers
+// with equal w_var / molecular value (tracer.py:223-231, 273-275).
+//   R = double: the FP64 form; R = float: FP32 (C = false) or compensated FP32
+//   (C = true: f holds ntr FP32 planes and then ntr int16 residual planes, and
+//   so does the scratch buffer).
+template <typename R, bool C>
+static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, const double* w,
+                            const double* const* wv, const double* nuh, double dt, const double* mol,
+                            double* h_out, hipStream_t s)
+{
+    const Grid* t = tr->tgrid;
+    const size_t n3 = t->n3(), nn = n3 * (size_t)ntr;
     int rc;
-    if ((rc = tr->scratch.ensure(n3 * (size_t)ntr * sizeof(R))) || (rc = tr->hA.ensure(n3 * sizeof(double))) ||
+    const size_t per = C ? sizeof(float) + sizeof(short) : sizeof(R);
+    if ((rc = tr->scratch.ensure(nn * per)) || (rc = tr->hA.ensure(n3 * sizeof(double))) ||
         (rc = tr->hB.ensure(n3 * sizeof(double))))
         return rc;
     R* S = static_cast<R*>(tr->scratch.ptr);
+    short* fl = C ? reinterpret_cast<short*>(f + nn) : nullptr;
+    short* SL = C ? reinterpret_cast<short*>(S + nn) : nullptr;
     double* hA = static_cast<double*>(tr->hA.ptr);
     double* hB = static_cast<double*>(tr->hB.ptr);
     double* hfinal = h_out ? h_out : hA;
@@ -355,29 +401,88 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     const int d1 = tr->ufirst ? 1 : 2;  // first direction of stages 1-2 (operators.py:118-121)
     const int d2 = 3 - d1;
     const int ty = tr->tile_y ? tr->tile_y : auto_tile_y(ntr);
-    // FP32: the w stage leaves the thickness of the columns it does not update in hA
-    // (stages 1-2 wrote every cell there), and stages 4-5 read those cells from hA
-    // instead of a copy in hB; into hA itself they store updated cells only.
+    int* stats = static_cast<int*>(tr->stats.ptr);
+    // FP32 forms: the w stage leaves the thickness of the columns it does not update
+    // in hA (stages 1-2 wrote every cell there), and stages 4-5 read those cells from
+    // hA instead of a copy in hB; into hA itself they store updated cells only.
     constexpr bool f32 = std::is_same<R, float>::value;
-    if ((rc = launch_hpair<R>(d1, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, SR_PAIR1, s))) return rc;
-    if ((rc = launch_wcol<R>(tr->scheme, A, dt, ntr, tr->tpb_w, S, f, hA, hB, w, w, static_cast<int*>(tr->stats.ptr),
-                             s, !f32)))
-        return rc;
-    if ((rc = launch_hpair<R>(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, SR_PAIR2, s,
-                              f32 ? hA : hB, !(f32 && hfinal == hA))))
-        return rc;
-    if constexpr (std::is_same<R, float>::value) {
-        if (t->nz >= 2 && t->nz <= tr->pcr_max_nz && !tr->ea4)
-            rc = launch_vdiff_pcr(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s);
+    if constexpr (C)
+        rc = launch_hpair_c(d1, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, fl, S, SL, t->ho, hA, s);
+    else
+        rc = launch_hpair<R>(d1, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, SR_PAIR1, s);
+    if (rc) return rc;
+    rc = for_runs<const double*>(ntr, wv, nullptr, [&](int t0, int t1, const double* wvar) -> int {
+        const double* wu = wvar ? wvar : w;
+        const size_t o = (size_t)t0 * n3;
+        if constexpr (C)
+            return launch_wcol_c(tr->scheme, A, dt, t1 - t0, tr->tpb_w, S + o, SL + o, f + o, fl + o, hA, hB, w, wu,
+                                 stats, s, false);
         else
-            rc = launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, ntr, tr->tpb_d, S, f, s, tr->ea4);
-        if (rc) return rc;
-        return launch_count_step(const_cast<unsigned long long*>(A.rstep), s);
-    } else {
-        // FP64: the reference's prepare + per-tracer apply (tracer.py:273-276) in one
-        // launch, scratch -> tracers, bit for bit
-        return launch_vdiff_batch_f64(*t, t->nz, t->mask, t->hn, nuh, dt, tr->cnpar, molecular, ntr, S, f, s);
+            return launch_wcol<R>(tr->scheme, A, dt, t1 - t0, tr->tpb_w, S + o, f + o, hA, hB, w, wu, stats, s, !f32);
+    });
+    if (rc) return rc;
+    if constexpr (C)
+        rc = launch_hpair_c(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, fl, S, SL, hB, hfinal, s, hA,
+                            hfinal != hA);
+    else
+        rc = launch_hpair<R>(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, SR_PAIR2, s,
+                             f32 ? hA : hB, !(f32 && hfinal == hA));
+    if (rc) return rc;
+    rc = for_runs<double>(ntr, mol, 0.0, [&](int t0, int t1, double molecular) -> int {
+        const size_t o = (size_t)t0 * n3;
+        const int nr = t1 - t0;
+        const double* ea4 = tr->ea4 ? tr->ea4 + o : nullptr;
+        if constexpr (C) {
+            return launch_vdiff_c(A, dt, tr->cnpar, molecular, nuh, nr, tr->tpb_d, S + o, SL + o, f + o, fl + o, s,
+                                  ea4);
+        } else if constexpr (f32) {
+            if (t->nz >= 2 && t->nz <= tr->pcr_max_nz && !ea4)
+                return launch_vdiff_pcr(A, dt, tr->cnpar, molecular, nuh, nr, tr->tpb_d, S + o, f + o, s);
+            return launch_vdiff_f32(A, dt, tr->cnpar, molecular, nuh, nr, tr->tpb_d, S + o, f + o, s, ea4);
+        } else {
+            // FP64: the reference's prepare + per-tracer apply (tracer.py:273-276) in one
+            // launch, scratch -> tracers, bit for bit
+            return launch_vdiff_batch_f64(*t, t->nz, t->mask, t->hn, nuh, dt, tr->cnpar, molecular, nr, S + o, f + o,
+                                          s);
+        }
+    });
+    if (rc) return rc;
+    if constexpr (f32 && !C) return launch_count_step(const_cast<unsigned long long*>(A.rstep), s);
+    return GETM_OK;
+}
+
+// Checks shared by transport_step and transport_step_mixed, then the step.
+static int transport_step_any(Transport* tr, int ntracers, void* tracers, const double* u, const double* v,
+                              const double* w, const double* const* w_var, const double* nuh, const double* Ah_u,
+                              const double* Ah_v, double timestep, const double* molecular, double* h_out,
+                              void* stream)
+{
+    if (!tr) return GETM_ERR_ARG;
+    if (ntracers < 0 || (ntracers > 0 && !tracers) || !u || !v || !w || !nuh) return GETM_ERR_ARG;
+    if ((Ah_u == nullptr) != (Ah_v == nullptr)) return GETM_ERR_ARG;
+    const Grid* t = tr->tgrid;
+    if (!t->ho || !t->hn || !t->mask || !t->iarea || !tr->ugrid->hn || !tr->vgrid->hn || !tr->ugrid->mask ||
+        !tr->vgrid->mask || !tr->ugrid->idx || !tr->ugrid->dy || !tr->vgrid->dx || !tr->vgrid->idy)
+        return GETM_ERR_ARG;
+    if (t->nz > 64 && tr->precision != GETM_F64) return GETM_ERR_CAPACITY;
+    hipStream_t s = as_stream(stream);
+    if (ntracers == 0) {
+        if (h_out) GETM_HIP_OK(hipMemcpyAsync(h_out, t->ho, t->n3() * sizeof(double), hipMemcpyDeviceToDevice, s));
+        return GETM_OK;
     }
+    int rc;
+    if ((rc = step_capacity(tr, ntracers))) return rc;
+    StepArgs A = step_args(tr, u, v, Ah_u, Ah_v);
+    if (tr->precision == GETM_F32 && tr->rounding)
+        A.rstep = tr->rstep_ext ? tr->rstep_ext : static_cast<unsigned long long*>(tr->rstep_own.ptr);
+    if (tr->precision == GETM_F32C)
+        return transport_step_t<float, true>(tr, A, ntracers, static_cast<float*>(tracers), w, w_var, nuh, timestep,
+                                             molecular, h_out, s);
+    if (tr->precision == GETM_F32)
+        return transport_step_t<float, false>(tr, A, ntracers, static_cast<float*>(tracers), w, w_var, nuh,
+                                              timestep, molecular, h_out, s);
+    return transport_step_t<double, false>(tr, A, ntracers, static_cast<double*>(tracers), w, w_var, nuh, timestep,
+                                           molecular, h_out, s);
 }
 
 extern "C" {
@@ -387,27 +492,19 @@ int transport_step(void* p, int ntracers, void* tracers, const double* u, const 
                    double* h_out, void* stream)
 {
     Transport* tr = static_cast<Transport*>(p);
-    if (!tr) return GETM_ERR_ARG;
-    if (ntracers < 0 || (ntracers > 0 && !tracers) || !u || !v || !w || !nuh) return GETM_ERR_ARG;
-    if ((Ah_u == nullptr) != (Ah_v == nullptr)) return GETM_ERR_ARG;
-    const Grid* t = tr->tgrid;
-    if (!t->ho || !t->hn || !t->mask || !t->iarea || !tr->ugrid->hn || !tr->vgrid->hn || !tr->ugrid->mask ||
-        !tr->vgrid->mask || !tr->ugrid->idx || !tr->ugrid->dy || !tr->vgrid->dx || !tr->vgrid->idy)
-        return GETM_ERR_ARG;
-    if (t->nz > 64 && tr->precision == GETM_F32) return GETM_ERR_CAPACITY;
-    hipStream_t s = as_stream(stream);
-    if (ntracers == 0) {
-        if (h_out) GETM_HIP_OK(hipMemcpyAsync(h_out, t->ho, t->n3() * sizeof(double), hipMemcpyDeviceToDevice, s));
-        return GETM_OK;
-    }
-    StepArgs A = step_args(tr, u, v, Ah_u, Ah_v);
-    if (tr->precision == GETM_F32 && tr->rounding)
-        A.rstep = tr->rstep_ext ? tr->rstep_ext : static_cast<unsigned long long*>(tr->rstep_own.ptr);
-    if (tr->precision == GETM_F32)
-        return transport_step_t<float>(tr, A, ntracers, static_cast<float*>(tracers), w, nuh, timestep, molecular,
-                                       h_out, s);
-    return transport_step_t<double>(tr, A, ntracers, static_cast<double*>(tracers), w, nuh, timestep, molecular, h_out,
-                                    s);
+    if (ntracers < 0) return GETM_ERR_ARG;
+    std::vector<double> mol((size_t)ntracers, molecular);
+    return transport_step_any(tr, ntracers, tracers, u, v, w, nullptr, nuh, Ah_u, Ah_v, timestep, mol.data(), h_out,
+                              stream);
+}
+
+int transport_step_mixed(void* p, int ntracers, void* tracers, const double* u, const double* v, const double* w,
+                         const double* const* w_var, const double* nuh, const double* Ah_u, const double* Ah_v,
+                         double timestep, const double* molecular, double* h_out, void* stream)
+{
+    if (ntracers > 0 && !molecular) return GETM_ERR_ARG;
+    return transport_step_any(static_cast<Transport*>(p), ntracers, tracers, u, v, w, w_var, nuh, Ah_u, Ah_v,
+                              timestep, molecular, h_out, stream);
 }
 
 int transport_inventory(void* p, int ntracers, const void* tracers, const double* h, double* inventory, void* stream)
@@ -423,6 +520,12 @@ int transport_inventory(void* p, int ntracers, const void* tracers, const double
     double* part = static_cast<double*>(tr->part.ptr);
     const double* hh = h ? h : t->hn;
     hipStream_t s = as_stream(stream);
+    if (tr->precision == GETM_F32C) {
+        const float* f = static_cast<const float*>(tracers);
+        return launch_inventory_c(A, hh, t->area, ntracers, f,
+                                  reinterpret_cast<const short*>(f + t->n3() * (size_t)ntracers), part, inventory,
+                                  nullptr, nullptr, s);
+    }
     if (tr->precision == GETM_F32)
         return launch_inventory<float>(A, hh, t->area, ntracers, static_cast<const float*>(tracers), part, inventory,
                                        nullptr, nullptr, s);
@@ -434,7 +537,7 @@ int transport_conserve(void* p, int ntracers, void* tracers, const double* targe
 {
     Transport* tr = static_cast<Transport*>(p);
     if (!tr || ntracers < 1 || !tracers || !target) return GETM_ERR_ARG;
-    if (tr->precision != GETM_F32) return GETM_OK;
+    if (tr->precision == GETM_F64) return GETM_OK;
     const Grid* t = tr->tgrid;
     if (!t->area || !t->mask || !t->hn) return GETM_ERR_ARG;
     if (ntracers > 65535) return GETM_ERR_CAPACITY;
@@ -450,6 +553,12 @@ int transport_conserve(void* p, int ntracers, void* tracers, const double* targe
     double* ratio = static_cast<double*>(tr->ratio.ptr);
     hipStream_t s = as_stream(stream);
     float* f = static_cast<float*>(tracers);
+    if (tr->precision == GETM_F32C) {
+        // x (1 + (target - I) / I) on every wet cell, exact to the storage rounding
+        short* lo = reinterpret_cast<short*>(f + t->n3() * (size_t)ntracers);
+        if ((rc = launch_inventory_c(A, t->hn, t->area, ntracers, f, lo, part, nullptr, target, ratio, s))) return rc;
+        return launch_fix_c(A, ntracers, f, lo, ratio, s);
+    }
     if ((rc = launch_inventory<float>(A, t->hn, t->area, ntracers, f, part, nullptr, target, ratio, s))) return rc;
     return launch_fix_f32(A, ntracers, f, part, ratio, t->area, static_cast<double*>(tr->part2.ptr), target,
                           static_cast<double*>(tr->resid.ptr), s);
@@ -488,7 +597,7 @@ int transport_set_sources(void* p, const double* ea4)
 {
     Transport* tr = static_cast<Transport*>(p);
     if (!tr) return GETM_ERR_ARG;
-    if (ea4 && (tr->precision != GETM_F32 || tr->cnpar != 1.0)) return GETM_ERR_ARG;
+    if (ea4 && (tr->precision == GETM_F64 || tr->cnpar != 1.0)) return GETM_ERR_ARG;
     tr->ea4 = ea4;
     return GETM_OK;
 }

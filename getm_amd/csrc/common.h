@@ -168,6 +168,45 @@ struct SRound {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Compensated FP32 storage (precision GETM_F32C).  A tracer value x is stored
+// as hi = (float)x, rounded to nearest, plus an int16 residual
+//   lo = rint((x - hi) / 2^(e - 38)),
+// with e the exponent of hi (hi in [2^e, 2^(e+1)); e = -126 for zero and FP32
+// subnormals): x - hi in units of 2^-15 of hi's ulp, so |lo| <= 2^14 and the
+// stored value carries 39 significant bits.  The kernels decode both into an
+// FP64 value, work with it and encode the result once per launch; the store's
+// rounding (at most 2^-16 of an FP32 ulp) is too small to accumulate over any
+// practical number of steps, so no stochastic rounding is needed.
+// ---------------------------------------------------------------------------
+// 2^(e - 38) for the exponent e of hi, built from hi's exponent field eb
+// (e = eb - 127, fields 0 and 1 both meaning -126): FP64 exponent field
+// eb - 165 + 1023 = eb + 858.
+__device__ __forceinline__ double res_unit(float hi)
+{
+    unsigned eb = (__float_as_uint(hi) >> 23) & 0xffu;
+    eb = eb > 1u ? eb : 1u;
+    return __hiloint2double((int)((eb + 858u) << 20), 0);
+}
+
+// x = hi + lo 2^(e - 38), exact (39 significant bits).
+__device__ __forceinline__ double cdecode(float hi, short lo)
+{
+    return fma((double)lo, res_unit(hi), (double)hi);
+}
+
+// hi = (float)x, lo = rint((x - hi) 2^(38 - e)); x - hi is exact in FP64 and the
+// scaling by 2^(38 - e) (FP64 exponent field 1023 + 165 - eb) is exact.
+__device__ __forceinline__ void cencode(double x, float& hi, short& lo)
+{
+    hi = (float)x;
+    const double r = x - (double)hi;
+    unsigned eb = (__float_as_uint(hi) >> 23) & 0xffu;
+    eb = eb > 1u ? eb : 1u;
+    const double inv = __hiloint2double((int)((1188u - eb) << 20), 0);
+    lo = (short)__double2int_rn(r * inv);
+}
+
 }  // namespace getm
 
 // Instantiate a templated launcher for every scheme; `body` is called with a

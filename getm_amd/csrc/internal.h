@@ -197,6 +197,16 @@ int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, cons
 int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
                      const float* fin, float* fout, hipStream_t s,
                      const double* ea4 = nullptr);
+// compensated FP32 (GETM_F32C): FP32 values plus int16 residuals (common.h cencode / cdecode)
+int launch_hpair_c(int d1, int scheme, int ty, const StepArgs& A, double dt, int ntr, int tpb, const float* fin,
+                   const short* lin, float* fout, short* lout, const double* hin, double* hout, hipStream_t s,
+                   const double* hin_dry = nullptr, bool write_dry = true);
+int launch_wcol_c(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const float* fin, const short* lin,
+                  float* fout, short* lout, const double* hin, double* hout, const double* w, const double* wv,
+                  int* stats, hipStream_t s, bool copy_dry);
+int launch_vdiff_c(const StepArgs& A, double dt, double cnpar, double molecular, const double* nuh, int ntr, int tpb,
+                   const float* fin, const short* lin, float* fout, short* lout, hipStream_t s,
+                   const double* ea4 = nullptr);
 // advance the step counter of the stochastic rounding (single counter, or every domain's)
 int launch_count_step(unsigned long long* counter, hipStream_t s);
 int launch_count_steps_multi(const EnsDom<float>* doms, int ndom, hipStream_t s);
@@ -207,6 +217,9 @@ int launch_inventory(const StepArgs& A, const double* h, const double* area, int
                      double* inv, const double* target, double* ratio, hipStream_t s);
 int launch_fix_f32(const StepArgs& A, int ntr, float* f, const double* part, const double* ratio, const double* area,
                    double* part2, const double* target, double* resid, hipStream_t s);
+int launch_inventory_c(const StepArgs& A, const double* h, const double* area, int ntr, const float* f,
+                       const short* lo, double* part, double* inv, const double* target, double* ratio, hipStream_t s);
+int launch_fix_c(const StepArgs& A, int ntr, float* f, short* lo, const double* ratio, hipStream_t s);
 size_t fix_part_doubles(const StepArgs& A, int ntr);
 size_t inventory_part_doubles(const StepArgs& A, int ntr);
 

@@ -55,10 +55,12 @@ __device__ __forceinline__ double q_of(float c)
 }
 
 // Partial sums per (tracer, block): mass (hi, lo) and the sum of c/ulp(c) (FP32 only).
-template <typename R>
+// COMP: compensated FP32 tracers (f the FP32 values, lo their int16 residuals).
+template <typename R, bool COMP = false>
 __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, const double* __restrict__ h,
                                                            const double* __restrict__ area, int ntr,
-                                                           const R* __restrict__ f, double* __restrict__ part)
+                                                           const R* __restrict__ f, double* __restrict__ part,
+                                                           const short* __restrict__ lo = nullptr)
 {
     __shared__ double s_m[INV_THREADS], s_c[INV_THREADS], s_q[INV_THREADS];
     const int tid = threadIdx.y * IBX + threadIdx.x;
@@ -75,8 +77,13 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, con
             const double a = area[p];
             for (int k = 0; k < A.nz; k++) {
                 const R c = fc[(size_t)k * A.n2];
-                neumaier(m, mc, (a * h[(size_t)k * A.n2 + p]) * (double)c);
-                if constexpr (std::is_same<R, float>::value) q += q_of(c);
+                if constexpr (COMP) {
+                    const double x = cdecode(c, lo[(size_t)t * A.n3 + p + (size_t)k * A.n2]);
+                    neumaier(m, mc, (a * h[(size_t)k * A.n2 + p]) * x);
+                } else {
+                    neumaier(m, mc, (a * h[(size_t)k * A.n2 + p]) * (double)c);
+                    if constexpr (std::is_same<R, float>::value) q += q_of(c);
+                }
             }
         }
     }
@@ -399,6 +406,58 @@ __global__ void __launch_bounds__(64) k_greedy_f32(const StepArgs A, int ntr, fl
     }
 }
 
+// Compensated FP32 fixer: every wet cell of tracer t becomes x (1 + ratio[t])
+// in FP64, encoded again; the inventory then equals the target up to the
+// storage rounding of each cell (2^-40 of its value).
+__global__ void __launch_bounds__(INV_THREADS) k_fix_c(const StepArgs A, float* __restrict__ f,
+                                                       short* __restrict__ lo, const double* __restrict__ ratio)
+{
+    const int i = blockIdx.x * IBX + threadIdx.x, j = blockIdx.y * IBY + threadIdx.y;
+    const int t = blockIdx.z;
+    if (i >= A.nx_ || j >= A.ny_) return;
+    const size_t p = (size_t)j * A.nx_ + i;
+    if (!(i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[p] == 1)) return;
+    // x (1 + r) - x is below half the residual unit 2^-39 |x| (every cell keeps
+    // its stored value) when |r| < 2^-40: the inventory is already at the target
+    // to the storage precision
+    const double r = ratio[t];
+    if (!(fabs(r) >= 0x1p-40)) return;
+    float* fc = f + (size_t)t * A.n3 + p;
+    short* lc = lo + (size_t)t * A.n3 + p;
+    for (int k = 0; k < A.nz; k++) {
+        const size_t o = (size_t)k * A.n2;
+        const double x = cdecode(fc[o], lc[o]);
+        float hi;
+        short l;
+        cencode(fma(x, r, x), hi, l);
+        fc[o] = hi;
+        lc[o] = l;
+    }
+}
+
+int launch_inventory_c(const StepArgs& A, const double* h, const double* area, int ntr, const float* f,
+                       const short* lo, double* part, double* inv, const double* target, double* ratio, hipStream_t s)
+{
+    dim3 block(IBX, IBY, 1);
+    dim3 grid((A.nx_ + IBX - 1) / IBX, (A.ny_ + IBY - 1) / IBY, ntr);
+    const int nblk = grid.x * grid.y;
+    ProfileScope ps(GETM_K_INVENTORY, s);
+    hipLaunchKernelGGL((k_inventory<float, true>), grid, block, 0, s, A, h, area, ntr, f, part, lo);
+    hipLaunchKernelGGL(k_inventory_reduce, dim3(ntr), dim3(64), 0, s, ntr, nblk, part, inv, target, ratio);
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+int launch_fix_c(const StepArgs& A, int ntr, float* f, short* lo, const double* ratio, hipStream_t s)
+{
+    dim3 block(IBX, IBY, 1);
+    dim3 grid((A.nx_ + IBX - 1) / IBX, (A.ny_ + IBY - 1) / IBY, ntr);
+    ProfileScope ps(GETM_K_FIX, s);
+    hipLaunchKernelGGL(k_fix_c, grid, block, 0, s, A, f, lo, ratio);
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
 template <typename R>
 int launch_inventory(const StepArgs& A, const double* h, const double* area, int ntr, const R* f, double* part,
                      double* inv, const double* target, double* ratio, hipStream_t s)
@@ -407,7 +466,7 @@ int launch_inventory(const StepArgs& A, const double* h, const double* area, int
     dim3 grid((A.nx_ + IBX - 1) / IBX, (A.ny_ + IBY - 1) / IBY, ntr);
     const int nblk = grid.x * grid.y;
     ProfileScope ps(GETM_K_INVENTORY, s);
-    hipLaunchKernelGGL((k_inventory<R>), grid, block, 0, s, A, h, area, ntr, f, part);
+    hipLaunchKernelGGL((k_inventory<R>), grid, block, 0, s, A, h, area, ntr, f, part, nullptr);
     hipLaunchKernelGGL(k_inventory_reduce, dim3(ntr), dim3(64), 0, s, ntr, nblk, part, inv, target, ratio);
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;

@@ -153,7 +153,7 @@ class _Slot:
         self.steps = 0
         self.wet_cols = dom.n_wet_columns
         # a transport object of this grid provides inventories and the FP32 fixer
-        self.tc = TracerCollection(g, advection_scheme=req.scheme, precision="f32")
+        self.tc = TracerCollection(g, advection_scheme=req.scheme, precision="f32-sr")
         self.target = self._inventory()
         self.conserve = conserve
         # step count of the request's stochastic rounding, carried by the request

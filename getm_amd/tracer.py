@@ -1,28 +1,41 @@
-"""Tracer transport driver mirroring pygetm.tracer.TracerCollection
-(python/pygetm/tracer.py:125-276): split advection of every tracer followed by
-implicit vertical diffusion of every tracer.
+"""Tracer transport driver mirroring pygetm.tracer (python/pygetm/tracer.py:22-276):
+``Tracer`` fields with their sources, surface fluxes, vertical velocities and
+molecular diffusivities, and ``TracerCollection``, whose ``advance`` is split
+advection of every tracer followed by implicit vertical diffusion of every
+tracer.
 
-Tracers live in one stacked device tensor ``(T, nz, ny_, nx_)``.  With
-``batched=True`` (default) a step is one ``transport_step`` call of the
-C-ABI, which streams all tracers through shared stage coefficients; with
-``batched=False`` it is the reference's call sequence of per-tracer operators
-(FP64 only).  ``precision="f32"`` stores tracers in FP32 with the thickness
-update kept in FP64.
+Tracers live in one stacked device tensor ``values`` of shape
+``(T, nz, ny_, nx_)``.  With ``batched=True`` (default) a step is one call of
+the C-ABI's batched step, which streams all tracers through shared stage
+coefficients; with ``batched=False`` it is the reference's call sequence of
+per-tracer operators (FP64 only).
 
-FP32 anomalies (``set_values(..., offsets=...)``): every FP32 kernel updates a
-tracer by increments formed from differences of its values (limited face values
-shift with the field, the implicit column solve acts on increments), so a step
-of c - o equals the step of c minus o up to rounding.  Storing each tracer as
-its anomaly about an offset o_t near its smallest value makes the stored
-magnitudes, and with them the ulp that every rounding of the stored values
-scales with, several times smaller (salinity 30-35 stored about 30: 8x).  The
-stored array then holds c - o_t; ``absolute`` adds the offsets back in FP64 and
-``inventory_device(absolute=True)`` adds o_t times the water volume.
+Precisions:
+
+- ``"f64"``: the reference's method, bit-identical to it.
+- ``"f32"``: the FP32 form with compensated storage.  Each tracer value is an
+  FP32 number (``values``) plus an int16 residual (``residuals``) that extends
+  it by 15 bits.  The limiter and the tridiagonal elimination run in FP32 on
+  differences of the decoded values; fluxes, cell updates and the thickness
+  run in FP64, and every launch rounds its results once, to 2^-16 of an FP32
+  ulp.  Neither rounding errors nor inventory defects accumulate.
+  ``absolute()`` returns the decoded values.
+- ``"f32-sr"``: plain FP32 storage, every store rounded stochastically.  This
+  is the form of the ensembles and the serving loop.
+
+FP32 anomalies (``set_values(..., offsets=...)``): every FP32 update is formed
+from differences of a tracer's values, so a step of c - o equals the step of
+c minus o up to rounding.  Storing a tracer as its anomaly about an offset o_t
+near its smallest value makes the stored magnitudes smaller, and with them
+the ulp that every rounding scales with.  ``absolute`` adds the offsets back
+in FP64, and ``inventory_device(absolute=True)`` adds o_t times the water
+volume.
 """
 from __future__ import annotations
 
 import ctypes
 import json
+import logging
 from typing import List, Optional
 
 import numpy as np
@@ -31,36 +44,125 @@ import torch
 from . import _lib, operators
 from .core import CENTERS, INTERFACES, Array, Grid
 
-_DTYPES = {"f64": (torch.float64, 1), "f32": (torch.float32, 2)}
+_PRECISIONS = {"f64": (torch.float64, _lib.GETM_F64), "f32": (torch.float32, _lib.GETM_F32C),
+               "f32-sr": (torch.float32, _lib.GETM_F32)}
 _ROUNDING = {"nearest": 0, "stochastic": 1}
 
 
+def residual_unit(hi: torch.Tensor) -> torch.Tensor:
+    """2^(e - 38) for the exponent e of each FP32 value (e = -126 for zero and
+    subnormals): the unit of the int16 residuals of compensated storage."""
+    eb = ((hi.view(torch.int32) >> 23) & 0xFF).clamp_(min=1)
+    return torch.ldexp(torch.ones(hi.shape, dtype=torch.float64, device=hi.device), eb - 165)
+
+
+def decode(hi: torch.Tensor, lo: torch.Tensor) -> torch.Tensor:
+    """FP64 values of compensated FP32 storage: hi + lo 2^(e - 38), exact."""
+    return hi.double() + lo.double() * residual_unit(hi)
+
+
+def encode(x: torch.Tensor, hi: torch.Tensor, lo: torch.Tensor) -> None:
+    """Store FP64 values as hi = (float)x (rounded to nearest) and
+    lo = rint((x - hi) / 2^(e - 38)), the kernels' encoding."""
+    h = x.to(torch.float32)
+    r = torch.nan_to_num(x - h.double(), nan=0.0, posinf=0.0, neginf=0.0)
+    hi.copy_(h)
+    lo.copy_(torch.round(r / residual_unit(h)).clamp_(-32768, 32767).to(torch.int16))
+
+
+class _Storage:
+    """The collection's tracer arrays, shared with its Tracer objects (which hold
+    this, not the collection, so that no reference cycle keeps a collection and
+    its device buffers alive after its last user lets go)."""
+
+    __slots__ = ("values", "residuals")
+
+    def __init__(self):
+        self.values = None
+        self.residuals = None
+
+
+class Tracer(Array):
+    """A transported tracer, pygetm.tracer.Tracer (tracer.py:22-123): its field
+    (the collection's ``values[index]``) and the attributes that
+    ``TracerCollection.advance`` reads at every step -- ``source`` and
+    ``surface_flux`` (multiplied by ``source_scale``, in tracer units per time
+    times layer thickness), ``vertical_velocity`` (movement through the water,
+    positive upwards) and ``molecular_diffusivity``.  Rivers, open boundaries
+    and precipitation are outside the transport step and not modelled."""
+
+    __slots__ = ("source", "surface_flux", "source_scale", "vertical_velocity", "molecular_diffusivity",
+                 "_store", "_index", "attrs")
+
+    def __init__(self, collection: "TracerCollection", index: int, name: str = "", source: Optional[Array] = None,
+                 surface_flux: Optional[Array] = None, source_scale: float = 1.0,
+                 vertical_velocity: Optional[Array] = None, molecular_diffusivity: float = 0.0, **kwargs):
+        g = collection.grid
+        if source is not None and (source.grid is not g or source.z != CENTERS):
+            raise ValueError("source must be a layer field of the tracer's grid")
+        if surface_flux is not None and (surface_flux.grid is not g or surface_flux.z):
+            raise ValueError("surface_flux must be a 2-D field of the tracer's grid")
+        if vertical_velocity is not None and (vertical_velocity.grid is not g or vertical_velocity.z != CENTERS):
+            raise ValueError("vertical_velocity must be a layer field of the tracer's grid")
+        self.grid = g
+        self.z = CENTERS
+        self.name = name
+        self._store = collection._storage
+        self._index = index
+        self.source = source
+        self.surface_flux = surface_flux
+        self.source_scale = float(source_scale)
+        self.vertical_velocity = vertical_velocity
+        self.molecular_diffusivity = float(molecular_diffusivity)
+        # units, long_name, fill_value and the river/precipitation switches of the
+        # reference's Tracer are kept as metadata only
+        self.attrs = dict(kwargs)
+
+    @property
+    def all_values(self) -> torch.Tensor:
+        return self._store.values[self._index]
+
+    @all_values.setter
+    def all_values(self, value) -> None:
+        self._store.values[self._index].copy_(torch.as_tensor(value))
+
+
 class TracerCollection:
-    def __init__(self, grid: Grid, advection_scheme=operators.AdvectionScheme.DEFAULT, cnpar: float = 1.0,
-                 precision: str = "f64", batched: bool = True, ufirst: bool = False, conserve: bool = False):
-        if precision not in _DTYPES:
-            raise ValueError(f"precision must be one of {list(_DTYPES)}")
-        if precision == "f32" and not batched:
-            raise ValueError("the FP32 form is only available as the batched step")
+    """pygetm.tracer.TracerCollection (tracer.py:125-276) on the GPU.
+
+    ``logger`` is optional; ``precision``, ``batched``, ``ufirst`` and
+    ``conserve`` select the implementation (see the module docstring).
+    ``conserve=True`` (FP32 forms) restores every tracer's inventory after each
+    step to its value before the step (sum of area * ho * c before, area * hn * c
+    after), so the collection must be closed and without sources then."""
+
+    def __init__(self, grid: Grid, logger: Optional[logging.Logger] = None,
+                 advection_scheme=operators.AdvectionScheme.DEFAULT, cnpar: float = 1.0, precision: str = "f64",
+                 batched: bool = True, ufirst: bool = False, conserve: bool = False):
+        if precision not in _PRECISIONS:
+            raise ValueError(f"precision must be one of {list(_PRECISIONS)}")
+        if precision != "f64" and not batched:
+            raise ValueError("the FP32 forms are only available as the batched step")
         self.grid = grid
+        self.logger = logger or logging.getLogger("getm_amd.tracer")
         self.precision = precision
+        self.compensated = precision == "f32"
         self.batched = batched
         self.cnpar = float(cnpar)
-        self.dtype, code = _DTYPES[precision]
-        self._molecular: List[float] = []
-        self._names: List[str] = []
-        self._extra: List[tuple] = []  # (source, surface_flux, source_scale, vertical_velocity) per tracer
-        self.values = torch.empty((0, grid.nz_, grid.ny_, grid.nx_), dtype=self.dtype, device=grid.device)
+        self.dtype, code = _PRECISIONS[precision]
+        self._storage = _Storage()
+        self._tracers: List[Tracer] = []
+        self._alloc(0)
         self.offsets = np.zeros(0)  # FP32 anomalies: stored value = c - offsets[t]
         self._ones: Optional[torch.Tensor] = None
         self.h = torch.empty((grid.nz_, grid.ny_, grid.nx_), dtype=torch.float64, device=grid.device)
         self.Ah_u: Optional[Array] = None
         self.Ah_v: Optional[Array] = None
-        # FP32 only: restore every tracer's inventory to its value before the step
-        self.conserve = bool(conserve) and precision == "f32"
+        self.conserve = bool(conserve) and precision != "f64"
         self._target: Optional[torch.Tensor] = None
-        self.rounding = "stochastic"
+        self.rounding = "stochastic" if precision == "f32-sr" else "nearest"
         self._counter: Optional[torch.Tensor] = None
+        self._wcombs: List[Array] = []
         lib = _lib.load()
         if batched:
             self.p = lib.transport_create(grid.p, grid.ugrid.p, grid.vgrid.p, int(advection_scheme), self.cnpar, code,
@@ -70,7 +172,7 @@ class TracerCollection:
             # the collection's own step counter of the stochastic rounding, visible to
             # state_dict (a restart continues the same stream of random bits)
             self._own_counter = torch.zeros(1, dtype=torch.int64, device=grid.device)
-            if precision == "f32":
+            if precision == "f32-sr":
                 _lib.check(lib.transport_set_rounding(self.p, _ROUNDING["stochastic"], _lib.ptr(self._own_counter)),
                            "transport_set_rounding")
         else:
@@ -80,10 +182,40 @@ class TracerCollection:
         if not batched:
             self._sequence_operators()
 
+    # ---- storage ----------------------------------------------------------
+    def _alloc(self, ntr: int) -> None:
+        """Uninitialised storage for ntr tracers: ``values`` and, compensated,
+        ``residuals`` as views of one buffer (the FP32 planes followed by the
+        int16 planes, the layout the C-ABI takes)."""
+        g = self.grid
+        shape = (ntr, g.nz_, g.ny_, g.nx_)
+        if self.compensated:
+            n = ntr * g.nz_ * g.ny_ * g.nx_
+            self._buf = torch.empty(3 * n, dtype=torch.int16, device=g.device)
+            self.values = self._buf[:2 * n].view(torch.float32).view(shape)
+            self.residuals = self._buf[2 * n:].view(shape)
+        else:
+            self.values = torch.empty(shape, dtype=self.dtype, device=g.device)
+            self._buf = self.values
+            self.residuals = None
+        self._storage.values, self._storage.residuals = self.values, self.residuals
+
+    def _store(self, index, x: torch.Tensor) -> None:
+        """Write values (any float dtype) into tracer slot ``index`` (an int) or
+        every slot (a slice, x stacked); the compensated form keeps what FP32
+        cannot hold in the residuals, encoding one tracer at a time so that the
+        FP64 temporaries stay the size of one tracer."""
+        if not self.compensated:
+            self.values[index].copy_(x)
+        elif isinstance(index, int):
+            encode(x.to(device=self.grid.device, dtype=torch.float64), self.values[index], self.residuals[index])
+        else:
+            for t in range(self.values.shape[0]):
+                encode(x[t].to(device=self.grid.device, dtype=torch.float64), self.values[t], self.residuals[t])
+
     def _sequence_operators(self) -> None:
         """Operators of the reference's per-tracer call sequence (made on first use
-        by an FP64 batched collection whose tracers have sources, surface fluxes
-        or vertical velocities)."""
+        by an FP64 batched collection whose tracers have sources or surface fluxes)."""
         if self._advection is not None:
             return
         grid = self.grid
@@ -100,47 +232,50 @@ class TracerCollection:
             self.p = None
 
     def __len__(self) -> int:
-        return self.values.shape[0]
+        return len(self._tracers)
 
-    def __getitem__(self, i: int) -> Array:
-        return Array(self.grid, self.values[i], CENTERS, self._names[i])
+    def __getitem__(self, i: int) -> Tracer:
+        return self._tracers[i]
 
-    def add(self, values=None, molecular_diffusivity: float = 0.0, name: str = "", source: Optional[Array] = None,
-            surface_flux: Optional[Array] = None, source_scale: float = 1.0,
-            vertical_velocity: Optional[Array] = None) -> Array:
-        """Append a tracer (device tensor or array of shape (nz, ny_, nx_)); returns its view.
+    def __iter__(self):
+        return iter(self._tracers)
 
-        ``source`` (layers), ``surface_flux`` (2-D), ``source_scale`` and
-        ``vertical_velocity`` (layers; movement through the water, e.g. sinking)
-        have the meaning of pygetm.tracer.Tracer's (tracer.py:35-95) and are
-        applied by :meth:`advance` as TracerCollection.advance does: in FP64 by the
-        per-tracer operator sequence, in FP32 (sources and surface fluxes, cnpar =
-        1) by the batched step's column solve."""
+    @property
+    def names(self) -> List[str]:
+        return [t.name for t in self._tracers]
+
+    def add(self, name: str = "", data=None, **kwargs) -> Tracer:
+        """Add a tracer (tracer.py:179-190): ``data`` (values of shape
+        (nz, ny_, nx_), device tensor or array; NaN if not given, as the
+        reference) and the keyword arguments of :class:`Tracer` (``source``,
+        ``surface_flux``, ``source_scale``, ``vertical_velocity``,
+        ``molecular_diffusivity``, and metadata).  Returns the tracer."""
         g = self.grid
-        if source is not None and (source.grid is not g or source.z != CENTERS):
-            raise ValueError("source must be a layer field of the tracer's grid")
-        if surface_flux is not None and (surface_flux.grid is not g or surface_flux.z):
-            raise ValueError("surface_flux must be a 2-D field of the tracer's grid")
-        if vertical_velocity is not None and (vertical_velocity.grid is not g or vertical_velocity.z != CENTERS):
-            raise ValueError("vertical_velocity must be a layer field of the tracer's grid")
-        new = torch.empty((len(self) + 1, g.nz_, g.ny_, g.nx_), dtype=self.dtype, device=g.device)
-        new[:-1].copy_(self.values)
-        if values is not None:
-            new[-1].copy_(torch.as_tensor(values, device=g.device))
-        self.values = new
+        old_v, old_r, n = self.values, self.residuals, len(self)
+        tracer = Tracer(self, n, name=name, **kwargs)
+        self._alloc(n + 1)
+        self.values[:n].copy_(old_v)
+        if self.compensated:
+            self.residuals[:n].copy_(old_r)
+        x = torch.full((g.nz_, g.ny_, g.nx_), float("nan"), dtype=torch.float64, device=g.device) if data is None \
+            else torch.as_tensor(data, device=g.device)
+        self._store(n, x)
         self.offsets = np.append(self.offsets, 0.0)
-        self._molecular.append(float(molecular_diffusivity))
-        self._names.append(name)
-        self._extra.append((source, surface_flux, float(source_scale), vertical_velocity))
-        return self[len(self) - 1]
+        self._tracers.append(tracer)
+        return tracer
 
-    def set_values(self, values: torch.Tensor, molecular_diffusivity: float = 0.0, offsets=None) -> None:
-        """Replace all tracers by a stacked (T, nz, ny_, nx_) tensor (copied).
+    def set_values(self, values: torch.Tensor, molecular_diffusivity: Optional[float] = None, offsets=None) -> None:
+        """Replace the values of all tracers by a stacked (T, nz, ny_, nx_) tensor
+        (copied; FP64 input keeps its precision in the compensated form).  With
+        as many tracers as the collection holds, the tracers keep their names,
+        sources, vertical velocities and molecular diffusivities; otherwise the
+        collection is rebuilt with plain tracers.  ``molecular_diffusivity``
+        given sets it for every tracer.
 
-        ``offsets`` (FP32 only) stores each tracer as its anomaly c - o_t: None
-        stores c itself; "auto" takes o_t = floor of the tracer's smallest value
-        over wet cells (integer offsets keep c - o_t exact for values above o_t);
-        a sequence of T numbers gives them."""
+        ``offsets`` (FP32 forms only) stores each tracer as its anomaly c - o_t:
+        None stores c itself; "auto" takes o_t = floor of the tracer's smallest
+        value over wet cells (integer offsets keep c - o_t exact for values above
+        o_t); a sequence of T numbers gives them."""
         g = self.grid
         if tuple(values.shape[1:]) != (g.nz_, g.ny_, g.nx_):
             raise ValueError(f"expected (T, {g.nz_}, {g.ny_}, {g.nx_}), got {tuple(values.shape)}")
@@ -148,8 +283,8 @@ class TracerCollection:
         if offsets is None:
             off = np.zeros(ntr)
         else:
-            if self.precision != "f32":
-                raise ValueError("offsets apply to the FP32 form only (the FP64 form is the reference's method)")
+            if self.precision == "f64":
+                raise ValueError("offsets apply to the FP32 forms only (the FP64 form is the reference's method)")
             if isinstance(offsets, str):
                 if offsets != "auto":
                     raise ValueError('offsets: None, "auto" or one number per tracer')
@@ -164,93 +299,117 @@ class TracerCollection:
         if off.any():
             o = torch.as_tensor(off, dtype=torch.float64, device=g.device).view(-1, 1, 1, 1)
             src = src.double() - o
-        self.values = src.to(dtype=self.dtype).contiguous().clone()
+        if ntr != len(self):
+            self._tracers = [Tracer(self, t, name=f"tracer{t}") for t in range(ntr)]
+            self._alloc(ntr)
+        self._store(slice(None), src)
         self.offsets = off
-        self._molecular = [float(molecular_diffusivity)] * values.shape[0]
-        self._names = [""] * values.shape[0]
-        self._extra = [(None, None, 1.0, None)] * values.shape[0]
+        if molecular_diffusivity is not None:
+            for t in self._tracers:
+                t.molecular_diffusivity = float(molecular_diffusivity)
+
+    # ---- the step ---------------------------------------------------------
+    def _combined_w(self, w: Array) -> List[Optional[torch.Tensor]]:
+        """Per tracer, w + its vertical velocity at interfaces (tracer.py:223-231,
+        interpolation.F90:41-54: 0.5 (c_k + c_{k+1}), end interfaces of w kept),
+        or None where a tracer has none or it is zero everywhere."""
+        out: List[Optional[torch.Tensor]] = []
+        k = 0
+        for tr in self._tracers:
+            vv = tr.vertical_velocity
+            if vv is None or not bool(vv.all_values.any()):
+                out.append(None)
+                continue
+            if k == len(self._wcombs):
+                self._wcombs.append(self.grid.array(z=INTERFACES, fill=0.0))
+            wc = self._wcombs[k].all_values
+            k += 1
+            wc.zero_()
+            wc[1:-1] = 0.5 * (vv.all_values[:-1] + vv.all_values[1:])
+            wc += w.all_values
+            out.append(wc)
+        return out
 
     def advance(self, timestep: float, u: Array, v: Array, w: Array, diffusivity: Array) -> None:
         """tracer.py:193-246: advection of all tracers with velocities at t + dt/2,
-        then vertical diffusion with dzo = dzn = hn (tracer.py:248-276).  An FP64
-        collection with tracer sources, surface fluxes or vertical velocities runs
-        the per-tracer sequence (bit-identical to the batched step for the other
-        tracers); the FP32 form takes sources and surface fluxes in its column
-        solve (cnpar = 1) and rejects per-tracer vertical velocities."""
-        vv_extra = any(e[3] is not None for e in self._extra)
-        src_extra = any(e[0] is not None or e[1] is not None for e in self._extra)
-        if self.precision == "f32":
-            if vv_extra:
-                raise ValueError("per-tracer vertical velocities need the FP64 form")
+        then vertical diffusion with dzo = dzn = hn (tracer.py:248-276), each
+        tracer with its own vertical velocity and molecular diffusivity.  The
+        FP64 batched step runs the per-tracer sequence when a tracer has sources
+        or a surface flux (bit-identical to the batched step for the others); the
+        FP32 forms take them in their column solve (cnpar = 1)."""
+        src_extra = any(t.source is not None or t.surface_flux is not None for t in self._tracers)
+        if self.precision != "f64":
             if src_extra and self.cnpar != 1.0:
-                raise ValueError("tracer sources in the FP32 form need cnpar = 1")
+                raise ValueError("tracer sources in the FP32 forms need cnpar = 1")
             if src_extra and self.conserve:
-                raise ValueError("the inventory fixer restores pre-step inventories; with sources call "
+                raise ValueError("the inventory fixer restores the inventory the tracers had; with sources call "
                                  "transport_conserve with targets that include them")
             sequence = False
         else:
-            sequence = vv_extra or src_extra
+            sequence = src_extra
         if not self.batched or sequence:
-            self._sequence_operators()
-            tracers = [self[i] for i in range(len(self))]
-            extra = dict(zip(map(id, tracers), self._extra))
-
-            def combined_w(tracer: Array) -> Array:
-                # tracer.py:223-231: w + the tracer's own vertical velocity at interfaces
-                # (interpolation.F90:41-54: 0.5 (c_k + c_{k+1}); end interfaces kept)
-                vv = extra[id(tracer)][3]
-                if vv is None or not bool(vv.all_values.any()):
-                    return w
-                wc = self._wcomb.all_values
-                wc[1:-1] = 0.5 * (vv.all_values[:-1] + vv.all_values[1:])
-                wc += w.all_values
-                return self._wcomb
-
-            self._advection.apply_3d_batch(u, v, w, timestep, tracers, Ah_u=self.Ah_u, Ah_v=self.Ah_v,
-                                           get_w=combined_w)
-            self.h.copy_(self._advection.h)
-            avmol = -1.0
-            for tracer, mol, (source, flux, scale, _) in zip(tracers, self._molecular, self._extra):
-                ea4 = None
-                if source is not None or flux is not None:
-                    # tracer.py:263-272
-                    ea4 = self._source.all_values
-                    if source is not None:
-                        ea4.copy_(source.all_values)
-                    else:
-                        ea4.zero_()
-                    if flux is not None:
-                        ea4[-1] += flux.all_values
-                    ea4 *= timestep * scale
-                if mol != avmol:
-                    avmol = mol
-                    self._vertical_diffusion.prepare(diffusivity, timestep, avmol)
-                self._vertical_diffusion.apply(tracer, ea4=self._source if ea4 is not None else None)
+            self._advance_sequence(timestep, u, v, w, diffusivity)
             return
-        mols = set(self._molecular)
-        if len(mols) > 1:
-            raise ValueError("the batched step needs one molecular diffusivity for all tracers")
-        mol = mols.pop() if mols else 0.0
-        if self.conserve and len(self):
+        if not len(self):
+            return
+        if self.conserve:
             # closed domain without sources: sum(area ho c) before = sum(area hn c) after
             self._target = self.inventory_device(self.grid.ho.all_values)
         ea4 = self._fp32_sources(timestep) if src_extra else None
+        lib = _lib.load()
         if ea4 is not None:
-            _lib.check(_lib.load().transport_set_sources(self.p, _lib.ptr(ea4)), "transport_set_sources")
+            _lib.check(lib.transport_set_sources(self.p, _lib.ptr(ea4)), "transport_set_sources")
+        wv = self._combined_w(w)
+        ntr = len(self)
+        wv_arr = (ctypes.c_void_p * ntr)(*[t.data_ptr() if t is not None else None for t in wv])
+        mol_arr = (ctypes.c_double * ntr)(*[t.molecular_diffusivity for t in self._tracers])
         try:
-            rc = _lib.load().transport_step(
-                self.p, len(self), _lib.ptr(self.values), _lib.ptr(u.all_values), _lib.ptr(v.all_values),
-                _lib.ptr(w.all_values), _lib.ptr(diffusivity.all_values),
+            rc = lib.transport_step_mixed(
+                self.p, ntr, _lib.ptr(self._buf), _lib.ptr(u.all_values), _lib.ptr(v.all_values),
+                _lib.ptr(w.all_values), ctypes.cast(wv_arr, ctypes.POINTER(ctypes.c_void_p)),
+                _lib.ptr(diffusivity.all_values),
                 _lib.ptr(self.Ah_u.all_values) if self.Ah_u is not None else None,
-                _lib.ptr(self.Ah_v.all_values) if self.Ah_v is not None else None, float(timestep), mol,
-                _lib.ptr(self.h), _lib.stream_handle())
-            _lib.check(rc, "transport_step")
+                _lib.ptr(self.Ah_v.all_values) if self.Ah_v is not None else None, float(timestep),
+                ctypes.cast(mol_arr, ctypes.POINTER(ctypes.c_double)), _lib.ptr(self.h), _lib.stream_handle())
+            _lib.check(rc, "transport_step_mixed")
         finally:
             if ea4 is not None:
-                _lib.load().transport_set_sources(self.p, None)
-        if self.conserve and len(self):
-            _lib.check(_lib.load().transport_conserve(self.p, len(self), _lib.ptr(self.values), _lib.ptr(self._target),
-                                                      _lib.stream_handle()), "transport_conserve")
+                lib.transport_set_sources(self.p, None)
+        if self.conserve:
+            _lib.check(lib.transport_conserve(self.p, ntr, _lib.ptr(self._buf), _lib.ptr(self._target),
+                                              _lib.stream_handle()), "transport_conserve")
+
+    def _advance_sequence(self, timestep, u, v, w, diffusivity) -> None:
+        """The reference's per-tracer operator sequence (FP64): apply_3d_batch with
+        each tracer's combined w, then prepare/apply per tracer with its sources."""
+        self._sequence_operators()
+        tracers = list(self._tracers)
+        wv = {id(t): c for t, c in zip(tracers, self._combined_w(w))}
+
+        def combined_w(tracer: Array) -> Array:
+            c = wv[id(tracer)]
+            return w if c is None else Array(self.grid, c, INTERFACES)
+
+        self._advection.apply_3d_batch(u, v, w, timestep, tracers, Ah_u=self.Ah_u, Ah_v=self.Ah_v,
+                                       get_w=combined_w)
+        self.h.copy_(self._advection.h)
+        avmol = -1.0
+        for tracer in tracers:
+            ea4 = None
+            if tracer.source is not None or tracer.surface_flux is not None:
+                # tracer.py:263-272
+                ea4 = self._source.all_values
+                if tracer.source is not None:
+                    ea4.copy_(tracer.source.all_values)
+                else:
+                    ea4.zero_()
+                if tracer.surface_flux is not None:
+                    ea4[-1] += tracer.surface_flux.all_values
+                ea4 *= timestep * tracer.source_scale
+            if tracer.molecular_diffusivity != avmol:
+                avmol = tracer.molecular_diffusivity
+                self._vertical_diffusion.prepare(diffusivity, timestep, avmol)
+            self._vertical_diffusion.apply(tracer, ea4=self._source if ea4 is not None else None)
 
     def _fp32_sources(self, timestep: float) -> torch.Tensor:
         """The ea4 terms of every tracer stacked (FP64; zero for tracers without
@@ -259,20 +418,21 @@ class TracerCollection:
         shape = (len(self), g.nz_, g.ny_, g.nx_)
         if getattr(self, "_ea4", None) is None or tuple(self._ea4.shape) != shape:
             self._ea4 = torch.zeros(shape, dtype=torch.float64, device=g.device)
-        for t, (source, flux, scale, _) in enumerate(self._extra):
+        for t, tr in enumerate(self._tracers):
             e = self._ea4[t]
-            if source is None and flux is None:
+            if tr.source is None and tr.surface_flux is None:
                 e.zero_()
                 continue
-            if source is not None:
-                e.copy_(source.all_values)
+            if tr.source is not None:
+                e.copy_(tr.source.all_values)
             else:
                 e.zero_()
-            if flux is not None:
-                e[-1] += flux.all_values
-            e *= timestep * scale
+            if tr.surface_flux is not None:
+                e[-1] += tr.surface_flux.all_values
+            e *= timestep * tr.source_scale
         return self._ea4
 
+    # ---- inventories and values -------------------------------------------
     def inventory_device(self, h: Optional[torch.Tensor] = None, absolute: bool = False) -> torch.Tensor:
         """Per-tracer sum of area * h * c over wet cells (FP64, compensated,
         deterministic) as a device tensor; h defaults to the T grid's hn.  Of the
@@ -281,32 +441,51 @@ class TracerCollection:
         if not self.batched:
             raise ValueError("inventories are computed by the batched transport object")
         out = torch.empty(len(self), dtype=torch.float64, device=self.grid.device)
-        _lib.check(_lib.load().transport_inventory(self.p, len(self), _lib.ptr(self.values),
+        if not len(self):
+            return out
+        _lib.check(_lib.load().transport_inventory(self.p, len(self), _lib.ptr(self._buf),
                                                    _lib.ptr(h) if h is not None else None, _lib.ptr(out),
                                                    _lib.stream_handle()), "transport_inventory")
         if absolute and self.offsets.any():
             g = self.grid
-            if self._ones is None:
-                self._ones = torch.ones((1, g.nz_, g.ny_, g.nx_), dtype=self.dtype, device=g.device)
-            vol = torch.empty(1, dtype=torch.float64, device=g.device)
-            _lib.check(_lib.load().transport_inventory(self.p, 1, _lib.ptr(self._ones),
-                                                       _lib.ptr(h) if h is not None else None, _lib.ptr(vol),
-                                                       _lib.stream_handle()), "transport_inventory")
+            vol = self.volume(h)
             out = out + torch.as_tensor(self.offsets, dtype=torch.float64, device=g.device) * vol
         return out
 
+    def volume(self, h: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Water volume sum(area * h) over wet cells (one-element FP64 tensor)."""
+        g = self.grid
+        hh = g.hn.all_values if h is None else h
+        wet = (g.mask.all_values == 1).to(hh.device)
+        return (g.area.all_values * hh * wet).sum().reshape(1)
+
     def absolute(self, i: Optional[int] = None) -> torch.Tensor:
-        """Tracer values c in FP64 (stored value plus offset): tracer i, or all
+        """Tracer values c in FP64 (decoded, plus the offset): tracer i, or all
         stacked."""
         if i is None:
             o = torch.as_tensor(self.offsets, dtype=torch.float64, device=self.grid.device).view(-1, 1, 1, 1)
-            return self.values.double() + o
-        return self.values[i].double() + float(self.offsets[i])
+            x = decode(self.values, self.residuals) if self.compensated else self.values.double()
+            return x + o
+        x = decode(self.values[i], self.residuals[i]) if self.compensated else self.values[i].double()
+        return x + float(self.offsets[i])
 
     def inventory(self, h: Optional[torch.Tensor] = None):
         """Host copy of inventory_device (simulation.py:1408-1466 report these)."""
         return self.inventory_device(h).cpu().numpy()
 
+    def check_finite(self, h: Optional[torch.Tensor] = None):
+        """Wet cells with a non-finite value (simulation.py:348-410 checks the
+        same after a step): None when every tracer is finite, else (tracer name,
+        k, j, i) of the first one in memory order."""
+        g = self.grid
+        wet = (g.mask.all_values == 1)
+        bad = ~torch.isfinite(self.values) & wet.unsqueeze(0).unsqueeze(0)
+        if not bool(bad.any()):
+            return None
+        t, k, j, i = (int(x) for x in torch.nonzero(bad)[0])
+        return self._tracers[t].name, k, j, i
+
+    # ---- launch shape and rounding ------------------------------------------
     def set_tuning(self, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
                    tracers_per_block_d: int = 16, tile_y: int = 0) -> None:
         """Launch shape of the batched kernels: tracers per block of the three kernel
@@ -319,24 +498,27 @@ class TracerCollection:
                    "transport_set_tuning")
 
     def set_solver(self, pcr_max_nz: int = 0) -> None:
-        """FP32 column solve: wave-level parallel cyclic reduction for columns of
-        2 <= nz <= pcr_max_nz (at most 32), per-lane Thomas elimination otherwise
-        (0: Thomas only).  The two agree to FP32 rounding."""
+        """FP32 column solve of the "f32-sr" form: wave-level parallel cyclic
+        reduction for columns of 2 <= nz <= pcr_max_nz (at most 32), per-lane
+        Thomas elimination otherwise (0: Thomas only).  The two agree to FP32
+        rounding.  The compensated form always eliminates per lane."""
         if not self.batched:
             return
         _lib.check(_lib.load().transport_set_solver(self.p, int(pcr_max_nz)), "transport_set_solver")
         self.pcr_max_nz = int(pcr_max_nz)
 
     def set_rounding(self, mode: str = "stochastic", counter: Optional[torch.Tensor] = None) -> None:
-        """Rounding of the FP32 tracer updates: "stochastic" (the default) rounds
-        each stored c + dc up or down with probability given by the exact
+        """Rounding of the "f32-sr" form's stores: "stochastic" (its default)
+        rounds each stored c + dc up or down with probability given by the exact
         residual, so rounding errors do not accumulate in a steady flow;
         "nearest" is plain FP32 arithmetic.  The random bits are a hash of (step
         count, stage, tracer, cell) -- deterministic, independent of launch shape
         and batching.  ``counter`` (a one-element int64 device tensor) holds the
         step count and is advanced by every step; None uses the collection's own
-        counter (zero at creation, saved by state_dict).  No effect on FP64."""
-        if not self.batched:
+        counter (zero at creation, saved by state_dict).  The other forms have
+        nothing to choose: FP64 is exact, the compensated form rounds at 2^-16 of
+        an FP32 ulp."""
+        if not self.batched or self.precision != "f32-sr":
             return
         if counter is not None and (counter.dtype != torch.int64 or counter.numel() != 1 or not counter.is_cuda):
             raise ValueError("the step counter is a one-element int64 device tensor")
@@ -348,29 +530,43 @@ class TracerCollection:
 
     # ---- restart: snapshot and restore of the state the step depends on ----
     def state_dict(self) -> dict:
-        """Host copy of the collection's state: tracer values (in their storage
-        precision), per-tracer names and molecular diffusivities, and for the FP32
-        form the rounding mode and step count.  A collection restored from it
-        continues bit for bit as the original would (the grid fields and the
-        velocities are inputs of every step, not state of the collection)."""
+        """Host copy of the collection's state: tracer values in their storage
+        precision (and the residuals of the compensated form), per-tracer names and
+        molecular diffusivities, offsets, and for the "f32-sr" form the rounding
+        mode and step count.  A collection restored from it continues bit for bit
+        as the original would (the grid fields and velocities are inputs of every
+        step; tracer sources and vertical velocities are references to the caller's
+        fields, which a restore keeps)."""
         torch.cuda.synchronize(self.grid.device)
         ctr = self._counter if self._counter is not None else getattr(self, "_own_counter", None)
-        return {"values": self.values.detach().cpu().numpy().copy(), "precision": self.precision,
-                "names": list(self._names), "molecular": list(self._molecular),
-                "rounding": self.rounding, "step": int(ctr.item()) if ctr is not None else 0,
-                "offsets": [float(o) for o in self.offsets]}
+        st = {"values": self.values.detach().cpu().numpy().copy(), "precision": self.precision,
+              "names": self.names, "molecular": [t.molecular_diffusivity for t in self._tracers],
+              "rounding": self.rounding, "step": int(ctr.item()) if ctr is not None else 0,
+              "offsets": [float(o) for o in self.offsets]}
+        if self.compensated:
+            st["residuals"] = self.residuals.detach().cpu().numpy().copy()
+        return st
 
     def load_state_dict(self, state: dict) -> None:
         """Restore a state_dict; the collection must have the same grid shape and
-        precision.  Tracer sources and vertical velocities are not part of it
-        (re-add them with ``add`` semantics if the run uses them)."""
+        precision.  With as many tracers as the state holds, the collection's
+        tracers keep their sources, surface fluxes and vertical velocities;
+        otherwise the collection is rebuilt with plain tracers."""
         if state["precision"] != self.precision:
             raise ValueError(f"state is {state['precision']}, collection is {self.precision}")
-        self.set_values(torch.from_numpy(np.ascontiguousarray(state["values"])))
-        self.offsets = np.asarray(state.get("offsets", np.zeros(len(self))), dtype=np.float64)
-        self._names = list(state["names"])
-        self._molecular = [float(m) for m in state["molecular"]]
-        if self.batched and self.precision == "f32":
+        vals = torch.from_numpy(np.ascontiguousarray(state["values"]))
+        ntr = vals.shape[0]
+        if ntr != len(self):
+            self._tracers = [Tracer(self, t) for t in range(ntr)]
+            self._alloc(ntr)
+        self.values.copy_(vals.to(self.grid.device))
+        if self.compensated:
+            self.residuals.copy_(torch.from_numpy(np.ascontiguousarray(state["residuals"])).to(self.grid.device))
+        self.offsets = np.asarray(state.get("offsets", np.zeros(ntr)), dtype=np.float64)
+        for t, name, mol in zip(self._tracers, state["names"], state["molecular"]):
+            t.name = name
+            t.molecular_diffusivity = float(mol)
+        if self.batched and self.precision == "f32-sr":
             self.set_rounding(state["rounding"], self._counter)
             ctr = self._counter if self._counter is not None else self._own_counter
             ctr.fill_(int(state["step"]))
@@ -378,14 +574,16 @@ class TracerCollection:
     def save(self, path: str) -> None:
         """Write state_dict to an .npz file (arrays and a JSON header; no pickles)."""
         st = self.state_dict()
-        meta = {k: v for k, v in st.items() if k != "values"}
-        np.savez(path, values=st["values"], meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+        arrays = {k: st.pop(k) for k in ("values", "residuals") if k in st}
+        np.savez(path, **arrays, meta=np.frombuffer(json.dumps(st).encode(), dtype=np.uint8))
 
     def load(self, path: str) -> None:
         """Restore a file written by ``save``."""
         with np.load(path, allow_pickle=False) as f:
             st = json.loads(bytes(f["meta"]).decode())
             st["values"] = f["values"]
+            if "residuals" in f:
+                st["residuals"] = f["residuals"]
         self.load_state_dict(st)
 
     def stats(self, reset: bool = True):

@@ -127,8 +127,13 @@ int c_vertical_diffusion_apply(void* vd, int nx, int ny, int nz, const int* mask
 
 /* ---- batched tracer step ------------------------------------------------- */
 
-/* Tracer storage/arithmetic precision of a transport object. */
-enum { GETM_F64 = 1, GETM_F32 = 2 };
+/* Tracer storage/arithmetic precision of a transport object.
+ * GETM_F32C (compensated FP32): tracers are stored as FP32 values plus int16
+ * residuals that extend them by 15 bits; the tracer buffer of transport_step,
+ * transport_inventory and transport_conserve is then ntracers FP32 planes
+ * (ntracers x nz x ny_ x nx_) immediately followed by ntracers int16 planes of
+ * the same shape, and every call on one buffer passes the same ntracers. */
+enum { GETM_F64 = 1, GETM_F32 = 2, GETM_F32C = 3 };
 
 /* Rounding of the FP32 tracer stores of the batched step. */
 enum { GETM_ROUND_NEAREST = 0, GETM_ROUND_STOCHASTIC = 1 };
@@ -142,7 +147,12 @@ enum { GETM_ROUND_NEAREST = 0, GETM_ROUND_STOCHASTIC = 1 };
  * ugrid/vgrid must have "hn" (face thicknesses) registered.
  * precision GETM_F64: tracers are double and results equal the per-tracer
  * calls bit for bit.  GETM_F32: tracers are float, stencil/limiter/tridiagonal
- * arithmetic is FP32, the thickness update is FP64. */
+ * arithmetic is FP32, the thickness update is FP64.  GETM_F32C: tracers are
+ * FP32 plus an int16 residual each (see above); limiter and tridiagonal
+ * elimination run in FP32 on differences of the FP64-decoded values, fluxes,
+ * cell updates and the thickness in FP64, and each launch rounds its results
+ * once (to 2^-16 of an FP32 ulp), so neither rounding errors nor inventory
+ * defects accumulate. */
 void* transport_create(void* tgrid, void* ugrid, void* vgrid, int scheme, double cnpar, int precision, int ufirst);
 void transport_finalize(void* tr);
 
@@ -154,6 +164,18 @@ void transport_finalize(void* tr);
 int transport_step(void* tr, int ntracers, void* tracers, const double* u, const double* v, const double* w,
                    const double* nuh, const double* Ah_u, const double* Ah_v, double timestep, double molecular,
                    double* h_out, void* stream);
+
+/* As transport_step, with a vertical velocity and a molecular diffusivity per
+ * tracer: tracer t is advected in the w stage by w_var[t] (the reference's
+ * get_w / combined_w, python/pygetm/tracer.py:223-231 and operators.py:108-109;
+ * the thickness still follows w) and diffused with molecular[t] (tracer.py:
+ * 273-275).  w_var is a host array of ntracers device pointers (NULL entries,
+ * or w_var NULL: w); molecular a host array of ntracers values.  The w stage
+ * and the diffusion run once per run of consecutive tracers with equal values,
+ * so the FP64 form stays bit-identical to the reference's per-tracer sequence. */
+int transport_step_mixed(void* tr, int ntracers, void* tracers, const double* u, const double* v, const double* w,
+                         const double* const* w_var, const double* nuh, const double* Ah_u, const double* Ah_v,
+                         double timestep, const double* molecular, double* h_out, void* stream);
 
 /* No reference counterpart: launch shape of the batched kernels -- the height
  * of the horizontal-stage tiles (4, 8 or 16 rows of 64 columns; 0 = the default,
@@ -206,7 +228,9 @@ int transport_inventory(void* tr, int ntracers, const void* tracers, const doubl
  * for FP32 tracers, restores each tracer's inventory (with "hn") to target[t]
  * (device, ntracers doubles) by changing a few cells by one ulp, chosen by
  * rounding a mass-proportional prefix sum over a fixed cell order.  Leaves the
- * inventory within one cell's rounding of the target.  No-op for FP64. */
+ * inventory within one cell's rounding of the target.  GETM_F32C: multiplies
+ * every wet cell by target[t] / inventory in FP64 (exact to the storage
+ * rounding).  No-op for FP64. */
 int transport_conserve(void* tr, int ntracers, void* tracers, const double* target, void* stream);
 
 /* z-sweep sub-stepping statistics of a transport object (as advection_get_stats). */

@@ -38,7 +38,7 @@ def single_domain(d, nsteps, solver=0):
     from getm_amd.tracer import TracerCollection
 
     g = make_grid(d)
-    tc = TracerCollection(g, advection_scheme=d.scheme, precision="f32")
+    tc = TracerCollection(g, advection_scheme=d.scheme, precision="f32-sr")
     tc.set_values(dev(d.tracers))
     if solver:
         tc.set_solver(solver)
