@@ -7,7 +7,10 @@ diffusion.F90:276-313) or wave-level parallel cyclic reduction for columns of
 up to ``pcr_max_nz`` levels.  Each candidate is timed with HIP events on the
 current stream, and the best is cached per workload signature (grid size
 bucket, nz, tracers) in a JSON file.  Launch shapes never change results; the
-two solvers agree to FP32 rounding (see tests).
+two solvers agree to FP32 rounding (see tests).  The timing steps run on the
+caller's tracers, which are restored afterwards with their rounding step
+counters: a tuned collection or ensemble continues from the state it had
+before tuning, whether the shape came from the cache or from timing.
 """
 from __future__ import annotations
 
@@ -81,17 +84,23 @@ class Autotuner:
         return best
 
     def tune_collection(self, tc, advance: Callable[[], None], **kw) -> Shape:
-        """Tune a batched getm_amd.tracer.TracerCollection; ``advance`` runs one step."""
+        """Tune a batched getm_amd.tracer.TracerCollection; ``advance`` runs one step.
+        The collection's state is restored afterwards."""
         g = tc.grid
         key = "step-" + tc.precision + "-" + signature(g.nx_ * g.ny_ * g.nz_, g.nz_, len(tc))
         def apply(s):
             tc.set_tuning(s[1], s[2], s[3], tile_y=s[0])
             tc.set_solver(s[4] if len(s) > 4 else 0)
 
-        return self.tune(key, apply, advance, **kw)
+        snap = tc.snapshot()
+        try:
+            return self.tune(key, apply, advance, **kw)
+        finally:
+            tc.restore(snap)
 
     def tune_ensemble(self, ens, **kw) -> Shape:
-        """Tune a getm_amd.ensemble.Ensemble (rebuilds its work tables per candidate)."""
+        """Tune a getm_amd.ensemble.Ensemble (rebuilds its work tables per candidate).
+        The ensemble's state is restored afterwards."""
         ncells = sum(g.nx_ * g.ny_ * g.nz_ for g in ens.grids)
         key = "ens-" + signature(ncells, max(g.nz_ for g in ens.grids), sum(ens.ntracers)) + f"-d{len(ens)}"
 
@@ -100,4 +109,8 @@ class Autotuner:
             ens.set_solver(s[4] if len(s) > 4 else 0)
             ens.commit()
 
-        return self.tune(key, apply, ens.step, **kw)
+        snap = ens.snapshot()
+        try:
+            return self.tune(key, apply, ens.step, **kw)
+        finally:
+            ens.restore(snap)

@@ -564,6 +564,17 @@ int transport_conserve(void* p, int ntracers, void* tracers, const double* targe
                           static_cast<double*>(tr->resid.ptr), s);
 }
 
+int transport_check_finite(void* p, int ntracers, const void* tracers, long long* result, void* stream)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    if (!tr || ntracers < 0 || (ntracers > 0 && (!tracers || !result))) return GETM_ERR_ARG;
+    const Grid* t = tr->tgrid;
+    if (!t->mask) return GETM_ERR_ARG;
+    const bool f32 = tr->precision != GETM_F64;  // GETM_F32C: the FP32 planes come first
+    return launch_check_finite(*t, ntracers, f32 ? static_cast<const float*>(tracers) : nullptr,
+                               f32 ? nullptr : static_cast<const double*>(tracers), result, as_stream(stream));
+}
+
 int transport_set_tuning(void* p, int tile_y, int tracers_per_block_h, int tracers_per_block_w, int tracers_per_block_d)
 {
     Transport* tr = static_cast<Transport*>(p);

@@ -221,6 +221,11 @@ int launch_inventory_c(const StepArgs& A, const double* h, const double* area, i
                        const short* lo, double* part, double* inv, const double* target, double* ratio, hipStream_t s);
 int launch_fix_c(const StepArgs& A, int ntr, float* f, short* lo, const double* ratio, hipStream_t s);
 size_t fix_part_doubles(const StepArgs& A, int ntr);
+
+// ---- non-finite detection (check.hip) ----
+// out: 2 ntr int64 -- per tracer the count of non-finite interior values with T
+// mask 1 or 2, and the in-tracer index of the first (INT64_MAX: none)
+int launch_check_finite(const Grid& g, int ntr, const float* f32, const double* f64, long long* out, hipStream_t s);
 size_t inventory_part_doubles(const StepArgs& A, int ntr);
 
 // ---- FP64 kernels (transport_fp64.hip) ----

@@ -4,8 +4,6 @@
 // called by pygetm's FABM coupling (python/pygetm/fabm.py:368) for the sinking and
 // floating of biogeochemical tracers.  FP64, bit-identical to the reference.
 // Arrays are the Fortran (nx, ny, nz) = C (nz, ny, nx) with k = 0 at the surface.
-#include <mutex>
-
 #include "common.h"
 #include "internal.h"
 
@@ -59,17 +57,6 @@ __global__ void __launch_bounds__(256) k_vadv_sources(int nx, int ny, int nz, in
     s[(size_t)(nz - 1) * n2 + p] = sk;
 }
 
-// The activity flag of each device (allocated on first use, kept until the process ends).
-static int* vadv_flag(int device)
-{
-    static std::mutex mu;
-    static int* flags[64] = {};
-    if (device < 0 || device >= 64) return nullptr;
-    std::lock_guard<std::mutex> lock(mu);
-    if (!flags[device] && hipMalloc((void**)&flags[device], sizeof(int)) != hipSuccess) flags[device] = nullptr;
-    return flags[device];
-}
-
 }  // namespace getm
 
 using namespace getm;
@@ -81,16 +68,18 @@ extern "C" int c_vertical_advection_to_sources(int nx, int ny, int nz, int halox
     if (nx < 1 || ny < 1 || nz < 1 || halox < 0 || haloy < 0) return GETM_ERR_ARG;
     if (!mask || !c || !w || !h || !s) return GETM_ERR_ARG;
     if (nx - 2 * halox < 1 || ny - 2 * haloy < 1) return GETM_OK;  // no interior point
-    int device = 0;
-    GETM_HIP_OK(hipGetDevice(&device));
-    int* flag = vadv_flag(device);
-    if (!flag) return GETM_ERR_HIP;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // the call's own activity flag, in stream-ordered memory: calls on different
+    // streams never share one, and nothing outlives the call
+    int* flag = nullptr;
+    GETM_HIP_OK(hipMallocAsync((void**)&flag, sizeof(int), st));
     dim3 block(64, 4);
     dim3 grid((nx - 2 * halox + 63) / 64, (ny - 2 * haloy + 3) / 4);
     hipLaunchKernelGGL(k_vadv_flag_clear, dim3(1), dim3(64), 0, st, flag);
     hipLaunchKernelGGL(k_vadv_active, grid, block, 0, st, nx, ny, nz, halox, haloy, w, flag);
     hipLaunchKernelGGL(k_vadv_sources, grid, block, 0, st, nx, ny, nz, halox, haloy, mask, c, w, h, s, flag);
-    GETM_HIP_OK(hipGetLastError());
+    const hipError_t launched = hipGetLastError();
+    GETM_HIP_OK(hipFreeAsync(flag, st));
+    GETM_HIP_OK(launched);
     return GETM_OK;
 }

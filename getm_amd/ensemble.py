@@ -8,7 +8,10 @@ Per-domain results do not depend on the other domains or on the launch shape
 and equal the single-domain FP32 batched step bit for bit.
 
 ``graph_chunk(n)`` captures n steps in a hipGraph (via torch.cuda.CUDAGraph on
-the current device) for replay without per-launch host overhead.
+the current device) for replay without per-launch host overhead.  ``snapshot``
+/ ``restore`` copy the ensemble's state (bound tracers and rounding step
+counters) out and back in place; graph_chunk and the autotuner use them so
+that their warm-up and timing steps leave no trace in the results.
 """
 from __future__ import annotations
 
@@ -45,6 +48,11 @@ class Ensemble:
         self.pcr_max_nz = 0
         self.rounding = "stochastic"
         self._counters = None
+        # the ensemble's own per-domain step counters, held here so that they can be
+        # snapshotted with the tracers (a caller's counters replace them)
+        dev = self.grids[0].device
+        self._own_counters = [torch.zeros(1, dtype=torch.int64, device=dev) for _ in range(n)]
+        self.set_rounding("stochastic")
 
     def __del__(self):
         p = getattr(self, "p", None)
@@ -91,17 +99,34 @@ class Ensemble:
         (zero at creation).  Takes effect at commit."""
         code = _ROUNDING[mode]
         n = len(self.grids)
-        arr = None
         if counters is not None:
             if len(counters) != n:
                 raise ValueError("one counter (or None) per domain is needed")
             for c in counters:
                 if c is not None and (c.dtype != torch.int64 or c.numel() != 1 or not c.is_cuda):
                     raise ValueError("step counters are one-element int64 device tensors")
-            arr = (ctypes.c_void_p * n)(*[(c.data_ptr() if c is not None else None) for c in counters])
+        used = [c if c is not None else own
+                for c, own in zip(counters if counters is not None else [None] * n, self._own_counters)]
+        arr = (ctypes.c_void_p * n)(*[c.data_ptr() for c in used])
         _lib.check(_lib.load().ensemble_set_rounding(self.p, code, arr), "ensemble_set_rounding")
-        self._counters = list(counters) if counters is not None else None
+        self._counters = used
         self.rounding = mode
+
+    def snapshot(self):
+        """Device copies of the state a step changes: every bound tracer tensor and
+        every domain's rounding step counter."""
+        return ([t.clone() if t is not None else None for t in self.tracers],
+                [c.clone() for c in self._counters])
+
+    def restore(self, snap) -> None:
+        """Copy a snapshot back into the bound tensors (in place: the library holds
+        their addresses)."""
+        tracers, counters = snap
+        for t, saved in zip(self.tracers, tracers):
+            if t is not None:
+                t.copy_(saved)
+        for c, saved in zip(self._counters, counters):
+            c.copy_(saved)
 
     def commit(self) -> None:
         """Allocate scratch and upload the work tables (synchronous)."""
@@ -111,8 +136,13 @@ class Ensemble:
         _lib.check(_lib.load().ensemble_step(self.p, _lib.stream_handle()), "ensemble_step")
 
     def graph_chunk(self, nsteps: int) -> "torch.cuda.CUDAGraph":
-        """Capture nsteps steps as one hipGraph; replay with ``g.replay()``."""
+        """Capture nsteps steps as one hipGraph; replay with ``g.replay()``.  The
+        warm-up step that sizes every lazily allocated buffer before the capture
+        is undone (tracers and step counters restored), and capturing runs
+        nothing, so the state is the one before the call."""
+        snap = self.snapshot()
         self.step()  # every lazily sized buffer exists before capture
+        self.restore(snap)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):

@@ -26,6 +26,8 @@ from typing import Optional
 
 import numpy as np
 
+from .vertical_coordinates import calculate_sigma
+
 HALO = 2
 M2_PERIOD = 44712.0
 
@@ -98,18 +100,6 @@ class Domain:
         return dataclasses.replace(self, **{f.name: getattr(self, f.name).copy()
                                             for f in dataclasses.fields(self)
                                             if isinstance(getattr(self, f.name), np.ndarray)})
-
-
-def calculate_sigma(nz: int, ddl: float = 0.0, ddu: float = 0.0) -> np.ndarray:
-    """Sigma layer fractions with tanh zooming, bottom first
-    (python/pygetm/vertical_coordinates.py:70-90)."""
-    if ddl <= 0.0 and ddu <= 0.0:
-        return np.full(nz, 1.0 / nz)
-    ddl, ddu = max(ddl, 0.0), max(ddu, 0.0)
-    ga = np.linspace(0.0, 1.0, nz + 1)
-    ga[1:-1] = np.tanh((ddl + ddu) * ga[1:-1] - ddl) + np.tanh(ddl)
-    ga[1:-1] /= np.tanh(ddl) + np.tanh(ddu)
-    return np.diff(ga)
 
 
 def uv_masks(tmask: np.ndarray):

@@ -473,17 +473,40 @@ class TracerCollection:
         """Host copy of inventory_device (simulation.py:1408-1466 report these)."""
         return self.inventory_device(h).cpu().numpy()
 
-    def check_finite(self, h: Optional[torch.Tensor] = None):
-        """Wet cells with a non-finite value (simulation.py:348-410 checks the
-        same after a step): None when every tracer is finite, else (tracer name,
-        k, j, i) of the first one in memory order."""
+    def non_finite(self) -> np.ndarray:
+        """Per tracer (rows): the number of non-finite values in interior cells
+        whose mask is 1 or 2, and the (k, j, i) of the first in memory order
+        (-1s if none); counted on the device (``transport_check_finite``)."""
+        n = len(self)
+        out = torch.empty(2 * max(n, 1), dtype=torch.int64, device=self.grid.device)
+        if n:
+            _lib.check(_lib.load().transport_check_finite(self.p, n, _lib.ptr(self._buf), _lib.ptr(out),
+                                                          _lib.stream_handle()), "transport_check_finite")
+        res = out[:2 * n].view(n, 2).cpu().numpy()
         g = self.grid
-        wet = (g.mask.all_values == 1)
-        bad = ~torch.isfinite(self.values) & wet.unsqueeze(0).unsqueeze(0)
-        if not bool(bad.any()):
-            return None
-        t, k, j, i = (int(x) for x in torch.nonzero(bad)[0])
-        return self._tracers[t].name, k, j, i
+        rows = np.full((n, 4), -1, dtype=np.int64)
+        for t, (count, first) in enumerate(res):
+            rows[t, 0] = count
+            if count:
+                rows[t, 1:] = (first // (g.ny_ * g.nx_), (first // g.nx_) % g.ny_, first % g.nx_)
+        return rows
+
+    def check_finite(self) -> bool:
+        """The tracers' part of Simulation.check_finite (simulation.py:348-410):
+        logs every tracer with non-finite values in unmasked interior cells,
+        with their count and the first one's (k, j, i), and returns whether all
+        values were finite."""
+        if not self.batched:
+            raise ValueError("the finiteness check runs on the batched transport object")
+        ok = True
+        g = self.grid
+        unmasked = int(((g.mask.values == 1) | (g.mask.values == 2)).sum()) * g.nz_
+        for tr, (count, k, j, i) in zip(self._tracers, self.non_finite()):
+            if count:
+                ok = False
+                self.logger.error(f"Field {tr.name} has {count} non-finite values (out of {unmasked} unmasked "
+                                  f"values); the first at (k, j, i) = ({k}, {j}, {i}).")
+        return ok
 
     # ---- launch shape and rounding ------------------------------------------
     def set_tuning(self, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
@@ -528,21 +551,41 @@ class TracerCollection:
         self._counter = counter
         self.rounding = mode
 
+    def snapshot(self):
+        """Device copies of the state a step changes (the tracer buffer, values and
+        residuals, and the rounding step counter), for ``restore``."""
+        ctr = self._counter if self._counter is not None else getattr(self, "_own_counter", None)
+        return self._buf.clone(), (ctr.clone() if ctr is not None else None)
+
+    def restore(self, snap) -> None:
+        """Copy a snapshot back in place (same tracer count as when it was taken)."""
+        buf, ctr = snap
+        if buf.shape != self._buf.shape:
+            raise ValueError("the snapshot holds another number of tracers")
+        self._buf.copy_(buf)
+        mine = self._counter if self._counter is not None else getattr(self, "_own_counter", None)
+        if mine is not None and ctr is not None:
+            mine.copy_(ctr)
+
     # ---- restart: snapshot and restore of the state the step depends on ----
     def state_dict(self) -> dict:
         """Host copy of the collection's state: tracer values in their storage
         precision (and the residuals of the compensated form), per-tracer names and
-        molecular diffusivities, offsets, and for the "f32-sr" form the rounding
-        mode and step count.  A collection restored from it continues bit for bit
-        as the original would (the grid fields and velocities are inputs of every
-        step; tracer sources and vertical velocities are references to the caller's
-        fields, which a restore keeps)."""
+        molecular diffusivities, offsets, for the "f32-sr" form the rounding mode
+        and step count, and the grid's layer thicknesses ho and hn (the reference
+        keeps T.ho in its restarts as part of the model state, simulation.py:620-622;
+        hn is the thickness the tracers were last stepped to).  A collection
+        restored from it continues bit for bit as the original would (velocities
+        and diffusivity are inputs of every step; tracer sources and vertical
+        velocities are references to the caller's fields, which a restore keeps)."""
         torch.cuda.synchronize(self.grid.device)
         ctr = self._counter if self._counter is not None else getattr(self, "_own_counter", None)
         st = {"values": self.values.detach().cpu().numpy().copy(), "precision": self.precision,
               "names": self.names, "molecular": [t.molecular_diffusivity for t in self._tracers],
               "rounding": self.rounding, "step": int(ctr.item()) if ctr is not None else 0,
-              "offsets": [float(o) for o in self.offsets]}
+              "offsets": [float(o) for o in self.offsets],
+              "ho": self.grid.ho.all_values.detach().cpu().numpy().copy(),
+              "hn": self.grid.hn.all_values.detach().cpu().numpy().copy()}
         if self.compensated:
             st["residuals"] = self.residuals.detach().cpu().numpy().copy()
         return st
@@ -554,6 +597,16 @@ class TracerCollection:
         otherwise the collection is rebuilt with plain tracers."""
         if state["precision"] != self.precision:
             raise ValueError(f"state is {state['precision']}, collection is {self.precision}")
+        missing = [k for k in ("values", "ho", "hn") if k not in state]
+        if missing:
+            # as the reference's restart (simulation.py:185-197)
+            raise ValueError(f"the state lacks field(s) of the model state: {', '.join(missing)}")
+        for name in ("ho", "hn"):
+            dst = getattr(self.grid, name).all_values
+            src = np.asarray(state[name])
+            if tuple(src.shape) != tuple(dst.shape):
+                raise ValueError(f"{name}: state has shape {src.shape}, grid has {tuple(dst.shape)}")
+            dst.copy_(torch.from_numpy(np.ascontiguousarray(src)).to(dst.device))
         vals = torch.from_numpy(np.ascontiguousarray(state["values"]))
         ntr = vals.shape[0]
         if ntr != len(self):
@@ -574,16 +627,16 @@ class TracerCollection:
     def save(self, path: str) -> None:
         """Write state_dict to an .npz file (arrays and a JSON header; no pickles)."""
         st = self.state_dict()
-        arrays = {k: st.pop(k) for k in ("values", "residuals") if k in st}
+        arrays = {k: st.pop(k) for k in ("values", "residuals", "ho", "hn") if k in st}
         np.savez(path, **arrays, meta=np.frombuffer(json.dumps(st).encode(), dtype=np.uint8))
 
     def load(self, path: str) -> None:
         """Restore a file written by ``save``."""
         with np.load(path, allow_pickle=False) as f:
             st = json.loads(bytes(f["meta"]).decode())
-            st["values"] = f["values"]
-            if "residuals" in f:
-                st["residuals"] = f["residuals"]
+            for k in ("values", "residuals", "ho", "hn"):
+                if k in f:
+                    st[k] = f[k]
         self.load_state_dict(st)
 
     def stats(self, reset: bool = True):

@@ -233,6 +233,15 @@ int transport_inventory(void* tr, int ntracers, const void* tracers, const doubl
  * rounding).  No-op for FP64. */
 int transport_conserve(void* tr, int ntracers, void* tracers, const double* target, void* stream);
 
+/* The tracer part of the reference's Simulation.check_finite
+ * (python/pygetm/simulation.py:348-410): for each of ntracers stacked tracers
+ * (the object's precision; GETM_F32C: the FP32 values), result[2 t] = the
+ * number of non-finite values in interior cells whose T mask is 1 or 2 and
+ * result[2 t + 1] = the index within the tracer, k * ny_ * nx_ + j * nx_ + i,
+ * of the first of them (INT64_MAX if none).  result: device, 2 ntracers int64;
+ * the call initialises it (by a kernel, so it may be graph-captured). */
+int transport_check_finite(void* tr, int ntracers, const void* tracers, long long* result, void* stream);
+
 /* z-sweep sub-stepping statistics of a transport object (as advection_get_stats). */
 int transport_get_stats(void* tr, int* max_iters, int* capped_columns, int reset, void* stream);
 

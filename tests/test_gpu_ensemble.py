@@ -92,8 +92,9 @@ def test_ensemble_graph_chunks_replay_equal_eager():
     doms = small_ensemble(5, seed=11)
     eager = make_ensemble(doms)
     graphed = make_ensemble(doms)
-    g = graphed.graph_chunk(2)  # graph_chunk itself runs one eager step first
-    eager.step()
+    g = graphed.graph_chunk(2)  # its warm-up step is undone: the state is the initial one
+    for i in range(len(doms)):
+        assert_bitwise(host(graphed.tracers[i]), host(eager.tracers[i]), f"after capture, domain {i}")
     for rep in range(3):
         g.replay()
         torch.cuda.synchronize()
@@ -104,21 +105,31 @@ def test_ensemble_graph_chunks_replay_equal_eager():
             assert_bitwise(host(graphed.tracers[i]), host(eager.tracers[i]), f"replay {rep} domain {i}")
 
 
-def test_autotuner_keeps_results():
+def test_autotuner_keeps_results(tmp_path):
+    """Tuning leaves the tracers and rounding step counters as they were, whether it
+    times candidates (PCR and Thomas solves round differently) or applies a cached
+    shape: the tuned ensemble then steps exactly as an untuned one."""
     from getm_amd.autotune import Autotuner
 
     doms = small_ensemble(5, seed=13)
-    ens = make_ensemble(doms)
-    ref = make_ensemble(doms)
-    tuner = Autotuner()
-    n_before = 0
-    shape = tuner.tune_ensemble(ens, candidates=[(16, 0, 16, 16), (8, 2, 4, 4)], reps=1)
-    assert shape in [(16, 0, 16, 16), (8, 2, 4, 4)]
-    # tuning ran 2 candidates x (1 warm-up + 1 timed) steps
-    for _ in range(4):
-        ref.step()
-    for i in range(len(doms)):
-        assert_bitwise(host(ens.tracers[i]), host(ref.tracers[i]), f"domain {i}")
+    cands = [(16, 0, 16, 16, 0), (8, 2, 4, 4, 32)]
+    cache = str(tmp_path / "tune.json")
+    for run in ("timed", "cached"):
+        ens = make_ensemble(doms)
+        ref = make_ensemble(doms)
+        tuner = Autotuner(cache)
+        shape = tuner.tune_ensemble(ens, candidates=cands, reps=1)
+        assert shape in cands
+        for i in range(len(doms)):
+            assert_bitwise(host(ens.tracers[i]), host(ref.tracers[i]), f"{run}: after tuning, domain {i}")
+        ref.set_tuning(*shape[:4])
+        ref.set_solver(shape[4])
+        ref.commit()
+        for _ in range(2):
+            ens.step()
+            ref.step()
+        for i in range(len(doms)):
+            assert_bitwise(host(ens.tracers[i]), host(ref.tracers[i]), f"{run}: two steps later, domain {i}")
 
 
 def test_ensemble_pcr_by_depth_equals_single_domains():
@@ -133,3 +144,78 @@ def test_ensemble_pcr_by_depth_equals_single_domains():
         ens.step()
     for i, d in enumerate(doms):
         assert_bitwise(host(ens.tracers[i]), single_domain(d, 3, solver=24), f"domain {i} (nz {d.nz})")
+
+
+@pytest.mark.parametrize("scheme", [5, 1, 3])
+def test_ensemble_domains_follow_the_oracle(oracle_mod, scheme):
+    """Every domain of an uneven ensemble against the FP64 oracle (its restatement of
+    TracerCollection.advance): within 1e-5 of each tracer's largest value after 3 steps."""
+    doms = small_ensemble(scheme, seed=21)
+    ens = make_ensemble(doms)
+    for _ in range(3):
+        ens.step()
+    for i, d in enumerate(doms):
+        ref = d.tracers.astype(np.float32).astype(np.float64)  # the ensemble starts from FP32 values
+        for _ in range(3):
+            oracle_mod.transport_step(d, ref)
+        got = host(ens.tracers[i]).astype(np.float64)
+        wet = np.broadcast_to(d.tmask == 1, d.ho.shape)
+        for t in range(d.ntracers):
+            err = np.abs(got[t][wet] - ref[t][wet]).max() / np.abs(ref[t][wet]).max()
+            assert err <= 1e-5, (i, t, err)
+
+
+def test_96_domain_ensemble_graph_replay_equals_eager():
+    """The full uneven ensemble: 96 domains with nx, ny log-uniform in [64, 768], nz
+    uniform in [10, 60] and 1-32 blob tracers each.  A 100-step hipGraph replay equals
+    100 eager steps bit for bit, and sampled domains equal their single-domain steps."""
+    import torch
+
+    from getm_amd.core import CENTERS, INTERFACES, Grid
+    from getm_amd.ensemble import Ensemble
+    from getm_amd.precision import _blob_tracers
+
+    specs = inputs.ensemble_specs(96, seed=3)
+    grids, keep, fields = [], [], []
+    for nx, ny, nz, nt, seed in specs:
+        d = inputs.coastal(nx=nx, ny=ny, nz=nz, dx=2000.0, dt=30.0, scheme=5, seed=seed, ntracers=0)
+        g = Grid.from_domain(d)
+        f = (g.ugrid.wrap(dev(d.u), CENTERS), g.vgrid.wrap(dev(d.v), CENTERS), g.wrap(dev(d.w), INTERFACES),
+             g.wrap(dev(d.nuh), INTERFACES))
+        grids.append(g)
+        fields.append((d.dt, f))
+        keep.append(_blob_tracers(nt, g, seed, torch.float32))
+    print(f"96 domains: {sum(s[3] for s in specs)} tracers, nz {min(s[2] for s in specs)}-{max(s[2] for s in specs)}",
+          flush=True)
+    ens = Ensemble(grids, [s[3] for s in specs], scheme=5)
+    for i, (dt, (u, v, w, nuh)) in enumerate(fields):
+        ens.bind(i, keep[i], u, v, w, nuh, dt)
+    ens.commit()
+    start = ens.snapshot()
+    for _ in range(100):
+        ens.step()
+    eager = [host(t) for t in ens.tracers]
+    ens.restore(start)
+    graph = ens.graph_chunk(100)
+    graph.replay()
+    torch.cuda.synchronize()
+    for i in range(96):
+        assert_bitwise(host(ens.tracers[i]), eager[i], f"domain {i}")
+    print("100-step replay equals eager stepping", flush=True)
+    # sampled domains (the smallest, the largest, the deepest) against their single-domain step
+    ncells = [s[0] * s[1] * s[2] for s in specs]
+    sample = {int(np.argmin(ncells)), int(np.argmax(ncells)), int(np.argmax([s[2] for s in specs]))}
+    from getm_amd.tracer import TracerCollection
+
+    ens.restore(start)
+    for _ in range(3):
+        ens.step()
+    for i in sorted(sample):
+        tc = TracerCollection(grids[i], advection_scheme=5, precision="f32-sr")
+        tc.set_values(start[0][i])
+        dt, args = fields[i]
+        for _ in range(3):
+            tc.advance(dt, *args)
+        assert_bitwise(host(ens.tracers[i]), host(tc.values), f"domain {i} {specs[i][:4]}")
+        mi, capped = tc.stats()
+        assert capped == 0, (i, mi, capped)

@@ -64,3 +64,38 @@ def test_vadv_sources_flag_is_cleared_between_calls(oracle_mod):
     assert_bitwise(_gpu(halo, mask, c, w, h, s), want, "active call")
     w0 = np.zeros_like(w)
     assert _gpu(halo, mask, c, w0, h, s).tobytes() == s.tobytes()
+
+
+def test_vadv_sources_on_two_streams(oracle_mod):
+    """Two calls with different inputs, one of them inactive, enqueued on two
+    streams without synchronizing in between: each call has its own activity
+    flag, so both results equal the oracle's bit for bit."""
+    from getm_amd.operators import vertical_advection_to_sources
+
+    cases = [_case(7, 41, 33, 16), _case(8, 41, 33, 16)]
+    halo = cases[0][0]
+    inner = np.zeros(cases[1][5].shape[1:], bool)
+    inner[halo:-halo, halo:-halo] = True
+    cases[1][3][:, inner] = 0.0  # the second call is inactive
+    want = []
+    for _, mask, c, w, h, s in cases:
+        ws = s.copy()
+        oracle_mod.vertical_advection_to_sources(halo, halo, mask, c, w, h, ws)
+        want.append(ws)
+    assert not np.array_equal(want[0], cases[0][5]) and np.array_equal(want[1], cases[1][5])
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    dev_cases = [tuple(d(a) for a in case[1:]) for case in cases]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [[], []]
+    for _ in range(4):  # both streams filled before either is waited for
+        for idx, (st, (mask, c, w, h, s)) in enumerate(zip(streams, dev_cases)):
+            with torch.cuda.stream(st):
+                s_run = s.clone()
+                vertical_advection_to_sources(halo, halo, mask, c, w, h, s_run)
+                outs[idx].append(s_run)
+    for st in streams:
+        st.synchronize()
+    for idx in range(2):
+        for rep, s_run in enumerate(outs[idx]):
+            assert_bitwise(s_run.cpu().numpy(), want[idx], f"stream {idx}, call {rep}")

@@ -609,3 +609,37 @@ def test_sigma_fractions_and_thicknesses(oracle_mod):
     h = np.full((20, 2, 2), -1.0)
     oracle_mod.update_sigma(np.ascontiguousarray(surf), D, mask, h)
     assert np.array_equal(h[:, 0, 1], surf * 33.0) and (h[:, 1, 0] == -1.0).all()
+
+
+# ---------------------------------------------------------------- stage order of the step
+@pytest.mark.parametrize("scheme", [5, 3])
+def test_transport_step_runs_v_first(oracle_mod, scheme):
+    """The step's stage order is v, u, w, u, v (python/pygetm/operators.py:159-162:
+    adv1 = u_3d, adv2 = v_3d, swapped unless ufirst), each tracer starting every
+    stage from the thickness the previous stage left, then the diffusion with hn.
+    oracle.transport_step equals that composition of the single sweeps (pinned to
+    the reference Fortran above) bit for bit, and the u-first composition gives
+    different bits on these inputs, so a step that swapped the order would fail."""
+    d = inputs.tiny(nx=29, ny=23, nz=6, seed=9, scheme=scheme, ntracers=1, cfl=0.4, wscale=0.5)
+    o = oracle_mod
+    half = 0.5 * d.dt
+
+    def composed(order):
+        h, f = d.ho.copy(), d.tracers[0].copy()
+        sweep = {"u": lambda: o.advection_uv(1, scheme, d.umask, d.idxu, d.dyu, d.hu, d.u, d.tmask, d.iarea, None,
+                                             half, h, f),
+                 "v": lambda: o.advection_uv(2, scheme, d.vmask, d.dxv, d.idyv, d.hv, d.v, d.tmask, d.iarea, None,
+                                             half, h, f),
+                 "w": lambda: o.advection_w(scheme, d.w, d.w, d.tmask, d.dt, h, f)}
+        for s in order:
+            sweep[s]()
+        o.vertical_diffusion(d.nuh, d.dt, f, d.tmask, d.hn, cnpar=d.cnpar)
+        return f, h
+
+    v_first, hv = composed("vuwuv")
+    u_first, _ = composed("uvwvu")
+    step = d.tracers.copy()
+    hs = o.transport_step(d, step, return_h=True)
+    assert_bitwise(step[0], v_first, "transport_step against v, u, w, u, v")
+    assert_bitwise(hs, hv, "thickness")
+    assert not np.array_equal(v_first, u_first), "the inputs must tell the two orders apart"
