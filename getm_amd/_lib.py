@@ -67,6 +67,7 @@ _SIGNATURES = {
     "transport_set_rounding": (_i, [_vp, _i, _vp]),
     "transport_inventory": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "transport_conserve": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "transport_conserve_noalloc": (_i, [_vp, _i, _vp, _vp, _vp]),
     "transport_check_finite": (_i, [_vp, _i, _vp, _vp, _vp]),
     "ensemble_create": (_vp, [_i, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                ctypes.POINTER(_i), _i, _d, _i]),

@@ -3,6 +3,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -452,6 +453,62 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
 }
 
 // Checks shared by transport_step and transport_step_mixed, then the step.
+static int conserve_check(const Transport* tr, int ntracers, const void* tracers, const double* target)
+{
+    if (!tr || ntracers < 1 || !tracers || !target) return GETM_ERR_ARG;
+    if (tr->precision == GETM_F64) return GETM_OK;
+    const Grid* t = tr->tgrid;
+    if (!t->area || !t->mask || !t->hn) return GETM_ERR_ARG;
+    if (ntracers > 65535) return GETM_ERR_CAPACITY;
+    return GETM_OK;
+}
+
+// The fixer of the FP32 forms with its working memory given: part (inventory
+// partial sums), ratio (per tracer), and for GETM_F32 part2 (fixer partial sums)
+// and resid (2 per tracer).
+static int conserve_run(Transport* tr, const StepArgs& A, int ntracers, void* tracers, const double* target,
+                        double* part, double* part2, double* ratio, double* resid, hipStream_t s)
+{
+    const Grid* t = tr->tgrid;
+    float* f = static_cast<float*>(tracers);
+    int rc;
+    if (tr->precision == GETM_F32C) {
+        // x (1 + (target - I) / I) on every wet cell, exact to the storage rounding
+        short* lo = reinterpret_cast<short*>(f + t->n3() * (size_t)ntracers);
+        if ((rc = launch_inventory_c(A, t->hn, t->area, ntracers, f, lo, part, nullptr, target, ratio, s))) return rc;
+        return launch_fix_c(A, ntracers, f, lo, ratio, s);
+    }
+    if ((rc = launch_inventory<float>(A, t->hn, t->area, ntracers, f, part, nullptr, target, ratio, s))) return rc;
+    return launch_fix_f32(A, ntracers, f, part, ratio, t->area, part2, target, resid, s);
+}
+
+// Working memory of transport_conserve_noalloc: one array per GPU, in the
+// library's code object, held until the process ends; every call on a GPU
+// shares it.  Host code must not use the name `scratch` as an address (it is a
+// host-side stand-in): conserve_scratch looks up each device's address of it
+// once, with that device current, and callers use that.
+constexpr size_t kConserveScratchDoubles = size_t(4) << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kConserveScratchDoubles];
+
+static std::mutex scratch_lock;
+static double* scratch_at[256] = {};  // device address of `scratch` per device ordinal
+
+static int conserve_scratch(double** out)
+{
+    int dev = -1;
+    GETM_HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) return GETM_ERR_ARG;
+    std::lock_guard<std::mutex> guard(scratch_lock);
+    if (!scratch_at[dev]) {
+        void* p = nullptr;
+        GETM_HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return GETM_ERR_HIP;
+        scratch_at[dev] = static_cast<double*>(p);
+    }
+    *out = scratch_at[dev];
+    return GETM_OK;
+}
+
 static int transport_step_any(Transport* tr, int ntracers, void* tracers, const double* u, const double* v,
                               const double* w, const double* const* w_var, const double* nuh, const double* Ah_u,
                               const double* Ah_v, double timestep, const double* molecular, double* h_out,
@@ -536,32 +593,37 @@ int transport_inventory(void* p, int ntracers, const void* tracers, const double
 int transport_conserve(void* p, int ntracers, void* tracers, const double* target, void* stream)
 {
     Transport* tr = static_cast<Transport*>(p);
-    if (!tr || ntracers < 1 || !tracers || !target) return GETM_ERR_ARG;
-    if (tr->precision == GETM_F64) return GETM_OK;
-    const Grid* t = tr->tgrid;
-    if (!t->area || !t->mask || !t->hn) return GETM_ERR_ARG;
-    if (ntracers > 65535) return GETM_ERR_CAPACITY;
+    int rc = conserve_check(tr, ntracers, tracers, target);
+    if (rc || tr->precision == GETM_F64) return rc;
     const StepArgs A = step_args(tr, nullptr, nullptr, nullptr, nullptr);
-    int rc;
-    const size_t np = inventory_part_doubles(A, ntracers);
-    if ((rc = tr->part.ensure(np * sizeof(double))) ||
-        (rc = tr->part2.ensure(fix_part_doubles(A, ntracers) * sizeof(double))) ||
+    if ((rc = tr->part.ensure(inventory_part_doubles(A, ntracers) * sizeof(double))) ||
+        (tr->precision == GETM_F32 && (rc = tr->part2.ensure(fix_part_doubles(A, ntracers) * sizeof(double)))) ||
         (rc = tr->ratio.ensure((size_t)ntracers * sizeof(double))) ||
-        (rc = tr->resid.ensure(2 * (size_t)ntracers * sizeof(double))))
+        (tr->precision == GETM_F32 && (rc = tr->resid.ensure(2 * (size_t)ntracers * sizeof(double)))))
         return rc;
-    double* part = static_cast<double*>(tr->part.ptr);
-    double* ratio = static_cast<double*>(tr->ratio.ptr);
-    hipStream_t s = as_stream(stream);
-    float* f = static_cast<float*>(tracers);
-    if (tr->precision == GETM_F32C) {
-        // x (1 + (target - I) / I) on every wet cell, exact to the storage rounding
-        short* lo = reinterpret_cast<short*>(f + t->n3() * (size_t)ntracers);
-        if ((rc = launch_inventory_c(A, t->hn, t->area, ntracers, f, lo, part, nullptr, target, ratio, s))) return rc;
-        return launch_fix_c(A, ntracers, f, lo, ratio, s);
-    }
-    if ((rc = launch_inventory<float>(A, t->hn, t->area, ntracers, f, part, nullptr, target, ratio, s))) return rc;
-    return launch_fix_f32(A, ntracers, f, part, ratio, t->area, static_cast<double*>(tr->part2.ptr), target,
-                          static_cast<double*>(tr->resid.ptr), s);
+    return conserve_run(tr, A, ntracers, tracers, target, static_cast<double*>(tr->part.ptr),
+                        static_cast<double*>(tr->part2.ptr), static_cast<double*>(tr->ratio.ptr),
+                        static_cast<double*>(tr->resid.ptr), as_stream(stream));
+}
+
+int transport_conserve_noalloc(void* p, int ntracers, void* tracers, const double* target, void* stream)
+{
+    Transport* tr = static_cast<Transport*>(p);
+    int rc = conserve_check(tr, ntracers, tracers, target);
+    if (rc || tr->precision == GETM_F64) return rc;
+    const StepArgs A = step_args(tr, nullptr, nullptr, nullptr, nullptr);
+    const size_t np = inventory_part_doubles(A, ntracers);
+    const size_t n2 = tr->precision == GETM_F32 ? fix_part_doubles(A, ntracers) : 0;
+    const size_t nr = tr->precision == GETM_F32 ? 2 * (size_t)ntracers : 0;
+    if (np + n2 + (size_t)ntracers + nr > kConserveScratchDoubles) return GETM_ERR_CAPACITY;
+    double* base = nullptr;
+    if ((rc = conserve_scratch(&base))) return rc;
+    // every kernel writes its part of the scratch before reading it: nothing to clear
+    double* part = base;
+    double* part2 = part + np;
+    double* ratio = part2 + n2;
+    double* resid = ratio + ntracers;
+    return conserve_run(tr, A, ntracers, tracers, target, part, part2, ratio, resid, as_stream(stream));
 }
 
 int transport_check_finite(void* p, int ntracers, const void* tracers, long long* result, void* stream)

@@ -233,6 +233,19 @@ int transport_inventory(void* tr, int ntracers, const void* tracers, const doubl
  * rounding).  No-op for FP64. */
 int transport_conserve(void* tr, int ntracers, void* tracers, const double* target, void* stream);
 
+/* No reference counterpart: transport_conserve for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; the working
+ * memory is a 32 MiB array inside the library, one per GPU, held until the
+ * process ends, instead of buffers the transport object grows on demand.
+ * Needs B = ceil(nx_ / 64) * ceil(ny_ / 4) blocks of partial sums per tracer:
+ * GETM_F32C uses 3 B + 1 doubles per tracer, GETM_F32 7 B + 3; more than
+ * 4 Mi doubles in total returns GETM_ERR_CAPACITY (e.g. 1028 x 772 cells with
+ * halos: up to 426 GETM_F32C tracers, 182 GETM_F32 tracers).  Calls that
+ * overlap on one GPU (other streams or threads) share the array and race;
+ * calls ordered on one stream do not. */
+int transport_conserve_noalloc(void* tr, int ntracers, void* tracers, const double* target, void* stream);
+
 /* The tracer part of the reference's Simulation.check_finite
  * (python/pygetm/simulation.py:348-410): for each of ntracers stacked tracers
  * (the object's precision; GETM_F32C: the FP32 values), result[2 t] = the
