@@ -1,11 +1,17 @@
 """Benchmark of the tracer-transport step (advection + vertical diffusion).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f64|f32] [--tracers T]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload multi|coastal|ensemble|producers]
+                    [--precision f32|f32-sr|f64] [--tracers T]
 
-A step is one TracerCollection.advance (python/pygetm/tracer.py:193-276) over
-the synthetic coastal workload: 1024 x 768 x 40 cells, 2 km grid, 30% land,
-40 zoomed sigma layers, SUPERBEE, dt = 30 s, inputs resident in HBM before the
-timed region.  Prints one JSON line.
+A step is one TracerCollection.advance (python/pygetm/tracer.py:193-276).  The
+default workload is the multi-tracer batch: 256 passive tracers, each a sum of
+4 Gaussian blobs with amplitudes log-uniform in [1e-3, 10], on the synthetic
+coastal grid (1024 x 768 x 40 cells, 2 km, 30% land, 40 sigma layers, dt 30 s),
+P2_PDM, in the compensated FP32 form (FP32 values plus int16 residuals, FP64
+thickness) with the inventory fixer on: every timed step includes the pre-step
+inventory and the fixer.  Inputs are resident in HBM before the timed region.
+"--workload coastal" is the same grid with temperature and salinity, SUPERBEE.
+Prints one JSON line.
 
 metric: tracer cell-updates/s = wet cells (mask == 1) x nz x tracers x steps / time.
 roofline: the dominant kernel's algorithmic bytes per launch / its mean launch
@@ -17,15 +23,20 @@ serving: a short run of the serving loop (getm_amd.serve) on 24 dye-release
 requests: saturation throughput, then Poisson arrivals at 80% of it, with
 latency percentiles, inventory drift and the error against the FP64 method.
 accuracy (FP32 runs): the first (up to 16) tracers, which evolve independently
-of the others, rerun from the same initial values for --accuracy-steps steps in
-FP64 (the plain method), FP32 as benchmarked, FP32 with the inventory fixer, and
-FP32 storing anomalies about per-tracer offsets (the same kernels; offsets
-"auto"): max |c32 - c64| / max |c64| over wet cells, and the largest per-step
+of the others, rerun from the same (FP32) initial values for --accuracy-steps
+steps in FP64 (the plain method), in the benchmarked FP32 form without and with
+the inventory fixer, and for the stochastic-rounding form also storing anomalies
+about per-tracer offsets (the same kernels; offsets "auto"):
+max |c32 - c64| / max |c64| over wet cells, and the largest per-step
 inventory defect |I_after - I_before| / |I_before| (FP64 compensated sums, of the
 absolute values).
+fp64: the plain method (the batched FP64 step, bit-identical to the reference's
+algorithm) on the coastal grid with temperature and salinity, SUPERBEE, timed
+over --fp64-steps steps after the main run.
 cpu_baseline: the reference's Fortran sweeps (oracle/_ref) with the C restatement
-of the diffusion, FP64, on bounded slabs of the same workload, one process per
-host core (the C restatement alone where oracle/_ref is not built).
+of the diffusion, FP64, on bounded slabs of the same grid with the workload's
+limiter, one tracer, one process per host core (the C restatement alone where
+oracle/_ref is not built); the host CPU model is reported with it.
 
 --workload producers times the kernels on either side of the step at the same
 coastal size (one launch each, HIP events on the launch stream, median of
@@ -49,6 +60,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FORM_NAMES = {"f32": "compensated FP32 tracers (FP32 values + int16 residuals) with FP64 thickness",
+              "f32-sr": "FP32 tracers with stochastic rounding and FP64 thickness",
+              "f64": "FP64 bit-exact"}
 NX, NY, NZ = 1024, 768, 40
 METRIC = "tracer cell-updates/s (adv+diff step), % of HBM roofline, p50/p99 solve latency"
 
@@ -67,7 +81,7 @@ def recorded_traffic(kernel: str, workload: str):
     return None
 
 
-def accuracy_check(d, grid, fields, init, scheme, steps, rounding, log):
+def accuracy_check(d, grid, fields, init, scheme, steps, form, rounding, log):
     """FP32 error against the FP64 method and inventory defects (module docstring)."""
     import numpy as np
     import torch
@@ -85,9 +99,11 @@ def accuracy_check(d, grid, fields, init, scheme, steps, rounding, log):
             e = max(e, float((tc.absolute(t)[wet] - r).abs().max() / r.abs().max()))
         return e
 
-    for name, conserve, offsets in (("plain", False, None), ("with_fixer", True, None),
-                                    ("anomalies", False, "auto")):
-        tc, dd, _ = precision.run(d, grid, fields, init, scheme, steps, "f32", conserve, rounding, offsets)
+    runs = [("plain", False, None), ("with_fixer", True, None)]
+    if form == "f32-sr":
+        runs.append(("anomalies", False, "auto"))
+    for name, conserve, offsets in runs:
+        tc, dd, _ = precision.run(d, grid, fields, init, scheme, steps, form, conserve, rounding, offsets)
         out[name] = {"max_error_vs_fp64": err(tc), "max_step_inventory_defect": dd}
         if offsets is not None:
             out[name]["offsets"] = [float(o) for o in tc.offsets]
@@ -138,6 +154,10 @@ def algorithmic_bytes(kernel: str, n_wet_cols: int, nz: int, s: int, tracers: in
         return cells * (tr + 16) + n_wet_cols * (8 * (nz + 1) + 4)
     if kernel == "vdiff_f32":  # + hn, nuh at the nz-1 inner interfaces; mask
         return cells * (tr + 8) + n_wet_cols * (8 * (nz - 1) + 4)
+    if kernel == "inventory":  # tracers and h read; area and mask per column
+        return cells * (tracers * s + 8) + n_wet_cols * 12
+    if kernel == "fix":  # tracers read and written; mask per column
+        return cells * tracers * 2 * s + n_wet_cols * 4
     if kernel == "adv_uv_f64":   # f, h read+write; velocity, face thickness; masks + 3 metrics per column
         return cells * (16 + 16 + 16) + n_wet_cols * (8 + 24)
     if kernel == "adv_w_f64":    # f, h read+write; w at interfaces; mask
@@ -163,13 +183,13 @@ def _cpu_worker(args):
     from the reference sources) in the reference call order (v/2, u/2, w, u/2, v/2),
     then the C restatement of the diffusion; impl "c": the C restatement throughout.
     Both give bit-identical tracers."""
-    seed, seconds, impl = args
+    seed, seconds, impl, scheme = args
     import numpy as np
 
     import oracle
     from getm_amd import inputs
 
-    d = inputs.coastal(nx=NX // 4, ny=NY // 4, nz=NZ, seed=100 + seed, ntracers=1)
+    d = inputs.coastal(nx=NX // 4, ny=NY // 4, nz=NZ, seed=100 + seed, ntracers=1, scheme=scheme)
     half = 0.5 * d.dt
     ua = (d.umask, d.idxu, d.dyu, d.hu, d.u, d.tmask, d.iarea, None, half)
     va = (d.vmask, d.dxv, d.idyv, d.hv, d.v, d.tmask, d.iarea, None, half)
@@ -199,7 +219,21 @@ def _cpu_worker(args):
     return d.n_wet_cells * n / el
 
 
-def cpu_baseline(seconds: float, cores: int):
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or the platform's processor string."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(seconds: float, cores: int, scheme: int):
     """Summed rate of ``cores`` processes; the reference Fortran sweeps when oracle/_ref
     is built (shipped with the tree), else the C restatement."""
     import oracle
@@ -208,7 +242,7 @@ def cpu_baseline(seconds: float, cores: int):
     impl = "ref" if oracle.ref_available() else "c"
     ctx = mp.get_context("spawn")
     with ctx.Pool(cores) as pool:
-        rates = pool.map(_cpu_worker, [(i, seconds, impl) for i in range(cores)])
+        rates = pool.map(_cpu_worker, [(i, seconds, impl, scheme) for i in range(cores)])
     return sum(rates), cores, impl
 
 
@@ -237,6 +271,39 @@ def blob_tracers(ntr, grid, seed, dtype):
             gz = torch.exp(-0.5 * ((zs - z0) / sz) ** 2)
             acc += amp * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
         out[t] = acc.to(dtype)
+    return out
+
+
+def fp64_run(steps: int, warmup: int, log):
+    """The plain FP64 batched step on the coastal grid with temperature and
+    salinity, SUPERBEE: tracer cell-updates/s and ms per step."""
+    import numpy as np
+    import torch
+
+    from getm_amd import inputs
+    from getm_amd.core import CENTERS, INTERFACES, Grid
+    from getm_amd.tracer import TracerCollection
+
+    d = inputs.coastal(nx=NX, ny=NY, nz=NZ, scheme=5, ntracers=2, tracer_kind="TS")
+    grid = Grid.from_domain(d, device="cuda:0")
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    u, v = grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS)
+    w, nuh = grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES)
+    tc = TracerCollection(grid, advection_scheme=5, cnpar=1.0, precision="f64")
+    tc.set_values(dev(d.tracers), molecular_diffusivity=0.0)
+    for _ in range(warmup):
+        tc.advance(d.dt, u, v, w, nuh)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tc.advance(d.dt, u, v, w, nuh)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    out = {"workload": f"coastal {NX}x{NY}x{NZ}, temperature and salinity, SUPERBEE, dt 30 s, FP64 batched step "
+                       "(bit-identical to the reference's algorithm)",
+           "value": d.n_wet_columns * NZ * 2 * steps / el, "unit": "tracer cell-updates/s",
+           "ms_per_step": el * 1e3 / steps, "steps": steps}
+    log(f"[bench] fp64: {out}")
     return out
 
 
@@ -492,8 +559,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["coastal", "multi", "ensemble", "producers"], default="coastal",
-                    help="coastal: 1024x768x40 with T and S, SUPERBEE; multi: same grid, 256 blob tracers, P2_PDM; "
+    ap.add_argument("--workload", choices=["multi", "coastal", "ensemble", "producers"], default="multi",
+                    help="multi: 1024x768x40, 256 blob tracers, P2_PDM; coastal: same grid with T and S, SUPERBEE; "
                          "ensemble: 96 uneven domains in one launch per kernel kind, hipGraph chunk")
     ap.add_argument("--domains", type=int, default=96, help="ensemble workload: number of domains")
     ap.add_argument("--autotune", action="store_true", help="ensemble workload: autotune the launch shape first")
@@ -505,9 +572,12 @@ def main():
     ap.add_argument("--solver", type=int, default=0,
                     help="wave-PCR column solve for columns of up to this many layers (0: Thomas only)")
     ap.add_argument("--rounding", choices=["stochastic", "nearest"], default="stochastic",
-                    help="FP32 tracer stores: stochastic (default) or round-to-nearest")
-    ap.add_argument("--accuracy-steps", type=int, default=100,
+                    help="f32-sr tracer stores: stochastic (default) or round-to-nearest")
+    ap.add_argument("--no-conserve", action="store_true",
+                    help="FP32 forms: leave the inventory fixer out of the step")
+    ap.add_argument("--accuracy-steps", type=int, default=200,
                     help="FP32 runs: steps of the error check against the FP64 method")
+    ap.add_argument("--fp64-steps", type=int, default=10, help="steps of the FP64 block (0: skip it)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-cores", type=int, default=0)
@@ -552,8 +622,9 @@ def main():
     v = grid.vgrid.wrap(dev(d.v), CENTERS)
     w = grid.wrap(dev(d.w), INTERFACES)
     nuh = grid.wrap(dev(d.nuh), INTERFACES)
+    conserve = args.precision != "f64" and not args.no_conserve
     tc = TracerCollection(grid, advection_scheme=scheme, cnpar=1.0, precision=args.precision,
-                          batched=not args.per_tracer)
+                          batched=not args.per_tracer, conserve=conserve)
     if args.tuning:
         tc.set_tuning(*[int(x) for x in args.tuning.split(",")])
     if args.solver:
@@ -564,10 +635,10 @@ def main():
     else:
         tc.set_values(dev(d.tracers))
     del d.u, d.v, d.w, d.nuh, d.tracers
-    s_bytes = 4 if args.precision == "f32" else 8
+    s_bytes = {"f32": 6, "f32-sr": 4, "f64": 8}[args.precision]  # stored bytes per tracer value
     # initial values of the tracers whose FP32 error is checked (tracers evolve independently)
     n_check = min(ntr, 16)
-    init_check = tc.values[:n_check].clone() if args.precision == "f32" and not args.no_accuracy else None
+    init_check = tc.values[:n_check].clone() if args.precision != "f64" and not args.no_accuracy else None
 
     def step():
         tc.advance(d.dt, u, v, w, nuh)
@@ -605,28 +676,34 @@ def main():
     log = lambda m: print(m, file=sys.stderr, flush=True)
     accuracy = None
     if init_check is not None:
-        accuracy = accuracy_check(d, grid, (u, v, w, nuh), init_check, scheme, args.accuracy_steps, args.rounding, log)
+        accuracy = accuracy_check(d, grid, (u, v, w, nuh), init_check, scheme, args.accuracy_steps, args.precision,
+                                  args.rounding, log)
         del init_check
     workload_key = f"{args.workload}-{args.precision}-T{args.tracers}-s{args.scheme}" + (
         "-pertracer" if args.per_tracer else "")
     rec = recorded_traffic(dom, workload_key)
+    del tc, u, v, w, nuh, grid
+    torch.cuda.empty_cache()
+    fp64 = fp64_run(args.fp64_steps, 2, log) if args.fp64_steps > 0 else None
+    torch.cuda.empty_cache()
     serving = None
     if not args.no_serve:
-        del tc, u, v, w, nuh, grid
-        torch.cuda.empty_cache()
         serving = serving_run(args.serve_requests, log)
 
     cpu = None
     if not args.no_cpu:
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
-        rate, cores, impl = cpu_baseline(args.cpu_seconds, cores)
+        rate, cores, impl = cpu_baseline(args.cpu_seconds, cores, scheme)
         what = ("the reference's Fortran u2d/v2d/w3d (oracle/_ref, built from the reference sources) in the "
                 "reference call order + the C restatement of diffusion.F90 (not buildable here)"
                 if impl == "ref" else "FP64 oracle (C restatement of the reference Fortran)")
+        model = cpu_model()
         cpu = {"value": rate, "unit": "tracer cell-updates/s", "cores": cores,
-               "kind": "reference" if impl == "ref" else "port",
-               "sample": f"{what}; {cores} processes each stepping a {NX // 4}x{NY // 4}x{NZ} coastal slab, "
-                         f"1 tracer, FP64, ~{args.cpu_seconds:.0f} s each"}
+               "kind": "reference" if impl == "ref" else "port", "cpu_model": model,
+               "sample": f"{what}; {cores} processes on {model}, each stepping a {NX // 4}x{NY // 4}x{NZ} "
+                         f"coastal slab, 1 tracer, scheme {scheme}, FP64, ~{args.cpu_seconds:.0f} s each (the "
+                         "reference steps tracers one at a time, so its rate per tracer does not depend on their "
+                         "number)"}
 
     out = {
         "metric": METRIC,
@@ -639,19 +716,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision,
+        "dtype": {"f32": "f32+f64", "f32-sr": "f32", "f64": "f64"}[args.precision],
         "data": "synthetic (seeded coastal generator, continuity-consistent)",
         "config": {
-            "workload": f"coastal {NX}x{NY}x{NZ}, 2 km grid, 30% land, scheme {scheme}, dt 30 s, "
-                        f"{ntr} tracer(s), " + ("FP32 tracers with FP64 thickness" if args.precision == "f32"
-                                                 else "FP64 bit-exact") +
-                        (", per-tracer operators" if args.per_tracer else ", batched step"),
+            "workload": (("multi-tracer batch: " if kind == "blobs" else "single large domain: ") +
+                         f"coastal {NX}x{NY}x{NZ}, 2 km grid, 30% land, scheme {scheme}, dt 30 s, {ntr} " +
+                         ("blob tracers" if kind == "blobs" else "tracer(s) (temperature, salinity)") + ", " +
+                         FORM_NAMES[args.precision] + (" with the inventory fixer in every step" if conserve else "") +
+                         (", per-tracer operators" if args.per_tracer else ", batched step")),
             "grid": [NX, NY, NZ],
             "wet_columns": n_wet_cols,
             "tracers": args.tracers,
             "scheme": args.scheme,
             "precision": args.precision,
-            "rounding": args.rounding if args.precision == "f32" else None,
+            "conserve": conserve,
+            "bytes_per_tracer_value": s_bytes,
+            "rounding": args.rounding if args.precision == "f32-sr" else None,
             "step_hbm_fraction": b_step / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
             "kernel_ms_per_step": {k: v[0] / args.steps for k, v in ktimes.items()},
             "tuning": args.tuning or "default",
@@ -670,6 +750,7 @@ def main():
             "algorithmic_bytes": alg,
         },
         "accuracy": accuracy,
+        "fp64": fp64,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -3,13 +3,14 @@
     python -m getm_amd.precision --workload coastal --steps 500 --schemes 1,2,3,4,5,6
     python -m getm_amd.precision --workload multi --steps 200
 
-Runs the batched step from identical inputs four ways on the GPU: FP64 (the
-plain method, bit-identical to the reference algorithm -- see the GPU parity
-tests); FP32 with FP64 thickness and tracer stores rounded to nearest
-("f32_nearest"), or stochastically ("f32", the default form); the latter with
-the inventory fixer ("f32_fixed"); and stochastically rounded anomalies about
-per-tracer offsets (``offsets="auto"``, "f32_offsets"), compared on their
-absolute values and absolute inventories.  Reports,
+Runs the batched step from identical inputs on the GPU: FP64 (the plain method,
+bit-identical to the reference algorithm -- see the GPU parity tests) against
+the FP32 forms, all with FP64 thickness: the compensated form (FP32 values plus
+int16 residuals, "f32"), with the inventory fixer ("f32_fixed"); FP32 stores
+rounded stochastically ("f32-sr"), with the fixer ("f32-sr_fixed") and as
+anomalies about per-tracer offsets (``offsets="auto"``, "f32-sr_offsets"); and
+FP32 stores rounded to nearest ("f32-sr_nearest"), compared on their absolute
+values and absolute inventories.  Reports,
 per limiter, the error max |c32 - c64| / max |c64| over wet cells (max and
 median over tracers) after the run, and the largest per-step conservation
 defect |sum(area hn c_after) - sum(area ho c_before)| / |sum(area ho c_before)|
@@ -75,12 +76,22 @@ def run(d, grid, fields, init, scheme, steps, precision, conserve, rounding="sto
     return tc, float(worst.item()), time.perf_counter() - t0
 
 
+# (report name, precision, inventory fixer, rounding of the FP32 stores, offsets)
+FORMS = (("f32", "f32", False, "nearest", None),
+         ("f32_fixed", "f32", True, "nearest", None),
+         ("f32-sr", "f32-sr", False, "stochastic", None),
+         ("f32-sr_fixed", "f32-sr", True, "stochastic", None),
+         ("f32-sr_offsets", "f32-sr", False, "stochastic", "auto"),
+         ("f32-sr_nearest", "f32-sr", False, "nearest", None))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=["coastal", "multi"], default="coastal")
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--schemes", default="")
     ap.add_argument("--tracers", type=int, default=0)
+    ap.add_argument("--forms", default="", help="comma-separated subset of the FP32 forms (default: all)")
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
@@ -102,11 +113,10 @@ def main(argv=None):
         ref, d64, el64 = run(d, grid, fields, init, scheme, args.steps, "f64", False)
         log(f"[precision] scheme {scheme} f64: {args.steps} steps in {el64:.1f} s, per-step defect {d64:.1e}")
         rows = {"f64": {"max_step_conservation_defect": d64, "seconds": el64}}
-        for name, conserve, rounding, offsets in (("f32_nearest", False, "nearest", None),
-                                                  ("f32", False, "stochastic", None),
-                                                  ("f32_fixed", True, "stochastic", None),
-                                                  ("f32_offsets", False, "stochastic", "auto")):
-            tc, dd, el = run(d, grid, fields, init, scheme, args.steps, "f32", conserve, rounding, offsets)
+        for name, form, conserve, rounding, offsets in FORMS:
+            if args.forms and name not in args.forms.split(","):
+                continue
+            tc, dd, el = run(d, grid, fields, init, scheme, args.steps, form, conserve, rounding, offsets)
             err = []
             for t in range(ntr):
                 r = ref.values[t][wet]
