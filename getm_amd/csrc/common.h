@@ -76,6 +76,63 @@ __device__ __forceinline__ R face_value(R cfl, R fu, R fuu, R fd)
 
 __device__ __forceinline__ bool open_face(int m) { return m == 1 || m == 2; }
 
+// XCD-aware 1-D grid over nx * ny * nz logical blocks (x fastest).  Work-groups
+// go to the 8 XCDs round-robin (block b to XCD b % 8) and each XCD has its own
+// L2; block b takes logical block v = (b % 8) (gridDim.x / 8) + b / 8, so every
+// XCD walks a contiguous range of the logical order and the blocks resident on
+// one XCD are neighbours along x (then y): the cache lines that neighbouring
+// blocks share (rows are not aligned to lines) come from that XCD's L2.
+// gridDim.x is the block count rounded up to a multiple of 8; false: padding.
+__device__ __forceinline__ bool xcd_block(unsigned nx, unsigned ny, unsigned nz, unsigned& bx, unsigned& by,
+                                          unsigned& bz)
+{
+    const unsigned b = blockIdx.x, per = gridDim.x / 8;
+    unsigned v = (b % 8) * per + b / 8;
+    if (v >= nx * ny * nz) return false;
+    bx = v % nx;
+    v /= nx;
+    by = v % ny;
+    bz = v / ny;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Raw buffer access (CDNA buffer resources): the base and extent of an array
+// in SGPRs, a 32-bit per-lane byte offset and a wave-uniform byte offset, so
+// an access costs no 64-bit address arithmetic per lane and the per-level
+// offsets of a column stay scalar.  Lane offsets at or beyond `bytes` read 0
+// and write nothing (the hardware's range check); callers stay in range.
+// ---------------------------------------------------------------------------
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, unsigned bytes)
+{
+    // stride 0 (raw), the gfx9 default third dword (32-bit data format)
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float ld_f32(Rsrc r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ short ld_i16(Rsrc r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(short, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+}
+__device__ __forceinline__ double ld_f64(Rsrc r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void st_f32(Rsrc r, unsigned voff, unsigned soff, float x)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, voff, soff, 0);
+}
+__device__ __forceinline__ void st_i16(Rsrc r, unsigned voff, unsigned soff, short x)
+{
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, x), r, voff, soff, 0);
+}
+
+// Size of the 1-D grid of xcd_block for n logical blocks.
+inline unsigned xcd_grid(size_t n) { return (unsigned)((n + 7) / 8 * 8); }
+
 // 1/x to about one ulp of double for x > 0: the hardware estimate refined by two
 // Newton steps.  Used only where the result is rounded to FP32 afterwards (and
 // x is a positive layer thickness), never on the bit-exact FP64 path.

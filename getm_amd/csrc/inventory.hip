@@ -35,6 +35,7 @@
 namespace getm {
 
 constexpr int IBX = 64, IBY = 4, INV_THREADS = IBX * IBY;
+constexpr int INV_TCH = 8;  // tracers per block of k_inventory_tr
 
 __device__ __forceinline__ void neumaier(double& s, double& c, double x)
 {
@@ -102,6 +103,88 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, con
         pt[blk] = S;
         pt[nblk + blk] = C;
         pt[2 * nblk + blk] = Q;
+    }
+}
+
+// Sum of two compensated pairs (s1, c1) + (s2, c2).  Symmetric in its two
+// arguments (the error term comes from the larger of s1, s2 either way), so a
+// butterfly over a wave leaves the same pair in every lane.
+__device__ __forceinline__ void neumaier_merge(double& s, double& c, double s2, double c2)
+{
+    const double t = s + s2;
+    const double e = fabs(s) >= fabs(s2) ? (s - t) + s2 : (s2 - t) + s;
+    c = (c + c2) + e;
+    s = t;
+}
+
+// Partial sums per (tracer, block) of the FP64 and compensated forms, TCH
+// tracers per block: each lane sums its column over k in order for every
+// tracer of the chunk, with the column's area * h loaded once for all of them;
+// the block combines its lanes by a butterfly over each wave and the four
+// waves in order (a fixed order: deterministic), and writes the same
+// (S, C, 0) triple per block as k_inventory.  1-D XCD-aware grid over
+// (column block x, column block y, tracer chunk), so that neighbouring column
+// blocks share their L2.
+template <typename R, bool COMP, int TCH>
+__global__ void __launch_bounds__(INV_THREADS) k_inventory_tr(const StepArgs A, const double* __restrict__ h,
+                                                              const double* __restrict__ area, int ntr,
+                                                              const R* __restrict__ f,
+                                                              const short* __restrict__ lo,
+                                                              double* __restrict__ part)
+{
+    __shared__ double s_s[IBY][TCH], s_c[IBY][TCH];
+    const unsigned nbx = (A.nx_ + IBX - 1) / IBX, nby = (A.ny_ + IBY - 1) / IBY;
+    unsigned bx, by, bz;
+    if (!xcd_block(nbx, nby, (ntr + TCH - 1) / TCH, bx, by, bz)) return;
+    const int i = bx * IBX + threadIdx.x, j = by * IBY + threadIdx.y;
+    const int t0 = bz * TCH;
+    const int nt = ntr - t0 < TCH ? ntr - t0 : TCH;
+    const int nblk = nbx * nby;
+    const int blk = by * nbx + bx;
+    double m[TCH], mc[TCH];
+#pragma unroll
+    for (int tt = 0; tt < TCH; tt++) m[tt] = mc[tt] = 0.0;
+    if (i < A.nx_ && j < A.ny_) {
+        const size_t p = (size_t)j * A.nx_ + i;
+        const bool wet = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[p] == 1;
+        if (wet) {
+            const double a = area[p];
+            const size_t base = (size_t)t0 * A.n3 + p;
+            for (int k = 0; k < A.nz; k++) {
+                const size_t o = (size_t)k * A.n2;
+                const double ah = a * h[o + p];
+#pragma unroll
+                for (int tt = 0; tt < TCH; tt++) {
+                    if (tt < nt) {
+                        const size_t q = base + (size_t)tt * A.n3 + o;
+                        double x;
+                        if constexpr (COMP)
+                            x = cdecode(f[q], lo[q]);
+                        else
+                            x = (double)f[q];
+                        neumaier(m[tt], mc[tt], ah * x);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int tt = 0; tt < TCH; tt++) {
+        for (int d = 1; d < 64; d <<= 1) neumaier_merge(m[tt], mc[tt], __shfl_xor(m[tt], d), __shfl_xor(mc[tt], d));
+        if (threadIdx.x == 0) {
+            s_s[threadIdx.y][tt] = m[tt];
+            s_c[threadIdx.y][tt] = mc[tt];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.y == 0 && threadIdx.x < nt) {
+        const int tt = threadIdx.x;
+        double S = s_s[0][tt], C = s_c[0][tt];
+        for (int w = 1; w < IBY; w++) neumaier_merge(S, C, s_s[w][tt], s_c[w][tt]);
+        double* pt = part + (size_t)(t0 + tt) * 3 * nblk;
+        pt[blk] = S;
+        pt[nblk + blk] = C;
+        pt[2 * nblk + blk] = 0.0;
     }
 }
 
@@ -441,8 +524,9 @@ int launch_inventory_c(const StepArgs& A, const double* h, const double* area, i
     dim3 block(IBX, IBY, 1);
     dim3 grid((A.nx_ + IBX - 1) / IBX, (A.ny_ + IBY - 1) / IBY, ntr);
     const int nblk = grid.x * grid.y;
+    const dim3 tgrid(xcd_grid((size_t)nblk * ((ntr + INV_TCH - 1) / INV_TCH)));
     ProfileScope ps(GETM_K_INVENTORY, s);
-    hipLaunchKernelGGL((k_inventory<float, true>), grid, block, 0, s, A, h, area, ntr, f, part, lo);
+    hipLaunchKernelGGL((k_inventory_tr<float, true, INV_TCH>), tgrid, block, 0, s, A, h, area, ntr, f, lo, part);
     hipLaunchKernelGGL(k_inventory_reduce, dim3(ntr), dim3(64), 0, s, ntr, nblk, part, inv, target, ratio);
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
@@ -466,8 +550,16 @@ int launch_inventory(const StepArgs& A, const double* h, const double* area, int
     dim3 grid((A.nx_ + IBX - 1) / IBX, (A.ny_ + IBY - 1) / IBY, ntr);
     const int nblk = grid.x * grid.y;
     ProfileScope ps(GETM_K_INVENTORY, s);
-    hipLaunchKernelGGL((k_inventory<R>), grid, block, 0, s, A, h, area, ntr, f, part, nullptr);
-    hipLaunchKernelGGL(k_inventory_reduce, dim3(ntr), dim3(64), 0, s, ntr, nblk, part, inv, target, ratio);
+    if constexpr (std::is_same<R, double>::value) {
+        // FP64: no c/ulp sums (no fixer), tracers in chunks
+        const dim3 tgrid(xcd_grid((size_t)nblk * ((ntr + INV_TCH - 1) / INV_TCH)));
+        hipLaunchKernelGGL((k_inventory_tr<double, false, INV_TCH>), tgrid, block, 0, s, A, h, area, ntr, f, nullptr,
+                           part);
+        hipLaunchKernelGGL(k_inventory_reduce, dim3(ntr), dim3(64), 0, s, ntr, nblk, part, inv, target, ratio);
+    } else {
+        hipLaunchKernelGGL((k_inventory<R>), grid, block, 0, s, A, h, area, ntr, f, part, nullptr);
+        hipLaunchKernelGGL(k_inventory_reduce, dim3(ntr), dim3(64), 0, s, ntr, nblk, part, inv, target, ratio);
+    }
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }

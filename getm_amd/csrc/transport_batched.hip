@@ -40,7 +40,7 @@ int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, cons
     const int nb = nbatch(ntr, tpb);
     if (nb > 65535) return GETM_ERR_CAPACITY;
     dim3 block(64, 4, 1);
-    dim3 grid((A.nx_ + 63) / 64, (A.ny_ + 3) / 4, nb);
+    dim3 grid(xcd_grid((size_t)((A.nx_ + 63) / 64) * ((A.ny_ + 3) / 4) * nb));
     ProfileScope ps(GETM_K_WCOL, s);
     // tracers per sweep: the thickness and w loads of a sweep are shared by TB
     // tracers; 8 costs occupancy (125 against 96 VGPRs), so only for larger batches

@@ -22,7 +22,7 @@ int launch_wcol_c(int scheme, const StepArgs& A, double dt, int ntr, int tpb, co
     const int nb = nbatch(ntr, tpb);
     if (nb > 65535) return GETM_ERR_CAPACITY;
     dim3 block(64, 4, 1);
-    dim3 grid((A.nx_ + 63) / 64, (A.ny_ + 3) / 4, nb);
+    dim3 grid(xcd_grid((size_t)((A.nx_ + 63) / 64) * ((A.ny_ + 3) / 4) * nb));
     ProfileScope ps(GETM_K_WCOL, s);
     // two tracers per sweep: the FP64 column windows take twice the registers of FP32
     GETM_SCHEME_SWITCH(scheme, {
@@ -43,19 +43,29 @@ int launch_vdiff_c(const StepArgs& A, double dt, double cnpar, double molecular,
     const int nb = nbatch(ntr, tpb);
     if (nb > 65535) return GETM_ERR_CAPACITY;
     const bool cn1 = cnpar == 1.0;
-    const size_t shm = (size_t)(cn1 ? 2 : 4) * A.nz * 64 * sizeof(float);
-    dim3 grid((A.nx_ + 63) / 64, A.ny_, nb);
+    // FP32 factors e2, ru (and g3, g4 unless cnpar = 1), FP64 content weights, for
+    // every level of the depth class
+    const int nzc = (A.nz + 7) / 8 * 8;
+    const size_t shm = (size_t)(cn1 ? 2 : 4) * nzc * 64 * sizeof(float) + (size_t)nzc * 64 * sizeof(double);
+    int dev = 0, max_shm = 0;
+    GETM_HIP_OK(hipGetDevice(&dev));
+    GETM_HIP_OK(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (shm > (size_t)max_shm) return GETM_ERR_CAPACITY;
+    // 32-bit offsets within a tracer (FP64 sources and thickness included)
+    if ((size_t)A.n3 * sizeof(double) >= ((size_t)1 << 32)) return GETM_ERR_CAPACITY;
+    constexpr int NW = 4;  // waves per block, sharing the block's factorisation
+    dim3 grid(xcd_grid((size_t)((A.nx_ + 63) / 64) * A.ny_ * nb)), block(64, NW);
     ProfileScope ps(GETM_K_VDIFF_F32, s);
 #define GETM_VDC(NZM)                                                                                            \
     do {                                                                                                         \
         if (ea4)                                                                                                 \
-            hipLaunchKernelGGL((k_vdiff_c<NZM, true, true>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh,  \
-                               ntr, tpb, fin, lin, fout, lout, ea4);                                             \
+            hipLaunchKernelGGL((k_vdiff_c<NZM, true, true, NW>), grid, block, shm, s, A, dt, cnpar, molecular,     \
+                               nuh, ntr, tpb, fin, lin, fout, lout, ea4);                                        \
         else if (cn1)                                                                                            \
-            hipLaunchKernelGGL((k_vdiff_c<NZM, true, false>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, \
-                               ntr, tpb, fin, lin, fout, lout, nullptr);                                         \
+            hipLaunchKernelGGL((k_vdiff_c<NZM, true, false, NW>), grid, block, shm, s, A, dt, cnpar, molecular,    \
+                               nuh, ntr, tpb, fin, lin, fout, lout, nullptr);                                    \
         else                                                                                                     \
-            hipLaunchKernelGGL((k_vdiff_c<NZM, false, false>), grid, dim3(64), shm, s, A, dt, cnpar, molecular,     \
+            hipLaunchKernelGGL((k_vdiff_c<NZM, false, false, NW>), grid, block, shm, s, A, dt, cnpar, molecular,   \
                                nuh, ntr, tpb, fin, lin, fout, lout, nullptr);                                    \
     } while (0)
     switch ((A.nz - 1) / 8) {

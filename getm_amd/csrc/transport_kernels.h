@@ -52,6 +52,7 @@ constexpr int TX = 64;      // output tile width (one wave); the height TY is a 
 // Row offset j * nx_ (and other products of two array indices): both are below
 // 2^24 (create_grid), so the full-rate 24-bit multiply gives the exact product.
 __device__ __forceinline__ int mul24(int a, int b) { return (int)__umul24((unsigned)a, (unsigned)b); }
+
 constexpr int HX = TX + 4;  // input tile width incl. halo 2
 
 struct FaceC {
@@ -794,10 +795,11 @@ __global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(s
     if (v >= nitems) return;
     const int tbatch = v % nbatch;
     v /= nbatch;
-    const int k0 = (int)(v % ngroups) * KG;
-    v /= ngroups;
+    const int tx = v % ntx;
+    v /= ntx;
+    const int ty = v % nty;
+    const int k0 = (int)(v / nty) * KG;
     const int k1 = k0 + KG < A.nz ? k0 + KG : A.nz;
-    const int tx = v % ntx, ty = v / ntx;
     const int tb0 = tbatch * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
     const size_t off = (size_t)tb0 * A.n3;
@@ -1069,10 +1071,12 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
                                               const double* __restrict__ wv, int* __restrict__ stats, int copy_dry,
                                               const short* __restrict__ lin, short* __restrict__ lout)
 {
-    const int tb0 = blockIdx.z * tpb;
+    unsigned bx, by, bz;
+    if (!xcd_block((A.nx_ + 63) / 64, (A.ny_ + 3) / 4, (ntr + tpb - 1) / tpb, bx, by, bz)) return;
+    const int tb0 = bz * tpb;
     const int tb1 = tb0 + tpb < ntr ? tb0 + tpb : ntr;
-    wcol_body<S, R, TB, C>(A, dt, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y, tb0,
-                           tb1, blockIdx.z == 0, copy_dry != 0, fin, fout, hin, hout, w, wv, stats, lin, lout);
+    wcol_body<S, R, TB, C>(A, dt, bx * 64 + threadIdx.x, by * 4 + threadIdx.y, tb0, tb1, bz == 0, copy_dry != 0, fin,
+                           fout, hin, hout, w, wv, stats, lin, lout);
 }
 
 // Ensemble: work item {domain, bx, by, batch}; scratch -> tracers, hA -> hB (updated columns only).
@@ -1297,154 +1301,211 @@ __global__ void __launch_bounds__(64) k_vdiff_f32(const StepArgs A, double dt, d
                                 SRC ? ea4_all + (size_t)tb0 * A.n3 : nullptr);
 }
 
-// Compensated FP32 column solve (GETM_F32C): the factorisation and elimination
-// of vdiff_body (FP32 factors in LDS, FP32 increments d), with the differences
-// that drive the solve taken of the FP64-decoded old values.  The old column is
-// held in registers as its FP32 values and its int16 residuals, two residuals
-// to a register, and decoded where used.  The exact increments conserve the
-// column's content, sum_k h_k d_k = 0 (the right side is a sum of differences,
-// and the matrix minus diag(h) has zero column sums); the FP32 elimination
-// leaves that sum off by ~1e-7 of |d|, so the defect D = sum_k h_k d_k,
-// accumulated in FP64, is removed as the uniform shift D / sum_k h_k, and the
-// column conserves to FP64 rounding.  The new value c (+ s) + (d - D / H) is
-// formed in FP64 and encoded once.
-template <int NZMAX, bool CN1, bool SRC = false>
-__device__ __forceinline__ void vdiff_body_c(const StepArgs& A, double dt, double cnpar, double molecular,
-                                             const double* __restrict__ nuh, int i, int j, int lane, int ntr,
-                                             const float* __restrict__ fin, const short* __restrict__ lin,
-                                             float* __restrict__ fout, short* __restrict__ lout,
-                                             const double* __restrict__ ea4 = nullptr)
+// Factorisation of vdiff_factor plus what the compensated solve needs per
+// column: the content weights w / H (FP64, in LDS), H = sum_k h_k.
+// Back-substitution d_k = q_k - ru_k d_{k-1} (FP32 ru as stored, FP64
+// arithmetic) makes d a linear map of the eliminated right side q, so the
+// column's content change sum_k h_k d_k equals sum_k w_k q_k with
+//   w_{nz-1} = h_{nz-1},  w_k = h_k - ru_{k+1} w_{k+1},
+// formed here once for every tracer of the column; the solve's uniform shift
+// (sum_k h_k d_k) / H is then sum_k (w_k / H) q_k.
+template <bool CN1>
+__device__ __forceinline__ void vdiff_factor_w(const StepArgs& A, double dt, double cnpar, double molecular,
+                                               const double* __restrict__ nuh, size_t p, int lane, float* e2,
+                                               float* ru, float* g3, float* g4, double* wl, int nzmax)
 {
-    extern __shared__ float s_coef[];  // [ncoef][nz][64]
+    const size_t n2 = A.n2;
+    const int nz = A.nz;
+    const double* hn = A.hn + p;
+    double ru_up = 0.0, an_k = 0.0, ao_k = 0.0, w_up = 0.0, H = 0.0;
+    float ru_up_f = 0.f;
+    double dz = hn[(size_t)(nz - 1) * n2];
+    for (int k = nz - 1; k >= 0; k--) {
+        double an_km1 = 0.0, ao_km1 = 0.0, dz_lo = 0.0;
+        if (k >= 1) {
+            dz_lo = hn[(size_t)(k - 1) * n2];
+            const double x = (2.0 * dt) * (nuh[(size_t)k * n2 + p] + molecular);
+            const double rs = rcp_nr(dz + dz_lo);
+            an_km1 = (cnpar * x) * rs;
+            ao_km1 = ((1.0 - cnpar) * x) * rs;
+        }
+        const double a2 = (dz + an_k) + an_km1;
+        const double den = k == nz - 1 ? a2 : a2 + an_k * ru_up;  // a2 - a3 ru_{k+1}, a3 = -an_k
+        const double rden = rcp_nr(den);
+        const double ruk = -an_km1 * rden;
+        const float ruk_f = float(ruk);
+        e2[k * 64 + lane] = float(an_k * rden);
+        ru[k * 64 + lane] = ruk_f;
+        if (!CN1) {
+            g3[k * 64 + lane] = float(ao_k * rden);
+            g4[k * 64 + lane] = float(ao_km1 * rden);
+        }
+        const double wk = k == nz - 1 ? dz : fma(-(double)ru_up_f, w_up, dz);
+        wl[k * 64 + lane] = wk;
+        H += dz;
+        w_up = wk;
+        ru_up = ruk;
+        ru_up_f = ruk_f;
+        an_k = an_km1;
+        ao_k = ao_km1;
+        dz = dz_lo;
+    }
+    const double rH = 1.0 / H;
+    for (int k = 0; k < nz; k++) wl[k * 64 + lane] *= rH;
+    // levels nz .. nzmax - 1 of the depth class solve nothing: zero factors and weights
+    for (int k = nz; k < nzmax; k++) {
+        e2[k * 64 + lane] = 0.f;
+        ru[k * 64 + lane] = 0.f;
+        if (!CN1) {
+            g3[k * 64 + lane] = 0.f;
+            g4[k * 64 + lane] = 0.f;
+        }
+        wl[k * 64 + lane] = 0.0;
+    }
+}
+
+// Compensated FP32 column solve (GETM_F32C): the factorisation and elimination
+// of vdiff_body (FP32 factors in LDS, FP32 right side q), with the differences
+// that drive the solve taken of the FP64-decoded old values.  NW waves share a
+// block's 64 columns and their factorisation (made by wave 0), each wave
+// solving its own tracers, one lane per column.  The exact increments conserve
+// the column's content, sum_k h_k d_k = 0 (the right side is a sum of
+// differences, and the matrix minus diag(h) has zero column sums); the FP32
+// elimination leaves that sum off by ~1e-7 of |d|, so the defect
+// D = sum_k h_k d_k is removed as the uniform shift
+// D / sum_k h_k = sum_k (w_k / H) q_k (FP64, weights from vdiff_factor_w), and
+// the column conserves to FP64 rounding.  Back-substitution runs in FP64 from
+// the bottom and stores each level as it goes: c (+ s) + (d - D / H), encoded
+// once.  The old column is held in registers as its FP32 values and its int16
+// residuals (two to a register), q as FP32, so every tracer value is read and
+// written once.  The solve always runs NZMAX levels: the levels above the
+// surface (nz .. NZMAX - 1) have zero factors and weights (so their q and d are
+// 0) and read the surface cell; only real levels are stored.  Memory is reached
+// through buffer resources: a lane offset per element size and a wave-uniform
+// layer offset (the launcher checks that a tracer's FP64 extent stays below
+// 4 GiB).
+template <int NZMAX, bool CN1, bool SRC, int NW>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3)))
+k_vdiff_c(const StepArgs A, double dt, double cnpar, double molecular, const double* __restrict__ nuh, int ntr_all,
+          int tpb, const float* __restrict__ fin_all, const short* __restrict__ lin_all, float* __restrict__ fout_all,
+          short* __restrict__ lout_all, const double* __restrict__ ea4_all)
+{
+    extern __shared__ float s_coef[];  // e2, ru[, g3, g4]: [NZMAX][64] FP32; then w / H: [NZMAX][64] FP64
     static_assert(NZMAX % 8 == 0, "depth classes are multiples of 8 layers");
     const int nz = A.nz;
-    auto has = [&](int k) { return k < NZMAX - 8 || k < nz; };
-    auto has_up = [&](int k) { return k < NZMAX - 9 || k < nz - 1; };
-    const size_t n2 = A.n2, n3 = A.n3;
+    const size_t n3 = A.n3;
     float* e2 = s_coef;
-    float* ru = s_coef + (size_t)nz * 64;
-    float* g3 = s_coef + (size_t)2 * nz * 64;
-    float* g4 = s_coef + (size_t)3 * nz * 64;
-    if (i >= A.nx_) return;
-    const size_t p = (size_t)j * A.nx_ + i;
-    const bool inside = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax;
+    float* ru = s_coef + NZMAX * 64;
+    float* g3 = s_coef + 2 * NZMAX * 64;
+    float* g4 = s_coef + 3 * NZMAX * 64;
+    double* wl = reinterpret_cast<double*>(s_coef + (CN1 ? 2 : 4) * NZMAX * 64);
+    unsigned bx, by, bz;
+    if (!xcd_block((A.nx_ + 63) / 64, A.ny_, (ntr_all + tpb - 1) / tpb, bx, by, bz)) return;
+    const int lane = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int i = bx * 64 + lane, j = by;
+    const bool inx = i < A.nx_;
+    const unsigned p = (unsigned)j * (unsigned)A.nx_ + (unsigned)(inx ? i : 0);
+    const bool inside = inx && i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax;
     const bool wetcol = inside && A.tmask[p] == 1;
     const bool solve = wetcol && nz > 1;
-    if (!solve && nz > 1) return;
-    const double* hn = A.hn + p;
-    double H = 0.0;
-    if (solve) {
-        vdiff_factor<CN1>(A, dt, cnpar, molecular, nuh, p, lane, e2, ru, g3, g4);
-        for (int k = 0; k < nz; k++) H += hn[(size_t)k * n2];
-    }
-    for (int t = 0; t < ntr; t++) {
-        const size_t o = (size_t)t * n3 + p;
-        // nz = 1: the column's value is in fin if the stages updated it, else in fout
-        const float* fi = (wetcol ? fin : fout) + o;
-        const short* li = (wetcol ? lin : lout) + o;
-        const double* ei = SRC ? ea4 + o : nullptr;
-        float* fo = fout + o;
-        short* lo_ = lout + o;
-        float hi[NZMAX];
-        unsigned lp[NZMAX / 2];  // residuals of levels 2m (low half) and 2m + 1 (high half)
-#pragma unroll
-        for (int k = 0; k < NZMAX; k++)
-            if (has(k)) {
-                hi[k] = fi[(size_t)k * n2];
-                const unsigned l = (unsigned short)li[(size_t)k * n2];
-                lp[k / 2] = k % 2 ? lp[k / 2] | (l << 16) : l;
-            }
-        // old value (plus ea4/h with sources) of level k
-        auto value = [&](int k) -> double {
-            const short l = (short)(k % 2 ? lp[k / 2] >> 16 : lp[k / 2] & 0xffffu);
-            const double c = cdecode(hi[k], l);
-            if constexpr (SRC) return c + ei[(size_t)k * n2] / hn[(size_t)k * n2];
-            else return c;
-        };
-        auto store = [&](int k, double x) {
+    if (wave == 0 && solve) vdiff_factor_w<CN1>(A, dt, cnpar, molecular, nuh, p, lane, e2, ru, g3, g4, wl, NZMAX);
+    __syncthreads();
+    // columns that are not solved keep their values in fout (no stage writes them),
+    // except that with nz = 1 every point takes the reference's (dz c)/dz
+    if (!inx || (!solve && nz > 1)) return;
+    const int tb0 = bz * tpb;
+    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
+    // lane byte offsets of the column's cell in an FP32, int16 and FP64 layer;
+    // the layer's own offset is wave-uniform (k n2 times the element size)
+    const unsigned v4 = p * 4u, v2 = p * 2u, v8 = p * 8u;
+    const Rsrc rh = make_rsrc(A.hn, (unsigned)(n3 * 8));
+    for (int t = wave; t < ntr; t += NW) {
+        // the layer size and the lane through an empty asm: the per-level offsets
+        // and LDS factor loads below are formed where used instead of being
+        // hoisted out of the tracer loop and held in registers across it
+        unsigned n2 = (unsigned)A.n2;
+        asm volatile("" : "+s"(n2));
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        // layer of level k (the levels above the surface read the surface cell)
+        auto lev = [&](int k) -> unsigned { return (unsigned)(k < nz ? k : nz - 1) * n2; };
+        const size_t o = (size_t)(tb0 + t) * n3;  // wave-uniform
+        const Rsrc rfi = make_rsrc(fin_all + o, (unsigned)(n3 * 4)), rli = make_rsrc(lin_all + o, (unsigned)(n3 * 2));
+        const Rsrc rfo = make_rsrc(fout_all + o, (unsigned)(n3 * 4)), rlo = make_rsrc(lout_all + o, (unsigned)(n3 * 2));
+        const Rsrc rei = make_rsrc(SRC ? ea4_all + o : A.hn, (unsigned)(n3 * 8));
+        auto store = [&](unsigned layer, double x) {
             float h;
             short l;
             cencode(x, h, l);
-            fo[(size_t)k * n2] = h;
-            lo_[(size_t)k * n2] = l;
+            st_f32(rfo, v4, layer * 4u, h);
+            st_i16(rlo, v2, layer * 2u, l);
+        };
+        auto src = [&](unsigned layer) -> double {  // ea4 / h of the cell (sources)
+            return ld_f64(rei, v8, layer * 8u) / ld_f64(rh, v8, layer * 8u);
         };
         if (nz == 1) {
-            // diffusion.F90:224-231: every point, halos and land included
-            const double dz = hn[0];
-            const double c = cdecode(hi[0], (short)(lp[0] & 0xffffu));
-            if constexpr (SRC) store(0, (dz * c + ei[0]) / dz);
+            // diffusion.F90:224-231: every point, halos and land included; the
+            // value is in fin if the stages updated the column, else in fout
+            const double dz = ld_f64(rh, v8, 0);
+            const double c = wetcol ? cdecode(ld_f32(rfi, v4, 0), ld_i16(rli, v2, 0))
+                                    : cdecode(ld_f32(rfo, v4, 0), ld_i16(rlo, v2, 0));
+            if constexpr (SRC) store(0, (dz * c + ld_f64(rei, v8, 0)) / dz);
             else store(0, (dz * c) / dz);
             continue;
         }
+        float hi[NZMAX];
+        unsigned lp[NZMAX / 2];  // residuals of levels 2m (low half) and 2m + 1 (high half)
+#pragma unroll
+        for (int k = 0; k < NZMAX; k++) {
+            hi[k] = ld_f32(rfi, v4, lev(k) * 4u);
+            const unsigned l = (unsigned short)ld_i16(rli, v2, lev(k) * 2u);
+            lp[k / 2] = k % 2 ? lp[k / 2] | (l << 16) : l;
+        }
+        // old value (plus ea4/h with sources) of level k
+        auto value = [&](int k) -> double {
+            const double c = cdecode(hi[k], (short)(k % 2 ? lp[k / 2] >> 16 : lp[k / 2] & 0xffffu));
+            if constexpr (SRC) return c + src(lev(k));
+            else return c;
+        };
         float q[NZMAX];
+        double shift = 0.0;
         {
-            // elimination from the surface down; window (c_{k+1}, c_k, c_{k-1})
-            double c_up = 0.0, c_k = 0.0;
+            // elimination from the top down; window (c_{k+1}, c_k, c_{k-1})
+            double c_up = 0.0, c_k = value(NZMAX - 1);
             float q_up = 0.f;
 #pragma unroll
             for (int k = NZMAX - 1; k >= 0; k--) {
-                if (has(k)) {
-                    if (!has_up(k)) c_k = value(k);  // the surface level starts the window
-                    const float ek = has_up(k) ? e2[k * 64 + lane] : 0.f;
-                    const float rk = ru[k * 64 + lane];  // 0 at k = 0
-                    const double c_dn = k > 0 ? value(k > 0 ? k - 1 : 0) : 0.0;
-                    const float dup = has_up(k) ? float(c_up - c_k) : 0.f;
-                    const float ddn = k > 0 ? float(c_dn - c_k) : 0.f;
-                    float qk = __builtin_fmaf(ek, dup + q_up, -(rk * ddn));
-                    if (!CN1) {
-                        const float e3 = has_up(k) ? g3[k * 64 + lane] : 0.f;
-                        const float e4 = g4[k * 64 + lane];
-                        qk += e3 * dup + e4 * ddn;
-                    }
-                    q[k] = qk;
-                    q_up = qk;
-                    c_up = c_k;
-                    c_k = c_dn;
-                }
+                const float ek = e2[k * 64 + ln];  // 0 at and above the surface
+                const float rk = ru[k * 64 + ln];  // 0 at the bottom and above the surface
+                const double c_dn = k > 0 ? value(k > 0 ? k - 1 : 0) : 0.0;
+                const float dup = k < NZMAX - 1 ? float(c_up - c_k) : 0.f;
+                const float ddn = k > 0 ? float(c_dn - c_k) : 0.f;
+                float qk = __builtin_fmaf(ek, dup + q_up, -(rk * ddn));
+                if (!CN1) qk += g3[k * 64 + ln] * dup + g4[k * 64 + ln] * ddn;
+                q[k] = qk;
+                shift = fma((double)qk, wl[k * 64 + ln], shift);
+                q_up = qk;
+                c_up = c_k;
+                c_k = c_dn;
             }
         }
-        // back-substitution: d_k over q_k, and the content defect sum_k h_k d_k
-        double D;
-        {
-            float d = q[0];
-            D = hn[0] * (double)d;
+        double d = 0.0;
 #pragma unroll
-            for (int k = 1; k < NZMAX; k++) {
-                if (has(k)) {
-                    d = __builtin_fmaf(-ru[k * 64 + lane], d, q[k]);
-                    q[k] = d;
-                    D = fma(hn[(size_t)k * n2], (double)d, D);
-                }
-            }
-        }
-        const double shift = D / H;
-#pragma unroll
-        for (int k = 0; k < NZMAX; k++)
-            if (has(k)) {
+        for (int k = 0; k < NZMAX; k++) {
+            d = k == 0 ? (double)q[0] : fma(-(double)ru[k * 64 + ln], d, (double)q[k]);
+            if (k < NZMAX - 8 || k < nz) {
                 // the registers pass through an empty asm, so the compiler decodes them
                 // again here instead of keeping the elimination's FP64 values live
                 float h = hi[k];
                 unsigned l = lp[k / 2];
                 asm volatile("" : "+v"(h), "+v"(l));
                 double c = cdecode(h, (short)(k % 2 ? l >> 16 : l & 0xffffu));
-                if constexpr (SRC) c += ei[(size_t)k * n2] / hn[(size_t)k * n2];
-                store(k, c + ((double)q[k] - shift));
+                if constexpr (SRC) c += src(lev(k));
+                store(lev(k), c + (d - shift));
             }
+        }
     }
-}
-template <int NZMAX, bool CN1, bool SRC>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_vdiff_c(const StepArgs A, double dt, double cnpar, double molecular,
-                                                const double* __restrict__ nuh, int ntr_all, int tpb,
-                                                const float* __restrict__ fin_all, const short* __restrict__ lin_all,
-                                                float* __restrict__ fout_all, short* __restrict__ lout_all,
-                                                const double* __restrict__ ea4_all)
-{
-    const int tb0 = blockIdx.z * tpb;
-    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
-    const size_t off = (size_t)tb0 * A.n3;
-    vdiff_body_c<NZMAX, CN1, SRC>(A, dt, cnpar, molecular, nuh, blockIdx.x * 64 + threadIdx.x, blockIdx.y, threadIdx.x,
-                                  ntr, fin_all + off, lin_all + off, fout_all + off, lout_all + off,
-                                  SRC ? ea4_all + off : nullptr);
 }
 
 // Ensemble: work item {domain, bx, row, batch}; scratch -> tracers.
