@@ -401,7 +401,8 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     const double half = 0.5 * dt;
     const int d1 = tr->ufirst ? 1 : 2;  // first direction of stages 1-2 (operators.py:118-121)
     const int d2 = 3 - d1;
-    const int ty = tr->tile_y ? tr->tile_y : auto_tile_y(ntr);
+    // compensated form with many tracers: 16-row tiles (less halo to decode and sweep)
+    const int ty = tr->tile_y ? tr->tile_y : (C && ntr >= 16 ? 16 : auto_tile_y(ntr));
     int* stats = static_cast<int*>(tr->stats.ptr);
     // FP32 forms: the w stage leaves the thickness of the columns it does not update
     // in hA (stages 1-2 wrote every cell there), and stages 4-5 read those cells from

@@ -141,29 +141,42 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory_tr(const StepArgs A, 
     const int nt = ntr - t0 < TCH ? ntr - t0 : TCH;
     const int nblk = nbx * nby;
     const int blk = by * nbx + bx;
+    // per lane: the column's sum for each tracer of the chunk (40-odd terms: plain
+    // FP64 sums, exact to ~1e-15 of the column's sum of magnitudes); compensation
+    // starts with the merge of the lanes' sums
     double m[TCH], mc[TCH];
 #pragma unroll
     for (int tt = 0; tt < TCH; tt++) m[tt] = mc[tt] = 0.0;
     if (i < A.nx_ && j < A.ny_) {
-        const size_t p = (size_t)j * A.nx_ + i;
+        const unsigned p = (unsigned)j * A.nx_ + i;
         const bool wet = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[p] == 1;
         if (wet) {
             const double a = area[p];
-            const size_t base = (size_t)t0 * A.n3 + p;
+            const unsigned n2 = (unsigned)A.n2;
+            const unsigned vs = p * (unsigned)sizeof(R), v2 = p * 2u, v8 = p * 8u;
+            const Rsrc rh = make_rsrc(h, (unsigned)(A.n3 * 8));
+            Rsrc rf[TCH], rl[TCH];
+#pragma unroll
+            for (int tt = 0; tt < TCH; tt++) {
+                const size_t o = (size_t)(t0 + (tt < nt ? tt : 0)) * A.n3;
+                rf[tt] = make_rsrc(f + o, (unsigned)(A.n3 * sizeof(R)));
+                rl[tt] = make_rsrc(COMP ? (const void*)(lo + o) : (const void*)(f + o), (unsigned)(A.n3 * 2));
+            }
+#pragma unroll 4
             for (int k = 0; k < A.nz; k++) {
-                const size_t o = (size_t)k * A.n2;
-                const double ah = a * h[o + p];
+                const unsigned L = (unsigned)k * n2;
+                const double ah = a * ld_f64(rh, v8, L * 8u);
+                // (tracers past the chunk's end read tracer t0 again; their sums are not written)
 #pragma unroll
                 for (int tt = 0; tt < TCH; tt++) {
-                    if (tt < nt) {
-                        const size_t q = base + (size_t)tt * A.n3 + o;
-                        double x;
-                        if constexpr (COMP)
-                            x = cdecode(f[q], lo[q]);
-                        else
-                            x = (double)f[q];
-                        neumaier(m[tt], mc[tt], ah * x);
-                    }
+                    double x;
+                    if constexpr (COMP)
+                        x = cdecode(ld_f32(rf[tt], vs, L * 4u), ld_i16(rl[tt], v2, L * 2u));
+                    else if constexpr (sizeof(R) == 8)
+                        x = ld_f64(rf[tt], vs, L * 8u);
+                    else
+                        x = (double)ld_f32(rf[tt], vs, L * 4u);
+                    m[tt] = fma(ah, x, m[tt]);
                 }
             }
         }

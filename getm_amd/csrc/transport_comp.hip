@@ -21,9 +21,31 @@ int launch_wcol_c(int scheme, const StepArgs& A, double dt, int ntr, int tpb, co
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
     if (nb > 65535) return GETM_ERR_CAPACITY;
+    ProfileScope ps(GETM_K_WCOL, s);
+    // at least 16 tracers in columns of up to 40 levels: per-level terms in LDS,
+    // 16 waves per row of 64 columns, each with its own tracers (no dry-column
+    // copies: the next stage reads those from hin)
+    constexpr int NW = 16;
+    if (!copy_dry && ntr >= NW && A.nz <= 40 && (size_t)A.n3 * 4 < ((size_t)1 << 32)) {
+        const dim3 grid(xcd_grid((size_t)((A.nx_ + 63) / 64) * A.ny_ * nb)), block(64, NW);
+#define GETM_WCC(NZM)                                                                                      \
+    GETM_SCHEME_SWITCH(scheme, {                                                                           \
+        hipLaunchKernelGGL((k_wcol_c<S_, NZM, NW>), grid, block, sizeof(WcolLds<S_, NZM, NW>), s, A, dt, ntr, \
+                           tpb, fin, fout, hin, hout, w, wv, stats, lin, lout);                              \
+    })
+        switch ((A.nz - 1) / 8) {
+            case 0: GETM_WCC(8); break;
+            case 1: GETM_WCC(16); break;
+            case 2: GETM_WCC(24); break;
+            case 3: GETM_WCC(32); break;
+            default: GETM_WCC(40); break;
+        }
+#undef GETM_WCC
+        GETM_HIP_OK(hipGetLastError());
+        return GETM_OK;
+    }
     dim3 block(64, 4, 1);
     dim3 grid(xcd_grid((size_t)((A.nx_ + 63) / 64) * ((A.ny_ + 3) / 4) * nb));
-    ProfileScope ps(GETM_K_WCOL, s);
     // two tracers per sweep: the FP64 column windows take twice the registers of FP32
     GETM_SCHEME_SWITCH(scheme, {
         hipLaunchKernelGGL((k_wcol<S_, double, 2, true>), grid, block, 0, s, A, dt, ntr, tpb, fin, fout, hin, hout, w,

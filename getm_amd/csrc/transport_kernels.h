@@ -1079,6 +1079,174 @@ __global__ void __launch_bounds__(256) k_wcol(const StepArgs A, double dt, int n
                            fout, hin, hout, w, wv, stats, lin, lout);
 }
 
+// Compensated w stage for one row of 64 columns and a batch of tracers, NW waves
+// per block.  The tracer-independent terms of every level are formed once per
+// block, in parallel over levels, into LDS: per cell the increment
+// coefficients a = (h0 - hnew)/hnew, b = dt/hnew of cell_update (FP64), per
+// face the transport w (FP64) and the limiter's cfl terms (LimF), with the
+// reference's expressions (advection.F90.template:239-322 for one sub-step).
+// Columns whose Courant maximum needs a single sub-step are then swept from
+// registers: each wave takes a contiguous share of the tracers and, per tracer,
+// loads the whole column (FP32 values and int16 residuals) at once, forms the
+// faces from a window of decoded values and stores every level once.  The few
+// columns that need sub-steps take wcol_body (thickness replay, FP64 windows).
+// Levels above the surface (nz .. NZMAX - 1) have zero coefficients and read
+// the surface cell, so the unrolled sweep has no level tests but the stores';
+// the reference's zero flux through the surface is the zero transport there.
+template <int S, int NZMAX, int NW>
+struct WcolLds {
+    double a[NZMAX][64], b[NZMAX][64], q[NZMAX][64], c0[NZMAX][64];
+    float ch[NZMAX][64];
+    float l[LimF<S>::NL > 0 ? LimF<S>::NL : 1][NZMAX][64];
+    int single[64];
+};
+
+template <int S, int NZMAX, int NW>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4)))
+k_wcol_c(const StepArgs A, double dt, int ntr_all, int tpb, const float* __restrict__ fin_all,
+         float* __restrict__ fout_all, const double* __restrict__ hin, double* __restrict__ hout,
+         const double* __restrict__ w, const double* __restrict__ wv, int* __restrict__ stats,
+         const short* __restrict__ lin_all, short* __restrict__ lout_all)
+{
+    static_assert(NZMAX % 8 == 0, "depth classes are multiples of 8 layers");
+    extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
+    WcolLds<S, NZMAX, NW>& L = *reinterpret_cast<WcolLds<S, NZMAX, NW>*>(s_raw);
+    constexpr int NL = LimF<S>::NL;
+    const int nz = A.nz;
+    const size_t n2 = A.n2, n3 = A.n3;
+    unsigned bx, by, bz;
+    if (!xcd_block((A.nx_ + 63) / 64, A.ny_, (ntr_all + tpb - 1) / tpb, bx, by, bz)) return;
+    const int lane = threadIdx.x, tid = threadIdx.y * 64 + threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int j = by;
+    auto column = [&](int l, size_t& p) {  // wet interior column of lane l of the row?
+        const int i = bx * 64 + l;
+        p = (size_t)j * A.nx_ + (i < A.nx_ ? i : 0);
+        return i < A.nx_ && i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[p] == 1;
+    };
+    // ---- per-level terms, items (level, lane)
+    for (int it = tid; it < NZMAX * 64; it += 64 * NW) {
+        const int k = it / 64, l = it % 64;
+        size_t p;
+        const bool wet = column(l, p);
+        double a = 0.0, b = 0.0, q = 0.0, c0 = 0.0;
+        LimF<S> lf;
+        lf.set(0.0);
+        if (wet && k < nz) {
+            // cell k: one sub-step of the thickness (template:277-290)
+            const double h0 = hin[(size_t)k * n2 + p];
+            const double hnew = h0 - dt * (w[(size_t)(k + 1) * n2 + p] - w[(size_t)k * n2 + p]);
+            const double rh = rcp_nr(hnew);
+            a = (h0 - hnew) * rh;
+            b = dt * rh;
+            if (k >= 1) {
+                // face k between cells k - 1 and k
+                const double wvk = wv[(size_t)k * n2 + p];
+                c0 = (fabs(wvk) * dt) / fmin2(hin[(size_t)(k - 1) * n2 + p], h0);
+                q = wvk;
+                lf.set(c0 * 1.0);
+            }
+        }
+        L.a[k][l] = a;
+        L.b[k][l] = b;
+        L.q[k][l] = q;
+        L.c0[k][l] = c0;
+        L.ch[k][l] = lf.ch;
+#pragma unroll
+        for (int m = 0; m < NL; m++) L.l[m][k][l] = lf.l[m];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // Courant maximum in the order of the reference's MAXVAL: one sub-step?
+        double c = -1.7976931348623157e308;
+        for (int k = 1; k <= nz - 1; k++) c = fmax2(c, L.c0[k][lane]);
+        L.single[lane] = fceiling(c) <= 1;
+    }
+    __syncthreads();
+    const int tb0 = bz * tpb;
+    const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
+    if (bz == 0) {
+        // thickness of single-step columns (the others' is written by wcol_body)
+        for (int it = tid; it < NZMAX * 64; it += 64 * NW) {
+            const int k = it / 64, l = it % 64;
+            size_t p;
+            if (k < nz && column(l, p) && L.single[l]) {
+                const double h0 = hin[(size_t)k * n2 + p];
+                hout[(size_t)k * n2 + p] = h0 - dt * (w[(size_t)(k + 1) * n2 + p] - w[(size_t)k * n2 + p]);
+            }
+        }
+    }
+    size_t p;
+    const bool wet = column(lane, p);
+    if (!wet) return;
+    // this wave's tracers
+    const int per = (ntr + NW - 1) / NW;
+    const int t_lo = tb0 + wave * per, t_hi = t_lo + per < tb0 + ntr ? t_lo + per : tb0 + ntr;
+    if (t_lo >= t_hi) return;
+    if (!L.single[lane]) {
+        wcol_body<S, double, 2, true>(A, dt, bx * 64 + lane, j, t_lo, t_hi, bz == 0 && t_lo == tb0, false, fin_all,
+                                      fout_all, hin, hout, w, wv, stats, lin_all, lout_all);
+        return;
+    }
+    const unsigned v4 = (unsigned)p * 4u, v2 = (unsigned)p * 2u;
+    for (int t = t_lo; t < t_hi; t++) {
+        // the layer size and the lane through an empty asm: per-level offsets and LDS
+        // loads are formed where used, not hoisted out of the tracer loop
+        unsigned m2 = (unsigned)n2;
+        asm volatile("" : "+s"(m2));
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        auto lev = [&](int k) -> unsigned { return (unsigned)(k < nz ? k : nz - 1) * m2; };
+        const size_t o = (size_t)t * n3;
+        const Rsrc rfi = make_rsrc(fin_all + o, (unsigned)(n3 * 4)), rli = make_rsrc(lin_all + o, (unsigned)(n3 * 2));
+        const Rsrc rfo = make_rsrc(fout_all + o, (unsigned)(n3 * 4)), rlo = make_rsrc(lout_all + o, (unsigned)(n3 * 2));
+        float hi[NZMAX];
+        unsigned lp[NZMAX / 2];
+#pragma unroll
+        for (int k = 0; k < NZMAX; k++) {
+            hi[k] = ld_f32(rfi, v4, lev(k) * 4u);
+            const unsigned l = (unsigned short)ld_i16(rli, v2, lev(k) * 2u);
+            lp[k / 2] = k % 2 ? lp[k / 2] | (l << 16) : l;
+        }
+        auto value = [&](int k) -> double {
+            return cdecode(hi[k], (short)(k % 2 ? lp[k / 2] >> 16 : lp[k / 2] & 0xffffu));
+        };
+        // window: fm2 = f[k-2], fm1 = f[k-1], f0 = f[k], f1 = f[k+1] at face k
+        double fm2 = 0.0, fm1 = value(0), f0 = value(NZMAX > 1 ? 1 : 0), fprev = 0.0;
+#pragma unroll
+        for (int k = 1; k <= NZMAX; k++) {
+            double fl = 0.0;
+            const double f1 = value(k + 1 < NZMAX ? k + 1 : NZMAX - 1);
+            if (k < NZMAX) {
+                const double q = L.q[k][ln];
+                LimF<S> lf;
+                lf.ch = L.ch[k][ln];
+#pragma unroll
+                for (int m = 0; m < NL; m++) lf.l[m] = L.l[m][k][ln];
+                const bool pos = q > 0.0;
+                const double fu = pos ? fm1 : f0;
+                const double fuu = pos ? (k > 1 ? fm2 : fm1) : f1;
+                const double fd = pos ? f0 : fm1;
+                fl = q * (fu + (double)lf.corr(float(fd - fu), float(fu - fuu)));
+            }
+            // cell k - 1, increment form in FP64: f + ((h0 - hnew)/hnew f - dt/hnew dF)
+            const double a = L.a[k - 1][ln], b = L.b[k - 1][ln];
+            const double x = fm1 + fma(a, fm1, -(b * (fl - fprev)));
+            if (k - 1 < NZMAX - 8 || k - 1 < nz) {
+                float h;
+                short l;
+                cencode(x, h, l);
+                st_f32(rfo, v4, (unsigned)(k - 1) * m2 * 4u, h);
+                st_i16(rlo, v2, (unsigned)(k - 1) * m2 * 2u, l);
+            }
+            fprev = fl;
+            fm2 = fm1;
+            fm1 = f0;
+            f0 = f1;
+        }
+    }
+}
+
 // Ensemble: work item {domain, bx, by, batch}; scratch -> tracers, hA -> hB (updated columns only).
 template <int S, typename R, int TB>
 __global__ void __launch_bounds__(256) k_wcol_multi(const EnsDom<R>* __restrict__ doms, const int4* __restrict__ work)
