@@ -274,6 +274,44 @@ def blob_tracers(ntr, grid, seed, dtype):
     return out
 
 
+def blob_tracers_host(ntr, grid, seed):
+    """The same tracers generated on the host with numpy (FP64, rounded to FP32),
+    yielded one at a time as (index, array) in order; several tracers are
+    computed at once in threads.  The parameters are drawn in the order of
+    blob_tracers."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    nz, ny_, nx_ = grid.nz_, grid.ny_, grid.nx_
+    xs, ys, zs = np.arange(nx_, dtype=np.float64), np.arange(ny_, dtype=np.float64), np.arange(nz, dtype=np.float64)
+    params = []
+    for t in range(ntr):
+        blobs = []
+        for _ in range(4):
+            amp = 10.0 ** rng.uniform(-3.0, 1.0)
+            x0, y0, z0 = rng.uniform(0, nx_), rng.uniform(0, ny_), rng.uniform(0, nz)
+            sx, sy, sz = rng.uniform(0.03, 0.15) * nx_, rng.uniform(0.03, 0.15) * ny_, rng.uniform(0.1, 0.5) * nz
+            blobs.append((amp, x0, y0, z0, sx, sy, sz))
+        params.append(blobs)
+
+    def one(t):
+        acc = np.zeros((nz, ny_, nx_), dtype=np.float64)
+        for amp, x0, y0, z0, sx, sy, sz in params[t]:
+            gx = np.exp(-0.5 * ((xs - x0) / sx) ** 2)
+            gy = np.exp(-0.5 * ((ys - y0) / sy) ** 2)
+            gz = np.exp(-0.5 * ((zs - z0) / sz) ** 2)
+            acc += amp * (gz[:, None, None] * gy[None, :, None]) * gx[None, None, :]
+        return acc.astype(np.float32)
+
+    workers = min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for t0 in range(0, ntr, workers):
+            for t, x in zip(range(t0, min(ntr, t0 + workers)), ex.map(one, range(t0, min(ntr, t0 + workers)))):
+                yield t, x
+
+
 def fp64_run(steps: int, warmup: int, log):
     """The plain FP64 batched step on the coastal grid with temperature and
     salinity, SUPERBEE: tracer cell-updates/s and ms per step."""
@@ -579,6 +617,9 @@ def main():
                     help="FP32 runs: steps of the error check against the FP64 method")
     ap.add_argument("--fp64-steps", type=int, default=10, help="steps of the FP64 block (0: skip it)")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--host-inputs", action="store_true",
+                    help="generate and encode the tracers on the host and copy them in, so that no torch "
+                         "compute kernel runs before the timed steps (hardware-counter runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -630,8 +671,14 @@ def main():
     if args.solver:
         tc.set_solver(args.solver)
     tc.set_rounding(args.rounding)
-    if kind == "blobs":
+    if kind == "blobs" and args.host_inputs:
+        tc.resize(ntr)
+        for t, x in blob_tracers_host(ntr, grid, seed=7):
+            tc.set_tracer(t, x)
+    elif kind == "blobs":
         tc.set_values(blob_tracers(ntr, grid, seed=7, dtype=tc.dtype))
+    elif args.host_inputs:
+        tc.set_values(torch.from_numpy(np.ascontiguousarray(d.tracers)))
     else:
         tc.set_values(dev(d.tracers))
     del d.u, d.v, d.w, d.nuh, d.tracers

@@ -70,6 +70,20 @@ def encode(x: torch.Tensor, hi: torch.Tensor, lo: torch.Tensor) -> None:
     lo.copy_(torch.round(r / residual_unit(h)).clamp_(-32768, 32767).to(torch.int16))
 
 
+def encode_host(x: np.ndarray):
+    """The same encoding on the host (numpy): returns (hi float32, lo int16).
+    FP32 input is exact in hi, so its residuals are zero."""
+    x = np.asarray(x)
+    if x.dtype == np.float32:
+        return x, np.zeros(x.shape, dtype=np.int16)
+    x = x.astype(np.float64, copy=False)
+    hi = x.astype(np.float32)
+    r = np.nan_to_num(x - hi.astype(np.float64), nan=0.0, posinf=0.0, neginf=0.0)
+    eb = np.maximum((hi.view(np.int32) >> 23) & 0xFF, 1)
+    lo = np.clip(np.round(r / np.ldexp(1.0, eb - 165)), -32768, 32767).astype(np.int16)
+    return hi, lo
+
+
 class _Storage:
     """The collection's tracer arrays, shared with its Tracer objects (which hold
     this, not the collection, so that no reference cycle keeps a collection and
@@ -80,6 +94,35 @@ class _Storage:
     def __init__(self):
         self.values = None
         self.residuals = None
+
+    def put(self, index, x) -> None:
+        """Write values (any float dtype; a device tensor, or host data as a CPU
+        tensor or array) into tracer slot ``index`` (an int) or every slot (a
+        slice, x stacked).  Compensated storage keeps what FP32 cannot hold in
+        the residuals, encoding one tracer at a time so that the FP64
+        temporaries stay the size of one tracer.  Host data are converted and
+        encoded on the host and then copied, so no device kernel runs."""
+        comp = self.residuals is not None
+        one = isinstance(index, int)
+        slots = [index] if one else range(self.values.shape[0])
+        if isinstance(x, np.ndarray) or not isinstance(x, torch.Tensor) or x.device.type == "cpu":
+            x = x.numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
+            for t in slots:
+                xt = x if one else x[t]
+                if comp:
+                    hi, lo = encode_host(xt)
+                    self.values[t].copy_(torch.from_numpy(np.ascontiguousarray(hi)))
+                    self.residuals[t].copy_(torch.from_numpy(np.ascontiguousarray(lo)))
+                else:
+                    dt = np.float64 if self.values.dtype == torch.float64 else np.float32
+                    self.values[t].copy_(torch.from_numpy(np.ascontiguousarray(xt, dtype=dt)))
+        elif not comp:
+            self.values[index].copy_(x)
+        else:
+            dev = self.values.device
+            for t in slots:
+                xt = x if one else x[t]
+                encode(xt.to(device=dev, dtype=torch.float64), self.values[t], self.residuals[t])
 
 
 class Tracer(Array):
@@ -124,7 +167,8 @@ class Tracer(Array):
 
     @all_values.setter
     def all_values(self, value) -> None:
-        self._store.values[self._index].copy_(torch.as_tensor(value))
+        # compensated storage: the residuals are encoded with the values
+        self._store.put(self._index, value)
 
 
 class TracerCollection:
@@ -200,18 +244,10 @@ class TracerCollection:
             self.residuals = None
         self._storage.values, self._storage.residuals = self.values, self.residuals
 
-    def _store(self, index, x: torch.Tensor) -> None:
-        """Write values (any float dtype) into tracer slot ``index`` (an int) or
-        every slot (a slice, x stacked); the compensated form keeps what FP32
-        cannot hold in the residuals, encoding one tracer at a time so that the
-        FP64 temporaries stay the size of one tracer."""
-        if not self.compensated:
-            self.values[index].copy_(x)
-        elif isinstance(index, int):
-            encode(x.to(device=self.grid.device, dtype=torch.float64), self.values[index], self.residuals[index])
-        else:
-            for t in range(self.values.shape[0]):
-                encode(x[t].to(device=self.grid.device, dtype=torch.float64), self.values[t], self.residuals[t])
+    def _store(self, index, x) -> None:
+        """Write values into tracer slot ``index`` (an int) or every slot (a
+        slice, x stacked); see _Storage.put."""
+        self._storage.put(index, x)
 
     def _sequence_operators(self) -> None:
         """Operators of the reference's per-tracer call sequence (made on first use
@@ -258,7 +294,7 @@ class TracerCollection:
         if self.compensated:
             self.residuals[:n].copy_(old_r)
         x = torch.full((g.nz_, g.ny_, g.nx_), float("nan"), dtype=torch.float64, device=g.device) if data is None \
-            else torch.as_tensor(data, device=g.device)
+            else data
         self._store(n, x)
         self.offsets = np.append(self.offsets, 0.0)
         self._tracers.append(tracer)
@@ -295,9 +331,10 @@ class TracerCollection:
                 off = np.asarray(offsets, dtype=np.float64).reshape(-1)
                 if off.size != ntr:
                     raise ValueError(f"expected {ntr} offsets, got {off.size}")
-        src = values.to(device=g.device)
+        # host data stay on the host until stored (converted and encoded there)
+        src = values if isinstance(values, torch.Tensor) else torch.as_tensor(np.asarray(values))
         if off.any():
-            o = torch.as_tensor(off, dtype=torch.float64, device=g.device).view(-1, 1, 1, 1)
+            o = torch.as_tensor(off, dtype=torch.float64, device=src.device).view(-1, 1, 1, 1)
             src = src.double() - o
         if ntr != len(self):
             self._tracers = [Tracer(self, t, name=f"tracer{t}") for t in range(ntr)]
@@ -307,6 +344,32 @@ class TracerCollection:
         if molecular_diffusivity is not None:
             for t in self._tracers:
                 t.molecular_diffusivity = float(molecular_diffusivity)
+
+    def resize(self, ntr: int) -> None:
+        """Hold ntr tracers: the first min(ntr, len) keep their values, names and
+        attributes, new ones are plain tracers whose values are undefined until
+        set (``set_tracer``).  Lets a caller fill a large collection one tracer
+        at a time, e.g. from host data, without a stacked copy of all of them."""
+        n = min(ntr, len(self))
+        old_v, old_r = self.values, self.residuals
+        self._alloc(ntr)
+        if n:
+            self.values[:n].copy_(old_v[:n])
+            if self.compensated:
+                self.residuals[:n].copy_(old_r[:n])
+        self._tracers = self._tracers[:n] + [Tracer(self, t, name=f"tracer{t}") for t in range(n, ntr)]
+        self.offsets = np.concatenate([self.offsets[:n], np.zeros(ntr - n)])
+
+    def set_tracer(self, index: int, data) -> None:
+        """Store the values (nz, ny_, nx_) of tracer ``index``: a device tensor,
+        or host data (CPU tensor or array), which are converted and encoded on
+        the host and copied in."""
+        if not 0 <= index < len(self):
+            raise IndexError(f"tracer {index} of {len(self)}")
+        g = self.grid
+        if tuple(data.shape) != (g.nz_, g.ny_, g.nx_):
+            raise ValueError(f"expected ({g.nz_}, {g.ny_}, {g.nx_}), got {tuple(data.shape)}")
+        self._store(int(index), data)
 
     # ---- the step ---------------------------------------------------------
     def _combined_w(self, w: Array) -> List[Optional[torch.Tensor]]:

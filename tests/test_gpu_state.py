@@ -116,3 +116,43 @@ def test_state_without_thickness_is_rejected():
     del st["ho"]
     with pytest.raises(ValueError, match="ho"):
         tc.load_state_dict(st)
+
+
+def _same_storage(a, b):
+    assert_bitwise(a.values.cpu().numpy(), b.values.cpu().numpy(), "values")
+    if a.residuals is not None:
+        assert np.array_equal(a.residuals.cpu().numpy(), b.residuals.cpu().numpy()), "residuals"
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32-sr"])
+def test_host_inputs_store_the_bits_of_device_inputs(precision):
+    """Host data (encoded on the host, then copied) and device data give the same
+    storage bit for bit, through set_values, resize + set_tracer, add and the
+    Tracer.all_values setter; the compensated form's residuals included."""
+    from getm_amd.tracer import TracerCollection
+
+    d, grid, dev, (u, v, w, nuh) = _setup(40, 32, 6, 7, 3)
+    x = d.tracers * (1.0 + 1e-9 * np.arange(d.tracers.size).reshape(d.tracers.shape) % 7)  # not FP32-exact
+    a = TracerCollection(grid, advection_scheme=3, precision=precision)
+    a.set_values(dev(x))
+    b = TracerCollection(grid, advection_scheme=3, precision=precision)
+    b.set_values(torch.from_numpy(x))
+    _same_storage(b, a)
+    c = TracerCollection(grid, advection_scheme=3, precision=precision)
+    c.resize(3)
+    for t in range(3):
+        c.set_tracer(t, x[t])
+    _same_storage(c, a)
+    e = TracerCollection(grid, advection_scheme=3, precision=precision)
+    for t in range(3):
+        e.add(f"t{t}", data=dev(x[t]) if t % 2 else x[t])
+    _same_storage(e, a)
+    e[1].all_values = x[0]
+    e[0].all_values = dev(x[1])
+    a.set_values(dev(x[[1, 0, 2]]))
+    _same_storage(e, a)
+    # resize keeps the first tracers and their attributes
+    e[0].molecular_diffusivity = 1e-7
+    e.resize(2)
+    assert len(e) == 2 and e[0].molecular_diffusivity == 1e-7
+    assert_bitwise(e.absolute()[:2].cpu().numpy(), a.absolute()[:2].cpu().numpy())

@@ -177,3 +177,28 @@ def test_serving_drift_of_tracers_without_mass_is_finite():
     assert relative_drift(torch.tensor([1000.5, 0.0, -4.0], dtype=torch.float64), target) == 5e-4
     assert relative_drift(torch.tensor([1000.0, 1e-9, -4.0], dtype=torch.float64), target) == 1e-9
     assert relative_drift(torch.tensor([1000.0, 0.0, -4.4], dtype=torch.float64), target) == abs(-4.4 + 4.0) / 4.0
+
+
+def test_host_encoding_equals_device_encoding_and_decodes_within_residual():
+    """encode_host (numpy, for host data) gives the bits of encode (torch), and
+    decoding recovers FP64 values to 2^-16 of an FP32 ulp."""
+    import torch
+
+    import numpy as np
+
+    from getm_amd.tracer import decode, encode, encode_host
+
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.standard_normal(5000) * 10.0 ** rng.uniform(-6, 6, 5000),
+                        [0.0, -0.0, 1e-39, -3e-42, 1.0, 35.0, 2.0 ** 100, np.nextafter(1.0, 2.0)]])
+    hi_t, lo_t = torch.empty(x.size, dtype=torch.float32), torch.empty(x.size, dtype=torch.int16)
+    encode(torch.from_numpy(x), hi_t, lo_t)
+    hi, lo = encode_host(x)
+    assert np.array_equal(hi.view(np.int32), hi_t.numpy().view(np.int32))
+    assert np.array_equal(lo, lo_t.numpy())
+    back = decode(torch.from_numpy(hi), torch.from_numpy(lo)).numpy()
+    ulp = np.spacing(np.abs(hi).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(back - x) <= ulp * 2.0 ** -16 + 1e-300)
+    # FP32 input is exact: zero residuals
+    h32, l32 = encode_host(x.astype(np.float32))
+    assert np.array_equal(h32, x.astype(np.float32)) and not l32.any()
