@@ -762,9 +762,13 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
                         at32(fo, (unsigned)ooff[m] * (unsigned)sizeof(R)) = out;
                 }
             }
-            __syncthreads();
         }
-        if (ntr == 0) __syncthreads();  // s_q and s_h are rewritten by the next layer
+        // no barrier between tracers is needed after the stage-B cells: they read s_a
+        // and s_fb, which the next tracer writes only after its first barrier, and the
+        // tile buffer the next tracer fills was last read two tracers back.  The last
+        // tracer's stage-B cells still read the pool, where the next layer's set-up
+        // writes s_q and s_h.
+        __syncthreads();
     }
 }
 

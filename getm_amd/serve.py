@@ -11,9 +11,17 @@ launches, and finished requests leave without waiting for the others.
 Per-request arithmetic does not depend on what else is in flight: a result is
 the same alone and in the mix, bit for bit.
 
-With ``conserve=True`` each request's tracer inventories are restored after
-every tick to their values at admission (FP64 inventories, FP32 ulp fixer),
-which holds because the scenario fields keep ho == hn (closed, steady flow).
+With ``conserve=True`` each request's tracer inventories are restored to their
+values at admission (FP64 inventories, FP32 ulp fixer), which holds because the
+scenario fields keep ho == hn (closed, steady flow): once, when the request
+completes (``fix="end"``, the default: 5 launches per request in all), or after
+every tick (``fix="tick"``: 5 launches per request and tick).
+
+Requests may carry a latency budget (``deadline``, from arrival) and a client
+cancellation time (``cancel_after``, from arrival); ``Server.cancel(rid)``
+cancels one from outside (e.g. an ``on_tick`` callback).  A cancelled or expired
+request leaves at the next tick boundary, frees its slot and budget, and is
+reported with its status.
 
 Latency is measured from a request's arrival to its result being complete on
 the device.  ``verify`` re-runs completed requests with the FP64 plain method
@@ -52,6 +60,8 @@ class Request:
     scheme: int
     seed: int
     arrival: float = 0.0  # seconds after the start of the run
+    deadline: Optional[float] = None  # latency budget from arrival; expired requests are dropped
+    cancel_after: Optional[float] = None  # the client cancels this long after arrival
 
     @property
     def cells(self) -> int:
@@ -73,6 +83,7 @@ class Result:
     inventory_drift: float  # max over tracers of |I - I0| / I0 at the end
     tracers: Optional[torch.Tensor] = None  # (T, nz, ny_, nx_) float32 on the device
     error: str = ""
+    status: str = "done"  # done | rejected | cancelled | expired
 
     @property
     def latency(self) -> float:
@@ -198,9 +209,13 @@ class Server:
 
     def __init__(self, hbm_budget: float = 250e9, max_slots: int = 64, conserve: bool = True,
                  keep_results: bool = False, tuning: Optional[Sequence[int]] = None, device=None,
-                 policy: str = "lockstep", quantum: float = 2.0 ** 25, horizon: int = 8):
+                 policy: str = "lockstep", quantum: float = 2.0 ** 25, horizon: int = 8, fix: str = "end"):
         if policy not in ("fair", "lockstep", "uniform"):
             raise ValueError("policy must be 'fair', 'lockstep' or 'uniform'")
+        if fix not in ("end", "tick"):
+            raise ValueError("fix must be 'end' (once per request) or 'tick' (after every tick)")
+        self.fix = fix
+        self._cancel: set = set()
         if policy == "uniform" and horizon < 1:
             raise ValueError("horizon must be at least one tick")
         self.policy = policy
@@ -233,12 +248,19 @@ class Server:
             self._members[group] = key
         return self._groups[group]
 
+    def cancel(self, rid: int) -> None:
+        """Cancel request ``rid``: it leaves (or is never admitted) at the next tick
+        boundary and is reported with status "cancelled"."""
+        self._cancel.add(rid)
+
     def run(self, requests: Sequence[Request], domains: Optional[Dict[int, inputs.Domain]] = None,
-            log=None, clock: str = "wall") -> List[Result]:
-        """Serve ``requests`` (arrival times relative to the call) until all are done.
-        ``domains`` may hold pre-generated inputs by request id (generated up front
-        otherwise).  With ``clock="ticks"`` arrivals and latencies count steps of the
-        loop instead of seconds, which makes admission order reproducible."""
+            log=None, clock: str = "wall", on_tick=None) -> List[Result]:
+        """Serve ``requests`` (arrival times relative to the call) until all are done,
+        cancelled or expired.  ``domains`` may hold pre-generated inputs by request
+        id (generated up front otherwise).  With ``clock="ticks"`` arrivals,
+        deadlines and latencies count steps of the loop instead of seconds, which
+        makes admission order reproducible.  ``on_tick(server, tick)`` is called
+        before every tick."""
         if clock not in ("wall", "ticks"):
             raise ValueError("clock must be 'wall' or 'ticks'")
         self._groups.clear()
@@ -256,7 +278,38 @@ class Server:
         t0 = time.perf_counter()
         now = (lambda: float(ticks)) if clock == "ticks" else (lambda: time.perf_counter() - t0)
         admitted_at: Dict[int, float] = {}
+        def leaving(req: Request, t: float) -> str:
+            """"cancelled" / "expired" if the request must leave at time t, else ""."""
+            if req.rid in self._cancel or (req.cancel_after is not None and t >= req.arrival + req.cancel_after):
+                return "cancelled"
+            if req.deadline is not None and t > req.arrival + req.deadline:
+                return "expired"
+            return ""
+
         while pending or active:
+            if on_tick:
+                on_tick(self, ticks)
+            # cancellations and expiry at the tick boundary, of arrived requests
+            t_now = now()
+            keep = []
+            for req in pending:
+                why = leaving(req, t_now) if req.arrival <= t_now else ""
+                if why:
+                    results.append(Result(req.rid, req.arrival, float("nan"), t_now, 0, 0, float("nan"),
+                                          error=why, status=why))
+                else:
+                    keep.append(req)
+            pending = keep
+            for s in list(active):
+                why = leaving(s.req, t_now)
+                if why:
+                    results.append(Result(s.req.rid, s.req.arrival, admitted_at[s.req.rid], t_now, s.steps,
+                                          s.wet_cols * s.req.nz * s.req.ntracers * s.steps, float("nan"),
+                                          error=why, status=why))
+                    used -= request_bytes(s.req)
+                    active.remove(s)
+                    if log:
+                        log(f"[serve] request {s.req.rid} {why} after {s.steps} steps")
             # admission: arrived requests, in arrival order, while slots and budget allow
             while pending and pending[0].arrival <= now() and len(active) < self.max_slots:
                 req = pending[0]
@@ -264,7 +317,7 @@ class Server:
                 if need > self.hbm_budget:
                     pending.pop(0)
                     results.append(Result(req.rid, req.arrival, now(), now(), 0, 0, float("nan"),
-                                          error="request exceeds the HBM budget"))
+                                          error="request exceeds the HBM budget", status="rejected"))
                     continue
                 if used + need > self.hbm_budget:
                     break
@@ -292,12 +345,15 @@ class Server:
                     ens.step()
             for group in sorted(groups):
                 for s in groups[group]:
-                    if self.conserve:
+                    if self.conserve and self.fix == "tick":
                         s.fix()
                     s.steps += group[1]
             ticks += 1
             done = [s for s in active if s.steps >= s.req.nsteps]
             if done:
+                if self.conserve and self.fix == "end":
+                    for s in done:
+                        s.fix()
                 torch.cuda.synchronize()
                 for s in done:
                     res = Result(s.req.rid, s.req.arrival, admitted_at[s.req.rid], now(), s.steps,
@@ -316,6 +372,7 @@ class Server:
                     del self._members[group]
         self._groups.clear()
         self._members.clear()
+        self._cancel.clear()
         return results
 
 
@@ -333,13 +390,18 @@ def reference_fp64(req: Request, dom: inputs.Domain, device=None) -> torch.Tenso
     return tc.values
 
 
-def verify(results: Sequence[Result], requests: Sequence[Request], domains: Dict[int, inputs.Domain]) -> dict:
+def verify(results: Sequence[Result], requests: Sequence[Request], domains: Dict[int, inputs.Domain],
+           max_updates: Optional[float] = None) -> dict:
     """Error of every completed result (kept with ``keep_results``) against the FP64
-    plain method: max over tracers of max|c32 - c64| / max|c64| on wet cells."""
+    plain method: max over tracers of max|c32 - c64| / max|c64| on wet cells.
+    ``max_updates`` limits the check to results of at most that many tracer
+    cell-updates (the FP64 rerun of the largest requests takes minutes)."""
     by_id = {r.rid: r for r in requests}
     errs = {}
     for res in results:
         if res.error or res.tracers is None:
+            continue
+        if max_updates is not None and res.wet_updates > max_updates:
             continue
         req = by_id[res.rid]
         ref = reference_fp64(req, domains[req.rid], device=res.tracers.device)
@@ -365,7 +427,9 @@ def summary(results: Sequence[Result], wall: float, rate: Optional[float] = None
     ok = [r for r in results if not r.error]
     lat = [r.latency for r in ok]
     upd = sum(r.wet_updates for r in ok)
-    out = {"completed": len(ok), "rejected": len(results) - len(ok), "wall_s": wall,
+    out = {"completed": len(ok), "rejected": sum(r.status == "rejected" for r in results),
+           "cancelled": sum(r.status == "cancelled" for r in results),
+           "expired": sum(r.status == "expired" for r in results), "wall_s": wall,
            "tracer_cell_updates_per_s": upd / wall if wall > 0 else float("nan"),
            "latency_p50_s": percentile(lat, 50), "latency_p99_s": percentile(lat, 99),
            "latency_p999_s": percentile(lat, 99.9),
@@ -377,7 +441,8 @@ def summary(results: Sequence[Result], wall: float, rate: Optional[float] = None
 
 
 def sweep(reqs: Sequence[Request], domains: Dict[int, inputs.Domain], fractions: Sequence[float],
-          server: Server, verify_results: bool = True, seed: int = 0, log=None) -> dict:
+          server: Server, verify_results: bool = True, seed: int = 0, log=None,
+          verify_max_updates: Optional[float] = None) -> dict:
     """Saturation run (every request arrives at t = 0), optionally checked against
     the FP64 method, then one Poisson-arrival run per fraction of the measured
     saturation throughput.  Returns the report as a dict."""
@@ -402,7 +467,7 @@ def sweep(reqs: Sequence[Request], domains: Dict[int, inputs.Domain], fractions:
         f"p50 {s0['latency_p50_s']:.3f} s, drift {s0['max_inventory_drift']:.1e}")
     errs = {}
     if verify_results:
-        errs = verify(sat, reqs, domains)
+        errs = verify(sat, reqs, domains, verify_max_updates)
         s0["max_error_vs_fp64"] = max(errs.values()) if errs else float("nan")
         s0["verified"] = len(errs)
         log(f"[serve] {len(errs)} results checked against FP64: max error {s0['max_error_vs_fp64']:.2e}")
@@ -444,6 +509,10 @@ def main(argv=None):
     ap.add_argument("--steps", default="50,2000")
     ap.add_argument("--no-verify", action="store_true", help="skip the FP64 check of the saturation run")
     ap.add_argument("--no-conserve", action="store_true", help="no inventory fixer (raw FP32 steps)")
+    ap.add_argument("--fix", choices=["end", "tick"], default="end",
+                    help="inventory fixer once per request at completion, or after every tick")
+    ap.add_argument("--verify-max-updates", type=float, default=0.0,
+                    help="check against FP64 only results of at most this many tracer cell-updates (0: all)")
     ap.add_argument("--policy", choices=["fair", "lockstep", "uniform"], default="lockstep")
     ap.add_argument("--horizon", type=int, default=8, help="ticks per request under the uniform policy")
     ap.add_argument("--quantum", type=float, default=2.0 ** 25,
@@ -459,11 +528,12 @@ def main(argv=None):
     log(f"[serve] generating {len(reqs)} scenarios")
     domains = {r.rid: r.domain() for r in reqs}
     server = Server(hbm_budget=args.budget_gb * 1e9, max_slots=args.max_slots, conserve=not args.no_conserve,
-                    policy=args.policy, quantum=args.quantum, horizon=args.horizon)
+                    policy=args.policy, quantum=args.quantum, horizon=args.horizon, fix=args.fix)
     report = sweep(reqs, domains, [float(x) for x in args.fractions.split(",") if x], server,
-                   verify_results=not args.no_verify, seed=args.seed, log=log)
+                   verify_results=not args.no_verify, seed=args.seed, log=log,
+                   verify_max_updates=args.verify_max_updates or None)
     report["requests"].update(size=args.size, layers=args.layers, tracers=args.tracers, steps=args.steps,
-                              conserve=not args.no_conserve, policy=args.policy, quantum=args.quantum,
+                              conserve=not args.no_conserve, fix=args.fix, policy=args.policy, quantum=args.quantum,
                               horizon=args.horizon if args.policy == "uniform" else None)
     print(json.dumps(report))
     if args.out:

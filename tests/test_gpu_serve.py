@@ -141,3 +141,37 @@ def test_uniform_policy_mix_equals_alone_bitwise(domains):
         assert torch.equal(res.tracers, solo[rid].tracers), f"request {rid} differs in the mix"
         assert res.inventory_drift <= 1e-12
         assert res.finished - res.admitted <= horizon, (rid, res.admitted, res.finished)
+
+
+def test_fix_per_tick_mix_equals_alone_and_conserves(domains):
+    """The fixer after every tick instead of once at the end: results still do not
+    depend on the mix, and inventories hold to 1e-12."""
+    solo = run_alone(domains, fix="tick")
+    server, mix = run_mix(domains, fix="tick")
+    for rid, res in mix.items():
+        assert torch.equal(res.tracers, solo[rid].tracers), f"request {rid} differs in the mix"
+        assert res.inventory_drift <= 1e-12
+
+
+def test_cancelled_and_expired_requests_leave_and_free_their_slot(domains, alone):
+    """A request cancelled by its client, one cancelled through Server.cancel and
+    one past its deadline leave at a tick boundary with their status; the others'
+    results are unchanged, and the freed slot admits a waiting request."""
+    reqs = [dataclasses.replace(REQUESTS[0], cancel_after=4),  # client cancels at tick 4
+            dataclasses.replace(REQUESTS[1], deadline=5),  # 9 steps cannot finish within 5 ticks
+            REQUESTS[2], REQUESTS[3], REQUESTS[4]]
+
+    def on_tick(server, tick):
+        if tick == 7:
+            server.cancel(3)
+
+    server = Server(keep_results=True, max_slots=2)
+    results = {r.rid: r for r in server.run(reqs, domains, clock="ticks", on_tick=on_tick)}
+    assert results[0].status == "cancelled" and results[0].steps == 4
+    assert results[1].status == "expired" and results[1].steps < REQUESTS[1].nsteps
+    assert results[3].status == "cancelled"
+    for rid in (2, 4):
+        assert results[rid].status == "done" and not results[rid].error
+        assert torch.equal(results[rid].tracers, alone[rid].tracers)
+    # with two slots, request 2 (arrival 3) could only start once 0 or 1 had left
+    assert results[2].admitted >= 4
