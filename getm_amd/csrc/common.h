@@ -129,6 +129,28 @@ __device__ __forceinline__ void st_i16(Rsrc r, unsigned voff, unsigned soff, sho
 {
     __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, x), r, voff, soff, 0);
 }
+__device__ __forceinline__ void st_f64(Rsrc r, unsigned voff, unsigned soff, double x)
+{
+    using u2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, voff, soff, 0);
+}
+// Typed loads and stores of tracer values (FP32 or FP64).
+template <typename T>
+__device__ __forceinline__ T ld_val(Rsrc r, unsigned voff)
+{
+    if constexpr (sizeof(T) == 8) return ld_f64(r, voff, 0);
+    else return ld_f32(r, voff, 0);
+}
+template <typename T>
+__device__ __forceinline__ void st_val(Rsrc r, unsigned voff, T x)
+{
+    if constexpr (sizeof(T) == 8) st_f64(r, voff, 0, x);
+    else st_f32(r, voff, 0, x);
+}
+// Lane offset that the range check of every resource of fewer than 2^32 - 1
+// bytes rejects (with a zero wave-uniform offset): the load returns 0, the store
+// writes nothing.  Lets a lane skip an access without an exec-mask branch.
+constexpr unsigned RSRC_SKIP = 0xffffffffu;
 
 // Size of the 1-D grid of xcd_block for n logical blocks.
 inline unsigned xcd_grid(size_t n) { return (unsigned)((n + 7) / 8 * 8); }

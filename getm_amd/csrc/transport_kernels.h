@@ -278,6 +278,10 @@ template <typename R, bool C = false>
 __device__ __forceinline__ void cell_set(CellK<R>& c, bool upd, double hold, double hnew, double iA, double dt)
 {
     c.upd = upd;
+    // cells that are not updated still run the (discarded) update: defined factors
+    c.a = R(0);
+    c.b = R(1);
+    c.iA = R(0);
     if (!upd) return;
     if constexpr (C) {
         const double rh = rcp_nr(hnew);
@@ -553,17 +557,26 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             }
         }
         const size_t lay = (size_t)k * A.n2;
+        // tracer tiles and output cells go through buffer resources of one layer of
+        // one tracer (base and extent in SGPRs), lanes with nothing to access taking
+        // the offset RSRC_SKIP: every lane issues the same loads and stores in every
+        // iteration of the tracer loop, without exec-mask branches, so the compiler
+        // counts the memory operations in flight and waits for a tile's loads
+        // without waiting for the previous tracer's stores.  Tracers t >= ntr get an
+        // empty resource (nothing is read).
+        const unsigned lay_bytes = (unsigned)A.n2 * (unsigned)sizeof(T);
         T pf[NPIN];
         short pl[C ? NPIN : 1];
         auto load_tile = [&](int t) {
-            const T* fi = fin + (size_t)t * A.n3 + lay;
+            const Rsrc rf = make_rsrc(fin + (size_t)t * A.n3 + lay, t < ntr ? lay_bytes : 0u);
 #pragma unroll
             for (int m = 0; m < NPIN; m++)
-                pf[m] = toff[m] >= 0 ? at32(fi, (unsigned)toff[m] * (unsigned)sizeof(T)) : T(0);
+                pf[m] = ld_val<T>(rf, toff[m] >= 0 ? (unsigned)toff[m] * (unsigned)sizeof(T) : RSRC_SKIP);
             if constexpr (C) {
-                const short* li = lin + (size_t)t * A.n3 + lay;
+                const Rsrc rl = make_rsrc(lin + (size_t)t * A.n3 + lay, t < ntr ? (unsigned)A.n2 * 2u : 0u);
 #pragma unroll
-                for (int m = 0; m < NPIN; m++) pl[m] = toff[m] >= 0 ? at32(li, (unsigned)toff[m] * 2u) : (short)0;
+                for (int m = 0; m < NPIN; m++)
+                    pl[m] = ld_i16(rl, toff[m] >= 0 ? (unsigned)toff[m] * 2u : RSRC_SKIP, 0);
             }
         };
         auto store_tile = [&](int buf) {
@@ -625,6 +638,9 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
         for (int m = 0; m < NPA; m++) {
             const int c = tid + m * NTHR;
             ca[m].upd = 0;
+            ca[m].a = R(0);
+            ca[m].b = R(1);
+            ca[m].iA = R(0);
             if (c < NA) {
                 const int xl = AX0 + c % AW, yl = AY0 + c / AW;
                 const int own = (yl + 2) * HX + (xl + 2);
@@ -668,6 +684,9 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             const int own = (yl + 2) * HX + (xl + 2);
             const int f1 = (yl - FBY0) * FBW + (xl - FBX0);
             cb[m].upd = 0;
+            cb[m].a = R(0);
+            cb[m].b = R(1);
+            cb[m].iA = R(0);
             cb[m].own = own;
             cb[m].fl1 = f1;
             cb[m].rb = sr_cell_bits(ss_b, (unsigned)lay + (unsigned)ooff[m]);
@@ -684,13 +703,27 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
 
         // ---- tracer loop: tile t+1 was loaded during iteration t-1 and is stored to
         // LDS now, tile t+2 is loaded during this iteration (three LDS tile buffers)
-        if (ntr > 1) load_tile(1);
+        // the prefetches past the last tracer read nothing (empty resource) and fill
+        // a tile buffer that no tracer reads; they keep the loop's memory operations
+        // the same in every iteration
+        load_tile(1);
+        // as many stores (to nowhere) as a tracer's stage-B cells issue: entering the
+        // loop, tile 1's loads then have the same stores after them as every later
+        // tile's loads have (the previous tracer's), so the wait for them is the same
+        {
+            const Rsrc none = make_rsrc(fout, 0u);
+#pragma unroll
+            for (int m = 0; m < NPB; m++) {
+                st_val<T>(none, RSRC_SKIP, T(0));
+                if constexpr (C) st_i16(none, RSRC_SKIP, 0, 0);
+            }
+        }
         __syncthreads();  // s_q and s_h (read above) share the pool with the tile buffers
         for (int t = 0; t < ntr; t++) {
             const R* sin = s_in(t % 3);
             const SRound sr_a = tracer_round(ss_a, tg0 + t), sr_b = tracer_round(ss_b, tg0 + t);
-            if (t + 1 < ntr) store_tile((t + 1) % 3);
-            if (t + 2 < ntr) load_tile(t + 2);
+            store_tile((t + 1) % 3);
+            load_tile(t + 2);
             // stage A faces
 #pragma unroll
             for (int m = 0; m < NPFA; m++) {
@@ -713,11 +746,12 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             for (int m = 0; m < NPA; m++) {
                 const int c = tid + m * NTHR;
                 if (c < NA) {
-                    R out = sin[ca[m].own];
-                    if (ca[m].upd)
-                        out = cell_update<R, C>(ca[m], dt, out, s_fa[ca[m].fl1 - (D1 == 1 ? 1 : FAW)],
-                                                s_fa[ca[m].fl1], sr_a);
-                    s_a[ca[m].own] = out;
+                    // formed for every cell and selected (no branch); cells not updated keep
+                    // their value
+                    const R f = sin[ca[m].own];
+                    const R upd = cell_update<R, C>(ca[m], dt, f, s_fa[ca[m].fl1 - (D1 == 1 ? 1 : FAW)],
+                                                    s_fa[ca[m].fl1], sr_a);
+                    s_a[ca[m].own] = ca[m].upd ? upd : f;
                 }
             }
             __syncthreads();
@@ -739,27 +773,26 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
             }
             __syncthreads();
             // stage B cells -> HBM
-            T* fo = fout + (size_t)t * A.n3 + lay;
+            const Rsrc ro = make_rsrc(fout + (size_t)t * A.n3 + lay, lay_bytes);
 #pragma unroll
             for (int m = 0; m < NPB; m++) {
-                R out = s_a[cb[m].own];
-                if (cb[m].upd)
-                    out = cell_update<R, C>(cb[m], dt, out, s_fb[cb[m].fl1 - (D2 == 1 ? 1 : FBW)], s_fb[cb[m].fl1],
-                                            sr_b);
+                const R f = s_a[cb[m].own];
+                const R upd = cell_update<R, C>(cb[m], dt, f, s_fb[cb[m].fl1 - (D2 == 1 ? 1 : FBW)], s_fb[cb[m].fl1],
+                                                sr_b);
+                const R out = cb[m].upd ? upd : f;
                 // FP32: only updated cells are stored (the scratch buffer's other cells are
                 // never read, and the tracers' own keep their values); FP64 stores every
                 // cell, as its step ends with a copy of the whole scratch buffer
+                const bool st = (C || std::is_same<R, float>::value) ? (bool)cb[m].upd : ooff[m] >= 0;
                 if constexpr (C) {
-                    if (cb[m].upd) {
-                        float hi;
-                        short lo;
-                        cencode(out, hi, lo);
-                        at32(fo, (unsigned)ooff[m] * 4u) = hi;
-                        at32(lout + (size_t)t * A.n3 + lay, (unsigned)ooff[m] * 2u) = lo;
-                    }
+                    float hi;
+                    short lo;
+                    cencode(out, hi, lo);
+                    const Rsrc rlo = make_rsrc(lout + (size_t)t * A.n3 + lay, (unsigned)A.n2 * 2u);
+                    st_f32(ro, st ? (unsigned)ooff[m] * 4u : RSRC_SKIP, 0, hi);
+                    st_i16(rlo, st ? (unsigned)ooff[m] * 2u : RSRC_SKIP, 0, lo);
                 } else {
-                    if (std::is_same<R, float>::value ? cb[m].upd : ooff[m] >= 0)
-                        at32(fo, (unsigned)ooff[m] * (unsigned)sizeof(R)) = out;
+                    st_val<T>(ro, st ? (unsigned)ooff[m] * (unsigned)sizeof(T) : RSRC_SKIP, out);
                 }
             }
         }
