@@ -388,6 +388,7 @@ def run_ensemble(args, log):
     if args.autotune:
         shape = Autotuner(None).tune_ensemble(ens, reps=2)
         log(f"[bench] autotuner picked {shape}")
+    tuning, tile_x = ens.tuning, ens.tile_x
     for _ in range(args.warmup):
         ens.step()
     torch.cuda.synchronize()
@@ -400,6 +401,36 @@ def run_ensemble(args, log):
     elapsed = time.perf_counter() - t0
     cells = ens.cells
     value = cells * args.steps / elapsed
+    # roofline: one more (eager) step with every kernel timed by HIP events on its
+    # launch stream; algorithmic bytes summed over the domains
+    from getm_amd import _lib
+
+    _lib.profile_enable(True)
+    for k in _lib.KERNELS:
+        _lib.profile_read(k, reset=True)
+    ens.step()
+    torch.cuda.synchronize()
+    _lib.profile_enable(False)
+    ktimes = {k: v for k, v in ((k, _lib.profile_read(k)) for k in _lib.KERNELS) if v[1] > 0}
+    dom = max(ktimes, key=lambda k: ktimes[k][0])
+    wet_cols = [int((g.mask.all_values == 1).sum().item()) for g in grids]
+    alg = sum(algorithmic_bytes(dom, wc, g.nz_, 4, nt) for wc, g, nt in zip(wet_cols, grids, [s[3] for s in specs]))
+    achieved = alg / (ktimes[dom][0] / 1e3) / 1e9  # every kernel kind is one launch per step
+    log("[bench] ensemble kernel times of one eager step (ms, launches): " +
+        ", ".join(f"{k}={v[0]:.3f}/{v[1]}" for k, v in ktimes.items()))
+    del ens, graph, tracers, keep
+    torch.cuda.empty_cache()
+    cpu = None
+    if not args.no_cpu:
+        cores = args.cpu_cores or min(16, os.cpu_count() or 1)
+        rate, cores, impl = cpu_baseline(args.cpu_seconds, cores, 5)
+        model = cpu_model()
+        cpu = {"value": rate, "unit": "tracer cell-updates/s", "cores": cores,
+               "kind": "reference" if impl == "ref" else "port", "cpu_model": model,
+               "sample": f"{'the reference Fortran sweeps (oracle/_ref) + the C restatement of diffusion.F90' if impl == 'ref' else 'FP64 oracle'}; "
+                         f"{cores} processes on {model}, each stepping a {NX // 4}x{NY // 4}x{NZ} coastal slab, 1 tracer, "
+                         f"SUPERBEE, FP64, ~{args.cpu_seconds:.0f} s each (the reference steps one domain and one "
+                         "tracer at a time, so its rate per tracer cell does not depend on the ensemble)"}
     out = {
         "metric": METRIC,
         "value": value,
@@ -418,12 +449,16 @@ def run_ensemble(args, log):
                         f"tracers), SUPERBEE, FP32 tracers with FP64 thickness, {args.steps}-step hipGraph chunk",
             "domains": len(specs),
             "tracer_cells_per_step": cells,
-            "tuning": list(shape) if shape else list(ens.tuning),
+            "tuning": list(shape) if shape else list(tuning),
+            "tile_x": tile_x,
             "autotuned": bool(args.autotune),
             "rounding": args.rounding,
+            "kernel_ms_per_step": {k: v[0] for k, v in ktimes.items()},
         },
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "traffic_source": None,
+                     "algorithmic_bytes": alg},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -603,7 +638,8 @@ def run_producers(args, log):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps (default 20; the ensemble workload: one 100-step hipGraph chunk)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["multi", "coastal", "ensemble", "producers"], default="multi",
                     help="multi: 1024x768x40, 256 blob tracers, P2_PDM; coastal: same grid with T and S, SUPERBEE; "
@@ -639,6 +675,8 @@ def main():
               "sharding over several GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
+    if not args.steps:
+        args.steps = 100 if args.workload == "ensemble" else 20
     if args.workload == "ensemble":
         return run_ensemble(args, lambda m: print(m, file=sys.stderr, flush=True))
     if args.workload == "producers":

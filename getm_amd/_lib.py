@@ -74,6 +74,7 @@ _SIGNATURES = {
     "ensemble_finalize": (None, [_vp]),
     "ensemble_bind": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _d, _d]),
     "ensemble_set_tuning": (_i, [_vp, _i, _i, _i, _i]),
+    "ensemble_set_tile_x": (_i, [_vp, _i]),
     "ensemble_set_solver": (_i, [_vp, _i]),
     "ensemble_set_rounding": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
     "ensemble_commit": (_i, [_vp, _vp]),

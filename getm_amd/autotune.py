@@ -1,8 +1,8 @@
 """Launch-shape and solver autotuner for the batched step and ensembles.
 
-Searches the height of the horizontal-stage tiles (4, 8 or 16 rows of 64
-columns), the tracers per block of the three kernel kinds, and the column
-solver by depth: per-lane Thomas elimination (the reference's algorithm,
+Searches the height of the horizontal-stage tiles (4, 8 or 16 rows; for
+ensembles also their width, 32, 64 or 128 columns), the tracers per block of
+the three kernel kinds, and the column solver by depth: per-lane Thomas elimination (the reference's algorithm,
 diffusion.F90:276-313) or wave-level parallel cyclic reduction for columns of
 up to ``pcr_max_nz`` levels.  Each candidate is timed with HIP events on the
 current stream, and the best is cached per workload signature (grid size
@@ -22,11 +22,19 @@ from typing import Callable, Dict, Iterable, Optional, Tuple
 
 import torch
 
-Shape = Tuple[int, ...]  # (tile_y, tracers/block h, w, d[, pcr_max_nz])
+Shape = Tuple[int, ...]  # (tile_y, tracers/block h, w, d[, pcr_max_nz[, tile_x]])
 
 DEFAULT_CANDIDATES = [
     (ty, h, w, d, pcr)
     for ty, h, (w, d), pcr in itertools.product((16, 8, 4), (0, 32), ((16, 16), (64, 64)), (0, 16, 32))
+]
+
+# ensembles: tile width x height (the shapes ensemble_set_tile_x accepts), the
+# column solver by depth, and two tracer batchings
+ENSEMBLE_TILES = ((32, 8), (32, 16), (64, 4), (64, 8), (64, 16), (128, 4), (128, 8))
+ENSEMBLE_CANDIDATES = [
+    (ty, h, w, d, pcr, tx)
+    for (tx, ty), (h, w, d), pcr in itertools.product(ENSEMBLE_TILES, ((0, 16, 16), (32, 64, 64)), (0, 32))
 ]
 
 
@@ -107,8 +115,10 @@ class Autotuner:
         def apply(s):
             ens.set_tuning(*s[:4])
             ens.set_solver(s[4] if len(s) > 4 else 0)
+            ens.set_tile_x(s[5] if len(s) > 5 else 64)
             ens.commit()
 
+        kw.setdefault("candidates", ENSEMBLE_CANDIDATES)
         snap = ens.snapshot()
         try:
             return self.tune(key, apply, ens.step, **kw)

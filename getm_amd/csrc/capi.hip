@@ -770,6 +770,15 @@ int ensemble_bind(void* p, int d, float* tracers, const double* u, const double*
     return GETM_OK;
 }
 
+int ensemble_set_tile_x(void* p, int tile_x)
+{
+    Ensemble* e = static_cast<Ensemble*>(p);
+    if (!e || (tile_x != 32 && tile_x != 64 && tile_x != 128)) return GETM_ERR_ARG;
+    e->tile_x = tile_x;
+    e->committed = 0;
+    return GETM_OK;
+}
+
 int ensemble_set_tuning(void* p, int tile_y, int tpb_h, int tpb_w, int tpb_d)
 {
     Ensemble* e = static_cast<Ensemble*>(p);
@@ -826,6 +835,8 @@ int ensemble_commit(void* p, void* stream)
     }
     const int mean_ntr = (int)((ntr_all + e->ndom - 1) / e->ndom);
     e->tile_y_used = e->tile_y ? e->tile_y : auto_tile_y(mean_ntr);
+    if (!ens_tile_ok(e->tile_x, e->tile_y_used)) return GETM_ERR_ARG;  // e.g. 128-wide tiles of 16 rows
+    e->tile_x_used = e->tile_x;
     e->hpair_kg = 1;  // one layer per work item: measured faster than layer groups on uneven ensembles
     int rc;
     if ((rc = e->scratch.ensure(nscr * sizeof(float))) || (rc = e->hbuf.ensure(nh * sizeof(double)))) return rc;
@@ -856,7 +867,7 @@ int ensemble_commit(void* p, void* stream)
         if (nbh > 32767 || t->nz > 65535) return GETM_ERR_CAPACITY;
         // logical order as k_hpair: tracer batch fastest, then layer group, then tile
         for (int y0 = 0; y0 < t->ny_; y0 += e->tile_y_used)
-            for (int x0 = 0; x0 < t->nx_; x0 += 64)
+            for (int x0 = 0; x0 < t->nx_; x0 += e->tile_x_used)
                 for (int k0 = 0; k0 < t->nz; k0 += e->hpair_kg)
                     for (int b = 0; b < nbh; b++) wh.push_back(make_int4(d, x0, y0, k0 | (b << 16)));
         for (int b = 0; b < nbw; b++)
@@ -918,11 +929,11 @@ int ensemble_step(void* p, void* stream)
     const EnsDom<float>* doms = static_cast<const EnsDom<float>*>(e->doms.ptr);
     const int d1 = e->ufirst ? 1 : 2;
     int rc;
-    if ((rc = launch_hpair_multi(1, d1, e->scheme, e->tile_y_used, e->hpair_kg, doms, static_cast<const int4*>(e->work_h.ptr), e->nwork_h,
+    if ((rc = launch_hpair_multi(1, d1, e->scheme, e->tile_y_used, e->tile_x_used, e->hpair_kg, doms, static_cast<const int4*>(e->work_h.ptr), e->nwork_h,
                                  s)))
         return rc;
     if ((rc = launch_wcol_multi(e->scheme, doms, static_cast<const int4*>(e->work_w.ptr), e->nwork_w, s))) return rc;
-    if ((rc = launch_hpair_multi(2, 3 - d1, e->scheme, e->tile_y_used, e->hpair_kg, doms, static_cast<const int4*>(e->work_h.ptr),
+    if ((rc = launch_hpair_multi(2, 3 - d1, e->scheme, e->tile_y_used, e->tile_x_used, e->hpair_kg, doms, static_cast<const int4*>(e->work_h.ptr),
                                  e->nwork_h, s)))
         return rc;
     const int4* wd = static_cast<const int4*>(e->work_d.ptr);

@@ -147,6 +147,8 @@ struct Ensemble {
     double cnpar = 1.0;
     int tile_y = 0, tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tile_y 0 = by tracers per domain
     int tile_y_used = 16;                               // the height of the committed work table
+    int tile_x = 64;                                    // width of the horizontal-stage tiles (32, 64, 128)
+    int tile_x_used = 64;                               // the width of the committed work table
     int hpair_kg = 1;                                   // layers per horizontal-stage work item (hpair_group)
     int nwork_h = 0, nwork_w = 0, nwork_d = 0, nzmax = 1;
     int nwork_d_class[8] = {};  // diffusion work items per depth class (8 c-7 .. 8 c + 8 layers), in this order
@@ -181,8 +183,14 @@ template <typename R>
 int launch_hpair(int d1, int scheme, int ty, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout,
                  const double* hin, double* hout, unsigned sr_stage, hipStream_t s, const double* hin_dry = nullptr,
                  bool write_dry = true);
-int launch_hpair_multi(int stage, int d1, int scheme, int ty, int kg, const EnsDom<float>* doms, const int4* work,
-                       int nwork, hipStream_t s);
+int launch_hpair_multi(int stage, int d1, int scheme, int ty, int tx, int kg, const EnsDom<float>* doms,
+                       const int4* work, int nwork, hipStream_t s);
+// tile shapes of the ensembles' horizontal stages: 32 x {8, 16}, 64 x {4, 8, 16}, 128 x {4, 8}
+inline bool ens_tile_ok(int tx, int ty)
+{
+    return (tx == 32 && (ty == 8 || ty == 16)) || (tx == 64 && (ty == 4 || ty == 8 || ty == 16)) ||
+           (tx == 128 && (ty == 4 || ty == 8));
+}
 int launch_wcol_multi(int scheme, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);
 // one depth class: every domain of the work items has 8 (cls + 1) - 7 .. 8 (cls + 1) layers
 int launch_vdiff_multi(bool cn1, int cls, const EnsDom<float>* doms, const int4* work, int nwork, hipStream_t s);

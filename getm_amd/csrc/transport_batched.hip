@@ -156,39 +156,50 @@ template int launch_hpair<double>(int, int, int, const StepArgs&, double, int, i
 // ---------------------------------------------------------------------------
 // ensemble launchers: one launch per kernel kind covers every domain
 // ---------------------------------------------------------------------------
-int launch_hpair_multi(int stage, int d1, int scheme, int ty, int kg, const EnsDom<float>* doms, const int4* work,
-                       int nwork, hipStream_t s)
+int launch_hpair_multi(int stage, int d1, int scheme, int ty, int tx, int kg, const EnsDom<float>* doms,
+                       const int4* work, int nwork, hipStream_t s)
 {
     if (nwork <= 0) return GETM_OK;
     if (kg != 1) return GETM_ERR_ARG;  // ensembles take one layer per work item
+    if (!ens_tile_ok(tx, ty)) return GETM_ERR_ARG;
     ProfileScope ps(stage == 1 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
-#define GETM_HPMK(TYV, KGV)                                                                                     \
+#define GETM_HPM(TXV, TYV)                                                                                      \
     GETM_SCHEME_SWITCH(scheme, {                                                                                \
         if (stage == 1 && d1 == 2)                                                                              \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 1, KGV>), dim3(nwork), dim3(TX * TYV), 0, s, doms, \
-                               work);                                                                           \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 1, 1, TXV>), dim3(nwork), dim3(TXV * TYV), 0, s, \
+                               doms, work);                                                                     \
         else if (stage == 1)                                                                                    \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 1, KGV>), dim3(nwork), dim3(TX * TYV), 0, s, doms, \
-                               work);                                                                           \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 1, 1, TXV>), dim3(nwork), dim3(TXV * TYV), 0, s, \
+                               doms, work);                                                                     \
         else if (d1 == 2)                                                                                       \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 2, KGV>), dim3(nwork), dim3(TX * TYV), 0, s, doms, \
-                               work);                                                                           \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 2, 1, TXV>), dim3(nwork), dim3(TXV * TYV), 0, s, \
+                               doms, work);                                                                     \
         else                                                                                                    \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 2, KGV>), dim3(nwork), dim3(TX * TYV), 0, s, doms, \
-                               work);                                                                           \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 2, 1, TXV>), dim3(nwork), dim3(TXV * TYV), 0, s, \
+                               doms, work);                                                                     \
     })
-#define GETM_HPM(TYV) GETM_HPMK(TYV, 1)
-    if (ty == 16) {
-        GETM_HPM(16);
-    } else if (ty == 8) {
-        GETM_HPM(8);
-    } else if (ty == 4) {
-        GETM_HPM(4);
+    if (tx == 64) {
+        if (ty == 16) {
+            GETM_HPM(64, 16);
+        } else if (ty == 8) {
+            GETM_HPM(64, 8);
+        } else {
+            GETM_HPM(64, 4);
+        }
+    } else if (tx == 32) {
+        if (ty == 16) {
+            GETM_HPM(32, 16);
+        } else {
+            GETM_HPM(32, 8);
+        }
     } else {
-        return GETM_ERR_ARG;
+        if (ty == 8) {
+            GETM_HPM(128, 8);
+        } else {
+            GETM_HPM(128, 4);
+        }
     }
 #undef GETM_HPM
-#undef GETM_HPMK
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }

@@ -850,8 +850,8 @@ __global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(s
 // table is built in the XCD-aware order of k_hpair (domain -1 = padding).
 // STAGE 1: stages 1-2 (tracers -> scratch, ho -> hA); STAGE 2: stages 4-5
 // (tracers -> scratch, hB -> hA).
-template <int S, typename R, int D1, int TY, int STAGE, int KG>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair_multi(const EnsDom<R>* __restrict__ doms,
+template <int S, typename R, int D1, int TY, int STAGE, int KG, int TXW = TX>
+__global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair_multi(const EnsDom<R>* __restrict__ doms,
                                                          const int4* __restrict__ work)
 {
     const int4 wk = work[blockIdx.x];
@@ -861,7 +861,7 @@ __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(si
     const int k1 = k0 + KG < D.A.nz ? k0 + KG : D.A.nz;
     const int ntr = D.ntr - tb0 < D.tpb_h ? D.ntr - tb0 : D.tpb_h;
     const size_t off = (size_t)tb0 * D.A.n3;
-    hpair_body<S, R, D1, TY, (KG > 1), false>(D.A, 0.5 * D.dt, wk.y, wk.z, k0, k1, ntr, D.tracers + off,
+    hpair_body<S, R, D1, TY, (KG > 1), false, TXW>(D.A, 0.5 * D.dt, wk.y, wk.z, k0, k1, ntr, D.tracers + off,
                                               D.scratch + off,
                              STAGE == 1 ? D.ho : D.hB, D.hA, STAGE == 1 ? SR_PAIR1 : SR_PAIR2, tb0,
                              // stages 4-5 end in hA, where stages 1-2 left the cells

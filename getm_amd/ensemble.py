@@ -45,6 +45,7 @@ class Ensemble:
         self._keep = [None] * n  # tensors bound to the library, kept alive here
         self.tracers: List[Optional[torch.Tensor]] = [None] * n
         self.tuning = (0, 0, 16, 16)
+        self.tile_x = 64
         self.pcr_max_nz = 0
         self.rounding = "stochastic"
         self._counters = None
@@ -86,6 +87,13 @@ class Ensemble:
                                                    int(tracers_per_block_w), int(tracers_per_block_d)),
                    "ensemble_set_tuning")
         self.tuning = (tile_y, tracers_per_block_h, tracers_per_block_w, tracers_per_block_d)
+
+    def set_tile_x(self, tile_x: int = 64) -> None:
+        """Width of the horizontal-stage tiles: 32, 64 or 128 columns (32 with 8 or 16
+        rows, 64 with 4, 8 or 16, 128 with 4 or 8); takes effect at commit and does
+        not change results."""
+        _lib.check(_lib.load().ensemble_set_tile_x(self.p, int(tile_x)), "ensemble_set_tile_x")
+        self.tile_x = int(tile_x)
 
     def set_solver(self, pcr_max_nz: int = 0) -> None:
         """Column solver per domain by depth (TracerCollection.set_solver); takes effect at commit."""

@@ -283,6 +283,14 @@ int ensemble_bind(void* ens, int domain, float* tracers, const double* u, const 
 int ensemble_set_tuning(void* ens, int tile_y, int tracers_per_block_h, int tracers_per_block_w,
                         int tracers_per_block_d);
 
+/* Width of the horizontal-stage tiles of an ensemble: 32, 64 (the default) or
+ * 128 columns; with the tile height it gives 32 x {8, 16}, 64 x {4, 8, 16} or
+ * 128 x {4, 8} (ensemble_commit returns GETM_ERR_ARG for another pair).  Takes
+ * effect at the next commit; results do not depend on it.  No counterpart in
+ * the reference (a launch shape of this implementation, searched by the
+ * autotuner). */
+int ensemble_set_tile_x(void* ens, int tile_x);
+
 /* Column solver as transport_set_solver, per domain by its nz; takes effect at
  * the next commit. */
 int ensemble_set_solver(void* ens, int pcr_max_nz);

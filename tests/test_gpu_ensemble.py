@@ -18,7 +18,7 @@ def small_ensemble(scheme, seed=5):
                                    scheme=scheme)
 
 
-def make_ensemble(doms, tuning=None):
+def make_ensemble(doms, tuning=None, tile_x=None):
     from getm_amd.core import CENTERS, INTERFACES
     from getm_amd.ensemble import Ensemble
 
@@ -29,6 +29,8 @@ def make_ensemble(doms, tuning=None):
                  g.vgrid.wrap(dev(d.v), CENTERS), g.wrap(dev(d.w), INTERFACES), g.wrap(dev(d.nuh), INTERFACES), d.dt)
     if tuning:
         ens.set_tuning(*tuning)
+    if tile_x:
+        ens.set_tile_x(tile_x)
     ens.commit()
     return ens
 
@@ -86,6 +88,35 @@ def test_ensemble_launch_shape_independent():
             assert_bitwise(o[i], outs[0][i], f"domain {i}")
 
 
+def test_ensemble_tile_widths_do_not_change_results():
+    """Every tile shape of the horizontal stages (32 x {8, 16}, 64 x {4, 8, 16},
+    128 x {4, 8}) gives the single-domain step's bits, also on domains narrower
+    than a tile and with a ragged last tile."""
+    from getm_amd.autotune import ENSEMBLE_TILES
+
+    doms = small_ensemble(3, seed=17) + [inputs.tiny(seed=90, scheme=3, ntracers=2, nx=130, ny=9, nz=4)]
+    ref = [single_domain(d, 2) for d in doms]
+    for tx, ty in ENSEMBLE_TILES:
+        ens = make_ensemble(doms, (ty, 0, 16, 16), tile_x=tx)
+        ens.step()
+        ens.step()
+        for i in range(len(doms)):
+            assert_bitwise(host(ens.tracers[i]), ref[i], f"tile {tx}x{ty}, domain {i}")
+
+
+def test_ensemble_rejects_unsupported_tile_shapes():
+    from getm_amd._lib import GetmError
+
+    doms = small_ensemble(5, seed=19)
+    ens = make_ensemble(doms)
+    with pytest.raises(GetmError):
+        ens.set_tile_x(48)
+    ens.set_tile_x(128)
+    ens.set_tuning(16, 0, 16, 16)  # 128 x 16 would be 2048 lanes
+    with pytest.raises(GetmError):
+        ens.commit()
+
+
 def test_ensemble_graph_chunks_replay_equal_eager():
     import torch
 
@@ -112,7 +143,7 @@ def test_autotuner_keeps_results(tmp_path):
     from getm_amd.autotune import Autotuner
 
     doms = small_ensemble(5, seed=13)
-    cands = [(16, 0, 16, 16, 0), (8, 2, 4, 4, 32)]
+    cands = [(16, 0, 16, 16, 0, 32), (8, 2, 4, 4, 32, 128), (4, 0, 16, 16, 0, 64)]
     cache = str(tmp_path / "tune.json")
     for run in ("timed", "cached"):
         ens = make_ensemble(doms)
@@ -124,6 +155,7 @@ def test_autotuner_keeps_results(tmp_path):
             assert_bitwise(host(ens.tracers[i]), host(ref.tracers[i]), f"{run}: after tuning, domain {i}")
         ref.set_tuning(*shape[:4])
         ref.set_solver(shape[4])
+        ref.set_tile_x(shape[5])
         ref.commit()
         for _ in range(2):
             ens.step()
