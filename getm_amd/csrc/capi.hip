@@ -403,13 +403,19 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     const int d2 = 3 - d1;
     // compensated form with many tracers: 16-row tiles (less halo to decode and sweep)
     const int ty = tr->tile_y ? tr->tile_y : (C && ntr >= 16 ? 16 : auto_tile_y(ntr));
+    // compensated form with many tracers: the y-first pair on 12-row tiles, two
+    // 768-lane blocks per CU (its first stage adds no rows, so the shorter tile
+    // costs it no halo work); the x-first pair on 16-row tiles
+    auto ty_of = [&](int dfirst) {
+        return (C && !tr->tile_y && ntr >= 16 && dfirst == 2 && !A.ah_u) ? 12 : ty;
+    };
     int* stats = static_cast<int*>(tr->stats.ptr);
     // FP32 forms: the w stage leaves the thickness of the columns it does not update
     // in hA (stages 1-2 wrote every cell there), and stages 4-5 read those cells from
     // hA instead of a copy in hB; into hA itself they store updated cells only.
     constexpr bool f32 = std::is_same<R, float>::value;
     if constexpr (C)
-        rc = launch_hpair_c(d1, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, fl, S, SL, t->ho, hA, s);
+        rc = launch_hpair_c(d1, tr->scheme, ty_of(d1), A, half, ntr, tr->tpb_h, f, fl, S, SL, t->ho, hA, s);
     else
         rc = launch_hpair<R>(d1, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, t->ho, hA, SR_PAIR1, s);
     if (rc) return rc;
@@ -424,7 +430,7 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     });
     if (rc) return rc;
     if constexpr (C)
-        rc = launch_hpair_c(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, fl, S, SL, hB, hfinal, s, hA,
+        rc = launch_hpair_c(d2, tr->scheme, ty_of(d2), A, half, ntr, tr->tpb_h, f, fl, S, SL, hB, hfinal, s, hA,
                             hfinal != hA);
     else
         rc = launch_hpair<R>(d2, tr->scheme, ty, A, half, ntr, tr->tpb_h, f, S, hB, hfinal, SR_PAIR2, s,
@@ -642,7 +648,9 @@ int transport_set_tuning(void* p, int tile_y, int tracers_per_block_h, int trace
 {
     Transport* tr = static_cast<Transport*>(p);
     if (!tr || tracers_per_block_h < 0 || tracers_per_block_w < 0 || tracers_per_block_d < 0) return GETM_ERR_ARG;
-    if (tile_y != 0 && tile_y != 4 && tile_y != 8 && tile_y != 16) return GETM_ERR_ARG;
+    // 12-row tiles: the compensated form only
+    if (tile_y != 0 && tile_y != 4 && tile_y != 8 && tile_y != 16 && !(tile_y == 12 && tr->precision == GETM_F32C))
+        return GETM_ERR_ARG;
     tr->tile_y = tile_y;
     tr->tpb_h = tracers_per_block_h;
     tr->tpb_w = tracers_per_block_w;

@@ -816,8 +816,17 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
 // Every tracer-batch block writes the same thickness.
 // FP32: at most 128 VGPRs (four waves per SIMD), two 512-lane blocks per CU at TY = 8.
 // KG: layers per block (1 or HPAIR_KG); HDIFF: horizontal diffusion (A.ah_u set).
+// Occupancy floor of the pair kernels: FP32 four waves per SIMD (128 VGPRs), FP64
+// two; the compensated form's 768-lane blocks (12-row tiles) six, so that two
+// blocks share a CU and cover each other's barrier phases.
+template <typename R, bool C, bool HDIFF>
+constexpr int hpair_waves_per_eu(int lanes)
+{
+    return sizeof(R) == 4 ? 4 : (C && !HDIFF && lanes == 768 ? 6 : 2);
+}
+
 template <int S, typename R, int D1, int TY, int KG, bool HDIFF, int TXW, bool C = false>
-__global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
+__global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(hpair_waves_per_eu<R, C, HDIFF>(TXW * TY)))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
                                                    int nty, int nbatch,
                                                    const std::conditional_t<C, float, R>* __restrict__ fin_all,
                                                    std::conditional_t<C, float, R>* __restrict__ fout_all,
@@ -1905,7 +1914,7 @@ inline int launch_hpair_impl(int d1, int scheme, int ty, const StepArgs& A, doub
     if (!hin_dry) hin_dry = hin;
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
-    if (ty != 4 && ty != 8 && ty != 16) return GETM_ERR_ARG;
+    if (ty != 4 && ty != 8 && ty != 16 && !(C && ty == 12)) return GETM_ERR_ARG;
     // Tile shape.  Stages 4-5 sweep x first, so their first stage covers the
     // tile's rows plus two on each side: 32-wide tiles of twice the height
     // (same lane count) cut that overhead from (ty + 4)/ty to (2 ty + 4)/(2 ty).
@@ -1944,6 +1953,8 @@ inline int launch_hpair_impl(int d1, int scheme, int ty, const StepArgs& A, doub
     if (d1 == 2) {
         if (ty == 16) {
             GETM_HP(2, 16, TX);
+        } else if (C && ty == 12) {
+            GETM_HP(2, 12, TX);
         } else if (ty == 8) {
             GETM_HP(2, 8, TX);
         } else {
@@ -1958,6 +1969,8 @@ inline int launch_hpair_impl(int d1, int scheme, int ty, const StepArgs& A, doub
     } else {
         if (ty == 16) {
             GETM_HP(1, 16, TX);
+        } else if (C && ty == 12) {
+            GETM_HP(1, 12, TX);
         } else if (ty == 8) {
             GETM_HP(1, 8, TX);
         } else {

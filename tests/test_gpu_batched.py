@@ -277,6 +277,37 @@ def test_batched_launch_shape_does_not_change_results(precision, ntracers):
         assert_bitwise(o[1], outs[0][1], "thickness")
 
 
+def test_compensated_tile_heights_do_not_change_results():
+    """The compensated form with 16 or more tracers runs its y-first pair on 12-row
+    tiles (two 768-lane blocks per CU) and its x-first pair on 16-row ones; 12-row
+    tiles for both pairs, 16-, 8- and 4-row tiles give the same bits.  Other forms
+    reject 12-row tiles."""
+    from getm_amd._lib import GetmError
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.tracer import TracerCollection
+
+    d = inputs.tiny(nx=75, ny=53, nz=7, seed=93, ntracers=20, cfl=0.5, wscale=2.0)
+    outs = []
+    for tuning in [None, (0, 16, 16, 12), (0, 16, 16, 16), (0, 16, 16, 8), (3, 16, 16, 4)]:
+        grid = make_grid(d)
+        tc = TracerCollection(grid, advection_scheme=d.scheme, cnpar=d.cnpar, precision="f32")
+        if tuning:
+            tc.set_tuning(*tuning)
+        tc.set_values(dev(d.tracers))
+        for _ in range(2):
+            tc.advance(d.dt, grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS),
+                       grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES))
+        outs.append((host(tc.values), host(tc.residuals), host(tc.h)))
+    for o in outs[1:]:
+        assert_bitwise(o[0], outs[0][0], "values")
+        assert np.array_equal(o[1], outs[0][1]), "residuals"
+        assert_bitwise(o[2], outs[0][2], "thickness")
+    for precision in ("f64", "f32-sr"):
+        tc = TracerCollection(make_grid(d), advection_scheme=d.scheme, precision=precision)
+        with pytest.raises(GetmError):
+            tc.set_tuning(0, 16, 16, 12)
+
+
 @pytest.mark.parametrize("precision", ["f64", "f32", "f32-sr"])
 def test_inventory_matches_exact_sum(precision):
     """Device inventories equal the exactly rounded host sum (math.fsum) to 1e-15."""
