@@ -163,20 +163,23 @@ int launch_hpair_multi(int stage, int d1, int scheme, int ty, int tx, int kg, co
     if (kg != 1) return GETM_ERR_ARG;  // ensembles take one layer per work item
     if (!ens_tile_ok(tx, ty)) return GETM_ERR_ARG;
     ProfileScope ps(stage == 1 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
-#define GETM_HPM(TXV, TYV)                                                                                      \
-    GETM_SCHEME_SWITCH(scheme, {                                                                                \
-        if (stage == 1 && d1 == 2)                                                                              \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 1, 1, TXV>), dim3(nwork), dim3(TXV * TYV), 0, s, \
-                               doms, work);                                                                     \
-        else if (stage == 1)                                                                                    \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 1, 1, TXV>), dim3(nwork), dim3(TXV * TYV), 0, s, \
-                               doms, work);                                                                     \
-        else if (d1 == 2)                                                                                       \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 2, 1, TXV>), dim3(nwork), dim3(TXV * TYV), 0, s, \
-                               doms, work);                                                                     \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 2, 1, TXV>), dim3(nwork), dim3(TXV * TYV), 0, s, \
-                               doms, work);                                                                     \
+    // tiles of 1024 cells: two cells per lane (512-lane blocks, two per CU)
+#define GETM_HPM(TXV, TYV)                                                                                    \
+    GETM_SCHEME_SWITCH(scheme, {                                                                              \
+        constexpr int L = TXV * TYV >= 1024 ? 2 : 1;                                                          \
+        const dim3 blk(TXV * TYV / L);                                                                        \
+        if (stage == 1 && d1 == 2)                                                                            \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 1, 1, TXV, L>), dim3(nwork), blk, 0, s, doms, \
+                               work);                                                                         \
+        else if (stage == 1)                                                                                  \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 1, 1, TXV, L>), dim3(nwork), blk, 0, s, doms, \
+                               work);                                                                         \
+        else if (d1 == 2)                                                                                     \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 2, TYV, 2, 1, TXV, L>), dim3(nwork), blk, 0, s, doms, \
+                               work);                                                                         \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_hpair_multi<S_, float, 1, TYV, 2, 1, TXV, L>), dim3(nwork), blk, 0, s, doms, \
+                               work);                                                                         \
     })
     if (tx == 64) {
         if (ty == 16) {

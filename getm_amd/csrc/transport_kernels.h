@@ -338,7 +338,7 @@ __device__ __forceinline__ SRound tracer_round(unsigned stage_seed, int tracer)
 // C: compensated FP32 storage (R = double): fin/fout hold the FP32 values and
 // lin/lout their int16 residuals; tiles are decoded to FP64 in LDS, faces are
 // FaceKC, cells the FP64 increment form, and updated cells are encoded again.
-template <int S, typename R, int D1, int TY, bool GROUP, bool HDIFF, int TXW = TX, bool C = false>
+template <int S, typename R, int D1, int TY, bool GROUP, bool HDIFF, int TXW = TX, bool C = false, int LPC = 1>
 __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0, int y0, int k0, int k1, int ntr,
                                            const std::conditional_t<C, float, R>* __restrict__ fin,
                                            std::conditional_t<C, float, R>* __restrict__ fout,
@@ -353,7 +353,8 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
     constexpr int TX = TXW, HX = TXW + 4;
     constexpr int HY = TY + 4;       // input tile height incl. halo 2
     constexpr int NT = HX * HY;      // cells of the input tile
-    constexpr int NTHR = TX * TY;    // one lane per output cell
+    constexpr int NTHR = TX * TY / LPC;  // LPC output cells per lane
+    static_assert(NTHR * LPC == TX * TY, "cells per lane must divide the tile");
     constexpr int D2 = 3 - D1;
     constexpr int SD1 = D1 == 1 ? 1 : HX, SD2 = D2 == 1 ? 1 : HX;  // LDS strides along the sweeps
     // region A cells
@@ -817,16 +818,18 @@ __device__ __forceinline__ void hpair_body(const StepArgs& A, double dt, int x0,
 // FP32: at most 128 VGPRs (four waves per SIMD), two 512-lane blocks per CU at TY = 8.
 // KG: layers per block (1 or HPAIR_KG); HDIFF: horizontal diffusion (A.ah_u set).
 // Occupancy floor of the pair kernels: FP32 four waves per SIMD (128 VGPRs), FP64
-// two; the compensated form's 768-lane blocks (12-row tiles) six, so that two
-// blocks share a CU and cover each other's barrier phases.
+// two; the compensated form's 768-lane blocks (12-row tiles) six and its 512-lane
+// blocks (16-row tiles, two cells per lane) four, so that two blocks share a CU and
+// cover each other's barrier phases.
+// LPC: output cells per lane (1, or 2 for the compensated form's 16-row tiles).
 template <typename R, bool C, bool HDIFF>
 constexpr int hpair_waves_per_eu(int lanes)
 {
-    return sizeof(R) == 4 ? 4 : (C && !HDIFF && lanes == 768 ? 6 : 2);
+    return sizeof(R) == 4 ? 4 : (C && !HDIFF && lanes == 768 ? 6 : (C && !HDIFF && lanes == 512 ? 4 : 2));
 }
 
-template <int S, typename R, int D1, int TY, int KG, bool HDIFF, int TXW, bool C = false>
-__global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(hpair_waves_per_eu<R, C, HDIFF>(TXW * TY)))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
+template <int S, typename R, int D1, int TY, int KG, bool HDIFF, int TXW, bool C = false, int LPC = 1>
+__global__ void __launch_bounds__(TXW * TY / LPC) __attribute__((amdgpu_waves_per_eu(hpair_waves_per_eu<R, C, HDIFF>(TXW * TY / LPC)))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
                                                    int nty, int nbatch,
                                                    const std::conditional_t<C, float, R>* __restrict__ fin_all,
                                                    std::conditional_t<C, float, R>* __restrict__ fout_all,
@@ -864,7 +867,7 @@ __global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(h
     const int tb0 = tbatch * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
     const size_t off = (size_t)tb0 * A.n3;
-    hpair_body<S, R, D1, TY, (KG > 1), HDIFF, TXW, C>(A, dt, tx * TXW, ty * TY, k0, k1, ntr, fin_all + off,
+    hpair_body<S, R, D1, TY, (KG > 1), HDIFF, TXW, C, LPC>(A, dt, tx * TXW, ty * TY, k0, k1, ntr, fin_all + off,
                                                       fout_all + off, hin, hout, sr_stage, tb0, hin_dry,
                                                       write_dry != 0, C ? lin_all + off : nullptr,
                                                       C ? lout_all + off : nullptr);
@@ -874,8 +877,8 @@ __global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(h
 // table is built in the XCD-aware order of k_hpair (domain -1 = padding).
 // STAGE 1: stages 1-2 (tracers -> scratch, ho -> hA); STAGE 2: stages 4-5
 // (tracers -> scratch, hB -> hA).
-template <int S, typename R, int D1, int TY, int STAGE, int KG, int TXW = TX>
-__global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair_multi(const EnsDom<R>* __restrict__ doms,
+template <int S, typename R, int D1, int TY, int STAGE, int KG, int TXW = TX, int LPC = 1>
+__global__ void __launch_bounds__(TXW * TY / LPC) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 2))) k_hpair_multi(const EnsDom<R>* __restrict__ doms,
                                                          const int4* __restrict__ work)
 {
     const int4 wk = work[blockIdx.x];
@@ -885,7 +888,7 @@ __global__ void __launch_bounds__(TXW * TY) __attribute__((amdgpu_waves_per_eu(s
     const int k1 = k0 + KG < D.A.nz ? k0 + KG : D.A.nz;
     const int ntr = D.ntr - tb0 < D.tpb_h ? D.ntr - tb0 : D.tpb_h;
     const size_t off = (size_t)tb0 * D.A.n3;
-    hpair_body<S, R, D1, TY, (KG > 1), false, TXW>(D.A, 0.5 * D.dt, wk.y, wk.z, k0, k1, ntr, D.tracers + off,
+    hpair_body<S, R, D1, TY, (KG > 1), false, TXW, false, LPC>(D.A, 0.5 * D.dt, wk.y, wk.z, k0, k1, ntr, D.tracers + off,
                                               D.scratch + off,
                              STAGE == 1 ? D.ho : D.hB, D.hA, STAGE == 1 ? SR_PAIR1 : SR_PAIR2, tb0,
                              // stages 4-5 end in hA, where stages 1-2 left the cells
@@ -1933,22 +1936,38 @@ inline int launch_hpair_impl(int d1, int scheme, int ty, const StepArgs& A, doub
     if (nitems > 0x7fffff00L) return GETM_ERR_CAPACITY;
     dim3 grid((unsigned)(8 * ((nitems + 7) / 8)));
     ProfileScope ps(d1 == 2 ? GETM_K_HPAIR_VU : GETM_K_HPAIR_UV, s);
-#define GETM_HPK(D1V, TYV, TXV, KGV, AHV)                                                                         \
-    GETM_SCHEME_SWITCH(scheme, {                                                                                    \
-        hipLaunchKernelGGL((k_hpair<S_, R, D1V, TYV, KGV, AHV, TXV, C>), grid, dim3(TXV * TYV), 0, s, A, dt, ntr,    \
-                           tpb, ntx, nty, nb, fin, fout, hin, hout, sr_stage, hin_dry, (int)write_dry, lin, lout);  \
+#define GETM_HPK(D1V, TYV, TXV, KGV, AHV, LPCV)                                                                     \
+    GETM_SCHEME_SWITCH(scheme, {                                                                                      \
+        hipLaunchKernelGGL((k_hpair<S_, R, D1V, TYV, KGV, AHV, TXV, C, LPCV>), grid, dim3(TXV * TYV / LPCV), 0, s, A,  \
+                           dt, ntr, tpb, ntx, nty, nb, fin, fout, hin, hout, sr_stage, hin_dry, (int)write_dry, lin,   \
+                           lout);                                                                                     \
     })
 #define GETM_HP(D1V, TYV, TXV)                            \
     if (ah) {                                             \
-        GETM_HPK(D1V, TYV, TXV, 1, true);                 \
+        GETM_HPK(D1V, TYV, TXV, 1, true, 1);              \
     } else if constexpr (std::is_same<R, float>::value) {  \
         if (kg == HPAIR_KG) {                             \
-            GETM_HPK(D1V, TYV, TXV, HPAIR_KG, false);     \
+            GETM_HPK(D1V, TYV, TXV, HPAIR_KG, false, 1);  \
         } else {                                          \
-            GETM_HPK(D1V, TYV, TXV, 1, false);            \
+            GETM_HPK(D1V, TYV, TXV, 1, false, 1);         \
         }                                                 \
     } else {                                              \
-        GETM_HPK(D1V, TYV, TXV, 1, false);                \
+        GETM_HPK(D1V, TYV, TXV, 1, false, 1);             \
+    }
+    // FP32 and compensated forms, 16-row tiles of one layer: two cells per lane
+    // (512-lane blocks, two per CU, each covering the other's barrier phases).  At
+    // 256 tracers: compensated x-first pair 43.3 -> 40.9 ms, y-first 42.1 -> 38.1
+    // ms; FP32 pairs 38.7 / 39.3 -> 28.4 / 29.1 ms (34.2 / 34.4 ms on 8-row tiles)
+    if constexpr (C || std::is_same<R, float>::value) {
+        if (!ah && ty == 16 && kg == 1) {
+            if (d1 == 2) {
+                GETM_HPK(2, 16, TX, 1, false, 2);
+            } else {
+                GETM_HPK(1, 16, TX, 1, false, 2);
+            }
+            GETM_HIP_OK(hipGetLastError());
+            return GETM_OK;
+        }
     }
     if (d1 == 2) {
         if (ty == 16) {

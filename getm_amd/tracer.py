@@ -575,8 +575,10 @@ class TracerCollection:
     def set_tuning(self, tracers_per_block_h: int = 0, tracers_per_block_w: int = 16,
                    tracers_per_block_d: int = 16, tile_y: int = 0) -> None:
         """Launch shape of the batched kernels: tracers per block of the three kernel
-        kinds (0 = all) and the height of the horizontal-stage tiles (4, 8, 16; 0 = the
-        default, 8 rows).  Does not change results."""
+        kinds (0 = all) and the height of the horizontal-stage tiles (4, 8, 16, and 12
+        for the compensated form; 0 = the default: 16 rows with two cells per lane for
+        the FP32 forms with 16 or more tracers, more than 8 per block, the compensated
+        form's y-first pair on 12 rows; 8 rows otherwise).  Does not change results."""
         if not self.batched:
             return
         _lib.check(_lib.load().transport_set_tuning(self.p, int(tile_y), int(tracers_per_block_h),

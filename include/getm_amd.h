@@ -178,8 +178,10 @@ int transport_step_mixed(void* tr, int ntracers, void* tracers, const double* u,
                          double timestep, const double* molecular, double* h_out, void* stream);
 
 /* No reference counterpart: launch shape of the batched kernels -- the height
- * of the horizontal-stage tiles (4, 8 or 16 rows of 64 columns; 0 = the default,
- * 8 rows) and, for the
+ * of the horizontal-stage tiles (4, 8 or 16 rows of 64 columns, and 12 for the
+ * compensated form; 0 = the default: 16 rows, two cells per lane, for the FP32
+ * forms with 16 or more tracers and more than 8 per block, the compensated
+ * form's y-first pair on 12 rows; 8 rows otherwise) and, for the
  * three kernel kinds (horizontal stage pairs, w stage, FP32 diffusion), the
  * tracers per block (the tracer batches of one step are spread over the grid;
  * 0 = all tracers in every block).  Results do not depend on these values. */
@@ -278,8 +280,8 @@ void ensemble_finalize(void* ens);
 int ensemble_bind(void* ens, int domain, float* tracers, const double* u, const double* v, const double* w,
                   const double* nuh, double timestep, double molecular);
 
-/* Launch shape as transport_set_tuning (tile_y 0: the default, 8 rows); takes
- * effect at the next commit. */
+/* Launch shape as transport_set_tuning (tile_y 0: the default, 16 rows, or 8
+ * with 128-column tiles); takes effect at the next commit. */
 int ensemble_set_tuning(void* ens, int tile_y, int tracers_per_block_h, int tracers_per_block_w,
                         int tracers_per_block_d);
 

@@ -277,35 +277,43 @@ def test_batched_launch_shape_does_not_change_results(precision, ntracers):
         assert_bitwise(o[1], outs[0][1], "thickness")
 
 
-def test_compensated_tile_heights_do_not_change_results():
-    """The compensated form with 16 or more tracers runs its y-first pair on 12-row
-    tiles (two 768-lane blocks per CU) and its x-first pair on 16-row ones; 12-row
-    tiles for both pairs, 16-, 8- and 4-row tiles give the same bits.  Other forms
-    reject 12-row tiles."""
+@pytest.mark.parametrize("precision", ["f32", "f32-sr"])
+def test_many_tracer_tile_heights_do_not_change_results(precision):
+    """With 16 or more tracers per block the FP32 forms run 16-row tiles with two
+    cells per lane (the compensated form's y-first pair: 12-row tiles, 768-lane
+    blocks); 12-row tiles for both pairs (compensated form only), 16-row tiles of
+    layer groups (4 tracers per block), 8- and 4-row tiles give the same bits.  The
+    other forms reject 12-row tiles."""
     from getm_amd._lib import GetmError
     from getm_amd.core import CENTERS, INTERFACES
     from getm_amd.tracer import TracerCollection
 
     d = inputs.tiny(nx=75, ny=53, nz=7, seed=93, ntracers=20, cfl=0.5, wscale=2.0)
+    tunings = [None, (0, 16, 16, 16), (4, 16, 16, 16), (0, 16, 16, 8), (3, 16, 16, 4)]
+    if precision == "f32":
+        tunings.append((0, 16, 16, 12))
     outs = []
-    for tuning in [None, (0, 16, 16, 12), (0, 16, 16, 16), (0, 16, 16, 8), (3, 16, 16, 4)]:
+    for tuning in tunings:
         grid = make_grid(d)
-        tc = TracerCollection(grid, advection_scheme=d.scheme, cnpar=d.cnpar, precision="f32")
+        tc = TracerCollection(grid, advection_scheme=d.scheme, cnpar=d.cnpar, precision=precision)
         if tuning:
             tc.set_tuning(*tuning)
         tc.set_values(dev(d.tracers))
         for _ in range(2):
             tc.advance(d.dt, grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS),
                        grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES))
-        outs.append((host(tc.values), host(tc.residuals), host(tc.h)))
+        res = host(tc.residuals) if tc.residuals is not None else None
+        outs.append((host(tc.values), res, host(tc.h)))
     for o in outs[1:]:
         assert_bitwise(o[0], outs[0][0], "values")
-        assert np.array_equal(o[1], outs[0][1]), "residuals"
+        if o[1] is not None:
+            assert np.array_equal(o[1], outs[0][1]), "residuals"
         assert_bitwise(o[2], outs[0][2], "thickness")
-    for precision in ("f64", "f32-sr"):
-        tc = TracerCollection(make_grid(d), advection_scheme=d.scheme, precision=precision)
-        with pytest.raises(GetmError):
-            tc.set_tuning(0, 16, 16, 12)
+    if precision == "f32-sr":
+        for other in ("f64", "f32-sr"):
+            tc = TracerCollection(make_grid(d), advection_scheme=d.scheme, precision=other)
+            with pytest.raises(GetmError):
+                tc.set_tuning(0, 16, 16, 12)
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32", "f32-sr"])
