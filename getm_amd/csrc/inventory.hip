@@ -9,13 +9,15 @@
 // Fixer (FP32 tracers): FP32 storage rounds every cell once per stage, which
 // makes the inventory drift by ~1e-11 relative per step.  The fixer restores
 // a target inventory M exactly up to one cell's rounding: the defect
-// D = M - I is shared in proportion to each cell's mass, which in units of
-// the cell's ulp is q_i = (D/I) * c_i / ulp(c_i); with S_i the prefix sum of
-// c/ulp in a fixed cell order, cell i changes by
-//   n_i = floor((D/I) S_i + u) - floor((D/I) S_{i-1} + u)  ulps   (n_i in {-1, 0, 1} in practice),
-// which is exact in FP32 and touches ~|D/I| * 2^23 cells per tracer.  The phase
-// u in [0, 1) is a hash of the bits of D/I, so a defect that repeats from step
-// to step does not flip the same cells every time (expected n_i = (D/I) q_i).
+// D = M - I is shared in proportion to each cell's absolute mass, which in
+// units of the cell's ulp is q_i = r |c_i| / ulp(c_i) with r = D / I_abs and
+// I_abs = sum of area * h * |c| (I_abs = I for a tracer of one sign; with mixed
+// signs no cell moves by more than |D| / I_abs of its value, however small I);
+// with S_i the prefix sum of |c|/ulp in a fixed cell order, cell i changes by
+//   n_i = floor(r S_i + u) - floor(r S_{i-1} + u)  ulps   (n_i in {-1, 0, 1} in practice),
+// which is exact in FP32 and touches ~|r| * 2^23 cells per tracer.  The phase
+// u in [0, 1) is a hash of the bits of r, so a defect that repeats from step
+// to step does not flip the same cells every time (expected n_i = q_i).
 // Flipped cells carry different masses, so a residual of a few tens of
 // cell-ulps remains (of the order of the largest cell's ulp mass when a tracer
 // sits in a few cells, e.g. a fresh point release).  The fix kernel
@@ -36,6 +38,7 @@ namespace getm {
 
 constexpr int IBX = 64, IBY = 4, INV_THREADS = IBX * IBY;
 constexpr int INV_TCH = 8;  // tracers per block of k_inventory_tr
+constexpr int INV_SLOTS = 4;  // partial sums per (tracer, block): S, C, sum |c|/ulp, absolute mass
 
 __device__ __forceinline__ void neumaier(double& s, double& c, double x)
 {
@@ -47,15 +50,16 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double x)
     s = t;
 }
 
-// c / ulp(c) for an FP32 value (0 for zero and subnormals).
+// |c| / ulp(c) for an FP32 value (0 for zero and subnormals).
 __device__ __forceinline__ double q_of(float c)
 {
     const int e = ilogbf(c);
     if (c == 0.f || e < -126) return 0.0;
-    return (double)c * exp2((double)(23 - e));
+    return (double)fabsf(c) * exp2((double)(23 - e));
 }
 
-// Partial sums per (tracer, block): mass (hi, lo) and the sum of c/ulp(c) (FP32 only).
+// Partial sums per (tracer, block), INV_SLOTS of them: mass (hi, lo), the sum
+// of |c|/ulp(c) (FP32 only, 0 otherwise) and the absolute mass (area h |c|).
 // COMP: compensated FP32 tracers (f the FP32 values, lo their int16 residuals).
 template <typename R, bool COMP = false>
 __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, const double* __restrict__ h,
@@ -63,13 +67,13 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, con
                                                            const R* __restrict__ f, double* __restrict__ part,
                                                            const short* __restrict__ lo = nullptr)
 {
-    __shared__ double s_m[INV_THREADS], s_c[INV_THREADS], s_q[INV_THREADS];
+    __shared__ double s_m[INV_THREADS], s_c[INV_THREADS], s_q[INV_THREADS], s_a[INV_THREADS];
     const int tid = threadIdx.y * IBX + threadIdx.x;
     const int i = blockIdx.x * IBX + threadIdx.x, j = blockIdx.y * IBY + threadIdx.y;
     const int t = blockIdx.z;
     const int nblk = gridDim.x * gridDim.y;
     const int blk = blockIdx.y * gridDim.x + blockIdx.x;
-    double m = 0.0, mc = 0.0, q = 0.0;
+    double m = 0.0, mc = 0.0, q = 0.0, ab = 0.0;
     if (i < A.nx_ && j < A.ny_) {
         const size_t p = (size_t)j * A.nx_ + i;
         const bool wet = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[p] == 1;
@@ -78,11 +82,14 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, con
             const double a = area[p];
             for (int k = 0; k < A.nz; k++) {
                 const R c = fc[(size_t)k * A.n2];
+                const double ah = a * h[(size_t)k * A.n2 + p];
                 if constexpr (COMP) {
                     const double x = cdecode(c, lo[(size_t)t * A.n3 + p + (size_t)k * A.n2]);
-                    neumaier(m, mc, (a * h[(size_t)k * A.n2 + p]) * x);
+                    neumaier(m, mc, ah * x);
+                    ab = fma(ah, fabs(x), ab);
                 } else {
-                    neumaier(m, mc, (a * h[(size_t)k * A.n2 + p]) * (double)c);
+                    neumaier(m, mc, ah * (double)c);
+                    ab = fma(ah, fabs((double)c), ab);
                     if constexpr (std::is_same<R, float>::value) q += q_of(c);
                 }
             }
@@ -91,18 +98,21 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, con
     s_m[tid] = m;
     s_c[tid] = mc;
     s_q[tid] = q;
+    s_a[tid] = ab;
     __syncthreads();
     if (tid == 0) {
-        double S = 0.0, C = 0.0, Q = 0.0;
+        double S = 0.0, C = 0.0, Q = 0.0, AB = 0.0;
         for (int l = 0; l < INV_THREADS; l++) {
             neumaier(S, C, s_m[l]);
             C += s_c[l];
             Q += s_q[l];
+            AB += s_a[l];
         }
-        double* pt = part + (size_t)t * 3 * nblk;
+        double* pt = part + (size_t)t * INV_SLOTS * nblk;
         pt[blk] = S;
         pt[nblk + blk] = C;
         pt[2 * nblk + blk] = Q;
+        pt[3 * nblk + blk] = AB;
     }
 }
 
@@ -122,7 +132,8 @@ __device__ __forceinline__ void neumaier_merge(double& s, double& c, double s2, 
 // tracer of the chunk, with the column's area * h loaded once for all of them;
 // the block combines its lanes by a butterfly over each wave and the four
 // waves in order (a fixed order: deterministic), and writes the same
-// (S, C, 0) triple per block as k_inventory.  1-D XCD-aware grid over
+// partial sums per block as k_inventory (the c/ulp sum 0; the absolute mass
+// for the compensated form only).  1-D XCD-aware grid over
 // (column block x, column block y, tracer chunk), so that neighbouring column
 // blocks share their L2.
 template <typename R, bool COMP, int TCH>
@@ -132,7 +143,7 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory_tr(const StepArgs A, 
                                                               const short* __restrict__ lo,
                                                               double* __restrict__ part)
 {
-    __shared__ double s_s[IBY][TCH], s_c[IBY][TCH];
+    __shared__ double s_s[IBY][TCH], s_c[IBY][TCH], s_a[IBY][TCH];
     const unsigned nbx = (A.nx_ + IBX - 1) / IBX, nby = (A.ny_ + IBY - 1) / IBY;
     unsigned bx, by, bz;
     if (!xcd_block(nbx, nby, (ntr + TCH - 1) / TCH, bx, by, bz)) return;
@@ -144,9 +155,11 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory_tr(const StepArgs A, 
     // per lane: the column's sum for each tracer of the chunk (40-odd terms: plain
     // FP64 sums, exact to ~1e-15 of the column's sum of magnitudes); compensation
     // starts with the merge of the lanes' sums
-    double m[TCH], mc[TCH];
+    double m[TCH], mc[TCH], ma[COMP ? TCH : 1];
 #pragma unroll
     for (int tt = 0; tt < TCH; tt++) m[tt] = mc[tt] = 0.0;
+#pragma unroll
+    for (int tt = 0; tt < (COMP ? TCH : 1); tt++) ma[tt] = 0.0;
     if (i < A.nx_ && j < A.ny_) {
         const unsigned p = (unsigned)j * A.nx_ + i;
         const bool wet = i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[p] == 1;
@@ -177,6 +190,7 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory_tr(const StepArgs A, 
                     else
                         x = (double)ld_f32(rf[tt], vs, L * 4u);
                     m[tt] = fma(ah, x, m[tt]);
+                    if constexpr (COMP) ma[tt] = fma(ah, fabs(x), ma[tt]);
                 }
             }
         }
@@ -184,20 +198,30 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory_tr(const StepArgs A, 
 #pragma unroll
     for (int tt = 0; tt < TCH; tt++) {
         for (int d = 1; d < 64; d <<= 1) neumaier_merge(m[tt], mc[tt], __shfl_xor(m[tt], d), __shfl_xor(mc[tt], d));
+        double a = 0.0;
+        if constexpr (COMP) {
+            a = ma[tt];
+            for (int d = 1; d < 64; d <<= 1) a += __shfl_xor(a, d);
+        }
         if (threadIdx.x == 0) {
             s_s[threadIdx.y][tt] = m[tt];
             s_c[threadIdx.y][tt] = mc[tt];
+            s_a[threadIdx.y][tt] = a;
         }
     }
     __syncthreads();
     if (threadIdx.y == 0 && threadIdx.x < nt) {
         const int tt = threadIdx.x;
-        double S = s_s[0][tt], C = s_c[0][tt];
-        for (int w = 1; w < IBY; w++) neumaier_merge(S, C, s_s[w][tt], s_c[w][tt]);
-        double* pt = part + (size_t)(t0 + tt) * 3 * nblk;
+        double S = s_s[0][tt], C = s_c[0][tt], AB = s_a[0][tt];
+        for (int w = 1; w < IBY; w++) {
+            neumaier_merge(S, C, s_s[w][tt], s_c[w][tt]);
+            AB += s_a[w][tt];
+        }
+        double* pt = part + (size_t)(t0 + tt) * INV_SLOTS * nblk;
         pt[blk] = S;
         pt[nblk + blk] = C;
         pt[2 * nblk + blk] = 0.0;
+        pt[3 * nblk + blk] = AB;
     }
 }
 
@@ -225,7 +249,8 @@ __device__ __forceinline__ double wave_exclusive_prefix(double x, int lane)
 }
 
 // One wave per tracer: inventory = compensated sum over blocks; for the fixer
-// also the exclusive prefix of the per-block c/ulp sums and the ratio D / I.
+// also the exclusive prefix of the per-block |c|/ulp sums and the ratio
+// D / I_abs of the defect to the absolute mass.
 // Lane l owns the contiguous block range [l seg, (l + 1) seg), so the prefix is
 // a lane-local running sum started at the lane's exclusive offset.
 __global__ void __launch_bounds__(64) k_inventory_reduce(int ntr, int nblk, double* __restrict__ part,
@@ -234,15 +259,18 @@ __global__ void __launch_bounds__(64) k_inventory_reduce(int ntr, int nblk, doub
 {
     const int t = blockIdx.x, lane = threadIdx.x;
     if (t >= ntr) return;
-    double* pt = part + (size_t)t * 3 * nblk;
+    double* pt = part + (size_t)t * INV_SLOTS * nblk;
     const int seg = (nblk + 63) / 64;
     const int b0 = lane * seg, b1 = b0 + seg < nblk ? b0 + seg : nblk;
-    double S = 0.0, C = 0.0, Q = 0.0;
+    double S = 0.0, C = 0.0, Q = 0.0, AB = 0.0;
     for (int b = b0; b < b1; b++) {
         neumaier(S, C, pt[b]);
         C += pt[nblk + b];
         Q += pt[2 * nblk + b];
+        AB += pt[3 * nblk + b];
     }
+    double abt = 0.0;  // absolute mass, lanes in order
+    for (int l = 0; l < 64; l++) abt += __shfl(AB, l);
     double run = wave_exclusive_prefix(Q, lane);
     for (int b = b0; b < b1; b++) {
         const double qb = pt[2 * nblk + b];
@@ -252,7 +280,10 @@ __global__ void __launch_bounds__(64) k_inventory_reduce(int ntr, int nblk, doub
     const double I = wave_neumaier_total(S, C);
     if (lane == 0) {
         if (inv) inv[t] = I;
-        if (target && ratio) ratio[t] = (I != 0.0) ? (target[t] - I) / I : 0.0;
+        // the fixers move each cell in proportion to its absolute mass: ratio
+        // D / I_abs (the FP64 form has no fixer and no absolute mass: D / I)
+        const double den = abt > 0.0 ? abt : I;
+        if (target && ratio) ratio[t] = (den != 0.0) ? (target[t] - I) / den : 0.0;
     }
 }
 
@@ -287,7 +318,7 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_f32(const StepArgs A, int n
     s_q[tid] = q;
     __syncthreads();
     if (tid == 0) {
-        double Q = part[(size_t)t * 3 * nblk + 2 * nblk + blk];
+        double Q = part[(size_t)t * INV_SLOTS * nblk + 2 * nblk + blk];
         for (int l = 0; l < INV_THREADS; l++) {
             const double ql = s_q[l];
             s_q[l] = Q;  // exclusive prefix of this lane
@@ -502,9 +533,10 @@ __global__ void __launch_bounds__(64) k_greedy_f32(const StepArgs A, int ntr, fl
     }
 }
 
-// Compensated FP32 fixer: every wet cell of tracer t becomes x (1 + ratio[t])
-// in FP64, encoded again; the inventory then equals the target up to the
-// storage rounding of each cell (2^-40 of its value).
+// Compensated FP32 fixer: every wet cell of tracer t becomes x + ratio[t] |x|
+// in FP64 (x (1 + ratio) for a tracer of one sign; ratio = D / I_abs), encoded
+// again; the inventory then equals the target up to the storage rounding of
+// each cell (2^-40 of its value).
 __global__ void __launch_bounds__(INV_THREADS) k_fix_c(const StepArgs A, float* __restrict__ f,
                                                        short* __restrict__ lo, const double* __restrict__ ratio)
 {
@@ -513,9 +545,9 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_c(const StepArgs A, float* 
     if (i >= A.nx_ || j >= A.ny_) return;
     const size_t p = (size_t)j * A.nx_ + i;
     if (!(i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax && A.tmask[p] == 1)) return;
-    // x (1 + r) - x is below half the residual unit 2^-39 |x| (every cell keeps
-    // its stored value) when |r| < 2^-40: the inventory is already at the target
-    // to the storage precision
+    // r |x| is below half the residual unit 2^-39 |x| (every cell keeps its
+    // stored value) when |r| < 2^-40: the inventory is already at the target to
+    // the storage precision
     const double r = ratio[t];
     if (!(fabs(r) >= 0x1p-40)) return;
     float* fc = f + (size_t)t * A.n3 + p;
@@ -525,7 +557,7 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_c(const StepArgs A, float* 
         const double x = cdecode(fc[o], lc[o]);
         float hi;
         short l;
-        cencode(fma(x, r, x), hi, l);
+        cencode(fma(fabs(x), r, x), hi, l);
         fc[o] = hi;
         lc[o] = l;
     }
@@ -594,7 +626,7 @@ int launch_fix_f32(const StepArgs& A, int ntr, float* f, const double* part, con
 size_t inventory_part_doubles(const StepArgs& A, int ntr)
 {
     const size_t nblk = (size_t)((A.nx_ + IBX - 1) / IBX) * ((A.ny_ + IBY - 1) / IBY);
-    return nblk * 3 * (size_t)ntr;
+    return nblk * INV_SLOTS * (size_t)ntr;
 }
 
 size_t fix_part_doubles(const StepArgs& A, int ntr)

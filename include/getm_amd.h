@@ -243,7 +243,7 @@ int transport_conserve(void* tr, int ntracers, void* tracers, const double* targ
  * Needs B = ceil(nx_ / 64) * ceil(ny_ / 4) blocks of partial sums per tracer:
  * GETM_F32C uses 3 B + 1 doubles per tracer, GETM_F32 7 B + 3; more than
  * 4 Mi doubles in total returns GETM_ERR_CAPACITY (e.g. 1028 x 772 cells with
- * halos: up to 426 GETM_F32C tracers, 182 GETM_F32 tracers).  Calls that
+ * halos: up to 319 GETM_F32C tracers, 159 GETM_F32 tracers).  Calls that
  * overlap on one GPU (other streams or threads) share the array and race;
  * calls ordered on one stream do not. */
 int transport_conserve_noalloc(void* tr, int ntracers, void* tracers, const double* target, void* stream);

@@ -705,3 +705,55 @@ def test_f32_step_without_thickness_output_equals_with_it(tpb, precision):
             _lib.check(rc, "transport_step")
         outs.append(host(tc.absolute()))
     assert_bitwise(outs[1], outs[0], "tracers without h_out")
+
+
+@pytest.mark.parametrize("precision", ["f32", "f32-sr"])
+def test_fixer_of_a_mean_zero_tracer_moves_cells_by_the_absolute_mass_ratio(precision):
+    """transport_conserve on a tracer of mixed signs whose inventory is ~0 next to
+    a positive one: the defect D is shared in proportion to each cell's absolute
+    mass (ratio D / I_abs, I_abs = sum area h |c|), so both reach their targets
+    and no cell of the mean-zero tracer moves by more than |D| / I_abs of its
+    value plus a storage unit.  Weights of signed mass (ratio D / I with I ~ 0)
+    would scale its cells by orders of magnitude."""
+    from getm_amd import _lib
+    from getm_amd.core import Grid
+    from getm_amd.tracer import TracerCollection
+
+    d = inputs.coastal(nx=120, ny=90, nz=12, seed=5, ntracers=2, tracer_kind="TS")
+    grid = Grid.from_domain(d)
+    shape = (grid.nz_, grid.ny_, grid.nx_)
+    gen = torch.Generator(device=grid.device).manual_seed(11)
+    pos = 1.0 + torch.rand(shape, generator=gen, dtype=torch.float64, device=grid.device)
+    anom = 2.0 * torch.rand(shape, generator=gen, dtype=torch.float64, device=grid.device) - 1.0
+
+    def collection(vals):
+        tc = TracerCollection(grid, advection_scheme=3, precision=precision)
+        tc.set_values(torch.stack(vals))
+        return tc
+
+    inv = collection([pos, anom, torch.ones_like(pos)]).inventory_device()
+    anom = anom - float(inv[1] / inv[2])  # remove the volume mean: inventory ~0
+    tc = collection([pos, anom])
+    before = tc.absolute().clone()
+    absmass = collection([before[0].abs(), before[1].abs()]).inventory_device()
+    inv = tc.inventory_device()
+    assert float(inv[1].abs() / absmass[1]) < 1e-6, "the anomaly's inventory should be ~0"
+    dfix = 1e-9 * absmass
+    target = inv + dfix
+    rc = _lib.load().transport_conserve(tc.p, len(tc), _lib.ptr(tc._buf), _lib.ptr(target), _lib.stream_handle())
+    _lib.check(rc, "transport_conserve")
+    after_inv = tc.inventory_device()
+    err = ((after_inv - target).abs() / absmass).cpu().numpy()
+    assert (err <= 1e-12).all(), err
+    after = tc.absolute()
+    wet = before[1] != after[1]
+    assert int(wet.sum()) > 0, "the fixer changed nothing"
+    rel = ((after[1] - before[1]).abs() / before[1].abs().clamp_min(1e-30))
+    if precision == "f32":
+        # x + r |x|, r = D / I_abs = 1e-9, encoded again (residual unit 2^-39 |x|)
+        assert rel.max().item() <= 1e-9 + 2.0 ** -38, rel.max().item()
+    else:
+        # dithering moves cells by at most one ulp (2^-23 relative), the greedy
+        # pass that takes the remainder a few cells by at most 4096 ulps
+        assert torch.quantile(rel.flatten()[::7].float(), 0.999).item() <= 2.0 ** -22
+        assert rel.max().item() <= 4096 * 2.0 ** -23, rel.max().item()

@@ -5,8 +5,8 @@ fixed array inside the library, against transport_conserve through the C-ABI:
   on a smaller grid (the array keeps the first call's partial sums), for both
   FP32 forms;
 - the size limit: 4 Mi doubles of working memory, i.e. with B blocks of 64 x 4
-  columns per layer, floor(4 Mi / (3 B + 1)) compensated tracers or
-  floor(4 Mi / (7 B + 3)) stochastic-rounding tracers; one tracer more is
+  columns per layer, floor(4 Mi / (4 B + 1)) compensated tracers or
+  floor(4 Mi / (8 B + 3)) stochastic-rounding tracers; one tracer more is
   rejected with GETM_ERR_CAPACITY before anything runs.
 """
 import numpy as np
@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 GETM_ERR_CAPACITY = 5  # include/getm_amd.h
 LIMIT = 4 << 20
-PER_TRACER = {"f32": (3, 1), "f32-sr": (7, 3)}  # doubles per tracer: a B + b
+PER_TRACER = {"f32": (4, 1), "f32-sr": (8, 3)}  # doubles per tracer: a B + b
 
 
 def _collection(nx, ny, nz, seed, ntr, precision):
