@@ -184,7 +184,8 @@ int transport_step_mixed(void* tr, int ntracers, void* tracers, const double* u,
  * form's y-first pair on 12 rows; 8 rows otherwise) and, for the
  * three kernel kinds (horizontal stage pairs, w stage, FP32 diffusion), the
  * tracers per block (the tracer batches of one step are spread over the grid;
- * 0 = all tracers in every block).  Results do not depend on these values. */
+ * 0 = all tracers in every block; defaults 0, 16 (64 for GETM_F32C) and 16).
+ * Results do not depend on these values. */
 int transport_set_tuning(void* tr, int tile_y, int tracers_per_block_h, int tracers_per_block_w,
                          int tracers_per_block_d);
 
