@@ -316,12 +316,15 @@ def test_many_tracer_tile_heights_do_not_change_results(precision):
                 tc.set_tuning(0, 16, 16, 12)
 
 
+@pytest.mark.parametrize("nx", [70, 96])
 @pytest.mark.parametrize("precision", ["f64", "f32", "f32-sr"])
-def test_inventory_matches_exact_sum(precision):
-    """Device inventories equal the exactly rounded host sum (math.fsum) to 1e-15."""
+def test_inventory_matches_exact_sum(precision, nx):
+    """Device inventories equal the exactly rounded host sum (math.fsum) to 1e-15,
+    on rows of 74 and of 100 cells (a row length that is and one that is not a
+    multiple of 4)."""
     from getm_amd.tracer import TracerCollection
 
-    d = inputs.coastal(nx=70, ny=52, nz=9, ntracers=3, tracer_kind="blobs", seed=4)
+    d = inputs.coastal(nx=nx, ny=52, nz=9, ntracers=3, tracer_kind="blobs", seed=4)
     grid = make_grid(d)
     tc = TracerCollection(grid, advection_scheme=d.scheme, precision=precision)
     tc.set_values(dev(d.tracers))
