@@ -130,8 +130,8 @@ struct Transport {
     DeviceBuffer hA, hB;   // FP64 thickness between stages
     DeviceBuffer stats;    // int32[2] as in Advection
     DeviceBuffer auxo, auxn, ru;  // FP64 diffusion (precision GETM_F64)
-    int tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tracers per block of the three kernel kinds (0 = all)
-    int tile_y = 0;                         // height of the horizontal-stage tiles (4, 8, 16; 0 = by tracer count)
+    int tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tracers per block of the three kernel kinds (0 = all; w: 64 for GETM_F32C)
+    int tile_y = 0;                         // height of the horizontal-stage tiles (4, 8, 12, 16; 0 = by tracer count)
     int pcr_max_nz = 0;                     // FP32 column solve by wave PCR for 2 <= nz <= this (<= 32; 0: Thomas only)
     DeviceBuffer part, part2, ratio, resid;  // inventory partial sums, fixer ratios and residuals
     int rounding = 1;                        // FP32 stores: 1 stochastic (default), 0 nearest
@@ -145,7 +145,7 @@ struct Transport {
 struct Ensemble {
     int ndom = 0, scheme = 5, ufirst = 0;
     double cnpar = 1.0;
-    int tile_y = 0, tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tile_y 0 = by tracers per domain
+    int tile_y = 0, tpb_h = 0, tpb_w = 16, tpb_d = 16;  // tile_y 0 = 16 rows (8 at 128 columns)
     int tile_y_used = 16;                               // the height of the committed work table
     int tile_x = 64;                                    // width of the horizontal-stage tiles (32, 64, 128)
     int tile_x_used = 64;                               // the width of the committed work table
@@ -165,10 +165,12 @@ struct Ensemble {
     unsigned long long** rstep_ext = nullptr;  // per-domain caller counters (NULL entries: own)
 };
 
-// Default tile height of the horizontal stage pair: 8 rows.  Three 512-lane
-// blocks per CU overlap each other's barrier phases; 16-row tiles have a smaller
-// halo share but fit one or two 1024-lane blocks per CU, and were slower at
-// 2 and at 256 tracers on the coastal grid.
+// Tile height of the horizontal stage pair for few tracers per block (up to 8
+// FP32 tracers, which take layer groups) and for the FP64 form: 8 rows.  Three
+// 512-lane blocks per CU overlap each other's barrier phases; 16-row tiles of
+// one cell per lane have a smaller halo share but fit one or two 1024-lane blocks
+// per CU, and were slower at 2 tracers on the coastal grid.  (With more FP32
+// tracers per block the step takes 16-row tiles of two cells per lane.)
 inline int auto_tile_y(int tracers_per_domain)
 {
     (void)tracers_per_domain;
