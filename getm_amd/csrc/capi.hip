@@ -685,7 +685,7 @@ int transport_set_sources(void* p, const double* ea4)
 {
     Transport* tr = static_cast<Transport*>(p);
     if (!tr) return GETM_ERR_ARG;
-    if (ea4 && (tr->precision == GETM_F64 || tr->cnpar != 1.0)) return GETM_ERR_ARG;
+    if (ea4 && tr->precision == GETM_F64) return GETM_ERR_ARG;
     tr->ea4 = ea4;
     return GETM_OK;
 }

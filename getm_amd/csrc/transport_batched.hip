@@ -68,7 +68,6 @@ int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecula
                      const float* fin, float* fout, hipStream_t s, const double* ea4)
 {
     if (A.nz > 64 || A.nz < 1) return GETM_ERR_CAPACITY;
-    if (ea4 && cnpar != 1.0) return GETM_ERR_ARG;  // sources: the cnpar = 1 form only
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
     if (nb > 65535) return GETM_ERR_CAPACITY;
@@ -78,9 +77,12 @@ int launch_vdiff_f32(const StepArgs& A, double dt, double cnpar, double molecula
     ProfileScope ps(GETM_K_VDIFF_F32, s);
 #define GETM_VD(NZM)                                                                                             \
     do {                                                                                                         \
-        if (ea4)                                                                                                 \
+        if (ea4 && cn1)                                                                                          \
             hipLaunchKernelGGL((k_vdiff_f32<NZM, true, true>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, \
                                ntr, tpb, fin, fout, ea4);                                                        \
+        else if (ea4)                                                                                            \
+            hipLaunchKernelGGL((k_vdiff_f32<NZM, false, true>), grid, dim3(64), shm, s, A, dt, cnpar, molecular,    \
+                               nuh, ntr, tpb, fin, fout, ea4);                                                   \
         else if (cn1)                                                                                            \
             hipLaunchKernelGGL((k_vdiff_f32<NZM, true, false>), grid, dim3(64), shm, s, A, dt, cnpar, molecular, nuh, \
                                ntr, tpb, fin, fout, nullptr);                                                    \

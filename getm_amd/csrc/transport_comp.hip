@@ -60,7 +60,6 @@ int launch_vdiff_c(const StepArgs& A, double dt, double cnpar, double molecular,
                    const float* fin, const short* lin, float* fout, short* lout, hipStream_t s, const double* ea4)
 {
     if (A.nz > 64 || A.nz < 1) return GETM_ERR_CAPACITY;
-    if (ea4 && cnpar != 1.0) return GETM_ERR_ARG;  // sources: the cnpar = 1 form only
     if (tpb <= 0 || tpb > ntr) tpb = ntr;
     const int nb = nbatch(ntr, tpb);
     if (nb > 65535) return GETM_ERR_CAPACITY;
@@ -80,8 +79,11 @@ int launch_vdiff_c(const StepArgs& A, double dt, double cnpar, double molecular,
     ProfileScope ps(GETM_K_VDIFF_F32, s);
 #define GETM_VDC(NZM)                                                                                            \
     do {                                                                                                         \
-        if (ea4)                                                                                                 \
+        if (ea4 && cn1)                                                                                          \
             hipLaunchKernelGGL((k_vdiff_c<NZM, true, true, NW>), grid, block, shm, s, A, dt, cnpar, molecular,     \
+                               nuh, ntr, tpb, fin, lin, fout, lout, ea4);                                        \
+        else if (ea4)                                                                                            \
+            hipLaunchKernelGGL((k_vdiff_c<NZM, false, true, NW>), grid, block, shm, s, A, dt, cnpar, molecular,    \
                                nuh, ntr, tpb, fin, lin, fout, lout, ea4);                                        \
         else if (cn1)                                                                                            \
             hipLaunchKernelGGL((k_vdiff_c<NZM, true, false, NW>), grid, block, shm, s, A, dt, cnpar, molecular,    \

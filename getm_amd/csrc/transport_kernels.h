@@ -1477,7 +1477,15 @@ __device__ __forceinline__ void vdiff_body(const StepArgs& A, double dt, double 
                         const float dup = has_up(k) ? cv(b, k + 1 < NZMAX ? k + 1 : k) - ck : 0.f;
                         const float ddn = k > 0 ? cv(b, k > 0 ? k - 1 : 0) - ck : 0.f;
                         float qk = __builtin_fmaf(ek, dup + q_up[b], -(rk * ddn));
-                        if (!CN1) qk += e3 * dup + e4 * ddn;
+                        if constexpr (!CN1 && SRC) {
+                            // the explicit part acts on the old values, without ea4/dz
+                            // (diffusion.F90:238-262: a4 = c dzo + ea4 + auxo differences of c)
+                            const float cu = has_up(k) ? c[b][k + 1 < NZMAX ? k + 1 : k] - c[b][k] : 0.f;
+                            const float cd = k > 0 ? c[b][k > 0 ? k - 1 : 0] - c[b][k] : 0.f;
+                            qk += e3 * cu + e4 * cd;
+                        } else if (!CN1) {
+                            qk += e3 * dup + e4 * ddn;
+                        }
                         q[b][k] = qk;
                         q_up[b] = qk;
                     }
@@ -1700,11 +1708,19 @@ k_vdiff_c(const StepArgs A, double dt, double cnpar, double molecular, const dou
             if constexpr (SRC) return c + src(lev(k));
             else return c;
         };
+        // old value without the sources, for the explicit part of cnpar < 1
+        auto raw = [&](int k) -> double {
+            return cdecode(hi[k], (short)(k % 2 ? lp[k / 2] >> 16 : lp[k / 2] & 0xffffu));
+        };
+        constexpr bool RAW = !CN1 && SRC;
         float q[NZMAX];
         double shift = 0.0;
         {
-            // elimination from the top down; window (c_{k+1}, c_k, c_{k-1})
+            // elimination from the top down; window (c_{k+1}, c_k, c_{k-1}), and the
+            // same window of the old values without ea4/h when the explicit part
+            // needs them (diffusion.F90:238-262: a4 = c dzo + ea4 + auxo differences of c)
             double c_up = 0.0, c_k = value(NZMAX - 1);
+            double r_up = 0.0, r_k = RAW ? raw(NZMAX - 1) : 0.0;
             float q_up = 0.f;
 #pragma unroll
             for (int k = NZMAX - 1; k >= 0; k--) {
@@ -1714,7 +1730,16 @@ k_vdiff_c(const StepArgs A, double dt, double cnpar, double molecular, const dou
                 const float dup = k < NZMAX - 1 ? float(c_up - c_k) : 0.f;
                 const float ddn = k > 0 ? float(c_dn - c_k) : 0.f;
                 float qk = __builtin_fmaf(ek, dup + q_up, -(rk * ddn));
-                if (!CN1) qk += g3[k * 64 + ln] * dup + g4[k * 64 + ln] * ddn;
+                if constexpr (RAW) {
+                    const double r_dn = k > 0 ? raw(k > 0 ? k - 1 : 0) : 0.0;
+                    const float rup = k < NZMAX - 1 ? float(r_up - r_k) : 0.f;
+                    const float rdn = k > 0 ? float(r_dn - r_k) : 0.f;
+                    qk += g3[k * 64 + ln] * rup + g4[k * 64 + ln] * rdn;
+                    r_up = r_k;
+                    r_k = r_dn;
+                } else if (!CN1) {
+                    qk += g3[k * 64 + ln] * dup + g4[k * 64 + ln] * ddn;
+                }
                 q[k] = qk;
                 shift = fma((double)qk, wl[k * 64 + ln], shift);
                 q_up = qk;

@@ -399,11 +399,9 @@ class TracerCollection:
         tracer with its own vertical velocity and molecular diffusivity.  The
         FP64 batched step runs the per-tracer sequence when a tracer has sources
         or a surface flux (bit-identical to the batched step for the others); the
-        FP32 forms take them in their column solve (cnpar = 1)."""
+        FP32 forms take them in their column solve."""
         src_extra = any(t.source is not None or t.surface_flux is not None for t in self._tracers)
         if self.precision != "f64":
-            if src_extra and self.cnpar != 1.0:
-                raise ValueError("tracer sources in the FP32 forms need cnpar = 1")
             if src_extra and self.conserve:
                 raise ValueError("the inventory fixer restores the inventory the tracers had; with sources call "
                                  "transport_conserve with targets that include them")

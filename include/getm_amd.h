@@ -214,8 +214,9 @@ int transport_set_rounding(void* tr, int mode, unsigned long long* step_counter)
  * 263-272, diffusion.F90:269-270): a device array of ntracers x nz x ny_ x nx_
  * FP64 values, (source + surface flux in the top layer) * dt * source_scale per
  * tracer, read by every transport_step until replaced; NULL clears it.  FP32
- * transports with cnpar = 1 only (GETM_ERR_ARG otherwise); the column solve is
- * then the per-lane elimination whatever transport_set_solver chose. */
+ * transports only (GETM_ERR_ARG for GETM_F64, which runs tracers with sources
+ * through the per-tracer operators); the column solve is then the per-lane
+ * elimination whatever transport_set_solver chose. */
 int transport_set_sources(void* tr, const double* ea4);
 
 /* Tracer inventories (the reference's domain integrals of tracers,

@@ -192,25 +192,28 @@ def test_restart_keeps_sources_and_vertical_velocities(tmp_path, precision):
     assert_bitwise(outs[1], outs[0], "restored run")
 
 
+@pytest.mark.parametrize("cnpar", [1.0, 0.5])
 @pytest.mark.parametrize("nz", [1, 12, 30])
-def test_f32_sources_and_fluxes_follow_fp64(oracle_mod, nz):
-    """The FP32 form's sources and surface fluxes (solved as the diffusion of
-    c + ea4/dz, cnpar = 1) against the FP64 per-tracer sequence on the smooth
-    coastal fields (T- and S-like values): within 1e-5 of the tracer's range
-    over 5 steps, for a tracer with sources and a surface flux, one with a flux
-    only and one with neither."""
+def test_f32_sources_and_fluxes_follow_fp64(oracle_mod, nz, cnpar):
+    """The FP32 forms' sources and surface fluxes (the implicit part of the
+    column solve acting on c + ea4/dz, the explicit part of cnpar < 1 on c) against
+    the FP64 per-tracer sequence on the smooth coastal fields (T- and S-like
+    values): within 1e-5 of the tracer's range over 5 steps, for a tracer with
+    sources and a surface flux, one with a flux only and one with neither."""
     from getm_amd.core import CENTERS, INTERFACES
     from getm_amd.tracer import TracerCollection
 
     d = inputs.coastal(nx=96, ny=80, nz=nz, seed=13, ntracers=3, tracer_kind="TS")
     rng = np.random.default_rng(nz)
-    src = rng.standard_normal(d.ho.shape) * 1e-5
+    # with cnpar < 1 sources large enough that acting on c + ea4/dz in the explicit
+    # part instead of on c would move the result by more than the tolerance
+    src = rng.standard_normal(d.ho.shape) * (1e-5 if cnpar == 1.0 else 1e-3)
     flux = rng.standard_normal(d.ho.shape[1:]) * 1e-4
     wet = np.broadcast_to(d.tmask == 1, d.ho.shape)
     out = {}
     for precision in ("f64", "f32", "f32-sr"):
         grid = make_grid(d)
-        tc = TracerCollection(grid, advection_scheme=5, cnpar=1.0, precision=precision)
+        tc = TracerCollection(grid, advection_scheme=5, cnpar=cnpar, precision=precision)
         tc.add("heat", data=dev(d.tracers[0]), molecular_diffusivity=1e-6, source=grid.wrap(dev(src), CENTERS),
                surface_flux=grid.wrap(dev(flux)), source_scale=0.5)
         tc.add("flux", data=dev(d.tracers[1]), molecular_diffusivity=1e-6, surface_flux=grid.wrap(dev(flux)))
@@ -224,19 +227,19 @@ def test_f32_sources_and_fluxes_follow_fp64(oracle_mod, nz):
         r = out["f64"][t][wet]
         for precision, tol in (("f32-sr", 1e-5), ("f32", 1e-6)):
             err = np.abs(out[precision][t][wet] - r).max() / np.abs(r).max()
-            print(f"{precision} nz {nz} tracer {t}: {err:.2e}")
+            print(f"{precision} nz {nz} cnpar {cnpar} tracer {t}: {err:.2e}")
             assert err <= tol, (precision, t, err)
     # the sources did change the first two tracers
-    plain = single_f64_without_sources(d)
+    plain = single_f64_without_sources(d, cnpar)
     assert np.abs(out["f64"][0][wet] - plain[0][wet]).max() > 1e-6 * np.abs(plain[0][wet]).max()
 
 
-def single_f64_without_sources(d):
+def single_f64_without_sources(d, cnpar=1.0):
     from getm_amd.core import CENTERS, INTERFACES
     from getm_amd.tracer import TracerCollection
 
     grid = make_grid(d)
-    tc = TracerCollection(grid, advection_scheme=5, cnpar=1.0, precision="f64")
+    tc = TracerCollection(grid, advection_scheme=5, cnpar=cnpar, precision="f64")
     tc.set_values(dev(d.tracers), molecular_diffusivity=1e-6)
     args = (grid.ugrid.wrap(dev(d.u), CENTERS), grid.vgrid.wrap(dev(d.v), CENTERS),
             grid.wrap(dev(d.w), INTERFACES), grid.wrap(dev(d.nuh), INTERFACES))
@@ -245,13 +248,15 @@ def single_f64_without_sources(d):
     return host(tc.values)
 
 
-def test_f32_sources_need_cnpar_one_and_no_fixer():
+def test_f32_sources_with_the_fixer_are_rejected():
+    """The fixer restores the inventory the tracers had before the step, which
+    sources change: a collection with both raises."""
     from getm_amd.core import CENTERS
     from getm_amd.tracer import TracerCollection
 
     d = inputs.tiny(nx=12, ny=10, nz=4, seed=3, ntracers=1)
     grid = make_grid(d)
-    for kw in (dict(cnpar=0.5), dict(cnpar=1.0, conserve=True)):
+    for kw in (dict(cnpar=0.5, conserve=True), dict(cnpar=1.0, conserve=True)):
         for precision in ("f32", "f32-sr"):
             tc = TracerCollection(grid, precision=precision, **kw)
             tc.add("s", data=dev(d.tracers[0]), source=grid.wrap(dev(np.zeros_like(d.ho)), CENTERS))
