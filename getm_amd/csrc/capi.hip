@@ -420,6 +420,10 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     // in hA (stages 1-2 wrote every cell there), and stages 4-5 read those cells from
     // hA instead of a copy in hB; into hA itself they store updated cells only.
     constexpr bool f32 = std::is_same<R, float>::value;
+    // the FP32 forms' stage pairs cover the interior only: the halo ring's
+    // thickness (which no stage changes) goes to hA and to h_out first
+    if constexpr (f32 || C)
+        if ((rc = launch_h_ring(A, t->ho, hA, hfinal != hA ? hfinal : nullptr, s))) return rc;
     if constexpr (C)
         rc = launch_hpair_c(d1, tr->scheme, ty_of(d1), A, half, ntr, tr->tpb_h, f, fl, S, SL, t->ho, hA, s);
     else

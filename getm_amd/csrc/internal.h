@@ -178,6 +178,8 @@ inline int auto_tile_y(int tracers_per_domain)
 }
 
 // ---- batched kernels (transport_batched.hip) ----
+// thickness of the halo ring (outside [imin, imax] x [jmin, jmax]): ho -> ha (and hb if not NULL)
+int launch_h_ring(const StepArgs& A, const double* ho, double* ha, double* hb, hipStream_t s);
 // tpb = tracers per block (tracer batches go into the grid); <= 0 means all
 template <typename R>
 // hin_dry: thickness of the cells the stages do not update (NULL: hin); write_dry:

@@ -32,6 +32,32 @@ int launch_count_steps_multi(const EnsDom<float>* doms, int ndom, hipStream_t s)
     return GETM_OK;
 }
 
+// Thickness of the halo ring (the cells outside [imin, imax] x [jmin, jmax]),
+// which no stage changes: ho into ha and, if given, hb, every layer.  The stage
+// pairs of the FP32 forms cover the interior only and leave those cells to this.
+__global__ void __launch_bounds__(256) k_h_ring(const StepArgs A, const double* __restrict__ ho,
+                                                double* __restrict__ ha, double* __restrict__ hb)
+{
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (size_t)A.n2) return;
+    const int i = (int)(p % (size_t)A.nx_), j = (int)(p / (size_t)A.nx_);
+    if (i >= A.imin && i <= A.imax && j >= A.jmin && j <= A.jmax) return;
+    for (int k = 0; k < A.nz; k++) {
+        const size_t o = (size_t)k * A.n2 + p;
+        ha[o] = ho[o];
+        if (hb) hb[o] = ho[o];
+    }
+}
+int launch_h_ring(const StepArgs& A, const double* ho, double* ha, double* hb, hipStream_t s)
+{
+    const size_t nblk = ((size_t)A.n2 + 255) / 256;
+    if (nblk == 0) return GETM_OK;
+    if (nblk > 0x7fffffff) return GETM_ERR_CAPACITY;
+    hipLaunchKernelGGL(k_h_ring, dim3((unsigned)nblk), dim3(256), 0, s, A, ho, ha, hb);
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
 template <typename R>
 int launch_wcol(int scheme, const StepArgs& A, double dt, int ntr, int tpb, const R* fin, R* fout, const double* hin,
                 double* hout, const double* w, const double* wv, int* stats, hipStream_t s, bool copy_dry)

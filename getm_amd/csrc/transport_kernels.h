@@ -830,7 +830,7 @@ constexpr int hpair_waves_per_eu(int lanes)
 
 template <int S, typename R, int D1, int TY, int KG, bool HDIFF, int TXW, bool C = false, int LPC = 1>
 __global__ void __launch_bounds__(TXW * TY / LPC) __attribute__((amdgpu_waves_per_eu(hpair_waves_per_eu<R, C, HDIFF>(TXW * TY / LPC)))) k_hpair(const StepArgs A, double dt, int ntr_all, int tpb, int ntx,
-                                                   int nty, int nbatch,
+                                                   int nty, int nbatch, int ox, int oy,
                                                    const std::conditional_t<C, float, R>* __restrict__ fin_all,
                                                    std::conditional_t<C, float, R>* __restrict__ fout_all,
                                                    const double* __restrict__ hin,
@@ -867,7 +867,7 @@ __global__ void __launch_bounds__(TXW * TY / LPC) __attribute__((amdgpu_waves_pe
     const int tb0 = tbatch * tpb;
     const int ntr = ntr_all - tb0 < tpb ? ntr_all - tb0 : tpb;
     const size_t off = (size_t)tb0 * A.n3;
-    hpair_body<S, R, D1, TY, (KG > 1), HDIFF, TXW, C, LPC>(A, dt, tx * TXW, ty * TY, k0, k1, ntr, fin_all + off,
+    hpair_body<S, R, D1, TY, (KG > 1), HDIFF, TXW, C, LPC>(A, dt, ox + tx * TXW, oy + ty * TY, k0, k1, ntr, fin_all + off,
                                                       fout_all + off, hin, hout, sr_stage, tb0, hin_dry,
                                                       write_dry != 0, C ? lin_all + off : nullptr,
                                                       C ? lout_all + off : nullptr);
@@ -1954,7 +1954,17 @@ inline int launch_hpair_impl(int d1, int scheme, int ty, const StepArgs& A, doub
     const bool narrow = std::is_same<R, float>::value && d1 == 1 && ty <= 8 && tpb <= HPAIR_KG;
     const int txw = narrow ? 32 : TX;
     const int tye = narrow ? 2 * ty : ty;
-    const int ntx = (A.nx_ + txw - 1) / txw, nty = (A.ny_ + tye - 1) / tye;
+    // FP32 and compensated forms: tiles over the interior [imin, imax] x [jmin,
+    // jmax] only (they store updated cells only, and the thickness of the halo
+    // ring, which no stage changes, is copied by launch_h_ring); 64-wide tiles
+    // over the 1028 columns of a 1024-column domain with its halos would add a
+    // 17th tile for 2 interior columns.  FP64 covers every cell: its step copies
+    // the whole scratch buffer back.
+    const bool inner = C || std::is_same<R, float>::value;
+    const int ox = inner ? A.imin : 0, oy = inner ? A.jmin : 0;
+    const int wx = inner ? A.imax - A.imin + 1 : A.nx_, wy = inner ? A.jmax - A.jmin + 1 : A.ny_;
+    if (wx <= 0 || wy <= 0) return GETM_OK;  // no interior: no cell to update
+    const int ntx = (wx + txw - 1) / txw, nty = (wy + tye - 1) / tye;
     const bool ah = A.ah_u != nullptr;
     const int kg = ah ? 1 : hpair_group(tpb, std::is_same<R, float>::value);
     const long nitems = (long)nb * ((A.nz + kg - 1) / kg) * ntx * nty;
@@ -1964,8 +1974,8 @@ inline int launch_hpair_impl(int d1, int scheme, int ty, const StepArgs& A, doub
 #define GETM_HPK(D1V, TYV, TXV, KGV, AHV, LPCV)                                                                     \
     GETM_SCHEME_SWITCH(scheme, {                                                                                      \
         hipLaunchKernelGGL((k_hpair<S_, R, D1V, TYV, KGV, AHV, TXV, C, LPCV>), grid, dim3(TXV * TYV / LPCV), 0, s, A,  \
-                           dt, ntr, tpb, ntx, nty, nb, fin, fout, hin, hout, sr_stage, hin_dry, (int)write_dry, lin,   \
-                           lout);                                                                                     \
+                           dt, ntr, tpb, ntx, nty, nb, ox, oy, fin, fout, hin, hout, sr_stage, hin_dry,               \
+                           (int)write_dry, lin, lout);                                                                \
     })
 #define GETM_HP(D1V, TYV, TXV)                            \
     if (ah) {                                             \
