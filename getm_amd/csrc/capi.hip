@@ -35,6 +35,8 @@ void getm::DeviceBuffer::release()
 }
 
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+static StepArgs grid_args(const Grid* t, const Grid* gu, const Grid* gv, const double* u, const double* v,
+                          const double* Ah_u, const double* Ah_v);
 
 extern "C" {
 
@@ -107,6 +109,8 @@ void advection_finalize(void* adv)
     a->wsh.release();
     a->wsf.release();
     a->stats.release();
+    a->hA.release();
+    a->hB.release();
     delete a;
 }
 
@@ -178,25 +182,44 @@ int advection_apply_3d(void* adv, void* tgrid, void* ugrid, void* vgrid, const d
     if ((rc = check_grids(t, gu)) || (rc = check_grids(t, gv))) return rc;
     if (!gu->hn || !gv->hn) return GETM_ERR_ARG;
     if (!w_var) w_var = w;
-    const int nk = t->nz;
-    if ((rc = a->work.ensure(t->n3() * sizeof(double)))) return rc;
+    const size_t n3 = t->n3();
+    if ((rc = a->work.ensure(n3 * sizeof(double))) || (rc = a->hA.ensure(n3 * sizeof(double))) ||
+        (rc = a->hB.ensure(n3 * sizeof(double))))
+        return rc;
     hipStream_t s = as_stream(stream);
     double* tmp = static_cast<double*>(a->work.ptr);
+    double* hA = static_cast<double*>(a->hA.ptr);
+    double* hB = static_cast<double*>(a->hB.ptr);
+    // adv1/adv2 as in operators.py:118-121: v first unless ufirst.  The batched
+    // FP64 kernels with one tracer: each pair of x/y sweeps (v/2 then u/2, and
+    // u/2 then v/2) in one kernel through an LDS tile, bit-identical to the single
+    // sweeps, with the thickness of each stage in hA / hB and the last in h;
+    // var -> tmp -> var (w stage) -> tmp, copied back to var.
+    const int d1 = ufirst ? 1 : 2, d2 = 3 - d1;
     const double half = 0.5 * timestep;
-    // adv1/adv2 as in operators.py:118-121: v first unless ufirst
-    const int d1 = ufirst ? 1 : 2, d2 = ufirst ? 2 : 1;
-    const Grid* g1 = ufirst ? gu : gv;
-    const Grid* g2 = ufirst ? gv : gu;
-    const double* vel1 = ufirst ? u : v;
-    const double* vel2 = ufirst ? v : u;
-    const double* ah1 = ufirst ? Ah_u : Ah_v;
-    const double* ah2 = ufirst ? Ah_v : Ah_u;
-    // ping-pong var -> tmp -> var (w in place) -> tmp -> var
-    if ((rc = uv_sweep(a, d1, nk, t, g1, vel1, ah1, half, h, g1->hn, var, tmp, s))) return rc;
-    if ((rc = uv_sweep(a, d2, nk, t, g2, vel2, ah2, half, h, g2->hn, tmp, var, s))) return rc;
-    if ((rc = w_sweep(a, t, w, w_var, timestep, h, var, s))) return rc;
-    if ((rc = uv_sweep(a, d2, nk, t, g2, vel2, ah2, half, h, g2->hn, var, tmp, s))) return rc;
-    if ((rc = uv_sweep(a, d1, nk, t, g1, vel1, ah1, half, h, g1->hn, tmp, var, s))) return rc;
+    if ((Ah_u == nullptr) != (Ah_v == nullptr)) {
+        // horizontal diffusion in one direction only: the single sweeps, each with
+        // its own Ah (the pair kernel takes both or neither), var -> tmp -> var
+        // (w in place) -> tmp -> var
+        const Grid* g1 = d1 == 1 ? gu : gv;
+        const Grid* g2 = d1 == 1 ? gv : gu;
+        const double* vel1 = d1 == 1 ? u : v;
+        const double* vel2 = d1 == 1 ? v : u;
+        const double* ah1 = d1 == 1 ? Ah_u : Ah_v;
+        const double* ah2 = d1 == 1 ? Ah_v : Ah_u;
+        if ((rc = uv_sweep(a, d1, t->nz, t, g1, vel1, ah1, half, h, g1->hn, var, tmp, s))) return rc;
+        if ((rc = uv_sweep(a, d2, t->nz, t, g2, vel2, ah2, half, h, g2->hn, tmp, var, s))) return rc;
+        if ((rc = w_sweep(a, t, w, w_var, timestep, h, var, s))) return rc;
+        if ((rc = uv_sweep(a, d2, t->nz, t, g2, vel2, ah2, half, h, g2->hn, var, tmp, s))) return rc;
+        return uv_sweep(a, d1, t->nz, t, g1, vel1, ah1, half, h, g1->hn, tmp, var, s);
+    }
+    const StepArgs A = grid_args(t, gu, gv, u, v, Ah_u, Ah_v);
+    if ((rc = launch_hpair<double>(d1, a->scheme, auto_tile_y(1), A, half, 1, 1, var, tmp, h, hA, 0u, s))) return rc;
+    if ((rc = launch_wcol<double>(a->scheme, A, timestep, 1, 1, tmp, var, hA, hB, w, w_var,
+                                  static_cast<int*>(a->stats.ptr), s, true)))
+        return rc;
+    if ((rc = launch_hpair<double>(d2, a->scheme, auto_tile_y(1), A, half, 1, 1, var, tmp, hB, h, 0u, s))) return rc;
+    GETM_HIP_OK(hipMemcpyAsync(var, tmp, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
     return GETM_OK;
 }
 
@@ -309,9 +332,9 @@ void transport_finalize(void* p)
 
 }  // extern "C"
 
-static StepArgs step_args(const Transport* tr, const double* u, const double* v, const double* Ah_u, const double* Ah_v)
+static StepArgs grid_args(const Grid* t, const Grid* gu, const Grid* gv, const double* u, const double* v,
+                          const double* Ah_u, const double* Ah_v)
 {
-    const Grid* t = tr->tgrid;
     StepArgs A;
     A.nx_ = t->nx_;
     A.ny_ = t->ny_;
@@ -323,22 +346,27 @@ static StepArgs step_args(const Transport* tr, const double* u, const double* v,
     A.n2 = t->n2();
     A.n3 = t->n3();
     A.tmask = t->mask;
-    A.umask = tr->ugrid->mask;
-    A.vmask = tr->vgrid->mask;
-    A.idxu = tr->ugrid->idx;
-    A.dyu = tr->ugrid->dy;
-    A.dxv = tr->vgrid->dx;
-    A.idyv = tr->vgrid->idy;
+    A.umask = gu->mask;
+    A.vmask = gv->mask;
+    A.idxu = gu->idx;
+    A.dyu = gu->dy;
+    A.dxv = gv->dx;
+    A.idyv = gv->idy;
     A.iarea = t->iarea;
     A.u = u;
     A.v = v;
-    A.hu = tr->ugrid->hn;
-    A.hv = tr->vgrid->hn;
+    A.hu = gu->hn;
+    A.hv = gv->hn;
     A.hn = t->hn;
     A.ah_u = Ah_u;
     A.ah_v = Ah_v;
     A.rstep = nullptr;
     return A;
+}
+
+static StepArgs step_args(const Transport* tr, const double* u, const double* v, const double* Ah_u, const double* Ah_v)
+{
+    return grid_args(tr->tgrid, tr->ugrid, tr->vgrid, u, v, Ah_u, Ah_v);
 }
 
 // Every grid-dimension limit of the step's launches, checked before the first

@@ -58,6 +58,7 @@ struct Advection {
     DeviceBuffer wsh;     // z-sweep sub-iteration thickness
     DeviceBuffer wsf;     // z-sweep sub-iteration tracer
     DeviceBuffer stats;   // int32[2]: largest sub-iteration count, columns capped at 200
+    DeviceBuffer hA, hB;  // thickness after stages 1-2 and after the w stage (advection_apply_3d)
 };
 
 struct VerticalDiffusion {
