@@ -139,9 +139,18 @@ int advection_uv_calculate(int direction, int nk, void* adv, void* tgrid, void* 
     if ((direction != 1 && direction != 2) || nk < 1 || !uv || !h || !huv || !var) return GETM_ERR_ARG;
     int rc = check_grids(t, g);
     if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    // in place, without a copy back: x with each row's old values staged in LDS;
+    // y in tiles of 32 rows, with the old rows around the tile boundaries saved first
+    if (direction == 1 && t->nx_ <= 2048)
+        return launch_adv_x_inplace_f64(a->scheme, nk, *t, *g, uv, Ah, timestep, h, huv, var, s);
+    if (direction == 2) {
+        if ((rc = a->work.ensure(std::max<size_t>(adv_y_snap_doubles(*t, nk), 1) * sizeof(double)))) return rc;
+        return launch_adv_y_inplace_f64(a->scheme, nk, *t, *g, uv, Ah, timestep, h, huv, var,
+                                        static_cast<double*>(a->work.ptr), s);
+    }
     const size_t bytes = t->n2() * (size_t)nk * sizeof(double);
     if ((rc = a->work.ensure(bytes))) return rc;
-    hipStream_t s = as_stream(stream);
     double* tmp = static_cast<double*>(a->work.ptr);
     if ((rc = uv_sweep(a, direction, nk, t, g, uv, Ah, timestep, h, huv, var, tmp, s))) return rc;
     GETM_HIP_OK(hipMemcpyAsync(var, tmp, bytes, hipMemcpyDeviceToDevice, s));

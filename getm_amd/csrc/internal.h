@@ -242,6 +242,13 @@ int launch_check_finite(const Grid& g, int ntr, const float* f32, const double* 
 size_t inventory_part_doubles(const StepArgs& A, int ntr);
 
 // ---- FP64 kernels (transport_fp64.hip) ----
+// in-place x half-sweep (rows of up to 2048 cells; GETM_ERR_CAPACITY beyond)
+int launch_adv_x_inplace_f64(int scheme, int nk, const Grid& t, const Grid& uv, const double* vel, const double* ah,
+                             double dt, double* h, const double* huv, double* f, hipStream_t s);
+// in-place y half-sweep; snap holds the old rows around the 32-row tile boundaries
+size_t adv_y_snap_doubles(const Grid& t, int nk);
+int launch_adv_y_inplace_f64(int scheme, int nk, const Grid& t, const Grid& uv, const double* vel, const double* ah,
+                             double dt, double* h, const double* huv, double* f, double* snap, hipStream_t s);
 int launch_adv_uv_f64(int direction, int scheme, int nk, const Grid& t, const Grid& uv,
                       const double* vel, const double* ah, double dt, double* h, const double* huv,
                       const double* f_in, double* f_out, hipStream_t s);

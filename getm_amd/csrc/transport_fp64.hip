@@ -28,7 +28,7 @@ template <int S, int DIR>
 __device__ __forceinline__ void face_flux_f64(const int* __restrict__ uvmask, const double* __restrict__ m1,
                                               const double* __restrict__ m2, const double* __restrict__ huv,
                                               const double* __restrict__ vel, const double* __restrict__ ah,
-                                              const double* __restrict__ f, size_t p2, size_t p3,
+                                              const double* __restrict__ f, size_t p2, size_t p3, ptrdiff_t pf,
                                               ptrdiff_t s2, ptrdiff_t s3, double dt, double& q, double& flux)
 {
     if (!open_face(uvmask[p2])) {
@@ -45,22 +45,24 @@ __device__ __forceinline__ void face_flux_f64(const int* __restrict__ uvmask, co
         q = (u * huv[p3]) * m1[p2];
         cfl = fabs((u * dt) * m2[p2]);
     }
+    // f[pf]: the tracer at the face's lower cell (f is the layer field itself,
+    // pf = p3, or one row of it, pf the column), s3 its stride along the sweep
     double fu, fuu, fd;
     if (u > 0.0) {
-        fu = f[p3];
+        fu = f[pf];
         fuu = fu;
-        if (open_face(uvmask[p2 - s2])) fuu = f[p3 - s3];
-        fd = f[p3 + s3];
+        if (open_face(uvmask[p2 - s2])) fuu = f[pf - s3];
+        fd = f[pf + s3];
     } else {
-        fu = f[p3 + s3];
+        fu = f[pf + s3];
         fuu = fu;
-        if (open_face(uvmask[p2 + s2])) fuu = f[p3 + 2 * s3];
-        fd = f[p3];
+        if (open_face(uvmask[p2 + s2])) fuu = f[pf + 2 * s3];
+        fd = f[pf];
     }
     fu = face_value<S, double>(cfl, fu, fuu, fd);
     flux = q * fu;
     if (ah) {  // template:97-105 / 197-205
-        const double df = f[p3 + s3] - f[p3];
+        const double df = f[pf + s3] - f[pf];
         if (DIR == 1)
             flux = flux - (((ah[p2] * huv[p3]) * df) * m1[p2]) * m2[p2];
         else
@@ -90,13 +92,157 @@ __global__ void __launch_bounds__(256) k_adv_uv_f64(
     }
     const ptrdiff_t s2 = DIR == 1 ? 1 : nx_;
     double q0, fl0, q1, fl1;
-    face_flux_f64<S, DIR>(uvmask, m1, m2, huv, vel, ah, f, p2 - s2, p3 - s2, s2, s2, dt, q0, fl0);
-    face_flux_f64<S, DIR>(uvmask, m1, m2, huv, vel, ah, f, p2, p3, s2, s2, dt, q1, fl1);
+    face_flux_f64<S, DIR>(uvmask, m1, m2, huv, vel, ah, f, p2 - s2, p3 - s2, p3 - s2, s2, s2, dt, q0, fl0);
+    face_flux_f64<S, DIR>(uvmask, m1, m2, huv, vel, ah, f, p2, p3, p3, s2, s2, dt, q1, fl1);
     const double hfo = h[p3] * f[p3];
     const double hnew = h[p3] - (dt * (q1 - q0)) * iarea[p2];
     h[p3] = hnew;
     const double advn = (fl1 - fl0) * iarea[p2];
     fout[p3] = (hfo - dt * advn) / hnew;
+}
+
+// x half-sweep in place: one wave per row (j, k), the row's old values first
+// copied into LDS (every face of the row reads only that row), then the row's
+// updated cells written over them; the arithmetic is k_adv_uv_f64's, operand for
+// operand.  Rows of up to ADV_X_ROW cells.
+constexpr int ADV_X_ROW = 2048;
+template <int S>
+__global__ void __launch_bounds__(256) k_adv_x_inplace_f64(
+    int nx_, int ny_, int imin, int imax, int jmin, int jmax, const int* __restrict__ uvmask,
+    const double* __restrict__ m1, const double* __restrict__ m2, const double* __restrict__ huv,
+    const double* __restrict__ vel, const int* __restrict__ tmask, const double* __restrict__ iarea,
+    const double* __restrict__ ah, double dt, double* __restrict__ h, double* __restrict__ f)
+{
+    extern __shared__ double s_rows[];  // [4][nx_]
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int j = blockIdx.x * 4 + w, k = blockIdx.y;
+    const size_t n2 = (size_t)nx_ * ny_;
+    double* row = s_rows + (size_t)w * nx_;
+    const size_t r3 = (size_t)k * n2 + (size_t)j * nx_;  // the row's first cell
+    if (j < ny_)
+        for (int c = lane; c < nx_; c += 64) row[c] = f[r3 + c];
+    __syncthreads();
+    if (j < jmin || j > jmax) return;
+    for (int i = lane; i < nx_; i += 64) {
+        const size_t p2 = (size_t)j * nx_ + i, p3 = r3 + i;
+        if (i < imin || i > imax || tmask[p2] != 1) continue;
+        double q0, fl0, q1, fl1;
+        face_flux_f64<S, 1>(uvmask, m1, m2, huv, vel, ah, row, p2 - 1, p3 - 1, i - 1, 1, 1, dt, q0, fl0);
+        face_flux_f64<S, 1>(uvmask, m1, m2, huv, vel, ah, row, p2, p3, i, 1, 1, dt, q1, fl1);
+        const double hfo = h[p3] * row[i];
+        const double hnew = h[p3] - (dt * (q1 - q0)) * iarea[p2];
+        h[p3] = hnew;
+        const double advn = (fl1 - fl0) * iarea[p2];
+        f[p3] = (hfo - dt * advn) / hnew;
+    }
+}
+
+// In-place x half-sweep of nk layers (GETM_ERR_CAPACITY for rows longer than ADV_X_ROW).
+int launch_adv_x_inplace_f64(int scheme, int nk, const Grid& t, const Grid& uv, const double* vel, const double* ah,
+                             double dt, double* h, const double* huv, double* f, hipStream_t s)
+{
+    const int nx_ = t.nx_, ny_ = t.ny_;
+    if (nx_ > ADV_X_ROW) return GETM_ERR_CAPACITY;
+    if (!uv.mask || !uv.idx || !uv.dy || !t.mask || !t.iarea) return GETM_ERR_ARG;
+    dim3 block(64, 4, 1), grid((ny_ + 3) / 4, nk, 1);
+    const size_t shm = (size_t)4 * nx_ * sizeof(double);
+    ProfileScope ps(GETM_K_ADV_UV_F64, s);
+    GETM_SCHEME_SWITCH(scheme, {
+        hipLaunchKernelGGL((k_adv_x_inplace_f64<S_>), grid, block, shm, s, nx_, ny_, t.imin, t.imax, t.jmin, t.jmax,
+                           uv.mask, uv.idx, uv.dy, huv, vel, t.mask, t.iarea, ah, dt, h, f);
+    });
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
+}
+
+// y half-sweep in place: tiles of 64 columns x ADV_Y_ROWS rows of a layer.  A
+// tile's faces read two rows beyond it on each side, which the neighbouring
+// tiles update, so k_snap_rows first copies the four rows around every tile
+// boundary (their old values; 4 / ADV_Y_ROWS of the layer); each tile then
+// stages its rows and those halo rows in LDS and writes its updated cells over
+// the old ones.  The arithmetic is k_adv_uv_f64's, operand for operand.
+constexpr int ADV_Y_ROWS = 32;
+__global__ void __launch_bounds__(256) k_snap_rows(int nx_, int ny_, int nb, const double* __restrict__ f,
+                                                   double* __restrict__ snap)
+{
+    // snap[k][b][q][i]: row ADV_Y_ROWS (b + 1) - 2 + q of layer k, q = 0..3
+    const int i = blockIdx.x * 64 + threadIdx.x, q = threadIdx.y, b = blockIdx.y, k = blockIdx.z;
+    const int r = ADV_Y_ROWS * (b + 1) - 2 + q;
+    if (i >= nx_ || r >= ny_) return;
+    snap[(((size_t)k * nb + b) * 4 + q) * nx_ + i] = f[(size_t)k * nx_ * ny_ + (size_t)r * nx_ + i];
+}
+template <int S>
+__global__ void __launch_bounds__(256) k_adv_y_inplace_f64(
+    int nx_, int ny_, int imin, int imax, int jmin, int jmax, const int* __restrict__ uvmask,
+    const double* __restrict__ m1, const double* __restrict__ m2, const double* __restrict__ huv,
+    const double* __restrict__ vel, const int* __restrict__ tmask, const double* __restrict__ iarea,
+    const double* __restrict__ ah, double dt, double* __restrict__ h, double* __restrict__ f, int nb,
+    const double* __restrict__ snap)
+{
+    constexpr int NR = ADV_Y_ROWS + 4;
+    __shared__ double s_t[NR * 64];  // rows j0 - 2 .. j0 + ADV_Y_ROWS + 1, 64 columns each
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int i = blockIdx.x * 64 + lane, ty = blockIdx.y, k = blockIdx.z;
+    const int j0 = ty * ADV_Y_ROWS;
+    const size_t n2 = (size_t)nx_ * ny_;
+    if (i < nx_) {
+        for (int rr = w; rr < NR; rr += 4) {
+            const int j = j0 - 2 + rr;
+            if (j < 0 || j >= ny_) continue;
+            double x;
+            if (j < j0)  // from the tile above: rows 0, 1 of boundary ty
+                x = snap[(((size_t)k * nb + (ty - 1)) * 4 + (j - (j0 - 2))) * nx_ + i];
+            else if (j >= j0 + ADV_Y_ROWS)  // from the tile below: rows 2, 3 of boundary ty + 1
+                x = snap[(((size_t)k * nb + ty) * 4 + (j - (j0 + ADV_Y_ROWS - 2))) * nx_ + i];
+            else
+                x = f[(size_t)k * n2 + (size_t)j * nx_ + i];
+            s_t[rr * 64 + lane] = x;
+        }
+    }
+    __syncthreads();
+    if (i >= nx_ || i < imin || i > imax) return;
+    for (int rr = 2 + w; rr < 2 + ADV_Y_ROWS; rr += 4) {
+        const int j = j0 - 2 + rr;
+        if (j < jmin || j > jmax) continue;
+        const size_t p2 = (size_t)j * nx_ + i, p3 = (size_t)k * n2 + p2;
+        if (tmask[p2] != 1) continue;
+        const ptrdiff_t s2 = nx_;
+        double q0, fl0, q1, fl1;
+        face_flux_f64<S, 2>(uvmask, m1, m2, huv, vel, ah, s_t, p2 - s2, p3 - s2, (rr - 1) * 64 + lane, s2, 64, dt,
+                            q0, fl0);
+        face_flux_f64<S, 2>(uvmask, m1, m2, huv, vel, ah, s_t, p2, p3, rr * 64 + lane, s2, 64, dt, q1, fl1);
+        const double hfo = h[p3] * s_t[rr * 64 + lane];
+        const double hnew = h[p3] - (dt * (q1 - q0)) * iarea[p2];
+        h[p3] = hnew;
+        const double advn = (fl1 - fl0) * iarea[p2];
+        f[p3] = (hfo - dt * advn) / hnew;
+    }
+}
+
+// In-place y half-sweep of nk layers; snap: a work array of at least
+// adv_y_snap_doubles(t, nk) doubles.
+size_t adv_y_snap_doubles(const Grid& t, int nk)
+{
+    const int nty = (t.ny_ + ADV_Y_ROWS - 1) / ADV_Y_ROWS;
+    return (size_t)nk * (nty > 1 ? nty - 1 : 0) * 4 * t.nx_;
+}
+int launch_adv_y_inplace_f64(int scheme, int nk, const Grid& t, const Grid& uv, const double* vel, const double* ah,
+                             double dt, double* h, const double* huv, double* f, double* snap, hipStream_t s)
+{
+    const int nx_ = t.nx_, ny_ = t.ny_;
+    if (!uv.mask || !uv.dx || !uv.idy || !t.mask || !t.iarea) return GETM_ERR_ARG;
+    const int ntx = (nx_ + 63) / 64, nty = (ny_ + ADV_Y_ROWS - 1) / ADV_Y_ROWS, nb = nty - 1;
+    if (nty > 65535 || nk > 65535) return GETM_ERR_CAPACITY;
+    ProfileScope ps(GETM_K_ADV_UV_F64, s);
+    if (nb > 0)
+        hipLaunchKernelGGL(k_snap_rows, dim3(ntx, nb, nk), dim3(64, 4, 1), 0, s, nx_, ny_, nb, f, snap);
+    GETM_SCHEME_SWITCH(scheme, {
+        hipLaunchKernelGGL((k_adv_y_inplace_f64<S_>), dim3(ntx, nty, nk), dim3(64, 4, 1), 0, s, nx_, ny_, t.imin,
+                           t.imax, t.jmin, t.jmax, uv.mask, uv.dx, uv.idy, huv, vel, t.mask, t.iarea, ah, dt, h, f, nb,
+                           snap);
+    });
+    GETM_HIP_OK(hipGetLastError());
+    return GETM_OK;
 }
 
 int launch_adv_uv_f64(int direction, int scheme, int nk, const Grid& t, const Grid& uv, const double* vel,
