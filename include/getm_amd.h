@@ -243,7 +243,7 @@ int transport_conserve(void* tr, int ntracers, void* tracers, const double* targ
  * memory is a 32 MiB array inside the library, one per GPU, held until the
  * process ends, instead of buffers the transport object grows on demand.
  * Needs B = ceil(nx_ / 64) * ceil(ny_ / 4) blocks of partial sums per tracer:
- * GETM_F32C uses 3 B + 1 doubles per tracer, GETM_F32 7 B + 3; more than
+ * GETM_F32C uses 4 B + 1 doubles per tracer, GETM_F32 8 B + 3; more than
  * 4 Mi doubles in total returns GETM_ERR_CAPACITY (e.g. 1028 x 772 cells with
  * halos: up to 319 GETM_F32C tracers, 159 GETM_F32 tracers).  Calls that
  * overlap on one GPU (other streams or threads) share the array and race;
