@@ -195,6 +195,8 @@ int advection_apply_3d(void* adv, void* tgrid, void* ugrid, void* vgrid, const d
     if ((rc = a->work.ensure(n3 * sizeof(double))) || (rc = a->hA.ensure(n3 * sizeof(double))) ||
         (rc = a->hB.ensure(n3 * sizeof(double))))
         return rc;
+    const bool pairs = (Ah_u == nullptr) == (Ah_v == nullptr);
+    if (pairs && (rc = a->work2.ensure(n3 * sizeof(double)))) return rc;
     hipStream_t s = as_stream(stream);
     double* tmp = static_cast<double*>(a->work.ptr);
     double* hA = static_cast<double*>(a->hA.ptr);
@@ -203,10 +205,10 @@ int advection_apply_3d(void* adv, void* tgrid, void* ugrid, void* vgrid, const d
     // FP64 kernels with one tracer: each pair of x/y sweeps (v/2 then u/2, and
     // u/2 then v/2) in one kernel through an LDS tile, bit-identical to the single
     // sweeps, with the thickness of each stage in hA / hB and the last in h;
-    // var -> tmp -> var (w stage) -> tmp, copied back to var.
+    // var -> tmp -> tmp2 (w stage) -> var (each writes every cell of its target).
     const int d1 = ufirst ? 1 : 2, d2 = 3 - d1;
     const double half = 0.5 * timestep;
-    if ((Ah_u == nullptr) != (Ah_v == nullptr)) {
+    if (!pairs) {
         // horizontal diffusion in one direction only: the single sweeps, each with
         // its own Ah (the pair kernel takes both or neither), var -> tmp -> var
         // (w in place) -> tmp -> var
@@ -224,12 +226,11 @@ int advection_apply_3d(void* adv, void* tgrid, void* ugrid, void* vgrid, const d
     }
     const StepArgs A = grid_args(t, gu, gv, u, v, Ah_u, Ah_v);
     if ((rc = launch_hpair<double>(d1, a->scheme, auto_tile_y(1), A, half, 1, 1, var, tmp, h, hA, 0u, s))) return rc;
-    if ((rc = launch_wcol<double>(a->scheme, A, timestep, 1, 1, tmp, var, hA, hB, w, w_var,
+    double* tmp2 = static_cast<double*>(a->work2.ptr);
+    if ((rc = launch_wcol<double>(a->scheme, A, timestep, 1, 1, tmp, tmp2, hA, hB, w, w_var,
                                   static_cast<int*>(a->stats.ptr), s, true)))
         return rc;
-    if ((rc = launch_hpair<double>(d2, a->scheme, auto_tile_y(1), A, half, 1, 1, var, tmp, hB, h, 0u, s))) return rc;
-    GETM_HIP_OK(hipMemcpyAsync(var, tmp, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-    return GETM_OK;
+    return launch_hpair<double>(d2, a->scheme, auto_tile_y(1), A, half, 1, 1, tmp2, var, hB, h, 0u, s);
 }
 
 int advection_get_stats(void* adv, int* max_iters, int* capped_columns, int reset, void* stream)

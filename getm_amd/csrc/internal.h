@@ -55,6 +55,7 @@ struct Advection {
     int scheme;
     const Grid* tgrid;
     DeviceBuffer work;    // out-of-place target of the x/y sweeps (nk layers)
+    DeviceBuffer work2;   // target of the w stage in advection_apply_3d
     DeviceBuffer wsh;     // z-sweep sub-iteration thickness
     DeviceBuffer wsf;     // z-sweep sub-iteration tracer
     DeviceBuffer stats;   // int32[2]: largest sub-iteration count, columns capped at 200

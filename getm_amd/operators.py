@@ -149,7 +149,14 @@ class Advection:
                        skip_initial_halo_exchange: bool = False,
                        get_w: Optional[Callable[[Array], Array]] = None):
         """Stage-outer, tracer-inner loop of operators.py:135-193: every tracer
-        starts each stage from the same thickness ``current_h``."""
+        starts each stage from the thickness the previous stage left, which for
+        the w stage is the last tracer's (its sub-step count follows that
+        tracer's w_var, and so do the roundings of h).  When every tracer has the
+        same vertical velocity, the thickness sequence is the same for all of
+        them, and each tracer runs its five stages in one ``advection_apply_3d``
+        call from the starting thickness instead (stage pairs on the GPU, no
+        copies of h between stages): the same bits, with ``h`` left at the final
+        thickness."""
         if get_w is None:
             get_w = lambda tracer: w
         vars = list(vars)
@@ -158,24 +165,28 @@ class Advection:
         assert w.grid is self.grid and w.z == INTERFACES
         for var in vars:
             assert var.grid is self.grid and var.z == CENTERS
+        w_vars = [get_w(var) for var in vars]
+        for w_var in w_vars:
+            assert w_var.grid is self.grid and w_var.z == INTERFACES
+        if all(w_var is w_vars[0] for w_var in w_vars):
+            for var in vars:
+                self.apply_3d(u, v, w, timestep, var, Ah_u, Ah_v, new_h=new_h, w_var=w_vars[0])
+            return
         adv1 = lambda var: self.u_3d(u, 0.5 * timestep, var, Ah_u)
         adv2 = lambda var: self.v_3d(v, 0.5 * timestep, var, Ah_v)
         if not self._ufirst:
             adv1, adv2 = adv2, adv1
         current_h = (self.grid.hn if new_h else self.grid.ho).all_values.clone()
         for stage in range(5):
-            for var in vars:
+            for var, w_var in zip(vars, w_vars):
                 self.h.copy_(current_h)
                 if stage in (0, 4):
                     adv1(var)
                 elif stage in (1, 3):
                     adv2(var)
                 else:
-                    w_var = get_w(var)
-                    assert w_var.grid is self.grid and w_var.z == INTERFACES
                     self.w_3d(w, w_var, timestep, var)
-            if vars:
-                current_h.copy_(self.h)
+            current_h.copy_(self.h)
 
     def stats(self, reset: bool = True):
         """(largest z sub-iteration count, columns capped at 200) since the last reset."""

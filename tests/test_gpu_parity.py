@@ -247,3 +247,64 @@ def test_seamount_100_steps_bitwise(oracle_mod):
     # thickness work array ends at hn (test_3d_barotropic.py:121-128)
     wet = np.broadcast_to(d.tmask == 1, d.ho.shape)
     np.testing.assert_allclose(host(adv.h)[wet], d.hn[wet], rtol=5e-14, atol=0)
+
+
+def _stage_outer_batch(adv, u, v, w, dt, vars, w_vars, Ah_u, Ah_v, ufirst):
+    """operators.py:135-193 as written: stage-outer, tracer-inner, the thickness
+    work array reset to the stage's starting thickness before every tracer."""
+    adv1 = lambda var: adv.u_3d(u, 0.5 * dt, var, Ah_u)
+    adv2 = lambda var: adv.v_3d(v, 0.5 * dt, var, Ah_v)
+    if not ufirst:
+        adv1, adv2 = adv2, adv1
+    current_h = adv.grid.ho.all_values.clone()
+    for stage in range(5):
+        for var, w_var in zip(vars, w_vars):
+            adv.h.copy_(current_h)
+            if stage in (0, 4):
+                adv1(var)
+            elif stage in (1, 3):
+                adv2(var)
+            else:
+                adv.w_3d(w, w_var, dt, var)
+        current_h.copy_(adv.h)
+
+
+@pytest.mark.parametrize("w_vars", ["shared", "own"])
+@pytest.mark.parametrize("ah", ["both", "u-only", "none"])
+@pytest.mark.parametrize("ufirst", [False, True])
+def test_apply_3d_batch_equals_the_stage_outer_loop(ah, ufirst, w_vars):
+    """apply_3d_batch against the reference's stage-outer loop of single sweeps,
+    three tracers, on a domain whose w stage sub-steps: with one vertical
+    velocity for all tracers (each tracer's five stages in one
+    advection_apply_3d call) and with their own (sinking and rising tracers,
+    whose sub-step counts differ, so that the last tracer's thickness carries
+    into stages 4-5): the same bits and the same final thickness."""
+    from getm_amd.core import CENTERS, INTERFACES
+    from getm_amd.operators import Advection
+
+    d = inputs.tiny(nx=40, ny=34, nz=9, seed=23, scheme=4, ntracers=3, cfl=0.9, wscale=2.0)
+    results = []
+    for batched in (True, False):
+        grid = make_grid(d)
+        u, v, w, _ = arrays(grid, d, d.tracers[0])
+        vars = [grid.wrap(dev(d.tracers[t]), CENTERS) for t in range(3)]
+        if w_vars == "shared":
+            wv = [w, w, w]
+        else:
+            wv = [w, grid.wrap(dev(1.6 * d.w), INTERFACES), grid.wrap(dev(0.4 * d.w), INTERFACES)]
+        Ah_u = grid.ugrid.wrap(dev(np.full(d.hu.shape[1:], 30.0))) if ah != "none" else None
+        Ah_v = grid.vgrid.wrap(dev(np.full(d.hv.shape[1:], 20.0))) if ah == "both" else None
+        adv = Advection(grid, d.scheme)
+        adv.ufirst = ufirst
+        adv.stats()
+        for _ in range(3):
+            if batched:
+                by = {id(var): x for var, x in zip(vars, wv)}
+                adv.apply_3d_batch(u, v, w, d.dt, vars, Ah_u=Ah_u, Ah_v=Ah_v, get_w=lambda var: by[id(var)])
+            else:
+                _stage_outer_batch(adv, u, v, w, d.dt, vars, wv, Ah_u, Ah_v, ufirst)
+        assert adv.stats()[0] > 1  # the w stage sub-stepped
+        results.append(([host(var.all_values) for var in vars], host(adv.h)))
+    for t in range(3):
+        assert_bitwise(results[0][0][t], results[1][0][t], f"tracer {t}")
+    assert_bitwise(results[0][1], results[1][1], "thickness")
