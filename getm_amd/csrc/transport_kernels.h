@@ -1993,7 +1993,7 @@ inline int launch_hpair_impl(int d1, int scheme, int ty, const StepArgs& A, doub
     // (512-lane blocks, two per CU, each covering the other's barrier phases).  At
     // 256 tracers: compensated x-first pair 43.3 -> 40.9 ms, y-first 42.1 -> 38.1
     // ms; FP32 pairs 38.7 / 39.3 -> 28.4 / 29.1 ms (34.2 / 34.4 ms on 8-row tiles)
-    if constexpr (C || std::is_same<R, float>::value) {
+    {
         if (!ah && ty == 16 && kg == 1) {
             if (d1 == 2) {
                 GETM_HPK(2, 16, TX, 1, false, 2);
