@@ -447,10 +447,12 @@ static int transport_step_t(Transport* tr, const StepArgs& A, int ntr, R* f, con
     const int tpb_h = tr->tpb_h <= 0 || tr->tpb_h > ntr ? ntr : tr->tpb_h;
     // FP32 and compensated forms with many tracers per block: 16-row tiles, two
     // cells per lane (less halo to load and sweep than 8-row tiles); and the FP64
-    // form without horizontal diffusion: the same tiles (coastal, 2 tracers: pairs
-    // 1.02 / 1.09 -> 0.99 / 1.01 ms against 8-row tiles)
+    // and compensated forms (FP64 working values) without horizontal diffusion with
+    // any number of tracers: the same tiles (coastal, 2 tracers: FP64 pairs 1.02 /
+    // 1.09 -> 0.99 / 1.01 ms, compensated 1.22 / 1.27 -> 1.06 / 1.06 ms against
+    // 8-row tiles)
     const bool tall = ((C || std::is_same<R, float>::value) && ntr >= 16 && hpair_group(tpb_h, !C) == 1) ||
-                      (std::is_same<R, double>::value && !C && !A.ah_u);
+                      ((std::is_same<R, double>::value || C) && !A.ah_u);
     const int ty = tr->tile_y ? tr->tile_y : (tall ? 16 : auto_tile_y(ntr));
     // compensated form with many tracers: the y-first pair on 12-row tiles, two
     // 768-lane blocks per CU (its first stage adds no rows, so the shorter tile

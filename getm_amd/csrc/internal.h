@@ -168,13 +168,13 @@ struct Ensemble {
 };
 
 // Tile height of the horizontal stage pair for few tracers per block (up to 8
-// FP32 tracers, which take layer groups) and for the FP64 form with horizontal
-// diffusion: 8 rows.  Three 512-lane blocks per CU overlap each other's barrier
+// FP32 tracers, which take layer groups) and for the FP64 and compensated forms
+// with horizontal diffusion: 8 rows.  Three 512-lane blocks per CU overlap each other's barrier
 // phases; 16-row tiles of one cell per lane have a smaller halo share but fit
 // one or two 1024-lane blocks per CU, and were slower at 2 tracers on the
-// coastal grid.  (With more FP32 tracers per block, and for the FP64 form
-// without horizontal diffusion, the pairs take 16-row tiles of two cells per
-// lane: 512-lane blocks, two per CU.)
+// coastal grid.  (With more FP32 tracers per block, and for the FP64 and
+// compensated forms without horizontal diffusion, the pairs take 16-row tiles
+// of two cells per lane: 512-lane blocks, two per CU.)
 inline int auto_tile_y(int tracers_per_domain)
 {
     (void)tracers_per_domain;

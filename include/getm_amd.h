@@ -181,8 +181,9 @@ int transport_step_mixed(void* tr, int ntracers, void* tracers, const double* u,
  * of the horizontal-stage tiles (4, 8 or 16 rows of 64 columns, and 12 for the
  * compensated form; 0 = the default: 16 rows, two cells per lane, for the FP32
  * forms with 16 or more tracers and more than 8 per block and for the FP64
- * form without horizontal diffusion, the compensated form's y-first pair on
- * 12 rows; 8 rows otherwise) and, for the
+ * and compensated forms without horizontal diffusion, the compensated form's
+ * y-first pair on 12 rows with 16 or more tracers; 8 rows otherwise) and, for
+ * the
  * three kernel kinds (horizontal stage pairs, w stage, FP32 diffusion), the
  * tracers per block (the tracer batches of one step are spread over the grid;
  * 0 = all tracers in every block; defaults 0, 16 (64 for GETM_F32C) and 16).
