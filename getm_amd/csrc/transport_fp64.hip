@@ -518,15 +518,20 @@ int launch_vdiff_apply_f64(const Grid& t, int nz, const int* mask, const double*
 
 // Batched FP64 diffusion of the transport step (dzo = dzn = hn, no ea2/ea4):
 // TracerCollection._diffuse (tracer.py:248-276) for every tracer, out of place
-// (the stage-5 scratch in, the tracers out), one lane per (column, tracer).
-// auxo/auxn are formed inline with the expressions of k_vdiff_prepare_f64 and the
-// solve is k_vdiff_apply_f64's, so results equal prepare + per-tracer apply bit
-// for bit.  The elimination stores ru(k) over the scratch value var(k), which it
-// no longer needs (it is in a register), so tracers need no other scratch.
-// Columns that are not solved copy their values; nz = 1 applies the reference's
-// (dz c)/dz to every point (diffusion.F90:224-231).
+// (the stage-5 scratch in, the tracers out), one lane per column and group of
+// up to TPL tracers.  auxo/auxn are formed inline with the expressions of
+// k_vdiff_prepare_f64 and the solve is k_vdiff_apply_f64's, so results equal
+// prepare + per-tracer apply bit for bit.  The matrix does not depend on the
+// tracer: its coefficients, the divisions that form them and ru are computed
+// once per column for the group (the same operands, so the same bits as per
+// tracer), and only the right sides q and the back-substitution are per
+// tracer.  The elimination stores ru(k) over the first tracer's scratch value
+// var(k), which it no longer needs (it is in a register), so tracers need no
+// other scratch.  Columns that are not solved copy their values; nz = 1 applies
+// the reference's (dz c)/dz to every point (diffusion.F90:224-231).
+template <int TPL>
 __global__ void __launch_bounds__(256) k_vdiff_batch_f64(
-    int nx_, int ny_, int nz, int imin, int imax, int jmin, int jmax, const int* __restrict__ mask,
+    int nx_, int ny_, int nz, int ntr, int imin, int imax, int jmin, int jmax, const int* __restrict__ mask,
     const double* __restrict__ hn, const double* __restrict__ nuh, double dt, double cnpar, double molecular,
     double* __restrict__ fin, double* __restrict__ fout)
 {
@@ -536,59 +541,90 @@ __global__ void __launch_bounds__(256) k_vdiff_batch_f64(
     const size_t n2 = (size_t)nx_ * ny_, n3 = n2 * (size_t)nz;
     const size_t p = (size_t)j * nx_ + i;
     const bool solve = nz > 1 && i >= imin && i <= imax && j >= jmin && j <= jmax && mask[p] == 1;
+    const int t0 = blockIdx.z * TPL;
+    const int nt = ntr - t0 < TPL ? ntr - t0 : TPL;  // tracers of this group (block-uniform)
 #define AT(a, k) (a)[(size_t)(k) * n2 + p]
-    {
-        const int t = blockIdx.z;
-        double* var = fin + (size_t)t * n3;
-        double* ru = var;  // ru(k) replaces var(k) once the elimination has passed row k
-        double* out = fout + (size_t)t * n3;
-        if (nz == 1) {
-            out[p] = (hn[p] * var[p]) / hn[p];
-            return;
+    double* var[TPL];
+    double* out[TPL];
+#pragma unroll
+    for (int u = 0; u < TPL; u++) {
+        const int t = t0 + (u < nt ? u : 0);
+        var[u] = fin + (size_t)t * n3;
+        out[u] = fout + (size_t)t * n3;
+    }
+    double* ru = var[0];  // ru(k) replaces the first tracer's var(k) once the elimination has passed row k
+    if (nz == 1) {
+#pragma unroll
+        for (int u = 0; u < TPL; u++)
+            if (u < nt) out[u][p] = (hn[p] * var[u][p]) / hn[p];
+        return;
+    }
+    if (!solve) {
+#pragma unroll
+        for (int u = 0; u < TPL; u++)
+            if (u < nt)
+                for (int k = 0; k < nz; k++) AT(out[u], k) = AT(var[u], k);
+        return;
+    }
+    // auxo(k), auxn(k) of interface k+1 (k_vdiff_prepare_f64)
+    auto aux = [&](int k, double& ao, double& an) {
+        const double x = (2.0 * dt) * (AT(nuh, k + 1) + molecular);
+        ao = ((1.0 - cnpar) * x) / (AT(hn, k + 1) + AT(hn, k));
+        an = (cnpar * x) / (AT(hn, k + 1) + AT(hn, k));
+    };
+    const int kt = nz - 1;
+    double ao_lo, an_lo;
+    aux(kt - 1, ao_lo, an_lo);
+    double vk1[TPL], vk[TPL], qup[TPL];
+    double a2 = AT(hn, kt) + an_lo;
+#pragma unroll
+    for (int u = 0; u < TPL; u++) {
+        if (u >= nt) continue;
+        vk1[u] = AT(var[u], kt);
+        vk[u] = AT(var[u], kt - 1);
+        const double a4 = vk1[u] * AT(hn, kt) + (vk[u] - vk1[u]) * ao_lo;
+        qup[u] = a4 / a2;
+        AT(out[u], kt) = qup[u];
+    }
+    double rup = (-an_lo) / a2;
+    AT(ru, kt) = rup;
+    for (int k = kt - 1; k >= 1; k--) {
+        const double ank = an_lo, aok = ao_lo;
+        aux(k - 1, ao_lo, an_lo);
+        a2 = (AT(hn, k) + ank) + an_lo;
+        const double den = a2 - (-ank) * rup;
+#pragma unroll
+        for (int u = 0; u < TPL; u++) {
+            if (u >= nt) continue;
+            const double vkm1 = AT(var[u], k - 1);
+            const double a4 = (vk[u] * AT(hn, k) + (vk1[u] - vk[u]) * aok) + (vkm1 - vk[u]) * ao_lo;
+            const double quk = (a4 - (-ank) * qup[u]) / den;
+            AT(out[u], k) = quk;
+            qup[u] = quk;
+            vk1[u] = vk[u];
+            vk[u] = vkm1;
         }
-        if (!solve) {
-            for (int k = 0; k < nz; k++) AT(out, k) = AT(var, k);
-            return;
-        }
-        // auxo(k), auxn(k) of interface k+1 (k_vdiff_prepare_f64)
-        auto aux = [&](int k, double& ao, double& an) {
-            const double x = (2.0 * dt) * (AT(nuh, k + 1) + molecular);
-            ao = ((1.0 - cnpar) * x) / (AT(hn, k + 1) + AT(hn, k));
-            an = (cnpar * x) / (AT(hn, k + 1) + AT(hn, k));
-        };
-        const int kt = nz - 1;
-        double ao_lo, an_lo;
-        aux(kt - 1, ao_lo, an_lo);
-        double vk1 = AT(var, kt), vk = AT(var, kt - 1);
-        double a2 = AT(hn, kt) + an_lo;
-        double a4 = vk1 * AT(hn, kt) + (vk - vk1) * ao_lo;
-        double rup = (-an_lo) / a2;
-        double qup = a4 / a2;
-        AT(ru, kt) = rup;
-        AT(out, kt) = qup;
-        for (int k = kt - 1; k >= 1; k--) {
-            const double ank = an_lo, aok = ao_lo;
-            aux(k - 1, ao_lo, an_lo);
-            const double vkm1 = AT(var, k - 1);
-            a2 = (AT(hn, k) + ank) + an_lo;
-            a4 = (vk * AT(hn, k) + (vk1 - vk) * aok) + (vkm1 - vk) * ao_lo;
-            const double den = a2 - (-ank) * rup;
-            const double ruk = (-an_lo) / den;
-            const double quk = (a4 - (-ank) * qup) / den;
-            AT(ru, k) = ruk;
-            AT(out, k) = quk;
-            rup = ruk;
-            qup = quk;
-            vk1 = vk;
-            vk = vkm1;
-        }
-        a2 = AT(hn, 0) + an_lo;
-        a4 = vk * AT(hn, 0) + (vk1 - vk) * ao_lo;
-        double v = (a4 - (-an_lo) * qup) / (a2 - (-an_lo) * rup);
-        AT(out, 0) = v;
-        for (int k = 1; k <= kt; k++) {
-            v = AT(out, k) - AT(ru, k) * v;
-            AT(out, k) = v;
+        const double ruk = (-an_lo) / den;
+        AT(ru, k) = ruk;
+        rup = ruk;
+    }
+    a2 = AT(hn, 0) + an_lo;
+    const double den0 = a2 - (-an_lo) * rup;
+    double v[TPL];
+#pragma unroll
+    for (int u = 0; u < TPL; u++) {
+        if (u >= nt) continue;
+        const double a4 = vk[u] * AT(hn, 0) + (vk1[u] - vk[u]) * ao_lo;
+        v[u] = (a4 - (-an_lo) * qup[u]) / den0;
+        AT(out[u], 0) = v[u];
+    }
+    for (int k = 1; k <= kt; k++) {
+        const double r = AT(ru, k);
+#pragma unroll
+        for (int u = 0; u < TPL; u++) {
+            if (u >= nt) continue;
+            v[u] = AT(out[u], k) - r * v[u];
+            AT(out[u], k) = v[u];
         }
     }
 #undef AT
@@ -599,10 +635,12 @@ int launch_vdiff_batch_f64(const Grid& t, int nz, const int* mask, const double*
 {
     if (ntr > 65535) return GETM_ERR_CAPACITY;
     ProfileScope ps(GETM_K_VDIFF_APPLY_F64, s);
+    // two tracers per lane: one set of coefficients, divisions and ru per column for both
+    constexpr int TPL = 2;
     dim3 block(64, 4, 1);
-    dim3 grid((t.nx_ + 63) / 64, (t.ny_ + 3) / 4, ntr);
-    hipLaunchKernelGGL(k_vdiff_batch_f64, grid, block, 0, s, t.nx_, t.ny_, nz, t.imin, t.imax, t.jmin, t.jmax, mask, hn,
-                       nuh, dt, cnpar, molecular, fin, fout);
+    dim3 grid((t.nx_ + 63) / 64, (t.ny_ + 3) / 4, (ntr + TPL - 1) / TPL);
+    hipLaunchKernelGGL(k_vdiff_batch_f64<TPL>, grid, block, 0, s, t.nx_, t.ny_, nz, ntr, t.imin, t.imax, t.jmin,
+                       t.jmax, mask, hn, nuh, dt, cnpar, molecular, fin, fout);
     GETM_HIP_OK(hipGetLastError());
     return GETM_OK;
 }
