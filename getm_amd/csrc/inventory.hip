@@ -3,8 +3,8 @@
 // Inventory of tracer t: sum over wet cells of area * h * c
 // (the reference's domain integral, python/pygetm/simulation.py:1408-1466),
 // accumulated in FP64 with Neumaier compensation in a fixed order (lane
-// column sums, then a sequential block sum, then a sequential sum over
-// blocks), so it is deterministic.
+// column sums, then a butterfly over each wave and the waves of a block in
+// order, then a fixed-order sum over blocks), so it is deterministic.
 //
 // Fixer (FP32 tracers): FP32 storage rounds every cell once per stage, which
 // makes the inventory drift by ~1e-11 relative per step.  The fixer restores
@@ -50,12 +50,27 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double x)
     s = t;
 }
 
-// |c| / ulp(c) for an FP32 value (0 for zero and subnormals).
+// Sum of two compensated pairs (s1, c1) + (s2, c2).  Symmetric in its two
+// arguments (the error term comes from the larger of s1, s2 either way), so a
+// butterfly over a wave leaves the same pair in every lane.
+__device__ __forceinline__ void neumaier_merge(double& s, double& c, double s2, double c2)
+{
+    const double t = s + s2;
+    const double e = fabs(s) >= fabs(s2) ? (s - t) + s2 : (s2 - t) + s;
+    c = (c + c2) + e;
+    s = t;
+}
+
+// |c| / ulp(c) for an FP32 value (0 for zero and subnormals): |c| 2^(23 - e)
+// with e = eb - 127 from the biased exponent eb, the power of two built as an
+// FP64 with exponent field 1023 + 150 - eb (exact, no exp2 per value).
+// Infinities and NaN give NaN.
 __device__ __forceinline__ double q_of(float c)
 {
-    const int e = ilogbf(c);
-    if (c == 0.f || e < -126) return 0.0;
-    return (double)fabsf(c) * exp2((double)(23 - e));
+    const unsigned eb = (__float_as_uint(c) >> 23) & 0xffu;
+    if (eb == 0u) return 0.0;
+    if (eb == 255u) return (double)c * 0.0;
+    return (double)fabsf(c) * __hiloint2double((int)((1173u - eb) << 20), 0);
 }
 
 // Partial sums per (tracer, block), INV_SLOTS of them: mass (hi, lo), the sum
@@ -67,7 +82,7 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, con
                                                            const R* __restrict__ f, double* __restrict__ part,
                                                            const short* __restrict__ lo = nullptr)
 {
-    __shared__ double s_m[INV_THREADS], s_c[INV_THREADS], s_q[INV_THREADS], s_a[INV_THREADS];
+    __shared__ double s_m[IBY], s_c[IBY], s_q[IBY], s_a[IBY];
     const int tid = threadIdx.y * IBX + threadIdx.x;
     const int i = blockIdx.x * IBX + threadIdx.x, j = blockIdx.y * IBY + threadIdx.y;
     const int t = blockIdx.z;
@@ -95,18 +110,26 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, con
             }
         }
     }
-    s_m[tid] = m;
-    s_c[tid] = mc;
-    s_q[tid] = q;
-    s_a[tid] = ab;
+    // a butterfly over each wave (compensated for the mass), then the waves in
+    // order: a fixed order, so deterministic
+    for (int d = 1; d < 64; d <<= 1) {
+        neumaier_merge(m, mc, __shfl_xor(m, d), __shfl_xor(mc, d));
+        q += __shfl_xor(q, d);
+        ab += __shfl_xor(ab, d);
+    }
+    if (threadIdx.x == 0) {
+        s_m[threadIdx.y] = m;
+        s_c[threadIdx.y] = mc;
+        s_q[threadIdx.y] = q;
+        s_a[threadIdx.y] = ab;
+    }
     __syncthreads();
     if (tid == 0) {
-        double S = 0.0, C = 0.0, Q = 0.0, AB = 0.0;
-        for (int l = 0; l < INV_THREADS; l++) {
-            neumaier(S, C, s_m[l]);
-            C += s_c[l];
-            Q += s_q[l];
-            AB += s_a[l];
+        double S = s_m[0], C = s_c[0], Q = s_q[0], AB = s_a[0];
+        for (int w = 1; w < IBY; w++) {
+            neumaier_merge(S, C, s_m[w], s_c[w]);
+            Q += s_q[w];
+            AB += s_a[w];
         }
         double* pt = part + (size_t)t * INV_SLOTS * nblk;
         pt[blk] = S;
@@ -114,17 +137,6 @@ __global__ void __launch_bounds__(INV_THREADS) k_inventory(const StepArgs A, con
         pt[2 * nblk + blk] = Q;
         pt[3 * nblk + blk] = AB;
     }
-}
-
-// Sum of two compensated pairs (s1, c1) + (s2, c2).  Symmetric in its two
-// arguments (the error term comes from the larger of s1, s2 either way), so a
-// butterfly over a wave leaves the same pair in every lane.
-__device__ __forceinline__ void neumaier_merge(double& s, double& c, double s2, double c2)
-{
-    const double t = s + s2;
-    const double e = fabs(s) >= fabs(s2) ? (s - t) + s2 : (s2 - t) + s;
-    c = (c + c2) + e;
-    s = t;
 }
 
 // Partial sums per (tracer, block) of the FP64 and compensated forms, TCH
@@ -294,8 +306,10 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_f32(const StepArgs A, int n
                                                          const double* __restrict__ ratio,
                                                          const double* __restrict__ area, double* __restrict__ part2)
 {
-    __shared__ double s_q[INV_THREADS], s_m[INV_THREADS], s_c[INV_THREADS], s_x[INV_THREADS];
+    __shared__ double s_w[IBY], s_m[IBY], s_c[IBY], s_x[IBY];
+    __shared__ int s_l[IBY];
     const int tid = threadIdx.y * IBX + threadIdx.x;
+    const int lane = threadIdx.x, wv = threadIdx.y;
     const int i = blockIdx.x * IBX + threadIdx.x, j = blockIdx.y * IBY + threadIdx.y;
     const int t = blockIdx.z;
     const int nblk = gridDim.x * gridDim.y;
@@ -315,21 +329,23 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_f32(const StepArgs A, int n
             for (int k = 0; k < A.nz; k++) q += q_of(fc[(size_t)k * A.n2]);
         }
     }
-    s_q[tid] = q;
-    __syncthreads();
-    if (tid == 0) {
-        double Q = part[(size_t)t * INV_SLOTS * nblk + 2 * nblk + blk];
-        for (int l = 0; l < INV_THREADS; l++) {
-            const double ql = s_q[l];
-            s_q[l] = Q;  // exclusive prefix of this lane
-            Q += ql;
-        }
+    // exclusive prefix of the lanes' sums in lane order, from the block's offset:
+    // a scan within each wave, then the totals of the waves before
+    double incl = q;
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o = __shfl_up(incl, d);
+        if (lane >= d) incl += o;
     }
+    double excl = __shfl_up(incl, 1);
+    if (lane == 0) excl = 0.0;
+    if (lane == 63) s_w[wv] = incl;
     __syncthreads();
+    double base = part[(size_t)t * INV_SLOTS * nblk + 2 * nblk + blk];
+    for (int w2 = 0; w2 < wv; w2++) base += s_w[w2];
     double m = 0.0, mc = 0.0, mx = 0.0;
     if (wet) {
         const double a = area[p];
-        double S0 = s_q[tid];
+        double S0 = base + excl;
         for (int k = 0; k < A.nz; k++) {
             float c = fc[(size_t)k * A.n2];
             const double S1 = S0 + q_of(c);
@@ -345,21 +361,36 @@ __global__ void __launch_bounds__(INV_THREADS) k_fix_f32(const StepArgs A, int n
             S0 = S1;
         }
     }
-    s_m[tid] = m;
-    s_c[tid] = mc;
-    s_x[tid] = mx;
+    // the block's fixed inventory (compensated butterfly over each wave, then the
+    // waves in order) and its largest cell mass with the lowest lane holding it
+    for (int d = 1; d < 64; d <<= 1) neumaier_merge(m, mc, __shfl_xor(m, d), __shfl_xor(mc, d));
+    int ml = mx > 0.0 ? tid : INV_THREADS;
+    for (int d = 1; d < 64; d <<= 1) {
+        const double ox = __shfl_xor(mx, d);
+        const int ol = __shfl_xor(ml, d);
+        if (ox > mx || (ox == mx && ol < ml)) {
+            mx = ox;
+            ml = ol;
+        }
+    }
+    if (lane == 0) {
+        s_m[wv] = m;
+        s_c[wv] = mc;
+        s_x[wv] = mx;
+        s_l[wv] = ml;
+    }
     __syncthreads();
     if (tid == 0) {
-        double S = 0.0, C = 0.0, X = 0.0;
-        int lx = -1;
-        for (int l = 0; l < INV_THREADS; l++) {
-            neumaier(S, C, s_m[l]);
-            C += s_c[l];
-            if (s_x[l] > X) {
-                X = s_x[l];
-                lx = l;
+        double S = s_m[0], C = s_c[0], X = s_x[0];
+        int L = s_l[0];
+        for (int w2 = 1; w2 < IBY; w2++) {
+            neumaier_merge(S, C, s_m[w2], s_c[w2]);
+            if (s_x[w2] > X || (s_x[w2] == X && s_l[w2] < L)) {
+                X = s_x[w2];
+                L = s_l[w2];
             }
         }
+        const int lx = X > 0.0 ? L : -1;
         double* pt = part2 + (size_t)t * 4 * nblk;
         pt[blk] = S;
         pt[nblk + blk] = C;
